@@ -1,0 +1,152 @@
+// ahip_device.hpp -- device helpers shared by the core kernels and by the
+// generated neuron-model libraries (CDNA4 / gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "annarchy_hip_model.h"
+
+namespace ahip {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------
+// Counter-based generator: Philox4x32-10 (Salmon, Moraes, Dror, Shaw,
+// "Parallel random numbers: as easy as 1, 2, 3", SC'11).  Key = (seed low,
+// seed high ^ stream), counter = (neuron, random-variable index, step low,
+// step high).  Every (neuron, variable, step) gets an independent draw, so a
+// run does not depend on the launch shape.
+// ---------------------------------------------------------------------------
+struct u32x4 {
+    uint32_t x, y, z, w;
+};
+
+__host__ __device__ inline void mulhilo(uint32_t a, uint32_t b, uint32_t& hi, uint32_t& lo) {
+    uint64_t p = (uint64_t)a * (uint64_t)b;
+    hi = (uint32_t)(p >> 32);
+    lo = (uint32_t)p;
+}
+
+__host__ __device__ inline u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint32_t hi0, lo0, hi1, lo1;
+        mulhilo(M0, c.x, hi0, lo0);
+        mulhilo(M1, c.z, hi1, lo1);
+        u32x4 n;
+        n.x = hi1 ^ c.y ^ k0;
+        n.y = lo1;
+        n.z = hi0 ^ c.w ^ k1;
+        n.w = lo0;
+        c = n;
+        k0 += W0;
+        k1 += W1;
+    }
+    return c;
+}
+
+// 53-bit uniform in [0, 1) from two 32-bit words.
+__host__ __device__ inline double u01_53(uint32_t a, uint32_t b) {
+    uint64_t v = (((uint64_t)a << 32) | b) >> 11;
+    return (double)v * (1.0 / 9007199254740992.0);
+}
+
+// Draw for (neuron i, variable k, step t): dist 0 = Uniform(a, b),
+// 1 = Normal(a, b) by the Box-Muller transform (cosine branch).
+__device__ inline double draw(uint64_t seed, uint32_t stream, int32_t i, int32_t k, int64_t t,
+                              int32_t dist, double a, double b) {
+    u32x4 c{(uint32_t)i, (uint32_t)k, (uint32_t)(uint64_t)t, (uint32_t)((uint64_t)t >> 32)};
+    u32x4 r = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32) ^ stream);
+    if (dist == 0) {
+        return a + (b - a) * u01_53(r.x, r.y);
+    }
+    double u1 = 1.0 - u01_53(r.x, r.y);  // (0, 1]
+    double u2 = u01_53(r.z, r.w);
+    double rad = sqrt(-2.0 * log(u1));
+    return a + b * (rad * cospi(2.0 * u2));
+}
+
+// ---------------------------------------------------------------------------
+// Wave reductions (64 lanes; butterfly over the lanes of a group of width W).
+// ---------------------------------------------------------------------------
+template <int W, typename T>
+__device__ inline T group_sum(T v) {
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
+    return v;
+}
+
+// Kahan-Babuska (Neumaier) compensated accumulator for FP32 psp sums.
+struct Kahan {
+    float s, c;
+    __device__ inline void add(float x) {
+        float t = s + x;
+        if (fabsf(s) >= fabsf(x))
+            c += (s - t) + x;
+        else
+            c += (x - t) + s;
+        s = t;
+    }
+    __device__ inline float value() const { return s + c; }
+};
+
+// Compensated reduction over a lane group: pairs (s, c) are combined with a
+// two-sum at every butterfly level.
+template <int W>
+__device__ inline float group_sum_kahan(float s, float c) {
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) {
+        float so = __shfl_xor(s, o, W);
+        float co = __shfl_xor(c, o, W);
+        float t = s + so;
+        float bp = t - s;
+        float err = (s - (t - bp)) + (so - bp);
+        s = t;
+        c = c + co + err;
+    }
+    return s + c;
+}
+
+__device__ inline int64_t sim_time(const ahip_pop_args& a) { return *a.t_base + a.step; }
+
+// Random variable k of neuron i at the current step: injected stream when one
+// is set (row = t - rand_t0, wrapped into the rows available), else Philox.
+template <typename T>
+__device__ inline T random_value(const ahip_pop_args& a, int32_t i, int32_t k, int64_t t,
+                                 int32_t dist, double pa, double pb) {
+    const void* inj = a.rand_inject[k];
+    if (inj) {
+        int64_t row = t - a.rand_t0;
+        int64_t n = a.rand_inject_steps;
+        row %= n;
+        if (row < 0) row += n;
+        return ((const T*)inj)[row * (int64_t)a.size + i];
+    }
+    return (T)draw(a.seed, a.stream_id, i, k, t, dist, pa, pb);
+}
+
+// Spike flags of one update block: wave ballot -> one 64-bit word per wave,
+// and the number of spikes of the block (256 threads = 4 waves).
+__device__ inline void publish_spikes(const ahip_pop_args& a, bool spk) {
+    __shared__ int32_t wave_cnt[AHIP_UPDATE_BLOCK / kWave];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    uint64_t m = __ballot(spk);
+    const int64_t word = (int64_t)blockIdx.x * (AHIP_UPDATE_BLOCK / kWave) + wid;
+    if (lane == 0) {
+        if (word * kWave < a.size) a.spike_mask[word] = m;
+        wave_cnt[wid] = __popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t s = 0;
+#pragma unroll
+        for (int w = 0; w < AHIP_UPDATE_BLOCK / kWave; w++) s += wave_cnt[w];
+        a.block_counts[blockIdx.x] = s;
+    }
+}
+
+}  // namespace ahip

@@ -1,0 +1,264 @@
+// containers.cpp -- device-resident storage formats of a rate-coded projection,
+// built on the host from the forward CSR view (rows = post ranks).
+//
+// Reference containers (host-side, include/*.hpp): CSRMatrix.hpp:142-193,
+// ELLMatrix.hpp:290-349 (column-major for the GPU, padding INT_MAX),
+// ELLRMatrix.hpp (row lengths), SELLMatrix.hpp:285-396 (slices without
+// sorting), BSRMatrix.hpp:149-280, DenseMatrix.hpp:44.  The device layouts
+// here are chosen for wave64 coalescing:
+//   CSR       row_ptr int64 over all post ranks, col int32, w
+//   ELL/ELLR  rows of the LIL only (empty rows are not stored), [k][row]
+//   SELL      C = 64/TPR rows per slice, rows sorted by length (descending)
+//             inside windows of sigma rows; slice width padded to a multiple
+//             of TPR; entry (r, k) at off + (k/TPR)*64 + r*TPR + k%TPR
+//   BSR       square tiles (2, 4 or 8), row-major inside a tile
+//   DENSE     row-major n_rows x n_cols
+//   CSR_PANEL CSR whose rows are split in column panels: panel-major order,
+//             aux[q][r] = first entry of row r in panel q
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace ahip {
+
+namespace {
+
+int upload(DevBuf& b, const void* src, size_t bytes, std::string& err) {
+    b.release();
+    if (bytes == 0) bytes = 8;  // keep a valid pointer for empty arrays
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        err = std::string("hipMalloc failed: ") + hipGetErrorString(e);
+        return AHIP_ENOMEM;
+    }
+    b.bytes = bytes;
+    if (src) {
+        e = hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            err = std::string("hipMemcpy failed: ") + hipGetErrorString(e);
+            return AHIP_EHIP;
+        }
+    } else {
+        hipMemset(b.p, 0, bytes);
+    }
+    return 0;
+}
+
+// Values in the network precision.
+int upload_vals(DevBuf& b, const std::vector<double>& v, int32_t precision, std::string& err) {
+    if (precision == AHIP_F64) return upload(b, v.data(), v.size() * sizeof(double), err);
+    std::vector<float> f(v.size());
+    for (size_t i = 0; i < v.size(); i++) f[i] = (float)v[i];
+    return upload(b, f.data(), f.size() * sizeof(float), err);
+}
+
+inline double wval(const Proj& p, int64_t j) { return p.single_w ? p.w_const : p.h_w[j]; }
+
+bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+}  // namespace
+
+int build_layout(const Proj& p, int32_t precision, const ahip_format_opts& o, RateLayout& L,
+                 std::string& err) {
+    const int32_t R = p.n_rows;
+    const std::vector<int64_t>& rp = p.h_row_ptr;
+    const std::vector<int32_t>& ci = p.h_col;
+    L = RateLayout();
+    L.format = o.format;
+    int rc;
+    switch (o.format) {
+        case AHIP_FMT_CSR: {
+            int lanes = o.lanes_per_row > 0 ? o.lanes_per_row : 64;
+            if (!pow2(lanes) || lanes > 64) { err = "CSR lanes_per_row must be a power of two <= 64"; return AHIP_EINVAL; }
+            L.lanes = lanes;
+            std::vector<double> w(p.nnz);
+            for (int64_t j = 0; j < p.nnz; j++) w[j] = wval(p, j);
+            if ((rc = upload(L.ptr, rp.data(), rp.size() * sizeof(int64_t), err))) return rc;
+            if ((rc = upload(L.idx, ci.data(), ci.size() * sizeof(int32_t), err))) return rc;
+            if ((rc = upload_vals(L.val, w, precision, err))) return rc;
+            L.stored = p.nnz;
+            return 0;
+        }
+        case AHIP_FMT_ELL:
+        case AHIP_FMT_ELLR: {
+            std::vector<int32_t> rows;
+            int32_t width = 0;
+            for (int32_t r = 0; r < R; r++) {
+                int64_t len = rp[r + 1] - rp[r];
+                if (len > 0) { rows.push_back(r); width = std::max<int32_t>(width, (int32_t)len); }
+            }
+            const int32_t n = (int32_t)rows.size();
+            std::vector<int32_t> col((size_t)n * width, INT32_MAX), rl(n);
+            std::vector<double> val((size_t)n * width, 0.0);
+            for (int32_t s = 0; s < n; s++) {
+                const int32_t r = rows[s];
+                rl[s] = (int32_t)(rp[r + 1] - rp[r]);
+                for (int64_t j = rp[r], k = 0; j < rp[r + 1]; j++, k++) {
+                    col[(size_t)k * n + s] = ci[j];
+                    val[(size_t)k * n + s] = wval(p, j);
+                }
+            }
+            L.width = width;
+            L.n_units = n;
+            L.stored = (int64_t)n * width;
+            if ((rc = upload(L.perm, rows.data(), rows.size() * sizeof(int32_t), err))) return rc;
+            if ((rc = upload(L.rl, rl.data(), rl.size() * sizeof(int32_t), err))) return rc;
+            if ((rc = upload(L.idx, col.data(), col.size() * sizeof(int32_t), err))) return rc;
+            if ((rc = upload_vals(L.val, val, precision, err))) return rc;
+            return 0;
+        }
+        case AHIP_FMT_SELL: {
+            int tpr = o.lanes_per_row > 0 ? o.lanes_per_row : 1;
+            if (!pow2(tpr) || tpr > 64) { err = "SELL threads per row must be a power of two <= 64"; return AHIP_EINVAL; }
+            const int C = 64 / tpr;
+            int sigma = o.sigma > 0 ? o.sigma : 1;
+            // row order: sort by length (descending, stable) inside sigma windows
+            std::vector<int32_t> order(R);
+            std::iota(order.begin(), order.end(), 0);
+            if (sigma > 1) {
+                for (int64_t w0 = 0; w0 < R; w0 += sigma) {
+                    int64_t w1 = std::min<int64_t>(R, w0 + sigma);
+                    std::stable_sort(order.begin() + w0, order.begin() + w1, [&](int32_t a, int32_t b) {
+                        return (rp[a + 1] - rp[a]) > (rp[b + 1] - rp[b]);
+                    });
+                }
+            }
+            const int32_t ns = (R + C - 1) / C;
+            std::vector<int64_t> off(ns + 1, 0);
+            std::vector<int32_t> perm((size_t)ns * C, -1);
+            for (int32_t s = 0; s < ns; s++) {
+                int64_t wmax = 0;
+                for (int r = 0; r < C; r++) {
+                    int64_t q = (int64_t)s * C + r;
+                    if (q < R) {
+                        perm[q] = order[q];
+                        wmax = std::max<int64_t>(wmax, rp[order[q] + 1] - rp[order[q]]);
+                    }
+                }
+                int64_t wpad = (wmax + tpr - 1) / tpr * tpr;
+                off[s + 1] = off[s] + wpad * C;
+            }
+            std::vector<int32_t> col(off[ns], 0);
+            std::vector<double> val(off[ns], 0.0);
+            for (int32_t s = 0; s < ns; s++) {
+                for (int r = 0; r < C; r++) {
+                    int64_t q = (int64_t)s * C + r;
+                    if (q >= R) break;
+                    const int32_t row = order[q];
+                    for (int64_t j = rp[row], k = 0; j < rp[row + 1]; j++, k++) {
+                        int64_t pos = off[s] + (k / tpr) * 64 + (int64_t)r * tpr + (k % tpr);
+                        col[pos] = ci[j];
+                        val[pos] = wval(p, j);
+                    }
+                }
+            }
+            L.lanes = tpr;
+            L.chunk = C;
+            L.sigma = sigma;
+            L.n_units = ns;
+            L.stored = off[ns];
+            if ((rc = upload(L.ptr, off.data(), off.size() * sizeof(int64_t), err))) return rc;
+            if ((rc = upload(L.perm, perm.data(), perm.size() * sizeof(int32_t), err))) return rc;
+            if ((rc = upload(L.idx, col.data(), col.size() * sizeof(int32_t), err))) return rc;
+            if ((rc = upload_vals(L.val, val, precision, err))) return rc;
+            return 0;
+        }
+        case AHIP_FMT_BSR: {
+            const int bs = o.chunk > 0 ? o.chunk : 4;
+            if (bs != 2 && bs != 4 && bs != 8) { err = "BSR tile must be 2, 4 or 8"; return AHIP_EINVAL; }
+            const int32_t nbr = (R + bs - 1) / bs;
+            std::vector<int32_t> brp(nbr + 1, 0), bci;
+            std::vector<double> val;
+            for (int32_t b = 0; b < nbr; b++) {
+                std::map<int32_t, std::vector<std::pair<int, double>>> tiles;
+                for (int32_t r = b * bs; r < std::min(R, (b + 1) * bs); r++)
+                    for (int64_t j = rp[r]; j < rp[r + 1]; j++)
+                        tiles[ci[j] / bs].push_back({(r % bs) * bs + (ci[j] % bs), wval(p, j)});
+                for (auto& kv : tiles) {
+                    bci.push_back(kv.first);
+                    size_t base = val.size();
+                    val.resize(base + (size_t)bs * bs, 0.0);
+                    for (auto& e : kv.second) val[base + e.first] = e.second;
+                }
+                brp[b + 1] = (int32_t)bci.size();
+            }
+            L.chunk = bs;
+            L.n_units = nbr;
+            L.stored = (int64_t)val.size();
+            if ((rc = upload(L.ptr, brp.data(), brp.size() * sizeof(int32_t), err))) return rc;
+            if ((rc = upload(L.idx, bci.data(), bci.size() * sizeof(int32_t), err))) return rc;
+            if ((rc = upload_vals(L.val, val, precision, err))) return rc;
+            return 0;
+        }
+        case AHIP_FMT_DENSE: {
+            const size_t total = (size_t)R * p.n_cols;
+            if (total > ((size_t)1 << 33)) { err = "dense layout too large"; return AHIP_EINVAL; }
+            std::vector<double> val(total, 0.0);
+            for (int32_t r = 0; r < R; r++)
+                for (int64_t j = rp[r]; j < rp[r + 1]; j++) val[(size_t)r * p.n_cols + ci[j]] = wval(p, j);
+            L.stored = (int64_t)total;
+            if ((rc = upload_vals(L.val, val, precision, err))) return rc;
+            return 0;
+        }
+        case AHIP_FMT_CSR_PANEL: {
+            int lanes = o.lanes_per_row > 0 ? o.lanes_per_row : 16;
+            if (lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64) { err = "panel lanes must be 8..64"; return AHIP_EINVAL; }
+            const size_t es = precision == AHIP_F64 ? 8 : 4;
+            int32_t pc = o.panel_cols > 0 ? o.panel_cols : (int32_t)(32768 / es);
+            if ((size_t)pc * es > 64 * 1024) { err = "panel exceeds 64 KiB of LDS"; return AHIP_EINVAL; }
+            const int32_t np = std::max<int32_t>(1, (p.n_cols + pc - 1) / pc);
+            std::vector<int64_t> aux((size_t)np * (R + 1), 0);
+            std::vector<int32_t> col(p.nnz);
+            std::vector<double> val(p.nnz);
+            // count entries per (panel, row)
+            std::vector<int64_t> cnt((size_t)np * R, 0);
+            for (int32_t r = 0; r < R; r++)
+                for (int64_t j = rp[r]; j < rp[r + 1]; j++) cnt[(size_t)(ci[j] / pc) * R + r]++;
+            int64_t off = 0;
+            for (int32_t q = 0; q < np; q++) {
+                for (int32_t r = 0; r < R; r++) {
+                    aux[(size_t)q * (R + 1) + r] = off;
+                    off += cnt[(size_t)q * R + r];
+                }
+                aux[(size_t)q * (R + 1) + R] = off;
+            }
+            std::vector<int64_t> cur(aux);
+            for (int32_t r = 0; r < R; r++)
+                for (int64_t j = rp[r]; j < rp[r + 1]; j++) {
+                    int32_t q = ci[j] / pc;
+                    int64_t pos = cur[(size_t)q * (R + 1) + r]++;
+                    col[pos] = ci[j];
+                    val[pos] = wval(p, j);
+                }
+            L.lanes = lanes;
+            L.panel_cols = pc;
+            L.n_units = np;
+            L.stored = p.nnz;
+            if ((rc = upload(L.aux, aux.data(), aux.size() * sizeof(int64_t), err))) return rc;
+            if ((rc = upload(L.idx, col.data(), col.size() * sizeof(int32_t), err))) return rc;
+            if ((rc = upload_vals(L.val, val, precision, err))) return rc;
+            return 0;
+        }
+        default:
+            err = "unknown storage format";
+            return AHIP_EINVAL;
+    }
+}
+
+// Algorithmic bytes of one psp launch, independent of the layout (padding is
+// not useful work): nnz*(s_w + 4) + (N_post+1)*4 + N_pre*s_r + 2*N_post*s_y.
+double psp_bytes(const Proj& p, int32_t precision) {
+    const double s = precision == AHIP_F64 ? 8.0 : 4.0;
+    return (double)p.nnz * (s + 4.0) + ((double)p.n_rows + 1.0) * 4.0 + (double)p.n_cols * s +
+           2.0 * (double)p.n_rows * s;
+}
+
+}  // namespace ahip

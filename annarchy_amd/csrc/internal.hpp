@@ -1,0 +1,154 @@
+// internal.hpp -- runtime structures and kernel launchers of libannarchy_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "annarchy_hip.h"
+#include "annarchy_hip_model.h"
+
+namespace ahip {
+
+// Device allocation owned by the network (RAII, moved, never copied).
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release();
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Device layout of a rate-coded projection in one storage format.
+struct RateLayout {
+    int32_t format = AHIP_FMT_CSR;
+    int32_t lanes = 64;       // CSR lanes per row / SELL threads per row
+    int32_t chunk = 64;       // SELL rows per slice / BSR tile
+    int32_t sigma = 1;
+    int32_t panel_cols = 0;
+    int64_t stored = 0;       // stored entries including padding
+    int32_t width = 0;        // ELL max row length
+    int32_t n_units = 0;      // SELL slices / BSR block rows / panels
+    DevBuf ptr;               // CSR row_ptr (int64) / SELL slice offsets (int64) / BSR block row ptr (int32)
+    DevBuf idx;               // column indices (int32), padded formats included
+    DevBuf val;               // weights (double or float)
+    DevBuf rl;                // ELLR row lengths / SELL slice widths (int32)
+    DevBuf perm;              // SELL: slice row -> post rank (int32)
+    DevBuf aux;               // CSR_PANEL: per-row panel offsets (int64)
+};
+
+struct Pop {
+    int32_t id = 0;
+    int32_t size = 0;
+    std::string lib_path;
+    void* dl = nullptr;
+    const ahip_model_desc* model = nullptr;
+    std::vector<DevBuf> slots;
+    std::vector<double> slot_host_global;   // host mirror of population-wide slots
+    DevBuf refractory, refractory_remaining, last_spike, spike_mask, block_counts;
+    DevBuf spiked, spike_count;             // int32[size], int32[1]
+    DevBuf rand_inject[AHIP_MAX_RAND];
+    int64_t rand_inject_steps[AHIP_MAX_RAND] = {0};
+    int64_t rand_t0 = 0;
+    int32_t nblocks() const { return (size + AHIP_UPDATE_BLOCK - 1) / AHIP_UPDATE_BLOCK; }
+    int32_t nwords() const { return (size + 63) / 64; }
+    int slot_index(const std::string& name) const;
+};
+
+struct Proj {
+    int32_t id = 0;
+    int32_t pre = 0, post = 0;
+    std::string target;
+    bool spiking = false;
+    int32_t n_rows = 0, n_cols = 0;
+    int64_t nnz = 0;
+    int32_t target_slot = -1;     // slot of post: _sum_<target> (rate) or g_<target> (spiking)
+    int32_t pre_slot = -1;        // rate: slot "r" of pre
+    bool accumulate = false;      // rate: add into the target instead of overwriting it
+    bool single_w = false;
+    double w_const = 0.0;
+    // host copy of the forward view (for re-layout and get/set)
+    std::vector<int64_t> h_row_ptr;
+    std::vector<int32_t> h_col;
+    std::vector<double> h_w;
+    RateLayout rate;
+    // spiking: backward (column) view
+    DevBuf col_ptr;   // int64[n_cols+1]
+    DevBuf row_idx;   // int32[nnz]
+    DevBuf w_col;     // T[nnz] column order (absent when single_w)
+    std::vector<int64_t> h_inv_idx;
+};
+
+struct Monitor {
+    int32_t id = 0;
+    int32_t pop = 0;
+    std::string attr;
+    int32_t slot = -1;
+    bool spikes = false;
+    bool after_psp = false;       // targets are recorded right after compute_psp
+    int64_t capacity = 0;
+    int64_t t_start = 0;
+    DevBuf data;                  // values [capacity][size] (T), or spike indices
+    DevBuf counts;                // spikes: int32[capacity]
+    DevBuf fill;                  // spikes: int64[1] indices written
+    int64_t idx_capacity = 0;
+};
+
+}  // namespace ahip
+
+struct ahip_net {
+    int32_t device = 0;
+    double dt = 1.0;
+    int32_t precision = AHIP_F64;
+    bool initialized = false;
+    uint64_t seed = 0;
+    int64_t t = 0;                 // host mirror of the simulation step
+    hipStream_t stream = nullptr;
+    ahip::DevBuf t_dev;            // int64 simulation step at the start of a batch
+    std::vector<std::unique_ptr<ahip::Pop>> pops;
+    std::vector<std::unique_ptr<ahip::Proj>> projs;
+    std::vector<std::unique_ptr<ahip::Monitor>> mons;
+    bool graph_enabled = true;
+    int32_t graph_batch = 0;
+    hipGraphExec_t graph_exec = nullptr;
+    int32_t graph_steps = 0;
+    bool dirty = true;             // structure changed since the last capture
+    size_t elem() const { return precision == AHIP_F64 ? sizeof(double) : sizeof(float); }
+};
+
+namespace ahip {
+
+// ---- kernel launchers (return hipError_t as int) ---------------------------
+int launch_rate_psp(const ahip_net* net, const Proj& p, const void* pre_r, void* target,
+                    hipStream_t s);
+int launch_spike_psp(const ahip_net* net, const Proj& p, const Pop& pre, void* target,
+                     hipStream_t s);
+int launch_compact(const Pop& p, hipStream_t s);
+int launch_advance_time(int64_t* t, int64_t n, hipStream_t s);
+int launch_fill(void* dst, int64_t n, int32_t elem, double value, hipStream_t s);
+int launch_fill_i32(int32_t* dst, int64_t n, int32_t value, hipStream_t s);
+int launch_fill_i64(int64_t* dst, int64_t n, int64_t value, hipStream_t s);
+int launch_record_values(const Monitor& m, const void* src, int32_t size, int32_t elem,
+                         const int64_t* t_base, int32_t step, hipStream_t s);
+int launch_record_spikes(const Monitor& m, const Pop& p, const int64_t* t_base, int32_t step,
+                         hipStream_t s);
+int launch_convert(const void* src, int32_t src_f64, void* dst, int32_t dst_f64, int64_t n,
+                   hipStream_t s);
+
+// Format builders (host, from the forward CSR view).
+int build_layout(const Proj& p, int32_t precision, const ahip_format_opts& o, RateLayout& out,
+                 std::string& err);
+double psp_bytes(const Proj& p, int32_t precision);
+
+}  // namespace ahip

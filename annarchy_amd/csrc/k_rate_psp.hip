@@ -1,0 +1,387 @@
+// k_rate_psp.hip -- rate-coded compute_psp: y = W . r over the device layouts.
+//
+// Semantics: generator/Projection/OpenMP/CSR.py:178-196 -- for each dendrite
+// (post rank), sum from 0 of w[j]*pre.r[col[j]], then target[post] += sum
+// after the per-step reset of the target (Population/OpenMPTemplates.py:
+// 269-276).  A projection that is the only (first) one into its target
+// overwrites the target instead of reset + add, which is the same value.
+// FP64: plain sums (row order differs from the sequential loop: results agree
+// to rounding).  FP32: Neumaier-compensated lane sums and a compensated lane
+// reduction.
+#include "ahip_device.hpp"
+#include "internal.hpp"
+
+namespace ahip {
+namespace {
+
+constexpr int kBlock = 256;
+
+template <typename T>
+struct Acc;
+template <>
+struct Acc<double> {
+    double s = 0.0;
+    __device__ inline void add(double x) { s += x; }
+    template <int W>
+    __device__ inline double reduce() { return group_sum<W>(s); }
+};
+template <>
+struct Acc<float> {
+    Kahan k{0.0f, 0.0f};
+    __device__ inline void add(float x) { k.add(x); }
+    template <int W>
+    __device__ inline float reduce() { return group_sum_kahan<W>(k.s, k.c); }
+};
+
+template <typename T>
+__device__ inline T ldnt(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+
+template <typename T>
+__device__ inline void store_target(T* y, int64_t r, T v, bool acc) {
+    if (acc)
+        y[r] += v;
+    else
+        y[r] = v;
+}
+
+// CSR, L lanes per row, rows indexed by post rank (row_ptr over all post
+// neurons).  Four independent loads in flight per lane.
+template <typename T, int L>
+__global__ __launch_bounds__(kBlock) void csr_rows(int32_t n_rows, const int64_t* __restrict__ row_ptr,
+                                                   const int32_t* __restrict__ col,
+                                                   const T* __restrict__ w,
+                                                   const T* __restrict__ x, T* __restrict__ y,
+                                                   int32_t acc) {
+    const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t row = gid / L;
+    const int lane = threadIdx.x & (L - 1);
+    if (row >= n_rows) return;  // whole groups leave together (L divides kBlock)
+    const int64_t beg = row_ptr[row], end = row_ptr[row + 1];
+    Acc<T> a0, a1;
+    int64_t j = beg + lane;
+    for (; j + 3 * L < end; j += 4 * L) {
+        const int32_t c0 = ldnt(col + j), c1 = ldnt(col + j + L);
+        const int32_t c2 = ldnt(col + j + 2 * L), c3 = ldnt(col + j + 3 * L);
+        const T w0 = ldnt(w + j), w1 = ldnt(w + j + L);
+        const T w2 = ldnt(w + j + 2 * L), w3 = ldnt(w + j + 3 * L);
+        a0.add(w0 * x[c0]);
+        a1.add(w1 * x[c1]);
+        a0.add(w2 * x[c2]);
+        a1.add(w3 * x[c3]);
+    }
+    for (; j < end; j += L) a0.add(ldnt(w + j) * x[ldnt(col + j)]);
+    a0.add(a1.template reduce<1>());  // fold the second accumulator in
+    const T s = a0.template reduce<L>();
+    if (lane == 0) store_target(y, row, s, acc != 0);
+}
+
+// ELLPACK, column-major over the stored rows (rows of the LIL); padding column
+// = INT32_MAX ends a row (CUDA/ELL.py:240-262).  ELLR reads the row length.
+template <typename T, bool R>
+__global__ __launch_bounds__(kBlock) void ell_rows(int32_t n_stored, int32_t width,
+                                                   const int32_t* __restrict__ post,
+                                                   const int32_t* __restrict__ rl,
+                                                   const int32_t* __restrict__ col,
+                                                   const T* __restrict__ w,
+                                                   const T* __restrict__ x, T* __restrict__ y,
+                                                   int32_t acc) {
+    const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n_stored) return;
+    Acc<T> a;
+    const int32_t len = R ? rl[r] : width;
+    for (int32_t k = 0; k < len; k++) {
+        const int64_t pos = (int64_t)k * n_stored + r;
+        const int32_t c = ldnt(col + pos);
+        if (!R && c == INT32_MAX) break;
+        a.add(ldnt(w + pos) * x[c]);
+    }
+    T s = a.template reduce<1>();
+    store_target(y, post[r], s, acc != 0);
+}
+
+// SELL-C-sigma with TPR threads per row: slice of C = 64/TPR rows, entry
+// (row r, k) at off + (k/TPR)*64 + r*TPR + k%TPR; slice width padded to a
+// multiple of TPR with column 0 / weight 0.  One wave per slice.
+template <typename T, int TPR>
+__global__ __launch_bounds__(kBlock) void sell_slices(int32_t n_slices, int32_t n_rows,
+                                                      const int64_t* __restrict__ off,
+                                                      const int32_t* __restrict__ perm,
+                                                      const int32_t* __restrict__ col,
+                                                      const T* __restrict__ w,
+                                                      const T* __restrict__ x, T* __restrict__ y,
+                                                      int32_t acc) {
+    constexpr int C = kWave / TPR;
+    const int64_t slice = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (slice >= n_slices) return;
+    const int64_t b = off[slice], e = off[slice + 1];
+    Acc<T> a0, a1;
+    int64_t p = b + lane;
+    for (; p + kWave < e; p += 2 * kWave) {
+        const int32_t c0 = ldnt(col + p), c1 = ldnt(col + p + kWave);
+        const T w0 = ldnt(w + p), w1 = ldnt(w + p + kWave);
+        a0.add(w0 * x[c0]);
+        a1.add(w1 * x[c1]);
+    }
+    if (p < e) a0.add(ldnt(w + p) * x[ldnt(col + p)]);
+    a0.add(a1.template reduce<1>());
+    const T s = a0.template reduce<TPR>();
+    const int r = lane / TPR;
+    const int64_t srow = slice * C + r;
+    if ((lane & (TPR - 1)) == 0 && srow < n_rows) {
+        const int32_t pr = perm[srow];
+        if (pr >= 0) store_target(y, pr, s, acc != 0);
+    }
+}
+
+// BSR with BS x BS tiles (row-major inside a tile); one wave per block row,
+// 64/(BS*BS) tiles in flight per wave.
+template <typename T, int BS>
+__global__ __launch_bounds__(kBlock) void bsr_rows(int32_t n_brows, int32_t n_rows, int32_t n_cols,
+                                                   const int32_t* __restrict__ brp,
+                                                   const int32_t* __restrict__ bci,
+                                                   const T* __restrict__ val,
+                                                   const T* __restrict__ x, T* __restrict__ y,
+                                                   int32_t acc) {
+    constexpr int TT = BS * BS;
+    constexpr int SLOTS = kWave / TT;
+    const int64_t br = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (br >= n_brows) return;
+    const int slot = lane / TT, e = lane % TT, tr = e / BS, tc = e % BS;
+    Acc<T> a;
+    for (int32_t t = brp[br] + slot; t < brp[br + 1]; t += SLOTS) {
+        const int32_t c = bci[t] * BS + tc;
+        const T v = ldnt(val + (int64_t)t * TT + e);
+        if (c < n_cols) a.add(v * x[c]);
+    }
+    // sum over the BS columns of a tile row (adjacent lanes), then over slots
+    T s;
+    if constexpr (sizeof(T) == 8) {
+        s = group_sum<BS>(a.s);
+#pragma unroll
+        for (int o = TT; o < kWave; o <<= 1) s += __shfl_xor(s, o, kWave);
+    } else {
+        float ss = a.k.s, cc = a.k.c;
+        float v = group_sum_kahan<BS>(ss, cc);
+#pragma unroll
+        for (int o = TT; o < kWave; o <<= 1) v += __shfl_xor(v, o, kWave);
+        s = v;
+    }
+    const int64_t row = br * BS + tr;
+    if (slot == 0 && tc == 0 && row < n_rows) store_target(y, row, s, acc != 0);
+}
+
+// Dense row-major W [n_rows x n_cols]: one wave per row.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void dense_rows(int32_t n_rows, int32_t n_cols,
+                                                     const T* __restrict__ W,
+                                                     const T* __restrict__ x, T* __restrict__ y,
+                                                     int32_t acc) {
+    const int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (row >= n_rows) return;
+    const T* wr = W + row * (int64_t)n_cols;
+    Acc<T> a0, a1;
+    int32_t k = lane;
+    for (; k + kWave < n_cols; k += 2 * kWave) {
+        a0.add(ldnt(wr + k) * x[k]);
+        a1.add(ldnt(wr + k + kWave) * x[k + kWave]);
+    }
+    if (k < n_cols) a0.add(ldnt(wr + k) * x[k]);
+    a0.add(a1.template reduce<1>());
+    const T s = a0.template reduce<kWave>();
+    if (lane == 0) store_target(y, row, s, acc != 0);
+}
+
+// CSR split into column panels: for every row, entries of panel q are
+// contiguous (the columns of a row stay ascending) and aux[q*(n_rows+1) + r]
+// gives where row r's panel-q entries start.  A workgroup stages the panel's
+// slice of x in LDS once and then serves every gather of its rows from LDS.
+// Partial sums of the panels are added in panel order by one workgroup per
+// row range (panel 0 writes, later panels add), so the sum is deterministic.
+template <typename T, int L>
+__global__ __launch_bounds__(kBlock) void csr_panel(int32_t n_rows, int32_t n_cols, int32_t panel_cols,
+                                                    int32_t n_panels, int32_t rows_per_block,
+                                                    const int64_t* __restrict__ aux,
+                                                    const int32_t* __restrict__ col,
+                                                    const T* __restrict__ w,
+                                                    const T* __restrict__ x, T* __restrict__ y,
+                                                    int32_t acc) {
+    extern __shared__ unsigned char lds_raw[];
+    T* xs = reinterpret_cast<T*>(lds_raw);
+    const int32_t row0 = blockIdx.x * rows_per_block;
+    const int32_t row1 = min(n_rows, row0 + rows_per_block);
+    const int lane = threadIdx.x & (L - 1);
+    const int grp = threadIdx.x / L;
+    constexpr int GROUPS = kBlock / L;
+    // running totals of the rows this thread group owns (rows_per_block / GROUPS each)
+    for (int32_t q = 0; q < n_panels; q++) {
+        const int32_t c0 = q * panel_cols;
+        const int32_t nc = min(panel_cols, n_cols - c0);
+        __syncthreads();
+        for (int32_t k = threadIdx.x; k < nc; k += kBlock) xs[k] = x[c0 + k];
+        __syncthreads();
+        const int64_t* pq = aux + (int64_t)q * (n_rows + 1);
+        for (int32_t r = row0 + grp; r < row1; r += GROUPS) {
+            const int64_t beg = pq[r], end = pq[r + 1];
+            Acc<T> a0, a1;
+            int64_t j = beg + lane;
+            for (; j + L < end; j += 2 * L) {
+                const int32_t cA = ldnt(col + j), cB = ldnt(col + j + L);
+                const T wA = ldnt(w + j), wB = ldnt(w + j + L);
+                a0.add(wA * xs[cA - c0]);
+                a1.add(wB * xs[cB - c0]);
+            }
+            if (j < end) a0.add(ldnt(w + j) * xs[ldnt(col + j) - c0]);
+            a0.add(a1.template reduce<1>());
+            const T s = a0.template reduce<L>();
+            if (lane == 0) {
+                if (q == 0)
+                    store_target(y, r, s, acc != 0);
+                else
+                    y[r] += s;
+            }
+        }
+    }
+}
+
+template <typename T>
+int launch_rate_t(const ahip_net* net, const Proj& p, const T* x, T* y, hipStream_t s) {
+    const RateLayout& L = p.rate;
+    const int32_t acc = p.accumulate ? 1 : 0;
+    (void)net;
+    switch (L.format) {
+        case AHIP_FMT_CSR: {
+            const int64_t threads = (int64_t)p.n_rows * L.lanes;
+            const int64_t nb = (threads + kBlock - 1) / kBlock;
+            const int64_t* rp = L.ptr.as<int64_t>();
+            const int32_t* ci = L.idx.as<int32_t>();
+            const T* w = L.val.as<T>();
+            if (nb == 0) return 0;
+            switch (L.lanes) {
+#define AHIP_CSR_CASE(N)                                                                     \
+    case N:                                                                                  \
+        hipLaunchKernelGGL((csr_rows<T, N>), dim3(nb), dim3(kBlock), 0, s, p.n_rows, rp, ci, \
+                           w, x, y, acc);                                                    \
+        break;
+                AHIP_CSR_CASE(1)
+                AHIP_CSR_CASE(2)
+                AHIP_CSR_CASE(4)
+                AHIP_CSR_CASE(8)
+                AHIP_CSR_CASE(16)
+                AHIP_CSR_CASE(32)
+                AHIP_CSR_CASE(64)
+#undef AHIP_CSR_CASE
+                default:
+                    return (int)hipErrorInvalidValue;
+            }
+            return (int)hipGetLastError();
+        }
+        case AHIP_FMT_ELL:
+        case AHIP_FMT_ELLR: {
+            const int32_t n_stored = L.n_units;
+            if (n_stored == 0) return 0;
+            const int64_t nb = (n_stored + kBlock - 1) / kBlock;
+            if (L.format == AHIP_FMT_ELL)
+                hipLaunchKernelGGL((ell_rows<T, false>), dim3(nb), dim3(kBlock), 0, s, n_stored,
+                                   L.width, L.perm.as<int32_t>(), L.rl.as<int32_t>(),
+                                   L.idx.as<int32_t>(), L.val.as<T>(), x, y, acc);
+            else
+                hipLaunchKernelGGL((ell_rows<T, true>), dim3(nb), dim3(kBlock), 0, s, n_stored,
+                                   L.width, L.perm.as<int32_t>(), L.rl.as<int32_t>(),
+                                   L.idx.as<int32_t>(), L.val.as<T>(), x, y, acc);
+            return (int)hipGetLastError();
+        }
+        case AHIP_FMT_SELL: {
+            const int64_t nb = ((int64_t)L.n_units * kWave + kBlock - 1) / kBlock;
+            if (nb == 0) return 0;
+            const int64_t* off = L.ptr.as<int64_t>();
+            const int32_t* perm = L.perm.as<int32_t>();
+            const int32_t* ci = L.idx.as<int32_t>();
+            const T* w = L.val.as<T>();
+            const int32_t n_srows = L.n_units * (kWave / L.lanes);
+            switch (L.lanes) {
+#define AHIP_SELL_CASE(N)                                                                    \
+    case N:                                                                                  \
+        hipLaunchKernelGGL((sell_slices<T, N>), dim3(nb), dim3(kBlock), 0, s, L.n_units,     \
+                           n_srows, off, perm, ci, w, x, y, acc);                            \
+        break;
+                AHIP_SELL_CASE(1)
+                AHIP_SELL_CASE(2)
+                AHIP_SELL_CASE(4)
+                AHIP_SELL_CASE(8)
+                AHIP_SELL_CASE(16)
+                AHIP_SELL_CASE(32)
+                AHIP_SELL_CASE(64)
+#undef AHIP_SELL_CASE
+                default:
+                    return (int)hipErrorInvalidValue;
+            }
+            return (int)hipGetLastError();
+        }
+        case AHIP_FMT_BSR: {
+            const int64_t nb = ((int64_t)L.n_units * kWave + kBlock - 1) / kBlock;
+            if (nb == 0) return 0;
+            if (L.chunk == 2)
+                hipLaunchKernelGGL((bsr_rows<T, 2>), dim3(nb), dim3(kBlock), 0, s, L.n_units,
+                                   p.n_rows, p.n_cols, L.ptr.as<int32_t>(), L.idx.as<int32_t>(),
+                                   L.val.as<T>(), x, y, acc);
+            else if (L.chunk == 4)
+                hipLaunchKernelGGL((bsr_rows<T, 4>), dim3(nb), dim3(kBlock), 0, s, L.n_units,
+                                   p.n_rows, p.n_cols, L.ptr.as<int32_t>(), L.idx.as<int32_t>(),
+                                   L.val.as<T>(), x, y, acc);
+            else if (L.chunk == 8)
+                hipLaunchKernelGGL((bsr_rows<T, 8>), dim3(nb), dim3(kBlock), 0, s, L.n_units,
+                                   p.n_rows, p.n_cols, L.ptr.as<int32_t>(), L.idx.as<int32_t>(),
+                                   L.val.as<T>(), x, y, acc);
+            else
+                return (int)hipErrorInvalidValue;
+            return (int)hipGetLastError();
+        }
+        case AHIP_FMT_DENSE: {
+            const int64_t nb = ((int64_t)p.n_rows * kWave + kBlock - 1) / kBlock;
+            if (nb == 0) return 0;
+            hipLaunchKernelGGL((dense_rows<T>), dim3(nb), dim3(kBlock), 0, s, p.n_rows, p.n_cols,
+                               L.val.as<T>(), x, y, acc);
+            return (int)hipGetLastError();
+        }
+        case AHIP_FMT_CSR_PANEL: {
+            const int32_t rpb = 256;
+            const int64_t nb = (p.n_rows + rpb - 1) / rpb;
+            if (nb == 0) return 0;
+            const size_t lds = (size_t)L.panel_cols * sizeof(T);
+            switch (L.lanes) {
+#define AHIP_PANEL_CASE(N)                                                                    \
+    case N:                                                                                   \
+        hipLaunchKernelGGL((csr_panel<T, N>), dim3(nb), dim3(kBlock), lds, s, p.n_rows,       \
+                           p.n_cols, L.panel_cols, L.n_units, rpb, L.aux.as<int64_t>(),       \
+                           L.idx.as<int32_t>(), L.val.as<T>(), x, y, acc);                    \
+        break;
+                AHIP_PANEL_CASE(8)
+                AHIP_PANEL_CASE(16)
+                AHIP_PANEL_CASE(32)
+                AHIP_PANEL_CASE(64)
+#undef AHIP_PANEL_CASE
+                default:
+                    return (int)hipErrorInvalidValue;
+            }
+            return (int)hipGetLastError();
+        }
+        default:
+            return (int)hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+int launch_rate_psp(const ahip_net* net, const Proj& p, const void* pre_r, void* target,
+                    hipStream_t s) {
+    if (net->precision == AHIP_F64)
+        return launch_rate_t<double>(net, p, (const double*)pre_r, (double*)target, s);
+    return launch_rate_t<float>(net, p, (const float*)pre_r, (float*)target, s);
+}
+
+}  // namespace ahip

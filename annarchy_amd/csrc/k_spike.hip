@@ -1,0 +1,193 @@
+// k_spike.hip -- spike compaction and event-driven spike propagation.
+#include "ahip_device.hpp"
+#include "internal.hpp"
+
+namespace ahip {
+namespace {
+
+constexpr int kCompactThreads = 1024;
+constexpr int kCompactPerThread = 4;
+constexpr int kCompactChunk = kCompactThreads * kCompactPerThread;  // 4096 neurons
+static_assert(kCompactChunk % AHIP_UPDATE_BLOCK == 0, "chunk must cover whole update blocks");
+
+// Block-wide exclusive scan of one int per thread (1024 threads, 16 waves).
+__device__ inline int32_t block_exclusive_scan(int32_t v, int32_t* lds_waves, int32_t& total) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    int32_t incl = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        int32_t n = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += n;
+    }
+    if (lane == kWave - 1) lds_waves[wid] = incl;
+    __syncthreads();
+    if (wid == 0) {
+        int32_t wv = lane < kCompactThreads / kWave ? lds_waves[lane] : 0;
+        int32_t wi = wv;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            int32_t n = __shfl_up(wi, o, kWave);
+            if (lane >= o) wi += n;
+        }
+        if (lane < kCompactThreads / kWave) lds_waves[lane] = wi - wv;  // exclusive wave offsets
+        if (lane == kCompactThreads / kWave - 1) lds_waves[kCompactThreads / kWave] = wi;
+    }
+    __syncthreads();
+    total = lds_waves[kCompactThreads / kWave];
+    return lds_waves[wid] + incl - v;
+}
+
+// Spike compaction, ascending neuron order (the order of the CPU gather,
+// Population/OpenMPGenerator.py:1124-1186 with a static schedule).
+// Workgroup b covers neurons [b*4096, (b+1)*4096): its global offset is the
+// sum of the spike counts of all update blocks before it (read from
+// block_counts written by the update kernel), then a block-wide scan over the
+// per-thread popcounts of the spike bit mask places every index.
+__global__ __launch_bounds__(kCompactThreads) void compact_spikes(int32_t size,
+                                                                  int32_t n_update_blocks,
+                                                                  const uint64_t* __restrict__ mask,
+                                                                  const int32_t* __restrict__ bcnt,
+                                                                  int32_t* __restrict__ spiked,
+                                                                  int32_t* __restrict__ count) {
+    __shared__ int32_t lds_waves[kCompactThreads / kWave + 1];
+    __shared__ int32_t lds_base[kCompactThreads / kWave];
+    const int32_t chunk0 = blockIdx.x * kCompactChunk;
+    const int32_t ub0 = chunk0 / AHIP_UPDATE_BLOCK;  // first update block of the chunk
+    // base = sum(bcnt[0..ub0))
+    int32_t part = 0;
+    for (int32_t k = threadIdx.x; k < ub0; k += kCompactThreads) part += bcnt[k];
+    part = group_sum<kWave>(part);
+    if ((threadIdx.x & (kWave - 1)) == 0) lds_base[threadIdx.x / kWave] = part;
+    __syncthreads();
+    int32_t base = 0;
+#pragma unroll
+    for (int w = 0; w < kCompactThreads / kWave; w++) base += lds_base[w];
+    // this thread's 4 neurons
+    const int32_t n0 = chunk0 + threadIdx.x * kCompactPerThread;
+    uint32_t bits = 0;
+    if (n0 < size) {
+        const uint64_t word = mask[n0 / 64];
+        bits = (uint32_t)(word >> (n0 % 64)) & 0xFu;
+        const int32_t valid = size - n0;
+        if (valid < kCompactPerThread) bits &= (1u << valid) - 1u;
+    }
+    int32_t total;
+    const int32_t excl = block_exclusive_scan(__popc(bits), lds_waves, total);
+    int32_t pos = base + excl;
+#pragma unroll
+    for (int k = 0; k < kCompactPerThread; k++)
+        if (bits & (1u << k)) spiked[pos++] = n0 + k;
+    // grand total: the last chunk knows base + its own total
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *count = base + total;
+    (void)n_update_blocks;
+}
+
+// Event-driven propagation over the compacted spike list, CSRC backward view
+// (generator/Projection/OpenMP/CSR.py:870-885; CUDA/CSR.py:575-657 uses one
+// block per spike with a host-read spike count).  Here the grid is fixed and
+// reads the count from device memory, so the step needs no host sync: a wave
+// takes spike s, lane 0 reads the column range of the pre-synaptic neuron and
+// broadcasts it, and the lanes add the synapse weights into the post-synaptic
+// conductances with device atomics.  With one weight per projection all
+// addends are equal, so the result does not depend on the order of the atomics.
+template <typename T, bool SINGLE_W>
+__global__ __launch_bounds__(256) void scatter_spikes(const int32_t* __restrict__ spiked,
+                                                      const int32_t* __restrict__ count,
+                                                      int32_t n_cols,
+                                                      const int64_t* __restrict__ col_ptr,
+                                                      const int32_t* __restrict__ row_idx,
+                                                      const T* __restrict__ w_col, T w_const,
+                                                      T* __restrict__ g) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int32_t n = *count;
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    for (int64_t s = (int64_t)blockIdx.x * 4 + wid; s < n; s += waves) {
+        int64_t b = 0, e = 0;
+        if (lane == 0) {
+            const int32_t pre = spiked[s];
+            if (pre >= 0 && pre < n_cols) { b = col_ptr[pre]; e = col_ptr[pre + 1]; }
+        }
+        b = __shfl(b, 0, kWave);
+        e = __shfl(e, 0, kWave);
+        for (int64_t syn = b + lane; syn < e; syn += kWave) {
+            const int32_t post = row_idx[syn];
+            const T v = SINGLE_W ? w_const : w_col[syn];
+            unsafeAtomicAdd(g + post, v);
+        }
+    }
+}
+
+template <typename T>
+int launch_scatter_t(const Proj& p, const Pop& pre, void* target, hipStream_t s) {
+    // enough waves to cover the pre population's spikes in one pass
+    int64_t waves = pre.size;
+    int64_t nb = (waves + 3) / 4;
+    if (nb > 4096) nb = 4096;
+    if (nb < 1) nb = 1;
+    if (p.single_w)
+        hipLaunchKernelGGL((scatter_spikes<T, true>), dim3(nb), dim3(256), 0, s,
+                           pre.spiked.as<int32_t>(), pre.spike_count.as<int32_t>(), p.n_cols,
+                           p.col_ptr.as<int64_t>(), p.row_idx.as<int32_t>(), (const T*)nullptr,
+                           (T)p.w_const, (T*)target);
+    else
+        hipLaunchKernelGGL((scatter_spikes<T, false>), dim3(nb), dim3(256), 0, s,
+                           pre.spiked.as<int32_t>(), pre.spike_count.as<int32_t>(), p.n_cols,
+                           p.col_ptr.as<int64_t>(), p.row_idx.as<int32_t>(), p.w_col.as<T>(),
+                           (T)0, (T*)target);
+    return (int)hipGetLastError();
+}
+
+// Spike recorder: appends the step's spike list to the monitor buffer.  One
+// workgroup; fill[0] is the number of indices written so far.
+__global__ __launch_bounds__(256) void record_spikes(const int32_t* __restrict__ spiked,
+                                                     const int32_t* __restrict__ count,
+                                                     const int64_t* __restrict__ t_base,
+                                                     int32_t step, int64_t t_start,
+                                                     int64_t capacity, int32_t* __restrict__ counts,
+                                                     int32_t* __restrict__ idx,
+                                                     int64_t idx_capacity,
+                                                     int64_t* __restrict__ fill) {
+    const int64_t slot = *t_base + step - t_start;
+    if (slot < 0 || slot >= capacity) return;
+    const int32_t n = *count;
+    const int64_t f = *fill;
+    __syncthreads();
+    for (int32_t k = threadIdx.x; k < n; k += 256)
+        if (f + k < idx_capacity) idx[f + k] = spiked[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        counts[slot] = n;
+        *fill = f + n;
+    }
+}
+
+}  // namespace
+
+int launch_compact(const Pop& p, hipStream_t s) {
+    const int32_t nb = (p.size + kCompactChunk - 1) / kCompactChunk;
+    if (nb == 0) return 0;
+    hipLaunchKernelGGL(compact_spikes, dim3(nb), dim3(kCompactThreads), 0, s, p.size, p.nblocks(),
+                       p.spike_mask.as<uint64_t>(), p.block_counts.as<int32_t>(),
+                       p.spiked.as<int32_t>(), p.spike_count.as<int32_t>());
+    return (int)hipGetLastError();
+}
+
+int launch_spike_psp(const ahip_net* net, const Proj& p, const Pop& pre, void* target,
+                     hipStream_t s) {
+    if (p.nnz == 0) return 0;
+    if (net->precision == AHIP_F64) return launch_scatter_t<double>(p, pre, target, s);
+    return launch_scatter_t<float>(p, pre, target, s);
+}
+
+int launch_record_spikes(const Monitor& m, const Pop& p, const int64_t* t_base, int32_t step,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(record_spikes, dim3(1), dim3(256), 0, s, p.spiked.as<int32_t>(),
+                       p.spike_count.as<int32_t>(), t_base, step, m.t_start, m.capacity,
+                       m.counts.as<int32_t>(), m.data.as<int32_t>(), m.idx_capacity,
+                       m.fill.as<int64_t>());
+    return (int)hipGetLastError();
+}
+
+}  // namespace ahip

@@ -1,0 +1,936 @@
+// runtime.cpp -- C-ABI of libannarchy_hip: network, populations, projections,
+// monitors and the captured simulation step.
+//
+// The step order follows the reference's singleStep()
+// (generator/Template/OpenMPBaseTemplate.py:217-325): compute_psp of every
+// projection (rate SpMV or spike scatter), record targets, population updates
+// (ODE + threshold + reset + refractory fused, random draws inside), spike
+// compaction, record variables, t++.  Unlike the reference CUDA host loop
+// (CUDABaseTemplate.py:474-589), nothing is read back per step: spike counts
+// stay on the device, grids are sized for the worst case, and batches of
+// steps are captured once in a HIP graph and replayed.
+#include <dlfcn.h>
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace ahip;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t _e = (expr);                                                             \
+        if (_e != hipSuccess)                                                               \
+            return fail(AHIP_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e),   \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+#define LCHK(expr)                                                                          \
+    do {                                                                                    \
+        int _e = (expr);                                                                    \
+        if (_e != 0)                                                                        \
+            return fail(AHIP_EHIP, "kernel launch %s failed: %s", #expr,                    \
+                        hipGetErrorString((hipError_t)_e));                                 \
+    } while (0)
+
+int dev_alloc(DevBuf& b, size_t bytes) {
+    b.release();
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(AHIP_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    HIPCHK(hipMemset(b.p, 0, bytes));
+    return 0;
+}
+
+Pop* get_pop(ahip_net* net, int32_t id) {
+    if (!net || id < 0 || id >= (int32_t)net->pops.size()) return nullptr;
+    return net->pops[id].get();
+}
+Proj* get_proj(ahip_net* net, int32_t id) {
+    if (!net || id < 0 || id >= (int32_t)net->projs.size()) return nullptr;
+    return net->projs[id].get();
+}
+Monitor* get_mon(ahip_net* net, int32_t id) {
+    if (!net || id < 0 || id >= (int32_t)net->mons.size()) return nullptr;
+    return net->mons[id].get();
+}
+
+void invalidate(ahip_net* net) {
+    net->dirty = true;
+    if (net->graph_exec) {
+        hipGraphExecDestroy(net->graph_exec);
+        net->graph_exec = nullptr;
+        net->graph_steps = 0;
+    }
+}
+
+bool layout_covers_all_rows(const Proj& p) {
+    if (p.rate.format == AHIP_FMT_ELL || p.rate.format == AHIP_FMT_ELLR)
+        return p.rate.n_units == p.n_rows;
+    return true;
+}
+
+struct StepPlan {
+    std::vector<int32_t> zero_before;   // per projection: 1 = fill target with 0 first
+};
+
+// Decide, per rate projection, whether it overwrites its target (first one
+// into that target and its layout writes every row), or adds into it.
+int prepare(ahip_net* net) {
+    if (!net->initialized) return fail(AHIP_ESTATE, "network not initialized");
+    for (auto& pp : net->projs) {
+        Proj& p = *pp;
+        if (p.nnz < 0) return fail(AHIP_ESTATE, "projection %d has no connectivity", p.id);
+    }
+    // pre-loop values (hoisted step sizes) from the host mirror of the globals
+    for (auto& pp : net->pops) {
+        Pop& p = *pp;
+        if (!p.model->preloop) continue;
+        std::vector<double> vals(p.slot_host_global);
+        p.model->preloop(p.slot_host_global.data(), net->dt, vals.data());
+        for (int32_t k = 0; k < p.model->n_slots; k++) {
+            if (p.model->slot_kind[k] != 3) continue;
+            double v = vals[k];
+            p.slot_host_global[k] = v;
+            if (net->precision == AHIP_F64) {
+                HIPCHK(hipMemcpy(p.slots[k].p, &v, sizeof(double), hipMemcpyHostToDevice));
+            } else {
+                float f = (float)v;
+                HIPCHK(hipMemcpy(p.slots[k].p, &f, sizeof(float), hipMemcpyHostToDevice));
+            }
+        }
+    }
+    return 0;
+}
+
+// Which projections write first into their target.
+void assign_accumulate(ahip_net* net, std::vector<int32_t>& zero_before) {
+    zero_before.assign(net->projs.size(), 0);
+    std::vector<std::pair<int32_t, int32_t>> seen;
+    for (auto& pp : net->projs) {
+        Proj& p = *pp;
+        if (p.spiking) { p.accumulate = true; continue; }
+        std::pair<int32_t, int32_t> key(p.post, p.target_slot);
+        bool first = true;
+        for (auto& k : seen)
+            if (k == key) first = false;
+        if (first) seen.push_back(key);
+        if (!first) {
+            p.accumulate = true;
+        } else if (layout_covers_all_rows(p)) {
+            p.accumulate = false;
+        } else {
+            p.accumulate = true;
+            zero_before[p.id] = 1;
+        }
+    }
+}
+
+ahip_pop_args make_args(ahip_net* net, Pop& p, int32_t step) {
+    ahip_pop_args a;
+    std::memset(&a, 0, sizeof(a));
+    a.size = p.size;
+    a.step = step;
+    a.t_base = net->t_dev.as<int64_t>();
+    a.dt = net->dt;
+    for (int32_t k = 0; k < p.model->n_slots && k < AHIP_MAX_SLOTS; k++) a.slot[k] = p.slots[k].p;
+    int64_t rows = 0;
+    for (int k = 0; k < AHIP_MAX_RAND; k++) {
+        a.rand_inject[k] = p.rand_inject[k].p;
+        if (p.rand_inject[k].p) rows = rows == 0 ? p.rand_inject_steps[k] : std::min(rows, p.rand_inject_steps[k]);
+    }
+    a.rand_inject_steps = rows > 0 ? rows : 1;
+    a.rand_t0 = p.rand_t0;
+    a.seed = net->seed;
+    a.stream_id = (uint32_t)p.id;
+    a.refractory = p.refractory.as<int32_t>();
+    a.refractory_remaining = p.refractory_remaining.as<int32_t>();
+    a.last_spike = p.last_spike.as<int64_t>();
+    a.spike_mask = p.spike_mask.as<uint64_t>();
+    a.block_counts = p.block_counts.as<int32_t>();
+    return a;
+}
+
+// Enqueue one simulation step (index `step` inside the current batch).
+int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_before) {
+    hipStream_t s = net->stream;
+    const size_t es = net->elem();
+    // 1. compute_psp
+    for (auto& pp : net->projs) {
+        Proj& p = *pp;
+        Pop& pre = *net->pops[p.pre];
+        Pop& post = *net->pops[p.post];
+        void* target = post.slots[p.target_slot].p;
+        if (p.spiking) {
+            LCHK(launch_spike_psp(net, p, pre, target, s));
+        } else {
+            if (zero_before[p.id]) LCHK(launch_fill(target, post.size, (int32_t)es, 0.0, s));
+            LCHK(launch_rate_psp(net, p, pre.slots[p.pre_slot].p, target, s));
+        }
+    }
+    // 2. record targets
+    for (auto& mp : net->mons) {
+        Monitor& m = *mp;
+        if (!m.after_psp) continue;
+        Pop& p = *net->pops[m.pop];
+        LCHK(launch_record_values(m, p.slots[m.slot].p, p.size, (int32_t)es,
+                                  net->t_dev.as<int64_t>(), step, s));
+    }
+    // 3. neuron updates (+ spike flags), 4. compaction
+    for (auto& pp : net->pops) {
+        Pop& p = *pp;
+        ahip_pop_args a = make_args(net, p, step);
+        ahip_launch_fn fn = net->precision == AHIP_F64 ? p.model->launch_f64 : p.model->launch_f32;
+        int rc = fn(&a, (void*)s);
+        if (rc != 0) return fail(AHIP_EHIP, "update of population %d failed: %s", p.id,
+                                 hipGetErrorString((hipError_t)rc));
+        if (p.model->spiking) LCHK(launch_compact(p, s));
+    }
+    // 5. record variables and spikes
+    for (auto& mp : net->mons) {
+        Monitor& m = *mp;
+        if (m.after_psp) continue;
+        Pop& p = *net->pops[m.pop];
+        if (m.spikes)
+            LCHK(launch_record_spikes(m, p, net->t_dev.as<int64_t>(), step, s));
+        else
+            LCHK(launch_record_values(m, p.slots[m.slot].p, p.size, (int32_t)es,
+                                      net->t_dev.as<int64_t>(), step, s));
+    }
+    return 0;
+}
+
+int enqueue_batch(ahip_net* net, int32_t nsteps, const std::vector<int32_t>& zb) {
+    for (int32_t j = 0; j < nsteps; j++) {
+        int rc = enqueue_step(net, j, zb);
+        if (rc) return rc;
+    }
+    LCHK(launch_advance_time(net->t_dev.as<int64_t>(), nsteps, net->stream));
+    return 0;
+}
+
+int build_graph(ahip_net* net, int32_t steps, const std::vector<int32_t>& zb) {
+    invalidate(net);
+    HIPCHK(hipStreamBeginCapture(net->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_batch(net, steps, zb);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(net->stream, &g);
+    if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(AHIP_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&net->graph_exec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        net->graph_exec = nullptr;
+        return fail(AHIP_EHIP, "graph instantiation failed: %s", hipGetErrorString(e));
+    }
+    net->graph_steps = steps;
+    net->dirty = false;
+    return 0;
+}
+
+int32_t auto_batch(ahip_net* net) {
+    if (net->graph_batch > 0) return net->graph_batch;
+    // keep a graph around a few thousand nodes
+    int64_t nodes = 1;
+    for (auto& p : net->pops) nodes += p->model->spiking ? 2 : 1;
+    nodes += (int64_t)net->projs.size() * 2 + (int64_t)net->mons.size();
+    int64_t b = 2000 / std::max<int64_t>(1, nodes);
+    return (int32_t)std::max<int64_t>(1, std::min<int64_t>(b, 200));
+}
+
+template <typename T>
+void to_host_t(const void* dev, double* out, int64_t n) {
+    std::vector<T> tmp(n);
+    hipMemcpy(tmp.data(), dev, n * sizeof(T), hipMemcpyDeviceToHost);
+    for (int64_t i = 0; i < n; i++) out[i] = (double)tmp[i];
+}
+
+}  // namespace
+
+void ahip::DevBuf::release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+int ahip::Pop::slot_index(const std::string& name) const {
+    for (int32_t k = 0; k < model->n_slots; k++)
+        if (name == model->slot_names[k]) return k;
+    return -1;
+}
+
+extern "C" {
+
+const char* ahip_last_error(void) { return g_err.c_str(); }
+
+const char* ahip_build_info(void) { return "annarchy_hip 0.1 gfx950 wave64"; }
+
+int ahip_device_count(int32_t* count) {
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    *count = n;
+    return 0;
+}
+
+int ahip_set_device(int32_t device) {
+    HIPCHK(hipSetDevice(device));
+    return 0;
+}
+
+int ahip_net_create(ahip_net** out) {
+    if (!out) return fail(AHIP_EINVAL, "null output pointer");
+    ahip_net* net = new ahip_net();
+    hipError_t e = hipGetDevice(&net->device);
+    if (e != hipSuccess) {
+        delete net;
+        return fail(AHIP_EHIP, "no HIP device: %s", hipGetErrorString(e));
+    }
+    e = hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete net;
+        return fail(AHIP_EHIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+    }
+    *out = net;
+    return 0;
+}
+
+int ahip_net_destroy(ahip_net* net) {
+    if (!net) return 0;
+    hipSetDevice(net->device);
+    hipStreamSynchronize(net->stream);
+    invalidate(net);
+    net->mons.clear();
+    net->projs.clear();
+    for (auto& p : net->pops) {
+        for (auto& s : p->slots) s.release();
+        // the model library stays mapped: kernels may still be referenced by HIP
+    }
+    net->pops.clear();
+    net->t_dev.release();
+    hipStreamDestroy(net->stream);
+    delete net;
+    return 0;
+}
+
+int ahip_net_initialize(ahip_net* net, double dt, int32_t precision) {
+    if (!net) return fail(AHIP_EINVAL, "null network");
+    if (!(dt > 0.0)) return fail(AHIP_EINVAL, "dt must be positive");
+    if (precision != AHIP_F64 && precision != AHIP_F32) return fail(AHIP_EINVAL, "bad precision");
+    if (net->initialized) return fail(AHIP_ESTATE, "network already initialized");
+    net->dt = dt;
+    net->precision = precision;
+    int rc = dev_alloc(net->t_dev, sizeof(int64_t));
+    if (rc) return rc;
+    net->initialized = true;
+    return 0;
+}
+
+int ahip_set_seed(ahip_net* net, uint64_t seed) {
+    if (!net) return fail(AHIP_EINVAL, "null network");
+    net->seed = seed;
+    invalidate(net);
+    return 0;
+}
+
+int ahip_set_graph_batch(ahip_net* net, int32_t steps) {
+    if (!net || steps < 0) return fail(AHIP_EINVAL, "bad graph batch");
+    net->graph_batch = steps;
+    invalidate(net);
+    return 0;
+}
+
+int ahip_set_graph_enabled(ahip_net* net, int32_t enabled) {
+    if (!net) return fail(AHIP_EINVAL, "null network");
+    net->graph_enabled = enabled != 0;
+    invalidate(net);
+    return 0;
+}
+
+int ahip_get_time(ahip_net* net, int64_t* t) {
+    if (!net || !t) return fail(AHIP_EINVAL, "null argument");
+    *t = net->t;
+    return 0;
+}
+
+int ahip_set_time(ahip_net* net, int64_t t) {
+    if (!net || !net->initialized) return fail(AHIP_ESTATE, "network not initialized");
+    HIPCHK(hipStreamSynchronize(net->stream));
+    HIPCHK(hipMemcpy(net->t_dev.p, &t, sizeof(int64_t), hipMemcpyHostToDevice));
+    net->t = t;
+    return 0;
+}
+
+int ahip_run(ahip_net* net, int64_t nsteps) {
+    if (!net) return fail(AHIP_EINVAL, "null network");
+    if (nsteps < 0) return fail(AHIP_EINVAL, "negative step count");
+    HIPCHK(hipSetDevice(net->device));
+    int rc = prepare(net);
+    if (rc) return rc;
+    std::vector<int32_t> zb;
+    assign_accumulate(net, zb);
+    int64_t left = nsteps;
+    const int32_t G = auto_batch(net);
+    if (net->graph_enabled && left >= G) {
+        if (net->dirty || !net->graph_exec || net->graph_steps != G) {
+            rc = build_graph(net, G, zb);
+            if (rc) return rc;
+        }
+        while (left >= G) {
+            HIPCHK(hipGraphLaunch(net->graph_exec, net->stream));
+            left -= G;
+        }
+    }
+    while (left > 0) {
+        const int32_t n = (int32_t)std::min<int64_t>(left, G);
+        rc = enqueue_batch(net, n, zb);
+        if (rc) return rc;
+        left -= n;
+    }
+    HIPCHK(hipStreamSynchronize(net->stream));
+    net->t += nsteps;
+    return 0;
+}
+
+int ahip_step(ahip_net* net) { return ahip_run(net, 1); }
+
+int ahip_run_until(ahip_net* net, int64_t max_steps, const int32_t* pops, int32_t npops,
+                   int32_t and_or, int64_t* done) {
+    if (!net || !done || (npops > 0 && !pops)) return fail(AHIP_EINVAL, "null argument");
+    for (int32_t k = 0; k < npops; k++) {
+        Pop* p = get_pop(net, pops[k]);
+        if (!p || !p->model->spiking) return fail(AHIP_EINVAL, "run_until needs spiking populations");
+    }
+    int64_t n = 0;
+    while (n < max_steps) {
+        int rc = ahip_run(net, 1);
+        if (rc) return rc;
+        n++;
+        int hits = 0;
+        for (int32_t k = 0; k < npops; k++) {
+            int32_t c = 0;
+            HIPCHK(hipMemcpy(&c, net->pops[pops[k]]->spike_count.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+            if (c > 0) hits++;
+        }
+        if (npops > 0 && ((and_or == 0 && hits > 0) || (and_or != 0 && hits == npops))) break;
+    }
+    *done = n;
+    return 0;
+}
+
+// ---- populations -----------------------------------------------------------
+
+int ahip_pop_create(ahip_net* net, int32_t size, const char* model_lib, int32_t* pop_id) {
+    if (!net || !model_lib || !pop_id) return fail(AHIP_EINVAL, "null argument");
+    if (!net->initialized) return fail(AHIP_ESTATE, "initialize the network first");
+    if (size <= 0) return fail(AHIP_EINVAL, "population size must be positive");
+    void* dl = dlopen(model_lib, RTLD_NOW | RTLD_LOCAL);
+    if (!dl) return fail(AHIP_EMODEL, "cannot load model library %s: %s", model_lib, dlerror());
+    auto describe = (ahip_model_describe_fn)dlsym(dl, "ahip_model_describe");
+    if (!describe) return fail(AHIP_EMODEL, "%s does not export ahip_model_describe", model_lib);
+    const ahip_model_desc* d = describe();
+    if (!d || d->abi != AHIP_MODEL_ABI)
+        return fail(AHIP_EMODEL, "%s: model ABI %d, runtime expects %d", model_lib, d ? d->abi : -1,
+                    AHIP_MODEL_ABI);
+    if (d->n_slots > AHIP_MAX_SLOTS || d->n_rand > AHIP_MAX_RAND)
+        return fail(AHIP_EMODEL, "%s: too many attributes", model_lib);
+    auto p = std::make_unique<Pop>();
+    p->id = (int32_t)net->pops.size();
+    p->size = size;
+    p->lib_path = model_lib;
+    p->dl = dl;
+    p->model = d;
+    const size_t es = net->elem();
+    p->slots.resize(d->n_slots);
+    p->slot_host_global.assign(d->n_slots, 0.0);
+    for (int32_t k = 0; k < d->n_slots; k++) {
+        const int64_t n = d->slot_local[k] ? size : 1;
+        int rc = dev_alloc(p->slots[k], n * es);
+        if (rc) return rc;
+        LCHK(launch_fill(p->slots[k].p, n, (int32_t)es, d->slot_init[k], net->stream));
+        if (!d->slot_local[k]) p->slot_host_global[k] = d->slot_init[k];
+    }
+    if (d->spiking) {
+        int rc;
+        if ((rc = dev_alloc(p->refractory, size * sizeof(int32_t)))) return rc;
+        if ((rc = dev_alloc(p->refractory_remaining, size * sizeof(int32_t)))) return rc;
+        if ((rc = dev_alloc(p->last_spike, size * sizeof(int64_t)))) return rc;
+        if ((rc = dev_alloc(p->spike_mask, (size_t)p->nwords() * sizeof(uint64_t)))) return rc;
+        if ((rc = dev_alloc(p->block_counts, (size_t)p->nblocks() * sizeof(int32_t)))) return rc;
+        if ((rc = dev_alloc(p->spiked, size * sizeof(int32_t)))) return rc;
+        if ((rc = dev_alloc(p->spike_count, sizeof(int32_t)))) return rc;
+        LCHK(launch_fill_i64(p->last_spike.as<int64_t>(), size, -10000, net->stream));
+    }
+    HIPCHK(hipStreamSynchronize(net->stream));
+    *pop_id = p->id;
+    net->pops.push_back(std::move(p));
+    invalidate(net);
+    return 0;
+}
+
+int ahip_pop_size(ahip_net* net, int32_t pop, int32_t* size) {
+    Pop* p = get_pop(net, pop);
+    if (!p || !size) return fail(AHIP_EINVAL, "bad population");
+    *size = p->size;
+    return 0;
+}
+
+int ahip_pop_set(ahip_net* net, int32_t pop, const char* attr, const double* values, int64_t n) {
+    Pop* p = get_pop(net, pop);
+    if (!p || !attr || !values) return fail(AHIP_EINVAL, "bad population or attribute");
+    int k = p->slot_index(attr);
+    if (k < 0) return fail(AHIP_EINVAL, "population %d has no attribute '%s'", pop, attr);
+    const int64_t want = p->model->slot_local[k] ? p->size : 1;
+    if (n != want) return fail(AHIP_EINVAL, "attribute '%s' expects %lld values, got %lld", attr,
+                               (long long)want, (long long)n);
+    HIPCHK(hipStreamSynchronize(net->stream));
+    if (net->precision == AHIP_F64) {
+        HIPCHK(hipMemcpy(p->slots[k].p, values, n * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(n);
+        for (int64_t i = 0; i < n; i++) f[i] = (float)values[i];
+        HIPCHK(hipMemcpy(p->slots[k].p, f.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    }
+    if (!p->model->slot_local[k]) p->slot_host_global[k] = values[0];
+    return 0;
+}
+
+int ahip_pop_get(ahip_net* net, int32_t pop, const char* attr, double* values, int64_t n) {
+    Pop* p = get_pop(net, pop);
+    if (!p || !attr || !values) return fail(AHIP_EINVAL, "bad population or attribute");
+    int k = p->slot_index(attr);
+    if (k < 0) return fail(AHIP_EINVAL, "population %d has no attribute '%s'", pop, attr);
+    const int64_t want = p->model->slot_local[k] ? p->size : 1;
+    if (n != want) return fail(AHIP_EINVAL, "attribute '%s' holds %lld values, got room for %lld",
+                               attr, (long long)want, (long long)n);
+    HIPCHK(hipStreamSynchronize(net->stream));
+    if (net->precision == AHIP_F64) {
+        HIPCHK(hipMemcpy(values, p->slots[k].p, n * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        to_host_t<float>(p->slots[k].p, values, n);
+    }
+    return 0;
+}
+
+static DevBuf* int_attr(Pop* p, const char* attr) {
+    if (!p->model->spiking) return nullptr;
+    if (!strcmp(attr, "refractory")) return &p->refractory;
+    if (!strcmp(attr, "refractory_remaining")) return &p->refractory_remaining;
+    return nullptr;
+}
+
+int ahip_pop_set_int(ahip_net* net, int32_t pop, const char* attr, const int32_t* values, int64_t n) {
+    Pop* p = get_pop(net, pop);
+    if (!p || !attr || !values) return fail(AHIP_EINVAL, "bad population or attribute");
+    DevBuf* b = int_attr(p, attr);
+    if (!b) return fail(AHIP_EINVAL, "population %d has no integer attribute '%s'", pop, attr);
+    if (n != p->size) return fail(AHIP_EINVAL, "'%s' expects %d values", attr, p->size);
+    HIPCHK(hipStreamSynchronize(net->stream));
+    HIPCHK(hipMemcpy(b->p, values, n * sizeof(int32_t), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int ahip_pop_get_int(ahip_net* net, int32_t pop, const char* attr, int32_t* values, int64_t n) {
+    Pop* p = get_pop(net, pop);
+    if (!p || !attr || !values) return fail(AHIP_EINVAL, "bad population or attribute");
+    DevBuf* b = int_attr(p, attr);
+    if (!b) return fail(AHIP_EINVAL, "population %d has no integer attribute '%s'", pop, attr);
+    if (n != p->size) return fail(AHIP_EINVAL, "'%s' holds %d values", attr, p->size);
+    HIPCHK(hipStreamSynchronize(net->stream));
+    HIPCHK(hipMemcpy(values, b->p, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int ahip_pop_get_last_spike(ahip_net* net, int32_t pop, int64_t* values, int64_t n) {
+    Pop* p = get_pop(net, pop);
+    if (!p || !values || !p->model->spiking) return fail(AHIP_EINVAL, "bad spiking population");
+    if (n != p->size) return fail(AHIP_EINVAL, "last_spike holds %d values", p->size);
+    HIPCHK(hipStreamSynchronize(net->stream));
+    HIPCHK(hipMemcpy(values, p->last_spike.p, n * sizeof(int64_t), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int ahip_pop_spikes(ahip_net* net, int32_t pop, int32_t* out, int32_t cap, int32_t* n) {
+    Pop* p = get_pop(net, pop);
+    if (!p || !n || !p->model->spiking) return fail(AHIP_EINVAL, "bad spiking population");
+    HIPCHK(hipStreamSynchronize(net->stream));
+    int32_t c = 0;
+    HIPCHK(hipMemcpy(&c, p->spike_count.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+    *n = c;
+    if (c > 0 && out) {
+        if (c > cap) return fail(AHIP_EINVAL, "spike buffer too small (%d > %d)", c, cap);
+        HIPCHK(hipMemcpy(out, p->spiked.p, c * sizeof(int32_t), hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int ahip_pop_reset_spiking(ahip_net* net, int32_t pop) {
+    Pop* p = get_pop(net, pop);
+    if (!p || !p->model->spiking) return fail(AHIP_EINVAL, "bad spiking population");
+    LCHK(launch_fill_i32(p->refractory_remaining.as<int32_t>(), p->size, 0, net->stream));
+    LCHK(launch_fill_i64(p->last_spike.as<int64_t>(), p->size, -10000, net->stream));
+    LCHK(launch_fill_i32(p->spike_count.as<int32_t>(), 1, 0, net->stream));
+    HIPCHK(hipStreamSynchronize(net->stream));
+    return 0;
+}
+
+int ahip_pop_inject_random(ahip_net* net, int32_t pop, int32_t k, const double* buf,
+                           int64_t nsteps) {
+    Pop* p = get_pop(net, pop);
+    if (!p || k < 0 || k >= p->model->n_rand) return fail(AHIP_EINVAL, "bad population or random variable");
+    HIPCHK(hipStreamSynchronize(net->stream));
+    invalidate(net);
+    if (!buf) {
+        p->rand_inject[k].release();
+        p->rand_inject_steps[k] = 0;
+        return 0;
+    }
+    if (nsteps <= 0) return fail(AHIP_EINVAL, "nsteps must be positive");
+    const int64_t n = nsteps * p->size;
+    int rc = dev_alloc(p->rand_inject[k], n * net->elem());
+    if (rc) return rc;
+    if (net->precision == AHIP_F64) {
+        HIPCHK(hipMemcpy(p->rand_inject[k].p, buf, n * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(n);
+        for (int64_t i = 0; i < n; i++) f[i] = (float)buf[i];
+        HIPCHK(hipMemcpy(p->rand_inject[k].p, f.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    }
+    p->rand_inject_steps[k] = nsteps;
+    p->rand_t0 = net->t;
+    return 0;
+}
+
+// ---- projections -----------------------------------------------------------
+
+int ahip_proj_create(ahip_net* net, int32_t pre, int32_t post, const char* target,
+                     int32_t spiking, int32_t* proj_id) {
+    if (!net || !target || !proj_id) return fail(AHIP_EINVAL, "null argument");
+    Pop* a = get_pop(net, pre);
+    Pop* b = get_pop(net, post);
+    if (!a || !b) return fail(AHIP_EINVAL, "bad pre/post population");
+    auto p = std::make_unique<Proj>();
+    p->id = (int32_t)net->projs.size();
+    p->pre = pre;
+    p->post = post;
+    p->target = target;
+    p->spiking = spiking != 0;
+    p->nnz = -1;
+    if (p->spiking) {
+        if (!a->model->spiking) return fail(AHIP_EINVAL, "spiking projection needs a spiking pre-synaptic population");
+        p->target_slot = b->slot_index(std::string("g_") + target);
+        if (p->target_slot < 0 || !b->model->slot_local[p->target_slot])
+            return fail(AHIP_EINVAL, "post-synaptic population has no conductance g_%s", target);
+    } else {
+        p->target_slot = b->slot_index(std::string("_sum_") + target);
+        if (p->target_slot < 0) return fail(AHIP_EINVAL, "post-synaptic population has no target sum(%s)", target);
+        p->pre_slot = a->slot_index("r");
+        if (p->pre_slot < 0 || !a->model->slot_local[p->pre_slot])
+            return fail(AHIP_EINVAL, "rate projection needs a pre-synaptic variable r");
+    }
+    *proj_id = p->id;
+    net->projs.push_back(std::move(p));
+    invalidate(net);
+    return 0;
+}
+
+static int build_backward(ahip_net* net, Proj& p) {
+    // CSRC backward view (include/CSRCMatrix.hpp:164-210): per pre neuron the
+    // post ranks ascending; weights are stored in column order.
+    std::vector<int64_t> col_ptr(p.n_cols + 1, 0);
+    for (int64_t j = 0; j < p.nnz; j++) col_ptr[p.h_col[j] + 1]++;
+    for (int32_t c = 0; c < p.n_cols; c++) col_ptr[c + 1] += col_ptr[c];
+    std::vector<int64_t> cur(col_ptr.begin(), col_ptr.end() - 1);
+    std::vector<int32_t> row_idx(p.nnz);
+    p.h_inv_idx.assign(p.nnz, 0);
+    for (int32_t r = 0; r < p.n_rows; r++)
+        for (int64_t j = p.h_row_ptr[r]; j < p.h_row_ptr[r + 1]; j++) {
+            int64_t q = cur[p.h_col[j]]++;
+            row_idx[q] = r;
+            p.h_inv_idx[q] = j;
+        }
+    int rc;
+    if ((rc = dev_alloc(p.col_ptr, col_ptr.size() * sizeof(int64_t)))) return rc;
+    HIPCHK(hipMemcpy(p.col_ptr.p, col_ptr.data(), col_ptr.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    if ((rc = dev_alloc(p.row_idx, row_idx.size() * sizeof(int32_t)))) return rc;
+    if (p.nnz) HIPCHK(hipMemcpy(p.row_idx.p, row_idx.data(), row_idx.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    p.w_col.release();
+    if (!p.single_w) {
+        if ((rc = dev_alloc(p.w_col, p.nnz * net->elem()))) return rc;
+        if (net->precision == AHIP_F64) {
+            std::vector<double> w(p.nnz);
+            for (int64_t q = 0; q < p.nnz; q++) w[q] = p.h_w[p.h_inv_idx[q]];
+            if (p.nnz) HIPCHK(hipMemcpy(p.w_col.p, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> w(p.nnz);
+            for (int64_t q = 0; q < p.nnz; q++) w[q] = (float)p.h_w[p.h_inv_idx[q]];
+            if (p.nnz) HIPCHK(hipMemcpy(p.w_col.p, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+        }
+    }
+    return 0;
+}
+
+int ahip_proj_init_csr(ahip_net* net, int32_t proj, int32_t n_rows, int32_t n_cols,
+                       const int64_t* row_ptr, const int32_t* col_idx, const double* w,
+                       double w_const, const ahip_format_opts* opts) {
+    Proj* p = get_proj(net, proj);
+    if (!p || !row_ptr || (!col_idx && row_ptr[n_rows] > 0)) return fail(AHIP_EINVAL, "bad projection or arrays");
+    Pop& pre = *net->pops[p->pre];
+    Pop& post = *net->pops[p->post];
+    if (n_rows != post.size) return fail(AHIP_EINVAL, "n_rows (%d) must equal the post size (%d)", n_rows, post.size);
+    if (n_cols != pre.size) return fail(AHIP_EINVAL, "n_cols (%d) must equal the pre size (%d)", n_cols, pre.size);
+    if (row_ptr[0] != 0) return fail(AHIP_EINVAL, "row_ptr[0] must be 0");
+    for (int32_t r = 0; r < n_rows; r++)
+        if (row_ptr[r + 1] < row_ptr[r]) return fail(AHIP_EINVAL, "row_ptr is not monotonic at row %d", r);
+    const int64_t nnz = row_ptr[n_rows];
+    for (int64_t j = 0; j < nnz; j++)
+        if (col_idx[j] < 0 || col_idx[j] >= n_cols)
+            return fail(AHIP_EINVAL, "column index %d out of range at %lld", col_idx[j], (long long)j);
+    HIPCHK(hipStreamSynchronize(net->stream));
+    p->n_rows = n_rows;
+    p->n_cols = n_cols;
+    p->nnz = nnz;
+    p->h_row_ptr.assign(row_ptr, row_ptr + n_rows + 1);
+    p->h_col.assign(col_idx, col_idx + nnz);
+    p->single_w = (w == nullptr);
+    p->w_const = w_const;
+    if (w)
+        p->h_w.assign(w, w + nnz);
+    else
+        p->h_w.clear();
+    invalidate(net);
+    if (p->spiking) return build_backward(net, *p);
+    ahip_format_opts o{AHIP_FMT_CSR, 64, 0, 1, 0};
+    if (opts) o = *opts;
+    std::string err;
+    int rc = build_layout(*p, net->precision, o, p->rate, err);
+    if (rc) return fail(rc, "%s", err.c_str());
+    return 0;
+}
+
+int ahip_proj_set_format(ahip_net* net, int32_t proj, const ahip_format_opts* opts) {
+    Proj* p = get_proj(net, proj);
+    if (!p || !opts || p->nnz < 0) return fail(AHIP_EINVAL, "bad projection");
+    if (p->spiking) return fail(AHIP_EINVAL, "spiking projections use the CSRC backward view");
+    HIPCHK(hipStreamSynchronize(net->stream));
+    invalidate(net);
+    std::string err;
+    RateLayout L;
+    int rc = build_layout(*p, net->precision, *opts, L, err);
+    if (rc) return fail(rc, "%s", err.c_str());
+    p->rate = std::move(L);
+    return 0;
+}
+
+int ahip_proj_get_format(ahip_net* net, int32_t proj, ahip_format_opts* opts) {
+    Proj* p = get_proj(net, proj);
+    if (!p || !opts) return fail(AHIP_EINVAL, "bad projection");
+    opts->format = p->rate.format;
+    opts->lanes_per_row = p->rate.lanes;
+    opts->chunk = p->rate.chunk;
+    opts->sigma = p->rate.sigma;
+    opts->panel_cols = p->rate.panel_cols;
+    return 0;
+}
+
+int ahip_proj_set_launch(ahip_net* net, int32_t proj, int32_t lanes_per_row) {
+    Proj* p = get_proj(net, proj);
+    if (!p || p->spiking || p->rate.format != AHIP_FMT_CSR) return fail(AHIP_EINVAL, "launch config applies to CSR rate projections");
+    if (lanes_per_row < 1 || lanes_per_row > 64 || (lanes_per_row & (lanes_per_row - 1)))
+        return fail(AHIP_EINVAL, "lanes_per_row must be a power of two <= 64");
+    p->rate.lanes = lanes_per_row;
+    invalidate(net);
+    return 0;
+}
+
+int ahip_proj_nb_synapses(ahip_net* net, int32_t proj, int64_t* nnz) {
+    Proj* p = get_proj(net, proj);
+    if (!p || !nnz) return fail(AHIP_EINVAL, "bad projection");
+    *nnz = p->nnz < 0 ? 0 : p->nnz;
+    return 0;
+}
+
+int ahip_proj_get_w(ahip_net* net, int32_t proj, double* w, int64_t nnz) {
+    Proj* p = get_proj(net, proj);
+    if (!p || !w || nnz != p->nnz) return fail(AHIP_EINVAL, "bad projection or size");
+    for (int64_t j = 0; j < nnz; j++) w[j] = p->single_w ? p->w_const : p->h_w[j];
+    return 0;
+}
+
+int ahip_proj_set_w(ahip_net* net, int32_t proj, const double* w, int64_t nnz) {
+    Proj* p = get_proj(net, proj);
+    if (!p || !w || nnz != p->nnz) return fail(AHIP_EINVAL, "bad projection or size");
+    HIPCHK(hipStreamSynchronize(net->stream));
+    p->single_w = false;
+    p->h_w.assign(w, w + nnz);
+    invalidate(net);
+    if (p->spiking) return build_backward(net, *p);
+    ahip_format_opts o{p->rate.format, p->rate.lanes, p->rate.chunk, p->rate.sigma, p->rate.panel_cols};
+    std::string err;
+    RateLayout L;
+    int rc = build_layout(*p, net->precision, o, L, err);
+    if (rc) return fail(rc, "%s", err.c_str());
+    p->rate = std::move(L);
+    return 0;
+}
+
+int ahip_proj_psp_bytes(ahip_net* net, int32_t proj, double* bytes) {
+    Proj* p = get_proj(net, proj);
+    if (!p || !bytes || p->nnz < 0) return fail(AHIP_EINVAL, "bad projection");
+    *bytes = psp_bytes(*p, net->precision);
+    return 0;
+}
+
+int ahip_proj_time_psp(ahip_net* net, int32_t proj, int32_t iters, double* avg_ms) {
+    Proj* p = get_proj(net, proj);
+    if (!p || !avg_ms || iters <= 0 || p->nnz < 0) return fail(AHIP_EINVAL, "bad projection or iteration count");
+    HIPCHK(hipSetDevice(net->device));
+    int rc = prepare(net);
+    if (rc) return rc;
+    std::vector<int32_t> zb;
+    assign_accumulate(net, zb);
+    Pop& pre = *net->pops[p->pre];
+    Pop& post = *net->pops[p->post];
+    void* target = post.slots[p->target_slot].p;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    // one untimed launch, then `iters` timed ones on the network stream
+    auto one = [&]() -> int {
+        if (p->spiking) return launch_spike_psp(net, *p, pre, target, net->stream);
+        return launch_rate_psp(net, *p, pre.slots[p->pre_slot].p, target, net->stream);
+    };
+    LCHK(one());
+    HIPCHK(hipEventRecord(e0, net->stream));
+    for (int32_t i = 0; i < iters; i++) LCHK(one());
+    HIPCHK(hipEventRecord(e1, net->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *avg_ms = (double)ms / iters;
+    return 0;
+}
+
+// ---- monitors ---------------------------------------------------------------
+
+int ahip_monitor_create(ahip_net* net, int32_t pop, const char* attr, int64_t capacity,
+                        int32_t* mon_id) {
+    Pop* p = get_pop(net, pop);
+    if (!p || !attr || !mon_id || capacity <= 0) return fail(AHIP_EINVAL, "bad monitor arguments");
+    auto m = std::make_unique<Monitor>();
+    m->id = (int32_t)net->mons.size();
+    m->pop = pop;
+    m->attr = attr;
+    m->capacity = capacity;
+    m->t_start = net->t;
+    int rc;
+    if (!strcmp(attr, "spike")) {
+        if (!p->model->spiking) return fail(AHIP_EINVAL, "spike recording needs a spiking population");
+        m->spikes = true;
+        // worst case every neuron every step, bounded to 256 Mi indices
+        int64_t cap = std::min<int64_t>((int64_t)p->size * capacity, (int64_t)1 << 28);
+        m->idx_capacity = cap;
+        if ((rc = dev_alloc(m->data, cap * sizeof(int32_t)))) return rc;
+        if ((rc = dev_alloc(m->counts, capacity * sizeof(int32_t)))) return rc;
+        if ((rc = dev_alloc(m->fill, sizeof(int64_t)))) return rc;
+    } else {
+        m->slot = p->slot_index(attr);
+        if (m->slot < 0 || !p->model->slot_local[m->slot])
+            return fail(AHIP_EINVAL, "population %d has no local attribute '%s'", pop, attr);
+        const int kind = p->model->slot_kind[m->slot];
+        m->after_psp = (kind == 2) || (p->model->spiking && !strncmp(attr, "g_", 2));
+        if ((rc = dev_alloc(m->data, (size_t)capacity * p->size * net->elem()))) return rc;
+    }
+    *mon_id = m->id;
+    net->mons.push_back(std::move(m));
+    invalidate(net);
+    return 0;
+}
+
+int ahip_monitor_steps(ahip_net* net, int32_t mon, int64_t* steps) {
+    Monitor* m = get_mon(net, mon);
+    if (!m || !steps) return fail(AHIP_EINVAL, "bad monitor");
+    int64_t s = net->t - m->t_start;
+    *steps = std::max<int64_t>(0, std::min<int64_t>(s, m->capacity));
+    return 0;
+}
+
+int ahip_monitor_get(ahip_net* net, int32_t mon, double* out, int64_t n) {
+    Monitor* m = get_mon(net, mon);
+    if (!m || !out || m->spikes) return fail(AHIP_EINVAL, "bad value monitor");
+    int64_t steps;
+    ahip_monitor_steps(net, mon, &steps);
+    const int64_t size = net->pops[m->pop]->size;
+    if (n != steps * size) return fail(AHIP_EINVAL, "monitor holds %lld values", (long long)(steps * size));
+    HIPCHK(hipStreamSynchronize(net->stream));
+    if (n == 0) return 0;
+    if (net->precision == AHIP_F64) {
+        HIPCHK(hipMemcpy(out, m->data.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        to_host_t<float>(m->data.p, out, n);
+    }
+    return 0;
+}
+
+int ahip_monitor_get_spikes(ahip_net* net, int32_t mon, int32_t* counts, int64_t steps,
+                            int32_t* idx, int64_t idx_cap, int64_t* total) {
+    Monitor* m = get_mon(net, mon);
+    if (!m || !m->spikes || !total) return fail(AHIP_EINVAL, "bad spike monitor");
+    int64_t have;
+    ahip_monitor_steps(net, mon, &have);
+    if (steps != have) return fail(AHIP_EINVAL, "monitor holds %lld steps", (long long)have);
+    HIPCHK(hipStreamSynchronize(net->stream));
+    int64_t fill = 0;
+    HIPCHK(hipMemcpy(&fill, m->fill.p, sizeof(int64_t), hipMemcpyDeviceToHost));
+    *total = fill;
+    if (fill > m->idx_capacity) return fail(AHIP_ESTATE, "spike monitor overflowed its buffer");
+    if (counts && steps > 0) HIPCHK(hipMemcpy(counts, m->counts.p, steps * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (idx) {
+        if (fill > idx_cap) return fail(AHIP_EINVAL, "index buffer too small (%lld)", (long long)fill);
+        if (fill > 0) HIPCHK(hipMemcpy(idx, m->data.p, fill * sizeof(int32_t), hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int ahip_monitor_clear(ahip_net* net, int32_t mon) {
+    Monitor* m = get_mon(net, mon);
+    if (!m) return fail(AHIP_EINVAL, "bad monitor");
+    HIPCHK(hipStreamSynchronize(net->stream));
+    m->t_start = net->t;
+    if (m->spikes) {
+        LCHK(launch_fill_i64(m->fill.as<int64_t>(), 1, 0, net->stream));
+        HIPCHK(hipStreamSynchronize(net->stream));
+    }
+    invalidate(net);
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,489 @@
+"""Neuron description -> C statements (the equation front end).
+
+Restates the parts of ANNarchy's parser that produce the per-variable C
+snippets spliced into the update template:
+
+* parameters / equations / flags: ``parser/Extraction.py:251-560``
+  (``: population`` makes an attribute population-wide; ``init=``, ``min=``,
+  ``max=``, method flags);
+* random distributions ``Uniform(a, b)`` / ``Normal(mu, sigma)`` become the
+  local variables ``rand_k`` drawn once per step (``Extraction.py:32-122``);
+* ``sum(target)`` becomes ``_sum_<target>[i]`` (``AnalyseNeuron.py:244-250``);
+* ODEs: explicit Euler ``_x = <sympy-simplified rhs>; x += dt*_x ;``
+  (``parser/Equation.py:276-307``) and exponential Euler with the global step
+  size hoisted into a pre-loop value (``Equation.py:559-604``, standardisation
+  ``:651-741``); assignments, increments and conditions
+  (``Equation.py:742-900``); C text from ``sympy.ccode(precision=8)``
+  (``Equation.py:153-197``), with the ``f`` suffix and ``expf``/``powf`` names
+  in single precision.
+
+The symbols use the reference's placeholders (``v%(local_index)s``,
+``tau%(global_index)s``, ``dt``, ``(double(t)*dt)``) so the emitted text is
+the text the reference splices into its loops.
+"""
+
+from __future__ import annotations
+
+import re
+from dataclasses import dataclass, field
+
+import sympy as sp
+from sympy.parsing.sympy_parser import convert_xor, parse_expr, standard_transformations
+
+DISTRIBUTIONS = {"Uniform": 0, "Normal": 1}
+
+_FUNCTIONS = {
+    "pos": sp.Function("positive"),
+    "positive": sp.Function("positive"),
+    "neg": sp.Function("negative"),
+    "negative": sp.Function("negative"),
+    "clip": sp.Function("clip"),
+    "ite": sp.Function("ite", nargs=3),
+    "modulo": sp.Function("modulo"),
+    "power": sp.Function("power", nargs=2),
+    "fabs": sp.Function("fabs"),
+    "round": sp.Function("round"),
+    "True": sp.Symbol("true"),
+    "False": sp.Symbol("false"),
+}
+_USER_FUNCTIONS = {
+    "pos": "positive",
+    "positive": "positive",
+    "neg": "negative",
+    "negative": "negative",
+    "modulo": "modulo",
+    "fabs": "fabs",
+    "clip": "clip",
+    "ite": "ite",
+    "power": "power",
+}
+
+
+class EquationError(ValueError):
+    pass
+
+
+@dataclass
+class Variable:
+    name: str
+    eq: str = ""
+    locality: str = "local"          # local | global
+    flags: list = field(default_factory=list)
+    bounds: dict = field(default_factory=dict)
+    init: float = 0.0
+    is_parameter: bool = False
+    ctype: str = "double"
+    # filled by the analysis
+    method: str = "explicit"
+    cpp: str = ""                    # ODE increment statement or assignment
+    switch: str | None = None        # ODE: x += dt*_x ;
+    pre_loop: dict = field(default_factory=dict)
+    bound_code: str = ""
+
+
+@dataclass
+class RandomVar:
+    name: str
+    dist: str
+    args: tuple
+    locality: str
+
+
+@dataclass
+class NeuronDescription:
+    name: str
+    spiking: bool
+    parameters: list
+    variables: list
+    randoms: list
+    targets: list
+    spike_cond: str = ""
+    spike_reset: list = field(default_factory=list)
+    refractory: float | str | None = None
+
+    @property
+    def attributes(self):
+        return [p.name for p in self.parameters] + [v.name for v in self.variables]
+
+    def local_names(self):
+        return [a.name for a in self.parameters + self.variables if a.locality == "local"]
+
+    def global_names(self):
+        return [a.name for a in self.parameters + self.variables if a.locality == "global"]
+
+
+# ---------------------------------------------------------------------------
+# String handling (parser/StringManipulation.py)
+# ---------------------------------------------------------------------------
+
+def _lines(text: str | None) -> list[str]:
+    if not text:
+        return []
+    out = []
+    for raw in text.replace(";", "\n").split("\n"):
+        line = raw.split("#", 1)[0].strip()
+        if line:
+            out.append(line)
+    return out
+
+
+def _split_constraint(line: str) -> tuple[str, str]:
+    # "eq : flags" -- the last colon that is not inside an ite(...) / brackets
+    depth = 0
+    for k in range(len(line) - 1, -1, -1):
+        c = line[k]
+        if c == ")":
+            depth += 1
+        elif c == "(":
+            depth -= 1
+        elif c == ":" and depth == 0:
+            return line[:k].strip(), line[k + 1:].strip()
+    return line.strip(), ""
+
+
+def _flags(constraint: str) -> tuple[dict, list]:
+    bounds, flags = {}, []
+    for part in [p.strip() for p in constraint.split(",") if p.strip()]:
+        if "=" in part:
+            k, v = part.split("=", 1)
+            bounds[k.strip()] = v.strip()
+        else:
+            flags.append(part)
+    return bounds, flags
+
+
+def _variable_name(eq: str) -> str:
+    e = eq.replace(" ", "")
+    m = re.search(r"d([A-Za-z_]\w*)/dt", e)
+    if m:
+        return m.group(1)
+    for op in ["+=", "-=", "*=", "/="]:
+        if op in e:
+            return e.split(op, 1)[0]
+    lhs = e.split("=", 1)[0]
+    return lhs
+
+
+def _parse_value(v: str) -> float:
+    v = v.strip()
+    if v in ("True", "true"):
+        return 1.0
+    if v in ("False", "false"):
+        return 0.0
+    return float(eval(v, {"__builtins__": {}}, {}))
+
+
+# ---------------------------------------------------------------------------
+# Symbolic translation
+# ---------------------------------------------------------------------------
+
+class _Translator:
+    def __init__(self, desc: NeuronDescription, precision: str = "double"):
+        self.desc = desc
+        self.precision = precision
+        d = {"dt": sp.Symbol("dt"), "t": sp.Symbol("(double(t)*dt)")}
+        d.update(_FUNCTIONS)
+        for n in desc.local_names():
+            d[n] = sp.Symbol(n + "%(local_index)s")
+        for n in desc.global_names():
+            d[n] = sp.Symbol(n + "%(global_index)s")
+        for r in desc.randoms:
+            idx = "%(local_index)s" if r.locality == "local" else "%(global_index)s"
+            d[r.name] = sp.Symbol(r.name + idx)
+        for tgt in desc.targets:
+            d["__sum_" + tgt + "__"] = sp.Symbol("_sum_" + tgt + "%(local_index)s")
+        self.local_dict = d
+
+    def parse(self, expr: str, extra: dict | None = None):
+        d = dict(self.local_dict)
+        if extra:
+            d.update(extra)
+        try:
+            return parse_expr(expr, local_dict=d,
+                              transformations=(standard_transformations + (convert_xor,)),
+                              evaluate=False)
+        except Exception as e:  # pragma: no cover - message path
+            raise EquationError(f"cannot analyse the expression: {expr} ({e})") from e
+
+    def ccode(self, e) -> str:
+        try:
+            code = sp.ccode(e, precision=8, strict=False, user_functions=_USER_FUNCTIONS)
+        except TypeError:  # sympy < 1.13
+            code = sp.ccode(e, precision=8, user_functions=_USER_FUNCTIONS)
+        if self.precision == "float":
+            # Equation.py:171-195: f suffix on literals, single-precision functions
+            matches = sorted(set(re.findall(r"[-]?[0-9]+\.[0-9]+", code)))
+            for m in matches:
+                code = code.replace(m, str(round(float(m), 8)) + "f")
+            for m in re.findall(r"[-]?[0-9]+\.[0-9]+f[f]+", code):
+                code = code.replace(m, str(round(float(m.split("f")[0]), 8)) + "f")
+            for fn in ["fabs", "pow", "exp", "sin", "cos", "tan", "round"]:
+                code = code.replace(fn + "(", fn + "f(")
+        return code
+
+    # -- ODEs ---------------------------------------------------------------
+    def explicit(self, name: str, expression: str) -> tuple[dict, str, str]:
+        expression = expression.replace("d" + name + "/dt", "_grad_var_")
+        g = sp.Symbol("_grad_var_")
+        a = self.parse(expression, {"_grad_var_": g})
+        eq = sp.solve(a, g, check=False, rational=False)[0]
+        eq = sp.simplify(eq, ratio=1.0)
+        code = f"{self.precision} _{name} = {self.ccode(eq)};"
+        switch = self.ccode(self.local_dict[name]) + " += dt*_" + name + " ;"
+        return {}, code, switch
+
+    def standardize(self, name: str, expression: str):
+        expression = expression.replace("d" + name + "/dt", "_gradvar_")
+        g = sp.Symbol("_gradvar_")
+        a = self.parse(expression, {"_gradvar_": g})
+        var = self.local_dict[name]
+        expanded = a.expand(modulus=None, power_base=False, power_exp=False, mul=True,
+                            log=False, multinomial=False)
+        coll = sp.collect(expanded, var, evaluate=False, exact=False)
+        if var not in coll or len(coll) > 2:
+            coll = sp.collect(sp.simplify(expanded), var, evaluate=False, exact=False)
+            if var not in coll or len(coll) > 2:
+                return None, None, None
+        factor_var = coll[var]
+        cg = sp.collect(sp.expand(a, g), g, evaluate=False, exact=False)
+        factor_grad = cg[g] if g in cg else sp.S(1.0)
+        real_tau = factor_grad / factor_var
+        normalized = a / factor_var
+        steady = sp.simplify(sp.together(real_tau * g + var - normalized))
+        stepsize = sp.simplify(sp.together(self.local_dict["dt"] / real_tau))
+        return real_tau, stepsize, steady
+
+    def exponential(self, name: str, expression: str, local_names) -> tuple[dict, str, str]:
+        real_tau, stepsize, steady = self.standardize(name, expression)
+        if real_tau is None:
+            return self.explicit(name, expression)
+        global_step = not any(self.local_dict[v] in stepsize.atoms() for v in local_names)
+        pre = {}
+        step_code = "(1.0 - exp(" + self.ccode(-stepsize) + "))"
+        if global_step:
+            pre = {"name": "__stepsize_" + name, "value": "1.0 - exp( " + self.ccode(-stepsize) + ")"}
+            step_code = "__stepsize_" + name
+        var = self.ccode(self.local_dict[name])
+        code = (f"{self.precision} _{name} =  {step_code}*({self.ccode(steady)} - "
+                f"{self.ccode(self.local_dict[name])});")
+        switch = var + " += _" + name + " ;"
+        return pre, code, switch
+
+    # -- assignments, increments, conditions --------------------------------
+    def assignment(self, expression: str) -> str:
+        name = expression[: expression.find("=")].strip()
+        rhs = expression[expression.find("=") + 1:]
+        a = self.parse(rhs)
+        return self.ccode(self.local_dict[name]) + " = " + self.ccode(a) + ";"
+
+    def increment(self, expression: str) -> str:
+        for op in ["+=", "-=", "*=", "/="]:
+            if op in expression:
+                name = expression[: expression.find(op)].strip()
+                rhs = expression[expression.find(op) + 2:]
+                a = self.parse(rhs)
+                return (self.ccode(self.local_dict[name]) + f" {op} " +
+                        self.ccode(sp.simplify(a, ratio=1.0)) + ";")
+        raise EquationError(expression)
+
+    def condition(self, expression: str) -> str:
+        expression = expression.strip()
+        expression = (expression.replace(" and ", " & ").replace(" or ", " | ")
+                      .replace(" is not ", " != ").replace(" not ", " Not ").replace(" is ", " == "))
+        if "==" in expression:
+            if "&" in expression or "|" in expression:
+                expression = re.sub(r"([\w\s.]+)==([\w\s.]+)", r"Equality(\1, \2)", expression)
+            else:
+                if expression[0] == "(" and expression[-1] == ")":
+                    expression = expression[1:-1]
+                terms = expression.split("==")
+                expression = "Equality(" + terms[0] + ", " + terms[1] + ")"
+        if "!=" in expression:
+            if "&" in expression or "|" in expression:
+                expression = re.sub(r"([\w\s.]+)!=([\w\s.]+)", r"Not(Equality(\1, \2))", expression)
+            else:
+                if expression[0] == "(" and expression[-1] == ")":
+                    expression = expression[1:-1]
+                terms = expression.split("!=")
+                expression = "Not(Equality(" + terms[0] + ", " + terms[1] + "))"
+        extra = {"Equality": sp.Eq, "Not": sp.Not}
+        return self.ccode(self.parse(expression, extra))
+
+    def expression(self, expr: str) -> str:
+        return self.ccode(self.parse(expr))
+
+
+def _equation_type(name: str, expression: str) -> str:
+    e = expression.replace(" ", "")
+    for op in ["+=", "-=", "*=", "/="]:
+        if op in e:
+            return "inc"
+    if "d" + name + "/dt" in e:
+        return "ODE"
+    split = e.split("=")
+    if len(split) == 1 or split[1] == "":
+        return "cond"
+    return "simple"
+
+
+# ---------------------------------------------------------------------------
+# Public entry point
+# ---------------------------------------------------------------------------
+
+def analyse_neuron(parameters: str = "", equations: str = "", spike: str | None = None,
+                   reset: str | None = None, refractory=None, name: str = "neuron",
+                   precision: str = "double") -> NeuronDescription:
+    """Analyse a neuron definition into C snippets (AnalyseNeuron.py:27-380)."""
+    params = []
+    for line in _lines(parameters):
+        eq, constraint = _split_constraint(line)
+        bounds, flags = _flags(constraint)
+        if "=" not in eq:
+            raise EquationError(f"parameter without value: {line}")
+        pname, val = eq.split("=", 1)
+        params.append(Variable(name=pname.strip(), eq=eq, locality="global" if "population" in flags else "local",
+                               flags=flags, bounds=bounds, init=_parse_value(val), is_parameter=True,
+                               ctype="int" if "int" in flags else precision))
+    variables = []
+    for line in _lines(equations):
+        eq, constraint = _split_constraint(line)
+        bounds, flags = _flags(constraint)
+        vname = _variable_name(eq)
+        init = 0.0
+        if "init" in bounds:
+            init = _parse_value(bounds.pop("init"))
+        variables.append(Variable(name=vname, eq=eq, locality="global" if "population" in flags else "local",
+                                  flags=flags, bounds=bounds, init=init,
+                                  ctype="int" if "int" in flags else precision))
+    declared = {v.name for v in params + variables}
+    spiking = spike is not None
+    # conductances of spiking neurons and targets of rate neurons
+    targets = []
+    for v in variables:
+        for m in re.finditer(r"sum\(\s*(\w+)\s*\)", v.eq):
+            if m.group(1) not in targets:
+                targets.append(m.group(1))
+    if spiking:
+        for v in variables:
+            if v.name.startswith("g_") and v.name[2:] not in targets:
+                targets.append(v.name[2:])
+    # random distributions -> rand_k
+    randoms = []
+    for v in variables:
+        transformed = v.eq
+        for dist in DISTRIBUTIONS:
+            for m in re.finditer(r"(?P<pre>[^\w.]|^)" + dist + r"\(([^()]+)\)", " " + v.eq):
+                args = [a.strip() for a in m.group(2).split(",")]
+                if len(args) != 2:
+                    raise EquationError(f"{dist} requires two arguments: {v.eq}")
+                rv = RandomVar(name=f"rand_{len(randoms)}", dist=dist,
+                               args=tuple(float(a) for a in args), locality=v.locality)
+                randoms.append(rv)
+                transformed = transformed.replace(dist + "(" + m.group(2) + ")", rv.name)
+        v.eq = transformed
+    for tgt in targets:
+        for v in variables:
+            v.eq = re.sub(r"sum\(\s*" + tgt + r"\s*\)", "__sum_" + tgt + "__", v.eq)
+    desc = NeuronDescription(name=name, spiking=spiking, parameters=params, variables=variables,
+                             randoms=randoms, targets=targets, refractory=refractory)
+    for v in variables:
+        if v.name not in declared:
+            raise EquationError(f"cannot identify the variable of: {v.eq}")
+    tr = _Translator(desc, precision)
+    local_names = desc.local_names() + [r.name for r in randoms if r.locality == "local"]
+    for v in variables:
+        kind = _equation_type(v.name, v.eq)
+        method = "explicit"
+        for m in ("exponential", "explicit"):
+            if m in v.flags:
+                method = m
+        for m in ("implicit", "semiimplicit", "midpoint", "rk4", "event-driven"):
+            if m in v.flags:
+                raise EquationError(f"numerical method '{m}' is not supported by this backend")
+        v.method = method
+        if kind == "ODE":
+            expr = v.eq
+            expr = expr.replace("=", "- (", 1) + ")" if "=" in expr else expr
+            expr = expr.replace(" ", "")
+            if method == "exponential":
+                v.pre_loop, v.cpp, v.switch = tr.exponential(v.name, expr, local_names)
+            else:
+                v.pre_loop, v.cpp, v.switch = tr.explicit(v.name, expr)
+        elif kind == "inc":
+            v.cpp = tr.increment(v.eq)
+        elif kind == "simple":
+            v.cpp = tr.assignment(v.eq)
+        else:
+            raise EquationError(f"not an equation: {v.eq}")
+        # bounds (generator/Utils.py:48-66)
+        code = ""
+        idx = "%(local_index)s" if v.locality == "local" else "%(global_index)s"
+        for b in ("min", "max"):
+            if b in v.bounds:
+                val = v.bounds[b]
+                try:
+                    float(val)
+                except ValueError:
+                    val = tr.expression(val)
+                op = "<" if b == "min" else ">"
+                code += f"if({v.name}{idx} {op} {val})\n    {v.name}{idx} = {val};\n"
+        v.bound_code = code
+    if spiking:
+        desc.spike_cond = tr.condition(spike)
+        for line in _lines(reset):
+            kind = "inc" if any(op in line for op in ["+=", "-=", "*=", "/="]) else "simple"
+            desc.spike_reset.append(tr.increment(line) if kind == "inc" else tr.assignment(line))
+    return desc
+
+
+def sort_odes(desc: NeuronDescription, locality: str):
+    """Group consecutive ODEs into blocks (generator/Utils.py:17-45)."""
+    blocks = []
+    is_ode = False
+    for v in desc.variables:
+        if v.cpp == "" or v.locality != locality:
+            continue
+        if v.switch is not None:
+            if is_ode:
+                blocks[-1][1].append(v)
+            else:
+                is_ode = True
+                blocks.append(("ode", [v]))
+        else:
+            if is_ode or not blocks:
+                is_ode = False
+                blocks.append(("non-ode", [v]))
+            else:
+                blocks[-1][1].append(v)
+    return blocks
+
+
+def equation_code(desc: NeuronDescription, locality: str, mask: str | None) -> str:
+    """Statement list of the update loop (generator/Utils.py:139-265).
+
+    ODE blocks: every increment first, then every switch; switches of
+    non-conductance variables are multiplied by the refractory mask
+    (Utils.py:69-77); non-ODE statements of non-conductances are wrapped in
+    ``if (mask[i]) { ... }``.
+    """
+    out = []
+    for kind, block in sort_odes(desc, locality):
+        if kind == "ode":
+            for v in block:
+                out.append(v.cpp)
+            for v in block:
+                sw = v.switch
+                if mask is not None and not v.name.startswith("g_"):
+                    sw = sw.replace(";", "* " + mask + "%(local_index)s;")
+                out.append(sw)
+                if v.bound_code:
+                    out.append(v.bound_code)
+        else:
+            for v in block:
+                code = v.cpp
+                if mask is not None and not v.name.startswith("g_"):
+                    code = "if (" + mask + "%(local_index)s) { " + code + " }"
+                out.append(code)
+                if v.bound_code:
+                    out.append(v.bound_code)
+    return "\n".join(out)

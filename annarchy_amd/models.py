@@ -1,0 +1,107 @@
+"""Neuron type definitions (the ``Neuron`` object of the reference API).
+
+``Neuron`` mirrors ``ANNarchy.core.Neuron`` (parameters / equations / spike /
+reset / refractory strings); the analysis into C snippets happens in
+:mod:`annarchy_amd.equations`, the kernel generation in
+:mod:`annarchy_amd.codegen`.  The models below are the ones of the benchmark
+networks, written exactly as ANNarchy users write them.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+
+@dataclass(frozen=True)
+class Neuron:
+    parameters: str = ""
+    equations: str = ""
+    spike: str | None = None
+    reset: str | None = None
+    refractory: float | str | None = None
+    name: str = "neuron"
+    description: str = ""
+
+    @property
+    def spiking(self) -> bool:
+        return self.spike is not None
+
+
+# Conductance-based LIF of the COBA benchmark (Vogels & Abbott 2005 as used by
+# Brette et al. 2007; ANNarchy's COBA example): dimensionless conductances in
+# units of the leak conductance, I = 20 mV background drive.
+COBA = Neuron(
+    parameters="""
+        El = -60.0          : population
+        Vr = -60.0          : population
+        Erev_exc = 0.0      : population
+        Erev_inh = -80.0    : population
+        Vt = -50.0          : population
+        tau = 20.0          : population
+        tau_exc = 5.0       : population
+        tau_inh = 10.0      : population
+        I = 20.0            : population
+    """,
+    equations="""
+        tau * dv/dt = (El - v) + g_exc * (Erev_exc - v) + g_inh * (Erev_inh - v ) + I
+
+        tau_exc * dg_exc/dt = - g_exc
+        tau_inh * dg_inh/dt = - g_inh
+    """,
+    spike="v > Vt",
+    reset="v = Vr",
+    refractory=5.0,
+    name="COBA",
+)
+
+# Rate neuron of the rate-network benchmark:
+#   tau dr/dt = -r + sum(exc) + N(0, 0.1),  r >= 0
+RateNoisy = Neuron(
+    parameters="""
+        tau = 10.0 : population
+    """,
+    equations="""
+        tau * dr/dt = -r + sum(exc) + Normal(0.0, 0.1) : min=0.0
+    """,
+    name="RateNoisy",
+)
+
+# Leaky rate neuron with a per-instance input gain, used by the batched sweep:
+#   tau dr/dt = -r + sum(exc) + gain * sum(in) ,  r >= 0
+RateGain = Neuron(
+    parameters="""
+        tau = 10.0 : population
+        gain = 1.0 : population
+    """,
+    equations="""
+        tau * dr/dt = -r + sum(exc) + gain * sum(inp) : min=0.0
+    """,
+    name="RateGain",
+)
+
+
+def poisson_neuron(rates: float) -> Neuron:
+    """PoissonPopulation with a constant rate (inputs/PoissonPopulation.py:171-185)."""
+    return Neuron(
+        parameters=f"rates = {float(rates)!r}",
+        equations="p = Uniform(0.0, 1.0) * 1000.0 / dt",
+        spike="p < rates",
+        name="Poisson",
+    )
+
+
+# Known-answer neurons of tests_python/Neuron/test_SpikingNeuron.py
+CounterNeuron = Neuron(
+    equations="v = v + 1.0",
+    spike="v == 3.0",
+    reset="v = 1.0",
+    name="Counter",
+)
+CounterNeuronRefractory = Neuron(
+    parameters="Vt = 3.0",
+    equations="v = v + 1.0",
+    spike="v == Vt",
+    reset="v = 1.0 ",
+    refractory=3.0,
+    name="CounterRef",
+)

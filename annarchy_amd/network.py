@@ -1,0 +1,501 @@
+"""Python driver of the backend: the counterpart of ANNarchy's Network /
+Population / Projection / Monitor objects for the simulation step.
+
+What it reproduces of the reference caller (core/Network.py, core/Population.py,
+core/Projection.py, core/Simulate.py):
+  * ``Network(dt, seed)`` owns ``numpy.random.default_rng(seed)``
+    (core/Network.py:147) used by the connectors in projection order;
+  * ``simulate(T)`` runs ``ceil(T/dt)`` steps (core/Simulate.py:63);
+  * refractory periods in ms become ``(value/dt * ones).astype(int)`` steps
+    (core/Population.py:669-692);
+  * a projection whose pre and post differ allows self-connections
+    (core/ConnectorMethods.py:287-288, :344-345);
+  * monitors record targets after compute_psp and other variables after the
+    update (OpenMPBaseTemplate.py:235-305).
+Everything below compile() goes through the C-ABI of libannarchy_hip.so.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from pathlib import Path
+
+import numpy as np
+
+from . import _lib
+from . import build as _build
+from . import codegen
+from . import connectivity as conn
+from .models import Neuron
+
+
+def model_library(neuron: Neuron, allow_build: bool = True) -> Path:
+    """Path of the compiled model library of `neuron` (built on demand when
+    hipcc is available and `allow_build`)."""
+    src = codegen.generate_from_text(neuron.name, neuron.parameters, neuron.equations,
+                                     neuron.spike, neuron.reset, neuron.refractory)
+    path = _build.model_lib_path(neuron.name, src.source)
+    if path.exists():
+        return path
+    if not allow_build or os.environ.get("AHIP_NO_BUILD"):
+        raise _lib.AhipError(f"model library {path.name} for {neuron.name} is not built")
+    return _build.build_model(neuron.name, src.source)
+
+
+class Population:
+    def __init__(self, net: "Network", geometry, neuron: Neuron, name: str | None = None):
+        self.net = net
+        self.geometry = (geometry,) if isinstance(geometry, int) else tuple(geometry)
+        self.size = int(np.prod(self.geometry))
+        self.neuron = neuron
+        self.name = name or f"pop{len(net.populations)}"
+        self.id = -1
+        self._init = {}
+        self._refractory = neuron.refractory
+        self.ranks = np.arange(self.size)
+        self.spiking = neuron.spiking
+        self._attrs = None
+
+    # -- attribute access (Population.py:375-538) --------------------------
+    def __getattr__(self, name):
+        if name.startswith("_") or name in ("net", "geometry", "size", "neuron", "id", "ranks",
+                                             "spiking", "name"):
+            raise AttributeError(name)
+        return self.get(name)
+
+    def __setattr__(self, name, value):
+        if name in ("net", "geometry", "size", "neuron", "name", "id", "ranks", "spiking") or name.startswith("_"):
+            object.__setattr__(self, name, value)
+        else:
+            self.set(name, value)
+
+    def _slot_info(self):
+        if self._attrs is None:
+            src = codegen.generate_from_text(self.neuron.name, self.neuron.parameters,
+                                             self.neuron.equations, self.neuron.spike,
+                                             self.neuron.reset, self.neuron.refractory)
+            self._attrs = {s[0]: s for s in src.slots}
+        return self._attrs
+
+    def set(self, name, value):
+        if name == "refractory":
+            self._refractory = value
+            if self.net.compiled:
+                self._upload_refractory()
+            return
+        info = self._slot_info()
+        if name not in info:
+            raise AttributeError(f"population {self.name} has no attribute {name}")
+        if not self.net.compiled:
+            self._init[name] = value
+            return
+        local = info[name][1]
+        vals = self._materialize(value, local)
+        _lib.check(_lib.lib().ahip_pop_set(self.net._h, self.id, name.encode(),
+                                           _lib.ptr(vals, C.c_double), vals.size))
+
+    def get(self, name):
+        if name == "refractory":
+            if not self.net.compiled:
+                return self._refractory
+            out = np.empty(self.size, dtype=np.int32)
+            _lib.check(_lib.lib().ahip_pop_get_int(self.net._h, self.id, b"refractory",
+                                                   _lib.ptr(out, C.c_int32), self.size))
+            return out
+        info = self._slot_info()
+        if name not in info:
+            raise AttributeError(f"population {self.name} has no attribute {name}")
+        if not self.net.compiled:
+            v = self._init.get(name, info[name][2])
+            return v
+        n = self.size if info[name][1] else 1
+        out = np.empty(n, dtype=np.float64)
+        _lib.check(_lib.lib().ahip_pop_get(self.net._h, self.id, name.encode(),
+                                           _lib.ptr(out, C.c_double), n))
+        return out.reshape(self.geometry) if n > 1 else float(out[0])
+
+    def _materialize(self, value, local):
+        n = self.size if local else 1
+        if isinstance(value, conn.RandomDistribution):
+            vals = np.asarray(value.get_values(n), dtype=np.float64)
+        else:
+            vals = np.asarray(value, dtype=np.float64).reshape(-1)
+            if vals.size == 1 and n > 1:
+                vals = np.full(n, vals[0])
+        if vals.size != n:
+            raise ValueError(f"expected {n} values")
+        return np.ascontiguousarray(vals)
+
+    def _upload_refractory(self):
+        if not self.spiking:
+            return
+        value = self._refractory if self._refractory is not None else 0.0
+        if isinstance(value, str):
+            return
+        dt = self.net.dt
+        if isinstance(value, conn.RandomDistribution):
+            refs = (value.get_values(self.size) / dt).astype(int)
+        elif isinstance(value, np.ndarray):
+            refs = (value / dt).astype(int).reshape(self.size)
+        else:
+            refs = (value / dt * np.ones(self.size)).astype(int)
+        refs = np.ascontiguousarray(refs, dtype=np.int32)
+        _lib.check(_lib.lib().ahip_pop_set_int(self.net._h, self.id, b"refractory",
+                                               _lib.ptr(refs, C.c_int32), self.size))
+
+    def __getitem__(self, sl):
+        ranks = self.ranks[sl]
+        return PopulationView(self, np.atleast_1d(ranks))
+
+    # -- spikes ---------------------------------------------------------------
+    def spikes(self) -> np.ndarray:
+        out = np.empty(self.size, dtype=np.int32)
+        n = C.c_int32()
+        _lib.check(_lib.lib().ahip_pop_spikes(self.net._h, self.id, _lib.ptr(out, C.c_int32),
+                                              self.size, C.byref(n)))
+        return out[: n.value].copy()
+
+    def inject_random(self, k: int, draws: np.ndarray | None):
+        """Feed rand_k from a host stream [steps, size] (parity runs)."""
+        if draws is None:
+            _lib.check(_lib.lib().ahip_pop_inject_random(self.net._h, self.id, k, None, 0))
+            return
+        d = np.ascontiguousarray(draws, dtype=np.float64)
+        steps = d.size // self.size
+        _lib.check(_lib.lib().ahip_pop_inject_random(self.net._h, self.id, k,
+                                                     _lib.ptr(d, C.c_double), steps))
+
+
+class PopulationView:
+    def __init__(self, pop: Population, ranks: np.ndarray):
+        self.population = pop
+        self.ranks = np.asarray(ranks)
+        self.size = len(self.ranks)
+
+    @property
+    def spiking(self):
+        return self.population.spiking
+
+
+def _base(obj):
+    return obj.population if isinstance(obj, PopulationView) else obj
+
+
+class Projection:
+    def __init__(self, net: "Network", pre, post, target: str):
+        self.net = net
+        self.pre, self.post, self.target = pre, post, target
+        self.id = -1
+        self._connector = None
+        self._format = _lib.opts("csr", 64)
+        self.csr: conn.CSR | None = None
+
+    # -- connectors (core/ConnectorMethods.py) ---------------------------------
+    def _store(self, kind, *args, storage_format=None, **fmt):
+        self._connector = (kind, args)
+        if storage_format is not None or fmt:
+            self._format = _lib.opts(storage_format or "csr", fmt.get("lanes", 0),
+                                     fmt.get("chunk", 0), fmt.get("sigma", 1), fmt.get("panel_cols", 0))
+        return self
+
+    def fixed_probability(self, probability, weights, delays=0.0, allow_self_connections=False,
+                          storage_format=None, **fmt):
+        self._check_delay(delays)
+        return self._store("fixed_probability", probability, weights, allow_self_connections,
+                           storage_format=storage_format, **fmt)
+
+    def fixed_number_pre(self, number, weights, delays=0.0, allow_self_connections=False,
+                         storage_format=None, **fmt):
+        self._check_delay(delays)
+        if number > _sizes(self.pre):
+            raise ValueError("fixed_number_pre: the number of pre-synaptic neurons exceeds the population size")
+        return self._store("fixed_number_pre", number, weights, allow_self_connections,
+                           storage_format=storage_format, **fmt)
+
+    def all_to_all(self, weights, delays=0.0, allow_self_connections=False, storage_format=None, **fmt):
+        self._check_delay(delays)
+        return self._store("all_to_all", weights, allow_self_connections,
+                           storage_format=storage_format, **fmt)
+
+    def one_to_one(self, weights, delays=0.0, storage_format=None, **fmt):
+        self._check_delay(delays)
+        return self._store("one_to_one", weights, storage_format=storage_format, **fmt)
+
+    def from_csr(self, csr: conn.CSR, storage_format=None, **fmt):
+        return self._store("csr", csr, storage_format=storage_format, **fmt)
+
+    @staticmethod
+    def _check_delay(d):
+        if not (isinstance(d, (int, float)) and d == 0.0):
+            raise NotImplementedError("synaptic delays are not supported by this backend yet")
+
+    def _build_csr(self) -> conn.CSR:
+        kind, args = self._connector
+        pre_b, post_b = _base(self.pre), _base(self.post)
+        pre_r, post_r = np.asarray(self.pre.ranks), np.asarray(self.post.ranks)
+        allow_self = self.pre is not self.post
+        rng = self.net.rng
+        if kind == "fixed_probability":
+            p, w, a = args
+            return conn.fixed_probability(pre_r, post_r, pre_b.size, post_b.size, rng, p, w, a or allow_self)
+        if kind == "fixed_number_pre":
+            n, w, a = args
+            return conn.fixed_number_pre(pre_r, post_r, pre_b.size, post_b.size, rng, n, w, a or allow_self)
+        if kind == "all_to_all":
+            w, a = args
+            return conn.all_to_all(pre_r, post_r, pre_b.size, post_b.size, w, a or allow_self)
+        if kind == "one_to_one":
+            (w,) = args
+            return conn.one_to_one(pre_r, post_r, pre_b.size, post_b.size, w)
+        if kind == "csr":
+            return args[0]
+        raise ValueError(kind)
+
+    def _instantiate(self):
+        L = _lib.lib()
+        pre_b, post_b = _base(self.pre), _base(self.post)
+        spiking = pre_b.spiking
+        pid = C.c_int32()
+        _lib.check(L.ahip_proj_create(self.net._h, pre_b.id, post_b.id, self.target.encode(),
+                                      int(spiking), C.byref(pid)))
+        self.id = pid.value
+        self.csr = self._build_csr()
+        c = self.csr
+        w = c.w
+        _lib.check(L.ahip_proj_init_csr(self.net._h, self.id, c.n_rows, c.n_cols,
+                                        _lib.ptr(c.row_ptr, C.c_int64), _lib.ptr(c.col_idx, C.c_int32),
+                                        _lib.ptr(w, C.c_double) if w is not None else None,
+                                        c.w_const, C.byref(self._format)))
+
+    # -- accessors ------------------------------------------------------------
+    @property
+    def nb_synapses(self) -> int:
+        return self.csr.nnz if self.csr is not None else 0
+
+    @property
+    def w(self) -> np.ndarray:
+        out = np.empty(self.nb_synapses)
+        _lib.check(_lib.lib().ahip_proj_get_w(self.net._h, self.id, _lib.ptr(out, C.c_double), out.size))
+        return out
+
+    @w.setter
+    def w(self, value):
+        v = np.ascontiguousarray(np.broadcast_to(np.asarray(value, dtype=np.float64), (self.nb_synapses,)))
+        _lib.check(_lib.lib().ahip_proj_set_w(self.net._h, self.id, _lib.ptr(v, C.c_double), v.size))
+
+    def set_format(self, fmt="csr", lanes=0, chunk=0, sigma=1, panel_cols=0):
+        self._format = _lib.opts(fmt, lanes, chunk, sigma, panel_cols)
+        if self.net.compiled:
+            _lib.check(_lib.lib().ahip_proj_set_format(self.net._h, self.id, C.byref(self._format)))
+
+    def get_format(self) -> dict:
+        o = _lib.FormatOpts()
+        _lib.check(_lib.lib().ahip_proj_get_format(self.net._h, self.id, C.byref(o)))
+        return {"format": _lib.FORMAT_NAMES[o.format], "lanes": o.lanes_per_row, "chunk": o.chunk,
+                "sigma": o.sigma, "panel_cols": o.panel_cols}
+
+    def psp_bytes(self) -> float:
+        b = C.c_double()
+        _lib.check(_lib.lib().ahip_proj_psp_bytes(self.net._h, self.id, C.byref(b)))
+        return b.value
+
+    def time_psp(self, iters=20) -> float:
+        ms = C.c_double()
+        _lib.check(_lib.lib().ahip_proj_time_psp(self.net._h, self.id, iters, C.byref(ms)))
+        return ms.value
+
+
+def _sizes(obj) -> int:
+    return obj.size
+
+
+class Monitor:
+    """Device-side recording (the reference records on the host every step,
+    generator/Monitor/BaseTemplates.py:5-149; here each recorded attribute has
+    a device buffer of `capacity` steps filled inside the captured step)."""
+
+    def __init__(self, net: "Network", obj, variables, capacity: int = 10000):
+        self.net = net
+        self.obj = obj
+        self.variables = [variables] if isinstance(variables, str) else list(variables)
+        self.capacity = capacity
+        self._ids = {}
+
+    def _instantiate(self):
+        pop = _base(self.obj)
+        for v in self.variables:
+            mid = C.c_int32()
+            _lib.check(_lib.lib().ahip_monitor_create(self.net._h, pop.id, v.encode(), self.capacity,
+                                                      C.byref(mid)))
+            self._ids[v] = mid.value
+
+    def _steps(self, mid) -> int:
+        s = C.c_int64()
+        _lib.check(_lib.lib().ahip_monitor_steps(self.net._h, mid, C.byref(s)))
+        return s.value
+
+    def get(self, name: str, keep: bool = False):
+        """Values [steps, size] or, for "spike", {rank: [time steps]}."""
+        mid = self._ids[name]
+        steps = self._steps(mid)
+        pop = _base(self.obj)
+        if name == "spike":
+            counts, idx = self.raw_spikes()
+            t0 = self.net.time - steps
+            out = {int(r): [] for r in np.asarray(self.obj.ranks)}
+            pos = 0
+            for s in range(steps):
+                for r in idx[pos: pos + counts[s]]:
+                    if int(r) in out:
+                        out[int(r)].append(t0 + s)
+                pos += counts[s]
+            if not keep:
+                _lib.check(_lib.lib().ahip_monitor_clear(self.net._h, mid))
+            return out
+        out = np.empty(steps * pop.size)
+        _lib.check(_lib.lib().ahip_monitor_get(self.net._h, mid, _lib.ptr(out, C.c_double), out.size))
+        if not keep:
+            _lib.check(_lib.lib().ahip_monitor_clear(self.net._h, mid))
+        out = out.reshape(steps, pop.size)
+        if isinstance(self.obj, PopulationView):
+            out = out[:, self.obj.ranks]
+        return out
+
+    def raw_spikes(self):
+        mid = self._ids["spike"]
+        steps = self._steps(mid)
+        counts = np.zeros(steps, dtype=np.int32)
+        total = C.c_int64()
+        _lib.check(_lib.lib().ahip_monitor_get_spikes(self.net._h, mid, _lib.ptr(counts, C.c_int32),
+                                                      steps, None, 0, C.byref(total)))
+        idx = np.zeros(max(1, total.value), dtype=np.int32)
+        _lib.check(_lib.lib().ahip_monitor_get_spikes(self.net._h, mid, _lib.ptr(counts, C.c_int32),
+                                                      steps, _lib.ptr(idx, C.c_int32), idx.size,
+                                                      C.byref(total)))
+        return counts, idx[: total.value]
+
+    # -- analysis (core/Monitor.py:1172-1330) -----------------------------------
+    @staticmethod
+    def mean_fr(spikes: dict, duration_ms: float) -> float:
+        n = sum(len(v) for v in spikes.values())
+        return n / len(spikes) / (duration_ms / 1000.0) if spikes else 0.0
+
+    @staticmethod
+    def inter_spike_interval(spikes: dict, dt: float, per_neuron: bool = False):
+        isis = [list(np.diff(np.asarray(ts) * dt)) for ts in spikes.values() if len(ts) > 1]
+        if per_neuron:
+            return isis
+        return [x for l in isis for x in l]
+
+    @staticmethod
+    def coefficient_of_variation(spikes: dict, dt: float) -> list:
+        """std(ISI)/mean(ISI) per neuron with at least two ISIs (Monitor.py:1210-1240)."""
+        out = []
+        for ts in spikes.values():
+            if len(ts) > 2:
+                isi = np.diff(np.asarray(ts) * dt)
+                out.append(float(np.std(isi) / np.mean(isi)))
+        return out
+
+
+class Network:
+    def __init__(self, dt: float = 1.0, seed: int | None = None, precision: str = "double"):
+        self.dt = float(dt)
+        self.seed = seed
+        self.rng = np.random.default_rng(seed)
+        self.precision = precision
+        self.populations: list[Population] = []
+        self.projections: list[Projection] = []
+        self.monitors: list[Monitor] = []
+        self.compiled = False
+        self._h = None
+
+    def create(self, geometry, neuron: Neuron, name: str | None = None) -> Population:
+        if self.compiled:
+            raise RuntimeError("the network is already compiled")
+        p = Population(self, geometry, neuron, name)
+        self.populations.append(p)
+        return p
+
+    def connect(self, pre, post, target: str) -> Projection:
+        if self.compiled:
+            raise RuntimeError("the network is already compiled")
+        p = Projection(self, pre, post, target)
+        self.projections.append(p)
+        return p
+
+    def monitor(self, obj, variables, capacity: int = 10000) -> Monitor:
+        m = Monitor(self, obj, variables, capacity)
+        self.monitors.append(m)
+        if self.compiled:
+            m._instantiate()
+        return m
+
+    def compile(self, allow_build: bool = True, graph_batch: int = 0):
+        """Build (or look up) the model libraries, create the device network,
+        run the connectors and upload everything (Compiler._instantiate,
+        generator/Compiler.py:796-1000)."""
+        L = _lib.lib()
+        h = C.c_void_p()
+        _lib.check(L.ahip_net_create(C.byref(h)))
+        self._h = h
+        prec = _lib.AHIP_F64 if self.precision in ("double", "float64", "f64") else _lib.AHIP_F32
+        _lib.check(L.ahip_net_initialize(h, self.dt, prec))
+        seed = self.seed if self.seed is not None else int(np.random.SeedSequence().entropy % (2**63))
+        _lib.check(L.ahip_set_seed(h, C.c_uint64(seed)))
+        if graph_batch:
+            _lib.check(L.ahip_set_graph_batch(h, graph_batch))
+        for p in self.populations:
+            lib_path = model_library(p.neuron, allow_build)
+            pid = C.c_int32()
+            _lib.check(L.ahip_pop_create(h, p.size, str(lib_path).encode(), C.byref(pid)))
+            p.id = pid.value
+        self.compiled = True
+        # connectors first, then attribute initial values: the order in which
+        # the reference draws from the shared generator (Compiler.py:796-1000)
+        for proj in self.projections:
+            proj._instantiate()
+        for p in self.populations:
+            for name, value in p._init.items():
+                p.set(name, value)
+            p._upload_refractory()
+        for m in self.monitors:
+            m._instantiate()
+        return self
+
+    # -- simulation (core/Simulate.py) ----------------------------------------
+    def simulate(self, duration: float):
+        nb_steps = math.ceil(duration / self.dt)
+        self.run(nb_steps)
+
+    def run(self, nsteps: int):
+        _lib.check(_lib.lib().ahip_run(self._h, int(nsteps)))
+
+    def step(self):
+        _lib.check(_lib.lib().ahip_step(self._h))
+
+    @property
+    def time(self) -> int:
+        t = C.c_int64()
+        _lib.check(_lib.lib().ahip_get_time(self._h, C.byref(t)))
+        return t.value
+
+    def get_current_step(self) -> int:
+        return self.time
+
+    def get_time(self) -> float:
+        return self.time * self.dt
+
+    def set_graph(self, enabled: bool = True, batch: int = 0):
+        _lib.check(_lib.lib().ahip_set_graph_enabled(self._h, int(enabled)))
+        _lib.check(_lib.lib().ahip_set_graph_batch(self._h, int(batch)))
+
+    def __del__(self):
+        try:
+            if self._h is not None and _lib._lib is not None:
+                _lib.lib().ahip_net_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass

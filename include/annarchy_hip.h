@@ -1,0 +1,168 @@
+/* annarchy_hip.h -- C-ABI of the MI355X simulation backend (libannarchy_hip.so).
+ *
+ * This is the drop-in boundary for ANNarchy's simulation step.  In the
+ * reference, Python drives a generated nanobind module ANNarchyWrapper<id>
+ * (generator/NanoBind/BaseTemplate.py:6-45) loaded by
+ * generator/Compiler.py:772-793 and instantiated by Compiler.py:796-1000; the
+ * entry points below replace the module functions and the per-object
+ * wrappers of that module one for one (each comment names the reference
+ * interface).  Conventions:
+ *   - every function returns 0 on success or a negative AHIP_E* code; the
+ *     message of the last failure is available from ahip_last_error()
+ *     (the reference returns bool and prints to std::cerr, Messages.py:94-112);
+ *   - host buffers are owned by the caller and only read/written during the
+ *     call; device memory is owned by the network;
+ *   - a network is not thread-safe; different networks may be used from
+ *     different threads.
+ */
+#ifndef ANNARCHY_HIP_H
+#define ANNARCHY_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AHIP_OK 0
+#define AHIP_EINVAL -1   /* bad argument (index, size, name) */
+#define AHIP_EHIP -2     /* HIP runtime error */
+#define AHIP_ENOMEM -3   /* device allocation failed */
+#define AHIP_ESTATE -4   /* call not valid in the current state */
+#define AHIP_EMODEL -5   /* model library could not be loaded */
+
+/* Storage formats of a rate-coded projection (core/Projection.py:691-789
+ * chooses among csr/ell/ellr/sell/bsr/dense in the reference). */
+#define AHIP_FMT_CSR 0    /* row-wise, L lanes per row (generator/Projection/CUDA/CSR.py:299-334) */
+#define AHIP_FMT_ELL 1    /* column-major ELLPACK (CUDA/ELL.py:240-262) */
+#define AHIP_FMT_ELLR 2   /* ELLPACK-R, explicit row lengths (CUDA/ELLR.py:239-255) */
+#define AHIP_FMT_SELL 3   /* SELL-C-sigma with row sorting inside sigma windows (CUDA/SELL.py:230-246 has no sigma) */
+#define AHIP_FMT_BSR 4    /* block sparse rows, square tiles (CUDA/BSR.py:229-252) */
+#define AHIP_FMT_DENSE 5  /* row-major dense (CUDA/Dense.py:287-316) */
+#define AHIP_FMT_CSR_PANEL 6 /* CSR split in column panels, pre-synaptic panel staged in LDS */
+
+/* Precision of a network (core/CTypes.py:30-74, ConfigManagement.py:25). */
+#define AHIP_F64 0
+#define AHIP_F32 1        /* FP32 state and weights, Kahan-compensated psp sums */
+
+typedef struct ahip_net ahip_net;
+
+typedef struct ahip_format_opts {
+    int32_t format;           /* AHIP_FMT_* */
+    int32_t lanes_per_row;    /* CSR: 1..64 (power of two); SELL: threads per row */
+    int32_t chunk;            /* SELL: rows per slice (C); BSR: tile size */
+    int32_t sigma;            /* SELL: sorting window in rows (1 = no sorting) */
+    int32_t panel_cols;       /* CSR_PANEL: columns per panel (0 = automatic) */
+} ahip_format_opts;
+
+/* ---- errors / device ------------------------------------------------------ */
+const char* ahip_last_error(void);
+/* set_device(int) of the CUDA wrapper (NanoBind/Generator.py:41-46). */
+int ahip_set_device(int32_t device);
+int ahip_device_count(int32_t* count);
+/* Library version string and the gfx target the kernels were built for. */
+const char* ahip_build_info(void);
+
+/* ---- network (module functions, NanoBind/BaseTemplate.py:17-28) ---------- */
+/* pyx_create() */
+int ahip_net_create(ahip_net** out);
+int ahip_net_destroy(ahip_net* net);
+/* pyx_initialize(dt) + dtype: FP64 or FP32 for the whole network. */
+int ahip_net_initialize(ahip_net* net, double dt, int32_t precision);
+/* set_seed(seed, num_sources, use_seed_seq): key of the on-device generator. */
+int ahip_set_seed(ahip_net* net, uint64_t seed);
+/* run(nbSteps): whole steps are captured in a HIP graph and replayed. */
+int ahip_run(ahip_net* net, int64_t nsteps);
+/* step() */
+int ahip_step(ahip_net* net);
+/* run_until(steps, populations, and_or): stops at the end of the first step
+ * where any (and_or=0) or all (and_or=1) listed spiking populations emitted a
+ * spike; *done = steps simulated.  (Simulate.py:173-266) */
+int ahip_run_until(ahip_net* net, int64_t max_steps, const int32_t* pops, int32_t npops,
+                   int32_t and_or, int64_t* done);
+/* get_time() / set_time(t) */
+int ahip_get_time(ahip_net* net, int64_t* t);
+int ahip_set_time(ahip_net* net, int64_t t);
+/* Number of steps captured per graph launch (0 = automatic). */
+int ahip_set_graph_batch(ahip_net* net, int32_t steps);
+/* Disable (0) or enable (1) graph capture; eager launches otherwise. */
+int ahip_set_graph_enabled(ahip_net* net, int32_t enabled);
+
+/* ---- populations (pop<id>_wrapper, NanoBind/Population.py:6-72) --------- */
+/* pop<id>_wrapper(size, max_delay): model_lib is the path of a generated
+ * neuron-model library (annarchy_hip_model.h). */
+int ahip_pop_create(ahip_net* net, int32_t size, const char* model_lib, int32_t* pop_id);
+/* Attribute property get/set (vector<double> copies in the reference,
+ * NanoBind/Population.py:27-54).  n = size for local attributes, 1 for
+ * population-wide ones.  Values are converted to the network precision. */
+int ahip_pop_set(ahip_net* net, int32_t pop, const char* attr, const double* values, int64_t n);
+int ahip_pop_get(ahip_net* net, int32_t pop, const char* attr, double* values, int64_t n);
+/* refractory (rw int vector, in steps) and refractory_remaining */
+int ahip_pop_set_int(ahip_net* net, int32_t pop, const char* attr, const int32_t* values, int64_t n);
+int ahip_pop_get_int(ahip_net* net, int32_t pop, const char* attr, int32_t* values, int64_t n);
+/* last_spike (int64 per neuron, -10000 initially, OpenMPTemplates.py:283-287) */
+int ahip_pop_get_last_spike(ahip_net* net, int32_t pop, int64_t* values, int64_t n);
+/* Spikes emitted in the last step, ascending (the `spiked` vector). */
+int ahip_pop_spikes(ahip_net* net, int32_t pop, int32_t* out, int32_t cap, int32_t* n);
+/* reset(): refractory_remaining = 0, last_spike = -10000, spiked cleared. */
+int ahip_pop_reset_spiking(ahip_net* net, int32_t pop);
+int ahip_pop_size(ahip_net* net, int32_t pop, int32_t* size);
+/* Injected random streams: buf holds nsteps*size draws of random variable k
+ * (rand_k), row 0 used at the current time step; replaces the on-device
+ * generator so a run can consume the same stream as a CPU run.  buf == NULL
+ * returns to the on-device generator. */
+int ahip_pop_inject_random(ahip_net* net, int32_t pop, int32_t k, const double* buf,
+                           int64_t nsteps);
+
+/* ---- projections (proj<id>_wrapper, NanoBind/Projection.py:6-95) -------- */
+/* proj<id>_wrapper(): pre/post population ids and the target name (exc,
+ * inh, ...); spiking = 1 makes an event-driven projection into g_<target>. */
+int ahip_proj_create(ahip_net* net, int32_t pre, int32_t post, const char* target,
+                     int32_t spiking, int32_t* proj_id);
+/* init_from_lil(post_ranks, pre_ranks, w, delays, requires_sorting)
+ * (ProjectionGenerator.py:743-797) as a bulk CSR: rows are post ranks
+ * (n_rows = post size, empty rows allowed), columns ascending inside a row.
+ * w: nnz weights in row order, or NULL for one constant weight w_const
+ * (core/Projection.py:823-839). */
+int ahip_proj_init_csr(ahip_net* net, int32_t proj, int32_t n_rows, int32_t n_cols,
+                       const int64_t* row_ptr, const int32_t* col_idx, const double* w,
+                       double w_const, const ahip_format_opts* opts);
+/* Re-layout an initialised rate projection in another format. */
+int ahip_proj_set_format(ahip_net* net, int32_t proj, const ahip_format_opts* opts);
+int ahip_proj_get_format(ahip_net* net, int32_t proj, ahip_format_opts* opts);
+/* nb_synapses(), nb_dendrites() */
+int ahip_proj_nb_synapses(ahip_net* net, int32_t proj, int64_t* nnz);
+/* get/set_local_attribute_all("w") in forward (row) order. */
+int ahip_proj_get_w(ahip_net* net, int32_t proj, double* w, int64_t nnz);
+int ahip_proj_set_w(ahip_net* net, int32_t proj, const double* w, int64_t nnz);
+/* Bytes the psp kernel must move per step (algorithmic, no padding). */
+int ahip_proj_psp_bytes(ahip_net* net, int32_t proj, double* bytes);
+/* Times `iters` back-to-back launches of the projection's psp kernel with HIP
+ * events recorded on the network stream; *avg_ms = mean launch duration. */
+int ahip_proj_time_psp(ahip_net* net, int32_t proj, int32_t iters, double* avg_ms);
+/* update_launch_config(nb, tpb) equivalent (Projection.py:2125-2150). */
+int ahip_proj_set_launch(ahip_net* net, int32_t proj, int32_t lanes_per_row);
+
+/* ---- monitors (device-side recording) ------------------------------------ */
+/* Record attr of pop each step into a device buffer of `capacity` steps
+ * (targets g_* and _sum_* are recorded after compute_psp, other variables
+ * after the update, OpenMPBaseTemplate.py:235-305).  attr "spike" records the
+ * spike list.  Recording starts at the current time step. */
+int ahip_monitor_create(ahip_net* net, int32_t pop, const char* attr, int64_t capacity,
+                        int32_t* mon_id);
+/* Steps recorded so far (<= capacity). */
+int ahip_monitor_steps(ahip_net* net, int32_t mon, int64_t* steps);
+/* Values: steps*size doubles, step-major. */
+int ahip_monitor_get(ahip_net* net, int32_t mon, double* out, int64_t n);
+/* Spikes: counts[steps] and the ascending indices of every step, concatenated;
+ * *total receives the number of indices (capacity check on idx_cap). */
+int ahip_monitor_get_spikes(ahip_net* net, int32_t mon, int32_t* counts, int64_t steps,
+                            int32_t* idx, int64_t idx_cap, int64_t* total);
+/* Clear the recorded data and restart recording at the current step. */
+int ahip_monitor_clear(ahip_net* net, int32_t mon);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,83 @@
+/* annarchy_hip_model.h -- plug-in ABI between the simulation runtime
+ * (libannarchy_hip.so) and a generated neuron-model library.
+ *
+ * A model library is produced from a neuron description: the equation
+ * snippets that the sympy front end emits are spliced into a HIP kernel
+ * template (one thread per neuron, fused ODE + bounds + threshold + reset +
+ * refractory + spike flags).  This mirrors the reference's per-population
+ * generated code: PopStruct::update()/spike_gather()
+ * (generator/Population/OpenMPGenerator.py:765-1237,
+ *  generator/Population/CUDATemplates.py:543-720), which the reference emits
+ * as text per network; here it is one shared object per model that the
+ * runtime loads with dlopen() and launches from inside its captured step.
+ *
+ * Plain pointers and sizes only; the stream is passed as an opaque pointer.
+ */
+#ifndef ANNARCHY_HIP_MODEL_H
+#define ANNARCHY_HIP_MODEL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AHIP_MODEL_ABI 3
+#define AHIP_MAX_SLOTS 48
+#define AHIP_MAX_RAND 8
+#define AHIP_UPDATE_BLOCK 256 /* threads per block of every update kernel */
+
+/* Arguments of one population update launch.  Device pointers unless noted. */
+typedef struct ahip_pop_args {
+    int32_t size;                       /* number of neurons */
+    int32_t step;                       /* step index inside the launched batch */
+    const int64_t* t_base;              /* simulation step at the start of the batch */
+    double dt;                          /* host value */
+    void* slot[AHIP_MAX_SLOTS];         /* local: size elements, global: 1 element */
+    const void* rand_inject[AHIP_MAX_RAND]; /* injected draws [steps][size] or NULL */
+    int64_t rand_inject_steps;          /* rows available in the injected buffers */
+    int64_t rand_t0;                    /* simulation step of row 0 of the injected buffers */
+    uint64_t seed;                      /* counter-based generator key */
+    uint32_t stream_id;                 /* population id, second key word */
+    int32_t* refractory;                /* steps, per neuron */
+    int32_t* refractory_remaining;
+    int64_t* last_spike;
+    uint64_t* spike_mask;               /* ceil(size/64) words, bit i = neuron spiked */
+    int32_t* block_counts;              /* ceil(size/256) spike counts per update block */
+} ahip_pop_args;
+
+typedef int (*ahip_launch_fn)(const ahip_pop_args* args, void* stream);
+/* Host-side evaluation of the hoisted step sizes (exponential Euler
+ * "pre_loop" values, parser/Equation.py:559-604) from the global slots; writes
+ * one value per pre-loop slot.  Host libm is used on purpose so the values
+ * equal those of a CPU build. */
+typedef void (*ahip_preloop_fn)(const double* globals_host, double dt, double* out);
+
+typedef struct ahip_model_desc {
+    int32_t abi;                        /* AHIP_MODEL_ABI */
+    const char* name;
+    int32_t spiking;                    /* 1: spiking (threshold/reset), 0: rate-coded */
+    int32_t n_slots;
+    const char* const* slot_names;      /* attributes, _sum_<target>, pre-loop step sizes */
+    const int32_t* slot_local;          /* 1: one value per neuron, 0: population-wide */
+    const double* slot_init;            /* initial value of each slot */
+    const int32_t* slot_kind;           /* 0 parameter, 1 variable, 2 psp sum, 3 pre-loop */
+    int32_t n_rand;                     /* random variables drawn per step */
+    const int32_t* rand_dist;           /* 0 Uniform(a,b), 1 Normal(mu,sigma) */
+    const double* rand_args;            /* two per random variable */
+    int32_t has_refractory;
+    int32_t has_global_update;          /* a one-thread kernel runs before the local one */
+    ahip_launch_fn launch_f64;
+    ahip_launch_fn launch_f32;
+    ahip_preloop_fn preloop;            /* NULL when there is no pre-loop value */
+    const char* source_hash;
+} ahip_model_desc;
+
+/* Every model library exports this symbol. */
+typedef const ahip_model_desc* (*ahip_model_describe_fn)(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,48 @@
+"""The C-ABI library loads and exports every entry point its headers declare
+(no device calls), and the generated model libraries export their
+descriptor.  CPU only."""
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+from annarchy_amd import _lib
+from annarchy_amd import build as B
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _declared(header: str) -> list[str]:
+    text = (ROOT / "include" / header).read_text()
+    return sorted(set(re.findall(r"\b(ahip_\w+)\s*\(", text)) - {"ahip_model_describe"})
+
+
+def test_core_library_exports_every_declared_symbol():
+    L = C.CDLL(str(B.CORE_LIB))
+    declared = _declared("annarchy_hip.h")
+    assert len(declared) >= 40
+    missing = [s for s in declared if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_binding_covers_the_header():
+    assert sorted(_lib.declared_symbols()) == _declared("annarchy_hip.h")
+
+
+def test_model_libraries_export_descriptor():
+    libs = sorted(B.LIB_DIR.glob("libahip_model_*.so"))
+    assert libs, "no model library built"
+    for p in libs:
+        L = C.CDLL(str(p))
+        assert hasattr(L, "ahip_model_describe")
+
+
+def test_error_path_without_device_calls():
+    L = _lib.lib()
+    rc = L.ahip_net_initialize(None, 1.0, 0)
+    assert rc == -1
+    assert b"null" in L.ahip_last_error()
+    rc = L.ahip_pop_set(None, 0, b"v", None, 0)
+    assert rc == -1

@@ -1,0 +1,157 @@
+"""Spiking path on the GPU: known answers of the reference tests and
+bit-exact agreement with the CPU oracle.
+
+* tests_python/Neuron/test_SpikingNeuron.py -- v trajectories with and
+  without refractory period;
+* tests_python/Neuron/test_Monitor.py:294-333 -- spike times [4, 6, 8] and,
+  with refractory=3, [2, 7];
+* tests_python/SpikeSynapse/test_SpikingTransmission.py:14-78 -- 5 emitters
+  firing at t=1, all-to-all w=1 -> g_exc == 5 at step 2 only;
+* the COBA network (4000 neurons, p=0.02, constant weights) against the
+  oracle over 1000 steps: spike lists, v and g_exc bit for bit (FMA
+  contraction off on both sides; one weight per projection makes the
+  atomic scatter order-independent);
+* Poisson drive with injected uniforms, bit for bit;
+* compaction: ascending, complete, no duplicates.
+"""
+
+import numpy as np
+import pytest
+
+import annarchy_amd as ah
+from annarchy_amd import models
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def test_counter_neuron_trajectory(gpu):
+    net = ah.Network(dt=1.0)
+    p1 = net.create(3, models.CounterNeuron)
+    p2 = net.create(3, models.CounterNeuronRefractory)
+    net.compile()
+    v1, v2 = [], []
+    for _ in range(7):
+        v1.append(float(p1.v[0]))
+        v2.append(float(p2.v[0]))
+        net.simulate(1)
+    v1.append(float(p1.v[0]))
+    v2.append(float(p2.v[0]))
+    assert v1[:6] == [0.0, 1.0, 2.0, 1.0, 2.0, 1.0]
+    assert v2 == [0.0, 1.0, 2.0, 1.0, 1.0, 1.0, 1.0, 2.0]
+
+
+def test_monitor_spike_times(gpu):
+    n2 = ah.Neuron(equations="v = v + 1.0", spike="v == 5.0", reset="v = 3.0", name="Mon2")
+    n3 = ah.Neuron(equations="v = v + 1.0", spike="v == 3.0", reset="v = 1.0 ", refractory=3.0,
+                   name="Mon3")
+    net = ah.Network(dt=1.0)
+    pop3 = net.create(3, n2)
+    pop4 = net.create(3, n3)
+    m3 = net.monitor(pop3, ["v", "spike"], capacity=20)
+    m4 = net.monitor(pop4, ["v", "spike"], capacity=20)
+    net.compile()
+    net.simulate(10)
+    assert m3.get("spike")[0] == [4, 6, 8]
+    v4 = m4.get("v")
+    expected = [1, 2, 1, 1, 1, 1, 2, 1, 1, 1]
+    np.testing.assert_array_equal(v4[:, 0], expected)
+    assert m4.get("spike")[1] == [2, 7]
+
+
+def test_spike_transmission_known_answer(gpu):
+    emit = ah.Neuron(spike="t==1", name="Emit")
+    recv = ah.Neuron(equations="g_exc = 0", spike="g_exc>30", name="Recv")
+    net = ah.Network(dt=1.0)
+    inp = net.create(5, emit)
+    out = net.create(2, recv)
+    net.connect(inp, out, "exc").all_to_all(1.0)
+    mon = net.monitor(out, ["g_exc"], capacity=10)
+    net.compile()
+    net.simulate(5)
+    g = mon.get("g_exc")
+    np.testing.assert_array_equal(g, [[0, 0], [0, 0], [5, 5], [0, 0], [0, 0]])
+
+
+def _coba_net(N=4000, seed=7, p=0.02, graph_batch=0, steps_cap=1000):
+    net = ah.Network(dt=0.1, seed=seed)
+    P = net.create(N, ah.COBA)
+    ne = int(0.8 * N)
+    Pe, Pi = P[:ne], P[ne:]
+    ce = net.connect(Pe, P, "exc").fixed_probability(p, 0.6)
+    ci = net.connect(Pi, P, "inh").fixed_probability(p, 6.7)
+    P.v = ah.Uniform(-60.0, -50.0, rng=net.rng)
+    mon = net.monitor(P, ["v", "g_exc", "spike"], capacity=steps_cap)
+    net.compile(graph_batch=graph_batch)
+    return net, P, ce, ci, mon
+
+
+def _oracle_coba(P, ce, ci, v0, steps):
+    projs = [orc.csrc_projection(ce.csr, 0), orc.csrc_projection(ci.csr, 1)]
+    ref = np.asarray(P.refractory, dtype=np.int32)
+    return orc.spiking_run(0, P.size, 0.1, v0, steps, projs, refractory=ref, trace=True)
+
+
+def test_coba_bit_exact_vs_oracle(gpu):
+    steps = 1000  # 100 ms
+    net, P, ce, ci, mon = _coba_net(graph_batch=50)
+    v0 = P.v.reshape(-1).copy()
+    assert np.all(P.refractory == 50)
+    net.run(steps)
+    ref = _oracle_coba(P, ce, ci, v0, steps)
+    counts, idx = mon.raw_spikes()
+    assert counts.sum() > 100, "network is silent"
+    np.testing.assert_array_equal(counts, ref["counts"])
+    np.testing.assert_array_equal(idx, ref["idx"])
+    v = mon.get("v", keep=True)
+    ge = mon.get("g_exc", keep=True)
+    assert np.array_equal(v, ref["trace_v"])
+    assert np.array_equal(ge, ref["trace_gexc"])
+    assert np.array_equal(P.g_inh.reshape(-1), ref["g_inh"])
+    rr = P.get("refractory") if False else None
+    last = np.empty(P.size, dtype=np.int64)
+    import ctypes as C
+    from annarchy_amd import _lib
+    _lib.check(_lib.lib().ahip_pop_get_last_spike(net._h, P.id, _lib.ptr(last, C.c_int64), P.size))
+    np.testing.assert_array_equal(last, ref["last_spike"])
+
+
+def test_compaction_ascending_complete(gpu):
+    net, P, ce, ci, mon = _coba_net(N=9000, seed=3, graph_batch=10, steps_cap=300)
+    net.run(300)
+    counts, idx = mon.raw_spikes()
+    pos = 0
+    for c in counts:
+        seg = idx[pos:pos + c]
+        assert np.all(np.diff(seg) > 0) and (c == 0 or (seg[0] >= 0 and seg[-1] < 9000))
+        pos += c
+    assert counts.sum() > 0
+
+
+def test_poisson_drive_bit_exact(gpu):
+    N, M, steps = 3000, 300, 400
+    net = ah.Network(dt=0.1, seed=11)
+    P = net.create(N, ah.COBA)
+    Q = net.create(M, models.poisson_neuron(50.0))
+    ce = net.connect(P[:2400], P, "exc").fixed_probability(0.02, 0.6)
+    ci = net.connect(P[2400:], P, "inh").fixed_probability(0.02, 6.7)
+    cq = net.connect(Q, P[:M], "exc").one_to_one(2.0)
+    P.v = ah.Uniform(-60.0, -50.0, rng=net.rng)
+    mon = net.monitor(P, ["spike"], capacity=steps)
+    monq = net.monitor(Q, ["spike"], capacity=steps)
+    net.compile(graph_batch=20)
+    P.I = 0.0
+    v0 = P.v.reshape(-1).copy()
+    unif = np.random.default_rng(5).random((steps, M))
+    Q.inject_random(0, unif)
+    net.run(steps)
+    projs = [orc.csrc_projection(ce.csr, 0), orc.csrc_projection(ci.csr, 1),
+             orc.csrc_projection(cq.csr, 0, pre_is_input=True)]
+    ref = orc.spiking_run(0, N, 0.1, v0, steps, projs, refractory=np.asarray(P.refractory),
+                          params=dict(I=0.0), poisson=dict(M=M, rates=50.0, uniforms=unif))
+    qc, _ = monq.raw_spikes()
+    np.testing.assert_array_equal(qc, ref["input_counts"])
+    counts, idx = mon.raw_spikes()
+    np.testing.assert_array_equal(counts, ref["counts"])
+    np.testing.assert_array_equal(idx, ref["idx"])
+    assert np.array_equal(P.v.reshape(-1), ref["v"])

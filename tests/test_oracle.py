@@ -1,0 +1,199 @@
+"""The CPU oracle pinned against the reference's golden vectors, known answers
+and closed forms (no GPU).
+
+Containers: tests_cpp golden layouts (tests/golden/container_layouts.json).
+Rate psp: the dense product W.r (tests_python/RateSynapse/test_RateDelays.py).
+Rate update: closed-form geometric decay / fixed point of the explicit Euler
+recurrence.  Spiking: test_SpikingNeuron / test_Monitor trajectories,
+test_SpikingTransmission's 5 x w=1, the free-membrane solution of the LIF and
+the conductance decay.  MLP: finite differences.
+"""
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from annarchy_amd import connectivity as conn
+from oracle import oracle as orc
+
+GOLD = json.loads((Path(__file__).parent / "golden" / "container_layouts.json").read_text())
+
+
+@pytest.mark.parametrize("m", ["A", "B"])
+def test_csr_golden(m):
+    g = GOLD[m]
+    rb, ci = orc.csr_from_lil(g["num_rows"], g["post_ranks"], g["pre_ranks"])
+    flat_vals = np.concatenate([np.asarray(v, float) for v in g["values"]])
+    assert np.array_equal(flat_vals, g["csr"])
+    # rows are indexed by post rank: empty rows have empty ranges
+    lens = np.diff(rb)
+    for k, r in enumerate(g["post_ranks"]):
+        assert lens[r] == len(g["pre_ranks"][k])
+    assert lens.sum() == len(g["csr"])
+    assert np.array_equal(ci, np.concatenate(g["pre_ranks"]))
+
+
+@pytest.mark.parametrize("m", ["A", "B"])
+def test_dense_golden(m):
+    g = GOLD[m]
+    d = orc.dense_from_lil(g["num_rows"], g["num_cols"], g["post_ranks"], g["pre_ranks"], g["values"], True)
+    assert np.array_equal(d, g["dense_row_major"])
+    if "dense_col_major" in g:
+        d = orc.dense_from_lil(g["num_rows"], g["num_cols"], g["post_ranks"], g["pre_ranks"], g["values"], False)
+        assert np.array_equal(d, g["dense_col_major"])
+
+
+@pytest.mark.parametrize("m", ["A", "B"])
+def test_ell_golden(m):
+    g = GOLD[m]
+    maxnzr, col, val, rl = orc.ell_from_lil(g["pre_ranks"], g["values"], row_major=True)
+    assert maxnzr == 4
+    assert np.array_equal(val, g["ell_row_major"])
+    assert np.all(col[val == 0] == np.iinfo(np.int32).max)
+    assert list(rl) == [len(x) for x in g["pre_ranks"]]
+    _, col, val, _ = orc.ell_from_lil(g["pre_ranks"], g["values"], row_major=False)
+    assert np.array_equal(val, g["ell_col_major"])
+
+
+@pytest.mark.parametrize("m", ["A", "B"])
+def test_sell_golden(m):
+    g = GOLD[m]
+    rp, col, val, mask = orc.sell_from_lil(g["num_rows"], g["post_ranks"], g["pre_ranks"], g["values"], 3, True)
+    assert np.array_equal(val, g["sell3_row_major"])
+    assert list(rp) == [0, 12, 18]
+    assert np.all(col[mask == 0] == 0)
+
+
+@pytest.mark.parametrize("m", ["A", "B"])
+def test_bsr_golden(m):
+    g = GOLD[m]
+    brp, bci, val = orc.bsr_from_lil(g["num_rows"], g["num_cols"], 2, g["post_ranks"], g["pre_ranks"], g["values"])
+    assert np.array_equal(val, g["bsr2"])
+    assert brp[-1] == len(bci)
+
+
+def test_csrc_inverse_brute_force():
+    rng = np.random.default_rng(0)
+    for trial in range(5):
+        nr, nc = rng.integers(1, 60, size=2)
+        dense = (rng.random((nr, nc)) < 0.2)
+        post = [r for r in range(nr) if dense[r].any()]
+        pre = [np.nonzero(dense[r])[0] for r in post]
+        if not post:
+            continue
+        rb, ci = orc.csr_from_lil(int(nr), post, pre)
+        col_ptr, row_idx, inv = orc.csrc_inverse(int(nr), int(nc), rb, ci)
+        for c in range(nc):
+            rows = row_idx[col_ptr[c]:col_ptr[c + 1]]
+            assert list(rows) == list(np.nonzero(dense[:, c])[0])  # ascending post ranks
+            for q in range(col_ptr[c], col_ptr[c + 1]):
+                assert ci[inv[q]] == c and rb[row_idx[q]] <= inv[q] < rb[row_idx[q] + 1]
+
+
+def test_spmv_dense_product():
+    rng = np.random.default_rng(1)
+    csr = conn.random_in_degree(500, 300, rng, rng.integers(0, 40, 300), conn.Normal(0, 1, rng=rng))
+    r = rng.random(500)
+    got = orc.spmv_csr(csr.row_ptr, csr.col_idx, csr.w, r, nthreads=4)
+    np.testing.assert_allclose(got, csr.dense() @ r, rtol=1e-12, atol=1e-12)
+    # += semantics: target accumulates
+    got2 = orc.spmv_csr(csr.row_ptr, csr.col_idx, csr.w, r, target=np.ones(300))
+    np.testing.assert_allclose(got2, csr.dense() @ r + 1.0, rtol=1e-12)
+
+
+def test_rate_run_closed_forms():
+    n, steps, tau, dt = 50, 40, 10.0, 1.0
+    empty = conn.from_arrays(n, n, np.zeros(n + 1, np.int64), np.zeros(0, np.int32), np.zeros(0))
+    r0 = np.linspace(0.1, 2.0, n)
+    r, _, tr = orc.rate_run(empty.row_ptr, empty.col_idx, np.zeros(1), r0, tau, dt, None, steps, trace=True)
+    k = np.arange(1, steps + 1)[:, None]
+    np.testing.assert_allclose(tr, r0[None, :] * (1 - dt / tau) ** k, rtol=1e-12)
+    # self-excitation w=0.5 of each neuron on itself: fixed point of r = r + dt/tau(-r + 0.5 r) is 0,
+    # with constant noise input c: r* = 2c
+    eye = conn.from_arrays(n, n, np.arange(n + 1, dtype=np.int64), np.arange(n, dtype=np.int32), np.full(n, 0.5))
+    c = 0.3
+    noise = np.full((2000, n), c)
+    r, _, _ = orc.rate_run(eye.row_ptr, eye.col_idx, eye.w, np.zeros(n), tau, dt, noise, 2000)
+    np.testing.assert_allclose(r, 2 * c, rtol=1e-12)
+    # bound r >= 0 (min=0.0): a negative drive clips at exactly zero
+    r, _, _ = orc.rate_run(empty.row_ptr, empty.col_idx, np.zeros(1), r0, tau, dt, np.full((5, n), -100.0), 5)
+    assert np.all(r == 0.0)
+
+
+def test_counter_neuron_known_answers():
+    # test_SpikingNeuron.py test_v: spike at v == 3, reset 1
+    out = orc.spiking_run(1, 3, 1.0, np.zeros(3), 6, params=dict(Vt=3.0, Vr=1.0), trace=True)
+    assert list(out["trace_v"][:, 0]) == [1.0, 2.0, 1.0, 2.0, 1.0, 2.0]
+    # test_v_ref: refractory 3 steps
+    out = orc.spiking_run(1, 3, 1.0, np.zeros(3), 7, params=dict(Vt=3.0, Vr=1.0),
+                          refractory=np.full(3, 3), trace=True)
+    assert list(out["trace_v"][:, 0]) == [1.0, 2.0, 1.0, 1.0, 1.0, 1.0, 2.0]
+    # test_Monitor.py:294-333: spikes at [4, 6, 8] (v == 5, reset 3) and [2, 7] with refractory
+    out = orc.spiking_run(1, 3, 1.0, np.zeros(3), 10, params=dict(Vt=5.0, Vr=3.0))
+    steps = [s for s in range(10) if out["counts"][s] > 0]
+    assert steps == [4, 6, 8]
+    out = orc.spiking_run(1, 3, 1.0, np.zeros(3), 10, params=dict(Vt=3.0, Vr=1.0), refractory=np.full(3, 3))
+    steps = [s for s in range(10) if out["counts"][s] > 0]
+    assert steps == [2, 7]
+    assert list(out["last_spike"]) == [7, 7, 7]
+
+
+def test_spike_transmission_known_answer():
+    # 5 inputs firing once at step 1 into 2 neurons, all-to-all w = 1
+    M, steps = 5, 5
+    unif = np.ones((steps, M))
+    unif[1, :] = 0.0
+    csr = conn.all_to_all(np.arange(M), np.arange(2), M, 2, 1.0, True)
+    proj = orc.csrc_projection(csr, 0, pre_is_input=True)
+    out = orc.spiking_run(1, 2, 1.0, np.zeros(2), steps, [proj], params=dict(Vt=1e9),
+                          poisson=dict(M=M, rates=1.0, uniforms=unif), trace=True)
+    assert list(out["input_counts"]) == [0, 5, 0, 0, 0]
+    assert out["trace_gexc"][1].tolist() == [0.0, 0.0]
+    assert out["trace_gexc"][2].tolist() == [5.0, 5.0]
+
+
+def test_lif_free_membrane_and_decay():
+    # no synapses: explicit Euler towards El + I with factor (1 - dt/tau)
+    N, dt, steps = 4, 0.1, 60
+    v0 = np.array([-60.0, -58.0, -55.0, -52.0])
+    out = orc.spiking_run(0, N, dt, v0, steps, params=dict(I=5.0), trace=True,
+                          g_exc=np.full(N, 0.0))
+    vinf = -60.0 + 5.0
+    k = np.arange(1, steps + 1)[:, None]
+    np.testing.assert_allclose(out["trace_v"], vinf + (v0 - vinf)[None, :] * (1 - dt / 20.0) ** k, rtol=1e-12)
+    # conductances decay geometrically and are not masked by refractoriness
+    out = orc.spiking_run(0, N, dt, np.full(N, -70.0), steps, params=dict(I=0.0),
+                          g_exc=np.full(N, 2.0), g_inh=np.full(N, 1.0), refractory=np.full(N, 50))
+    np.testing.assert_allclose(out["g_exc"], 2.0 * (1 - dt / 5.0) ** steps, rtol=1e-12)
+    np.testing.assert_allclose(out["g_inh"], 1.0 * (1 - dt / 10.0) ** steps, rtol=1e-12)
+
+
+def test_lif_refractory_period():
+    # strong drive: every spike is followed by exactly 50 refractory steps at v = Vr
+    out = orc.spiking_run(0, 1, 0.1, np.array([-50.5]), 400, params=dict(I=100.0),
+                          refractory=np.array([50]), trace=True)
+    spikes = [s for s in range(400) if out["counts"][s]]
+    assert len(spikes) >= 3
+    for a, b in zip(spikes, spikes[1:]):
+        assert np.all(out["trace_v"][a:a + 51, 0] == -60.0)
+        assert b - a > 50
+
+
+def test_mlp_gradient_finite_differences():
+    rng = np.random.default_rng(0)
+    dims = (16, 64, 64, 12)
+    n_par = 64 * 16 + 64 + 64 * 64 + 64 + 12 * 64 + 12
+    P = rng.normal(0, 0.3, n_par)
+    X = rng.normal(0, 1, (20, 16))
+    y = rng.integers(0, 12, 20)
+    loss, G = orc.mlp_loss_grad(X, y, P, dims)
+    assert n_par == 6028
+    for k in rng.choice(n_par, 40, replace=False):
+        h = 1e-6
+        Pp, Pm = P.copy(), P.copy()
+        Pp[k] += h
+        Pm[k] -= h
+        fd = (orc.mlp_loss_grad(X, y, Pp, dims, grad=False)[0] - orc.mlp_loss_grad(X, y, Pm, dims, grad=False)[0]) / (2 * h)
+        assert abs(fd - G[k]) < 1e-6 + 1e-4 * abs(fd)
