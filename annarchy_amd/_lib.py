@@ -45,12 +45,14 @@ _D = C.c_double
 _PI32 = C.POINTER(C.c_int32)
 _PI64 = C.POINTER(C.c_int64)
 _PD = C.POINTER(C.c_double)
+_PF = C.POINTER(C.c_float)
 
 _SIGS = {
     "ahip_last_error": ([], C.c_char_p),
     "ahip_build_info": ([], C.c_char_p),
     "ahip_set_device": ([_I32], _I32),
     "ahip_device_count": ([_PI32], _I32),
+    "ahip_probe_copy": ([_I64, _I32, _PD], _I32),
     "ahip_net_create": ([C.POINTER(_P)], _I32),
     "ahip_net_destroy": ([_P], _I32),
     "ahip_net_initialize": ([_P, _D, _I32], _I32),
@@ -87,6 +89,20 @@ _SIGS = {
     "ahip_monitor_get": ([_P, _I32, _PD, _I64], _I32),
     "ahip_monitor_get_spikes": ([_P, _I32, _PI32, _I64, _PI32, _I64, _PI64], _I32),
     "ahip_monitor_clear": ([_P, _I32], _I32),
+    "ahip_batch_last_error": ([], C.c_char_p),
+    "ahip_batch_create": ([_I32, _PI32, _I32, _D, _D, C.POINTER(_P)], _I32),
+    "ahip_batch_destroy": ([_P], _I32),
+    "ahip_batch_set_recurrent": ([_P, _PI64, _PI32, _PD, _I32], _I32),
+    "ahip_batch_set_input": ([_P, _PD, _PD, _D], _I32),
+    "ahip_batch_set_gain": ([_P, _PD], _I32),
+    "ahip_batch_set_rates": ([_P, _PD], _I32),
+    "ahip_batch_get_rates": ([_P, _PD], _I32),
+    "ahip_batch_get_input_sums": ([_P, _PD], _I32),
+    "ahip_batch_run": ([_P, _I64, _I32], _I32),
+    "ahip_batch_time": ([_P, _I32, _I32, _PD], _I32),
+    "ahip_batch_info": ([_P, _PI64, _PI32, _PD, _PD], _I32),
+    "ahip_mlp_train": ([_PF, _PF, _I32, _PF, _I32, C.c_float, C.c_float, _PF, _PF], _I32),
+    "ahip_mlp_predict": ([_PF, _I32, _PF, _PI32, _PF], _I32),
 }
 
 

@@ -33,6 +33,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CORE_SOURCES = [
     "runtime.cpp",
     "containers.cpp",
+    "batch.cpp",
     "k_rate_psp.hip",
     "k_spike.hip",
     "k_misc.hip",

@@ -145,6 +145,29 @@ int launch_record_spikes(const Monitor& m, const Pop& p, const int64_t* t_base, 
                          hipStream_t s);
 int launch_convert(const void* src, int32_t src_f64, void* dst, int32_t dst_f64, int64_t n,
                    hipStream_t s);
+int launch_copy16(const void* src, void* dst, int64_t n16, hipStream_t s);
+
+// Device state of the batched multi-instance rate sweep (k_batched.hip).
+struct BatchDev {
+    int32_t n_inst = 0, n_in = 0, total = 0;
+    int32_t m_tiles = 0, n_tiles = 0, k_steps = 0;  // GEMM tiling (32 x 32 x 8)
+    int32_t lanes = 16;
+    double dt = 1.0, tau = 10.0, omega = 0.0;
+    DevBuf row_ptr, col, w;          // block-diagonal CSR, FP32 weights
+    DevBuf inst_of_row, local_of_row;
+    DevBuf gain;                     // float [n_inst]
+    DevBuf r[2];                     // float [total] ping-pong
+    DevBuf sum_exc;                  // float [total], psp of the last step
+    DevBuf w_hi, w_lo;               // bf16 fragments [m_tiles][k_steps][64][4]
+    DevBuf x_hi, x_lo;               // bf16 fragments [n_tiles][k_steps][64][4]
+    DevBuf phase;                    // double [n_in][n_inst]
+    DevBuf y;                        // float [m_tiles*32][n_tiles*32]
+    DevBuf t_dev;                    // int64
+};
+int batch_launch_input(const BatchDev& B, int32_t step, hipStream_t s);
+int batch_launch_gemm(const BatchDev& B, hipStream_t s);
+int batch_launch_spmv(const BatchDev& B, int32_t parity, hipStream_t s);
+int batch_launch_advance(const BatchDev& B, int64_t n, hipStream_t s);
 
 // Format builders (host, from the forward CSR view).
 int build_layout(const Proj& p, int32_t precision, const ahip_format_opts& o, RateLayout& out,

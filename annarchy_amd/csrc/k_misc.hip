@@ -105,3 +105,23 @@ int launch_convert(const void* src, int32_t src_f64, void* dst, int32_t dst_f64,
 }
 
 }  // namespace ahip
+
+namespace ahip {
+namespace {
+// Streaming copy used to calibrate HBM counters and the achievable bandwidth:
+// 16-byte non-temporal loads and stores, grid-stride.
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void copy16(const v4u* __restrict__ src, v4u* __restrict__ dst,
+                                              int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        v4u v = __builtin_nontemporal_load(src + i);
+        __builtin_nontemporal_store(v, dst + i);
+    }
+}
+}  // namespace
+
+int launch_copy16(const void* src, void* dst, int64_t n16, hipStream_t s) {
+    hipLaunchKernelGGL(copy16, dim3(256 * 8 * 4), dim3(256), 0, s, (const v4u*)src, (v4u*)dst, n16);
+    return (int)hipGetLastError();
+}
+}  // namespace ahip

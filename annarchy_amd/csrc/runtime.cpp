@@ -292,6 +292,32 @@ const char* ahip_last_error(void) { return g_err.c_str(); }
 
 const char* ahip_build_info(void) { return "annarchy_hip 0.1 gfx950 wave64"; }
 
+int ahip_probe_copy(int64_t bytes, int32_t iters, double* avg_ms) {
+    if (bytes < 16 || iters <= 0 || !avg_ms) return fail(AHIP_EINVAL, "bad copy probe arguments");
+    const int64_t n16 = bytes / 16;
+    DevBuf a, b;
+    int rc;
+    if ((rc = dev_alloc(a, n16 * 16))) return rc;
+    if ((rc = dev_alloc(b, n16 * 16))) return rc;
+    hipStream_t s;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    LCHK(launch_copy16(a.p, b.p, n16, s));
+    HIPCHK(hipEventRecord(e0, s));
+    for (int32_t i = 0; i < iters; i++) LCHK(launch_copy16(a.p, b.p, n16, s));
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipStreamDestroy(s);
+    *avg_ms = (double)ms / iters;
+    return 0;
+}
+
 int ahip_device_count(int32_t* count) {
     int n = 0;
     HIPCHK(hipGetDeviceCount(&n));

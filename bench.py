@@ -117,6 +117,15 @@ def main():
     ms_step = el * 1000.0 / args.steps
     events = csr.nnz * args.steps / el
 
+    import ctypes as C
+    from annarchy_amd import _lib
+
+    copy_ms = C.c_double()
+    copy_bytes = 1 << 31
+    _lib.check(_lib.lib().ahip_probe_copy(copy_bytes, 10, C.byref(copy_ms)))
+    copy_gbs = 2.0 * copy_bytes / (copy_ms.value * 1e-3) / 1e9
+    log(f"streaming copy of {copy_bytes >> 20} MiB: {copy_ms.value:.3f} ms, {copy_gbs:.0f} GB/s (read + write)")
+
     psp_ms = proj.time_psp(50)
     psp_bytes = proj.psp_bytes()
     achieved = psp_bytes / (psp_ms * 1e-3) / 1e9
@@ -169,6 +178,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "hbm_gbs_step": psp_bytes / (ms_step * 1e-3) / 1e9,
+        "hbm_copy_gbs": copy_gbs,
     }
     print(json.dumps(out))
 

@@ -62,6 +62,11 @@ int ahip_set_device(int32_t device);
 int ahip_device_count(int32_t* count);
 /* Library version string and the gfx target the kernels were built for. */
 const char* ahip_build_info(void);
+/* Measurement helper with no counterpart in the reference: times `iters`
+ * launches of a streaming copy of `bytes` bytes (16-byte non-temporal loads
+ * and stores) with HIP events; *avg_ms = mean launch time.  Used to calibrate
+ * HBM byte counters and to state the achievable bandwidth. */
+int ahip_probe_copy(int64_t bytes, int32_t iters, double* avg_ms);
 
 /* ---- network (module functions, NanoBind/BaseTemplate.py:17-28) ---------- */
 /* pyx_create() */
@@ -160,6 +165,51 @@ int ahip_monitor_get_spikes(ahip_net* net, int32_t mon, int32_t* counts, int64_t
                             int32_t* idx, int64_t idx_cap, int64_t* total);
 /* Clear the recorded data and restart recording at the current step. */
 int ahip_monitor_clear(ahip_net* net, int32_t mon);
+
+/* ---- batched instances (parallel_run, core/Network.py:751-926) ------------
+ * Many rate-network instances of uneven size advanced together: instance i
+ * has N_i neurons  tau dr/dt = -r + sum(exc) + gain_i*sum(inp), r >= 0, a
+ * recurrent sparse projection (block-diagonal CSR over all instances, FP32
+ * with compensated sums) and an input population of n_in neurons
+ * r = 1 + 0.5*sin(omega*t*dt + phase) feeding the first min(N_i, n_in)
+ * neurons through one dense n_in x n_in weight matrix shared by every
+ * instance (a GEMM on MFMA, split-BF16 operands, FP32 accumulation).  The
+ * reference runs such sweeps as one process per instance. */
+typedef struct ahip_batch ahip_batch;
+const char* ahip_batch_last_error(void);
+int ahip_batch_create(int32_t n_inst, const int32_t* sizes, int32_t n_in, double dt, double tau,
+                      ahip_batch** out);
+int ahip_batch_destroy(ahip_batch* b);
+/* row_ptr[total+1], col (global row indices of the same instance), w (nnz) */
+int ahip_batch_set_recurrent(ahip_batch* b, const int64_t* row_ptr, const int32_t* col,
+                             const double* w, int32_t lanes);
+/* w_in: n_in x n_in row-major (row = post neuron); phase: n_in x n_inst */
+int ahip_batch_set_input(ahip_batch* b, const double* w_in, const double* phase, double omega);
+int ahip_batch_set_gain(ahip_batch* b, const double* gain);
+int ahip_batch_set_rates(ahip_batch* b, const double* r);
+int ahip_batch_get_rates(ahip_batch* b, double* r);
+/* Y = W_in . X of the last step, n_in x n_inst */
+int ahip_batch_get_input_sums(ahip_batch* b, double* y);
+int ahip_batch_run(ahip_batch* b, int64_t steps, int32_t graph_batch);
+/* which: 0 input update, 1 GEMM, 2 SpMV + update; mean launch time (ms) */
+int ahip_batch_time(ahip_batch* b, int32_t which, int32_t iters, double* avg_ms);
+int ahip_batch_info(ahip_batch* b, int64_t* nnz, int32_t* total, double* sparse_bytes,
+                    double* dense_flops);
+
+/* ---- learned storage-format selector --------------------------------------
+ * The reference picks a format with a decision tree on density and mean row
+ * length (core/Projection.py:691-789).  The selector here is an MLP
+ * 16-64-64-12 (ReLU, softmax) trained on the device from our own autotuner
+ * timings.  X: n x 16 features, T: n x 12 target distributions, params: 6028
+ * floats [W1 b1 W2 b2 W3 b3] (updated in place by training).
+ * epochs > 0: full-batch Adam; loss_curve (optional) receives the loss of
+ * every epoch.  epochs == 0: one forward/backward pass, grad_out receives
+ * dLoss/dparams and loss_curve[0] the loss. */
+int ahip_mlp_train(const float* X, const float* T, int32_t n, float* params, int32_t epochs,
+                   float lr, float weight_decay, float* loss_curve, float* grad_out);
+/* Class (argmax) and logits (optional, n x 12) of every sample. */
+int ahip_mlp_predict(const float* X, int32_t n, const float* params, int32_t* classes,
+                     float* logits);
 
 #ifdef __cplusplus
 }

@@ -438,6 +438,46 @@ int64_t orc_spiking_run(const orc_spiking_cfg* c, double* v, double* g_exc, doub
 }
 
 // ---------------------------------------------------------------------------
+// One instance of the batched parameter sweep, FP64, in the reference's step
+// order (OpenMPBaseTemplate.py:217-325): psp of both projections from the
+// rates of step t-1, then the updates of the input and exc populations.
+//   input (n_in):  r_in[k] = 1.0 + 0.5*sin(omega*(t*dt) + phase[k])
+//   exc (N):       _r = (_sum_exc[i] + gain*_sum_inp[i] - r[i])/tau;
+//                  r[i] += dt*_r;  if (r[i] < 0.0) r[i] = 0.0;
+//   exc -> exc:    CSR (row_begin, col_idx, w), sum from 0, j ascending
+//   inp -> exc:    dense rows 0 .. min(N, n_in)-1 of the shared n_in x n_in W_in
+// r_in starts at the value of step 0's update (the input population has no
+// state); trace (optional) receives r after every step.
+// ---------------------------------------------------------------------------
+void orc_sweep_instance_run(int32_t N, const int64_t* row_begin, const int32_t* col_idx,
+                            const double* w, int32_t n_in, const double* w_in,
+                            const double* phase, double omega, double gain, double tau, double dt,
+                            double* r, int32_t nsteps, int64_t t0, double* trace) {
+    std::vector<double> r_in(n_in, 0.0), s_exc(N), s_inp(N);
+    const int32_t rows_in = std::min(N, n_in);
+    int64_t t = t0;
+    for (int32_t s = 0; s < nsteps; s++) {
+        for (int32_t i = 0; i < N; i++) {
+            double sum = 0.0;
+            for (int64_t j = row_begin[i]; j < row_begin[i + 1]; j++) sum += w[j] * r[col_idx[j]];
+            s_exc[i] = sum;
+            double si = 0.0;
+            if (i < rows_in)
+                for (int32_t k = 0; k < n_in; k++) si += w_in[(int64_t)i * n_in + k] * r_in[k];
+            s_inp[i] = si;
+        }
+        for (int32_t k = 0; k < n_in; k++) r_in[k] = 1.0 + 0.5 * std::sin(omega * ((double)t * dt) + phase[k]);
+        for (int32_t i = 0; i < N; i++) {
+            double _r = (s_exc[i] + gain * s_inp[i] - r[i]) / tau;
+            r[i] += dt * _r;
+            if (r[i] < 0.0) r[i] = 0.0;
+        }
+        if (trace) std::memcpy(trace + (int64_t)s * N, r, sizeof(double) * N);
+        t++;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // MLP 16-64-64-12 (ReLU, softmax cross-entropy) used by the learned format
 // selector: forward and backward on the CPU in FP64, the gradient oracle for
 // the device training kernels.  Layout: W_l row-major [out][in], b_l [out].
@@ -452,9 +492,10 @@ static void dense_fwd(int n, int in, int out, const double* W, const double* b, 
         }
 }
 
-// Returns the mean cross-entropy; grads (same layout as params) accumulate the
-// mean gradient when non-NULL.  params = [W1 b1 W2 b2 W3 b3].
-double orc_mlp_loss_grad(int32_t n, const double* X, const int32_t* labels,
+// Returns the mean cross-entropy against target distributions T (n x d_out,
+// one-hot for hard labels); grads (same layout as params) accumulate the mean
+// gradient when non-NULL.  params = [W1 b1 W2 b2 W3 b3].
+double orc_mlp_loss_grad(int32_t n, const double* X, const double* T,
                          const double* params, double* grads, int32_t d_in, int32_t h1,
                          int32_t h2, int32_t d_out) {
     const double* W1 = params;
@@ -475,9 +516,13 @@ double orc_mlp_loss_grad(int32_t n, const double* X, const int32_t* labels,
         double se = 0.0;
         for (int o = 0; o < d_out; o++) se += std::exp(z[s * d_out + o] - m);
         double lse = m + std::log(se);
-        loss += lse - z[s * d_out + labels[s]];
+        double tsum = 0.0;
+        for (int o = 0; o < d_out; o++) {
+            loss += T[s * d_out + o] * (lse - z[s * d_out + o]);
+            tsum += T[s * d_out + o];
+        }
         for (int o = 0; o < d_out; o++)
-            dz[s * d_out + o] = (std::exp(z[s * d_out + o] - lse) - (o == labels[s] ? 1.0 : 0.0)) / n;
+            dz[s * d_out + o] = (tsum * std::exp(z[s * d_out + o] - lse) - T[s * d_out + o]) / n;
     }
     loss /= n;
     if (!grads) return loss;

@@ -85,7 +85,10 @@ def lib():
         L.orc_spiking_run.argtypes = [C.POINTER(OrcSpikingCfg), _PD, _PD, _PD, _PI32, _PI32, _PI64,
                                       _I32, _PI32, _PI32, _I64, _PD, _PD, _PI32]
         L.orc_spiking_run.restype = _I64
-        L.orc_mlp_loss_grad.argtypes = [_I32, _PD, _PI32, _PD, _PD, _I32, _I32, _I32, _I32]
+        L.orc_sweep_instance_run.argtypes = [_I32, _PI64, _PI32, _PD, _I32, _PD, _PD, _D, _D, _D, _D,
+                                             _PD, _I32, _I64, _PD]
+        L.orc_sweep_instance_run.restype = None
+        L.orc_mlp_loss_grad.argtypes = [_I32, _PD, _PD, _PD, _PD, _I32, _I32, _I32, _I32]
         L.orc_mlp_loss_grad.restype = _D
         _lib = L
     return _lib
@@ -298,11 +301,31 @@ def csrc_projection(csr, target, pre_is_input=False):
                 row_idx=row_idx, w_col=w_col, w_const=csr.w_const)
 
 
+def sweep_instance_run(csr, w_in, phase, omega, gain, tau, dt, r0, nsteps, trace=False):
+    """One instance of the batched sweep in FP64 (rows 0..min(N, n_in)-1 of w_in)."""
+    n = csr.n_rows
+    n_in = w_in.shape[1]
+    rp = np.ascontiguousarray(csr.row_ptr, dtype=np.int64)
+    ci = np.ascontiguousarray(csr.col_idx, dtype=np.int32)
+    ww = np.ascontiguousarray(csr.weights(), dtype=np.float64)
+    wi = np.ascontiguousarray(w_in, dtype=np.float64)
+    ph = np.ascontiguousarray(phase, dtype=np.float64)
+    r = np.array(r0, dtype=np.float64)
+    tr = np.zeros(nsteps * n) if trace else None
+    lib().orc_sweep_instance_run(n, _p(rp, _I64), _p(ci, _I32), _p(ww, _D), n_in, _p(wi, _D), _p(ph, _D),
+                                 omega, gain, tau, dt, _p(r, _D), nsteps, 0, _p(tr, _D))
+    return r, (tr.reshape(nsteps, n) if trace else None)
+
+
 # ---- MLP gradient oracle ------------------------------------------------------------
-def mlp_loss_grad(X, labels, params, dims=(16, 64, 64, 12), grad=True):
+def mlp_loss_grad(X, targets, params, dims=(16, 64, 64, 12), grad=True):
+    """targets: integer labels (n,) or target distributions (n, d_out)."""
     X = np.ascontiguousarray(X, dtype=np.float64)
-    y = np.ascontiguousarray(labels, dtype=np.int32)
+    t = np.asarray(targets)
+    if t.ndim == 1:
+        t = np.eye(dims[-1])[t.astype(int)]
+    y = np.ascontiguousarray(t, dtype=np.float64)
     P = np.ascontiguousarray(params, dtype=np.float64)
     G = np.zeros_like(P) if grad else None
-    loss = lib().orc_mlp_loss_grad(X.shape[0], _p(X, _D), _p(y, _I32), _p(P, _D), _p(G, _D), *dims)
+    loss = lib().orc_mlp_loss_grad(X.shape[0], _p(X, _D), _p(y, _D), _p(P, _D), _p(G, _D), *dims)
     return loss, G
