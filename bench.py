@@ -47,14 +47,20 @@ def build_c1(n, k, seed, lognormal=False):
     return conn.random_in_degree(n, n, rng, deg, w)
 
 
+KERNEL_OF_FORMAT = {"csr": "csr_rows", "sell": "sell_slices", "csr_panel": "csr_panel",
+                    "ell": "ell_rows", "ellr": "ell_rows", "bsr": "bsr_rows", "dense": "dense_rows",
+                    "tiles": "tile_psp"}
+
+
 def traffic_from_profiles(name):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary."""
     f = ROOT / "profiles" / name
     if not f.exists():
         return None
-    try:
-        return json.loads(f.read_text()).get("hbm_bytes_per_launch")
-    except Exception:
-        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("hbm_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline(csr, seconds, threads):
@@ -89,12 +95,19 @@ def main():
     ap.add_argument("--sigma", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2", "c3"],
+                    help="c1 (default): rate network 1e8 synapses; c0: COBA 4000 neurons; "
+                         "c2: COBA 400k neurons / 4e8 synapses; c3: batched sweep of 256 instances")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the simulation step of this backend is a "
               "single-device path (multi-GPU sharding is a design note, not built).", file=sys.stderr)
         sys.exit(2)
+
+    if args.config != "c1":
+        print(json.dumps(globals()["bench_" + args.config](args)))
+        return
 
     import annarchy_amd as ah
 
@@ -130,8 +143,9 @@ def main():
     psp_bytes = proj.psp_bytes()
     achieved = psp_bytes / (psp_ms * 1e-3) / 1e9
     dtype = "f64" if args.precision == "double" else "f32"
-    tag = f"c1_{'lognormal' if args.lognormal else 'fixed'}_{args.format}_{dtype}"
-    traffic = traffic_from_profiles(f"{tag}_traffic.json")
+    kname = KERNEL_OF_FORMAT.get(args.format, args.format)
+    tag = f"{kname}_{dtype}_c1_{'lognormal' if args.lognormal else 'fixed'}{args.degree}"
+    traffic = traffic_from_profiles(f"{tag}_traffic.txt")
     log(f"step {ms_step:.3f} ms, psp {psp_ms * 1e3:.1f} us, {achieved:.0f} GB/s")
 
     cpu = None
@@ -172,7 +186,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "csr_rows (compute_psp)",
+            "kernel": f"{kname} (compute_psp, {args.format})",
             "kernel_ms": psp_ms,
             "algorithmic_bytes": psp_bytes,
         },
@@ -181,6 +195,169 @@ def main():
         "hbm_copy_gbs": copy_gbs,
     }
     print(json.dumps(out))
+
+
+# ---------------------------------------------------------------------------
+# Other workloads (parity configurations and chosen forms); each returns one
+# JSON object of the same shape as the default line.
+# ---------------------------------------------------------------------------
+def _line(metric, value, unit, steps, warmup, ms_step, dtype, config, roofline, cpu, **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
+           "warmup": warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded)", "config": config,
+           "roofline": roofline, "cpu_baseline": cpu}
+    out.update(extra)
+    return out
+
+
+def _coba(N, ne, conn_fn, steps_cap, seed=7, poisson=None, weights=(0.6, 6.7), I=20.0,
+          precision="double"):
+    import annarchy_amd as ah
+    from annarchy_amd import models
+
+    net = ah.Network(dt=0.1, seed=seed, precision=precision)
+    P = net.create(N, ah.COBA)
+    projs = conn_fn(net, P, ne)
+    Q = None
+    if poisson is not None:
+        M, rate, w = poisson
+        Q = net.create(M, models.poisson_neuron(rate))
+        projs.append(net.connect(Q, P[:M], "exc").one_to_one(w))
+    P.v = ah.Uniform(-60.0, -50.0, rng=net.rng)
+    mon = net.monitor(P, ["spike"], capacity=steps_cap)
+    net.compile(allow_build=False)
+    P.I = I
+    return net, P, Q, projs, mon
+
+
+def _events(projs, P, counts, idx):
+    """Delivered events = sum over spikes of the out-degree (core/Utils.py:223-239)."""
+    out = np.zeros(P.size, dtype=np.int64)
+    for pr in projs:
+        if pr.pre is P or getattr(pr.pre, "population", None) is P:
+            out += np.bincount(pr.csr.col_idx, minlength=P.size)[: P.size]
+    return int(out[idx].sum())
+
+
+def bench_c0(args):
+    """COBA LIF, 4000 neurons (3200 E / 800 I), p = 0.02, 10,000 steps."""
+    from oracle import oracle as orc
+
+    steps = 10000
+    def conn(net, P, ne):
+        return [net.connect(P[:ne], P, "exc").fixed_probability(0.02, 0.6),
+                net.connect(P[ne:], P, "inh").fixed_probability(0.02, 6.7)]
+    net, P, _, projs, mon = _coba(4000, 3200, conn, steps)
+    net.set_graph(True, 200)
+    v0 = P.v.reshape(-1).copy()
+    t0 = time.perf_counter()
+    net.run(steps)
+    el = time.perf_counter() - t0
+    counts, idx = mon.raw_spikes()
+    events = _events(projs, P, counts, idx)
+    cpu = None
+    if not args.no_cpu:
+        po = [orc.csrc_projection(projs[0].csr, 0), orc.csrc_projection(projs[1].csr, 1)]
+        t1 = time.perf_counter()
+        ref = orc.spiking_run(0, 4000, 0.1, v0, steps, po, refractory=np.asarray(P.refractory, np.int32))
+        cel = time.perf_counter() - t1
+        exact = bool(np.array_equal(ref["counts"], counts) and np.array_equal(ref["idx"], idx))
+        cpu = {"value": steps / cel, "unit": "steps/s", "cores": 1, "kind": "port",
+               "sample": f"the full 10000-step run, single thread; spike trains bit-exact: {exact}"}
+    return _line("steps/s", steps / el, "steps/s", steps, 0, el * 1e3 / steps, "f64",
+                 {"workload": "COBA LIF N=4000 (3200E/800I) p=0.02, dt=0.1ms, 10000 steps"},
+                 None, cpu, synaptic_events_per_s=events / el, spikes=int(counts.sum()),
+                 mean_rate_hz=float(counts.sum() / 4000 / (steps * 1e-4)))
+
+
+def bench_c2(args):
+    """COBA LIF, 400k neurons (320k E / 80k I), in-degree 1000 (800 E + 200 I),
+    4e8 synapses with per-synapse weights, Poisson 5 Hz into 10%."""
+    from annarchy_amd import connectivity as cn
+
+    steps = args.steps if args.steps != 200 else 2000
+    N, ne = 400_000, 320_000
+    scale = 80.0 / 1000.0  # same mean input conductance as the 4000-neuron benchmark
+    prec = args.precision
+    t_b = time.perf_counter()
+    rng = np.random.default_rng(99)
+    we = cn.random_in_degree(ne, N, rng, np.full(N, 800), 0.6 * scale)
+    wi = cn.random_in_degree(N - ne, N, rng, np.full(N, 200), 6.7 * scale)
+    # per-synapse weights (4 bytes each in FP32), equal values
+    we.w = np.full(we.nnz, 0.6 * scale)
+    wi.w = np.full(wi.nnz, 6.7 * scale)
+    # pre columns are ranks of the whole population: shift the inhibitory ones
+    wi = cn.CSR(N, N, wi.row_ptr, (wi.col_idx + ne).astype(np.int32), wi.w)
+    we = cn.CSR(N, N, we.row_ptr, we.col_idx, we.w)
+    log(f"c2 connectivity {we.nnz + wi.nnz} synapses in {time.perf_counter() - t_b:.0f} s")
+
+    def conn(net, P, ne_):
+        return [net.connect(P, P, "exc").from_csr(we), net.connect(P, P, "inh").from_csr(wi)]
+    net, P, Q, projs, mon = _coba(N, ne, conn, steps + args.warmup, poisson=(N // 10, 5.0, 0.6),
+                                  precision=prec)
+    net.set_graph(True, 100)
+    net.run(args.warmup)
+    t0 = time.perf_counter()
+    net.run(steps)
+    el = time.perf_counter() - t0
+    counts, idx = mon.raw_spikes()
+    c_t = counts[args.warmup:]
+    idx_t = idx[int(counts[:args.warmup].sum()):]
+    events = _events(projs[:2], P, c_t, idx_t)
+    return _line("synaptic events/s", events / el, "synaptic events/s", steps, args.warmup,
+                 el * 1e3 / steps, "f64" if prec == "double" else "f32",
+                 {"workload": "COBA LIF N=400000 in-degree 1000 (800E/200I) 4e8 synapses, "
+                              "Poisson 5 Hz into 10%, dt=0.1ms", "precision": prec},
+                 None, None, spikes_per_step=float(c_t.mean()),
+                 mean_rate_hz=float(c_t.sum() / N / (steps * 1e-4)))
+
+
+def bench_c3(args):
+    """Batched sweep: 256 instances, N_i ~ U{500..8000}, recurrent p=0.05,
+    shared dense 2000 x 2000 input, gains linspace(0.1, 2.0)."""
+    from annarchy_amd.batched import RateSweep, time_plain
+
+    steps = args.steps if args.steps != 200 else 2000
+    rng = np.random.default_rng(2026)
+    sizes = rng.integers(500, 8001, 256)
+    t_b = time.perf_counter()
+    sw = RateSweep(sizes, n_in=2000, p=0.05, seed=3)
+    log(f"c3 sweep: {sw.total} neurons, {sw.nnz} synapses, built in {time.perf_counter() - t_b:.0f} s")
+    b = sw.batched(lanes=16)
+    b.run(20)
+    t0 = time.perf_counter()
+    b.run(steps)
+    el_b = time.perf_counter() - t0
+    info = b.info()
+    ms_spmv = b.time_kernel(2, 20)
+    ms_gemm = b.time_kernel(1, 50)
+    log(f"batched {el_b * 1e3 / steps:.3f} ms/step, spmv {ms_spmv * 1e3:.0f} us, gemm {ms_gemm * 1e3:.1f} us")
+    # plain per-instance FP64 loop, same initial state, same number of steps
+    nets = sw.plain_networks()
+    plain_steps = min(steps, 200)
+    el_p = time_plain(nets, plain_steps)
+    # error of the batched form against the FP64 plain form after `steps`
+    for net, exc in nets:
+        net.run(20 + steps - plain_steps)
+    got = b.rates()
+    worst, pop_err = 0.0, 0.0
+    for i, (net, exc) in enumerate(nets):
+        ref = exc.r.reshape(-1)
+        worst = max(worst, float(np.max(np.abs(got[i] - ref)) / max(np.max(np.abs(ref)), 1e-3)))
+        pop_err = max(pop_err, abs(float(got[i].mean() - ref.mean())) / max(abs(ref.mean()), 1e-9))
+    sps_b = steps / el_b
+    sps_p = plain_steps / el_p
+    achieved = info["sparse_bytes"] / (ms_spmv * 1e-3) / 1e9
+    return _line("steps/s (batched sweep)", sps_b, "steps/s", steps, 20, el_b * 1e3 / steps, "f32/bf16",
+                 {"workload": "256 rate instances N~U{500..8000} p=0.05 + shared dense 2000x2000 input",
+                  "neurons": sw.total, "synapses": sw.nnz},
+                 {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "batch_spmv_update",
+                  "kernel_ms": ms_spmv},
+                 None, plain_steps_per_s=sps_p, speedup_vs_plain=sps_b / sps_p,
+                 max_rel_rate_error_vs_fp64=worst, max_rel_population_rate_error=pop_err,
+                 gemm={"ms": ms_gemm, "tflops": 3 * info["dense_flops"] / (ms_gemm * 1e-3) / 1e12,
+                       "peak_bf16_tflops": 2500.0})
 
 
 if __name__ == "__main__":
