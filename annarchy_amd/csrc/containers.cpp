@@ -209,42 +209,63 @@ int build_layout(const Proj& p, int32_t precision, const ahip_format_opts& o, Ra
             return 0;
         }
         case AHIP_FMT_CSR_PANEL: {
-            int lanes = o.lanes_per_row > 0 ? o.lanes_per_row : 16;
-            if (lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64) { err = "panel lanes must be 8..64"; return AHIP_EINVAL; }
+            // tiles = (block of RB rows) x (panel of P columns); entries of a
+            // tile are contiguous (rows ascending, columns ascending inside a
+            // row); tile t = q * n_rb + rb, so consecutive workgroups share
+            // the same slice of the pre-synaptic vector.
+            int lanes = o.lanes_per_row > 0 ? o.lanes_per_row : 8;
+            if (lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32) { err = "tile lanes must be 4..32"; return AHIP_EINVAL; }
             const size_t es = precision == AHIP_F64 ? 8 : 4;
             int32_t pc = o.panel_cols > 0 ? o.panel_cols : (int32_t)(32768 / es);
             if ((size_t)pc * es > 64 * 1024) { err = "panel exceeds 64 KiB of LDS"; return AHIP_EINVAL; }
+            const int32_t rb_rows = o.chunk > 0 ? o.chunk : 512;
+            if (rb_rows < 16 || rb_rows > 8192) { err = "tile rows must be 16..8192"; return AHIP_EINVAL; }
             const int32_t np = std::max<int32_t>(1, (p.n_cols + pc - 1) / pc);
-            std::vector<int64_t> aux((size_t)np * (R + 1), 0);
+            const int32_t nrb = (R + rb_rows - 1) / rb_rows;
+            const int64_t ntiles = (int64_t)np * nrb;
+            // count entries per (tile, local row)
+            std::vector<int32_t> cnt((size_t)ntiles * rb_rows, 0);
+            for (int32_t r = 0; r < R; r++) {
+                const int32_t rb = r / rb_rows, lr = r % rb_rows;
+                for (int64_t j = rp[r]; j < rp[r + 1]; j++) {
+                    const int64_t t = (int64_t)(ci[j] / pc) * nrb + rb;
+                    cnt[(size_t)t * rb_rows + lr]++;
+                }
+            }
+            std::vector<int64_t> base(ntiles + 1, 0);
+            std::vector<int32_t> roff((size_t)ntiles * (rb_rows + 1), 0);
+            for (int64_t t = 0; t < ntiles; t++) {
+                int32_t acc = 0;
+                for (int32_t lr = 0; lr < rb_rows; lr++) {
+                    roff[(size_t)t * (rb_rows + 1) + lr] = acc;
+                    acc += cnt[(size_t)t * rb_rows + lr];
+                }
+                roff[(size_t)t * (rb_rows + 1) + rb_rows] = acc;
+                base[t + 1] = base[t] + acc;
+            }
             std::vector<int32_t> col(p.nnz);
             std::vector<double> val(p.nnz);
-            // count entries per (panel, row)
-            std::vector<int64_t> cnt((size_t)np * R, 0);
-            for (int32_t r = 0; r < R; r++)
-                for (int64_t j = rp[r]; j < rp[r + 1]; j++) cnt[(size_t)(ci[j] / pc) * R + r]++;
-            int64_t off = 0;
-            for (int32_t q = 0; q < np; q++) {
-                for (int32_t r = 0; r < R; r++) {
-                    aux[(size_t)q * (R + 1) + r] = off;
-                    off += cnt[(size_t)q * R + r];
-                }
-                aux[(size_t)q * (R + 1) + R] = off;
-            }
-            std::vector<int64_t> cur(aux);
-            for (int32_t r = 0; r < R; r++)
+            std::fill(cnt.begin(), cnt.end(), 0);
+            for (int32_t r = 0; r < R; r++) {
+                const int32_t rb = r / rb_rows, lr = r % rb_rows;
                 for (int64_t j = rp[r]; j < rp[r + 1]; j++) {
-                    int32_t q = ci[j] / pc;
-                    int64_t pos = cur[(size_t)q * (R + 1) + r]++;
+                    const int64_t t = (int64_t)(ci[j] / pc) * nrb + rb;
+                    const int64_t pos = base[t] + roff[(size_t)t * (rb_rows + 1) + lr] + cnt[(size_t)t * rb_rows + lr]++;
                     col[pos] = ci[j];
                     val[pos] = wval(p, j);
                 }
+            }
             L.lanes = lanes;
             L.panel_cols = pc;
+            L.chunk = rb_rows;
             L.n_units = np;
+            L.width = nrb;
             L.stored = p.nnz;
-            if ((rc = upload(L.aux, aux.data(), aux.size() * sizeof(int64_t), err))) return rc;
+            if ((rc = upload(L.ptr, base.data(), base.size() * sizeof(int64_t), err))) return rc;
+            if ((rc = upload(L.aux, roff.data(), roff.size() * sizeof(int32_t), err))) return rc;
             if ((rc = upload(L.idx, col.data(), col.size() * sizeof(int32_t), err))) return rc;
             if ((rc = upload_vals(L.val, val, precision, err))) return rc;
+            if ((rc = upload(L.perm, nullptr, (size_t)np * R * es, err))) return rc;  // partial sums
             return 0;
         }
         default:

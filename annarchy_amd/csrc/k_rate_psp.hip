@@ -196,56 +196,64 @@ __global__ __launch_bounds__(kBlock) void dense_rows(int32_t n_rows, int32_t n_c
     if (lane == 0) store_target(y, row, s, acc != 0);
 }
 
-// CSR split into column panels: for every row, entries of panel q are
-// contiguous (the columns of a row stay ascending) and aux[q*(n_rows+1) + r]
-// gives where row r's panel-q entries start.  A workgroup stages the panel's
-// slice of x in LDS once and then serves every gather of its rows from LDS.
-// Partial sums of the panels are added in panel order by one workgroup per
-// row range (panel 0 writes, later panels add), so the sum is deterministic.
+// Row-block x column-panel tiles.  The gather of the pre-synaptic rates is
+// the limit of the row kernels above (every w*x[c] is a random 8-byte read
+// through the vector L1, one cache line per lane), so here a workgroup owns
+// one tile: it stages the panel's slice of x (P values) in LDS with coalesced
+// loads, then streams the tile's synapses (contiguous, rows ascending) with L
+// lanes per row and serves every x[c] from LDS.  Tile t = q*n_rb + rb writes
+// the partial sums of its rows into part[q][row]; tile_reduce adds the panels
+// in ascending order, so the result does not depend on scheduling.
 template <typename T, int L>
-__global__ __launch_bounds__(kBlock) void csr_panel(int32_t n_rows, int32_t n_cols, int32_t panel_cols,
-                                                    int32_t n_panels, int32_t rows_per_block,
-                                                    const int64_t* __restrict__ aux,
-                                                    const int32_t* __restrict__ col,
-                                                    const T* __restrict__ w,
-                                                    const T* __restrict__ x, T* __restrict__ y,
-                                                    int32_t acc) {
+__global__ __launch_bounds__(kBlock) void tile_psp(int32_t n_rows, int32_t n_cols, int32_t panel_cols,
+                                                   int32_t rb_rows, int32_t n_rb,
+                                                   const int64_t* __restrict__ base,
+                                                   const int32_t* __restrict__ roff,
+                                                   const int32_t* __restrict__ col,
+                                                   const T* __restrict__ w,
+                                                   const T* __restrict__ x, T* __restrict__ part) {
     extern __shared__ unsigned char lds_raw[];
     T* xs = reinterpret_cast<T*>(lds_raw);
-    const int32_t row0 = blockIdx.x * rows_per_block;
-    const int32_t row1 = min(n_rows, row0 + rows_per_block);
+    const int32_t t = blockIdx.x;
+    const int32_t q = t / n_rb, rb = t % n_rb;
+    const int32_t c0 = q * panel_cols;
+    const int32_t nc = min(panel_cols, n_cols - c0);
+    for (int32_t k = threadIdx.x; k < nc; k += kBlock) xs[k] = x[c0 + k];
+    __syncthreads();
     const int lane = threadIdx.x & (L - 1);
     const int grp = threadIdx.x / L;
     constexpr int GROUPS = kBlock / L;
-    // running totals of the rows this thread group owns (rows_per_block / GROUPS each)
-    for (int32_t q = 0; q < n_panels; q++) {
-        const int32_t c0 = q * panel_cols;
-        const int32_t nc = min(panel_cols, n_cols - c0);
-        __syncthreads();
-        for (int32_t k = threadIdx.x; k < nc; k += kBlock) xs[k] = x[c0 + k];
-        __syncthreads();
-        const int64_t* pq = aux + (int64_t)q * (n_rows + 1);
-        for (int32_t r = row0 + grp; r < row1; r += GROUPS) {
-            const int64_t beg = pq[r], end = pq[r + 1];
-            Acc<T> a0, a1;
-            int64_t j = beg + lane;
-            for (; j + L < end; j += 2 * L) {
-                const int32_t cA = ldnt(col + j), cB = ldnt(col + j + L);
-                const T wA = ldnt(w + j), wB = ldnt(w + j + L);
-                a0.add(wA * xs[cA - c0]);
-                a1.add(wB * xs[cB - c0]);
-            }
-            if (j < end) a0.add(ldnt(w + j) * xs[ldnt(col + j) - c0]);
-            a0.add(a1.template reduce<1>());
-            const T s = a0.template reduce<L>();
-            if (lane == 0) {
-                if (q == 0)
-                    store_target(y, r, s, acc != 0);
-                else
-                    y[r] += s;
-            }
+    const int32_t row0 = rb * rb_rows;
+    const int32_t nr = min(rb_rows, n_rows - row0);
+    const int64_t b0 = base[t];
+    const int32_t* ro = roff + (int64_t)t * (rb_rows + 1);
+    T* pq = part + (int64_t)q * n_rows + row0;
+    for (int32_t lr = grp; lr < nr; lr += GROUPS) {
+        const int64_t beg = b0 + ro[lr], end = b0 + ro[lr + 1];
+        Acc<T> a0, a1;
+        int64_t j = beg + lane;
+        for (; j + L < end; j += 2 * L) {
+            const int32_t cA = ldnt(col + j), cB = ldnt(col + j + L);
+            const T wA = ldnt(w + j), wB = ldnt(w + j + L);
+            a0.add(wA * xs[cA - c0]);
+            a1.add(wB * xs[cB - c0]);
         }
+        if (j < end) a0.add(ldnt(w + j) * xs[ldnt(col + j) - c0]);
+        a0.add(a1.template reduce<1>());
+        const T s = a0.template reduce<L>();
+        if (lane == 0) pq[lr] = s;
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void tile_reduce(int32_t n_rows, int32_t n_panels,
+                                                      const T* __restrict__ part, T* __restrict__ y,
+                                                      int32_t acc) {
+    const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n_rows) return;
+    Acc<T> a;
+    for (int32_t q = 0; q < n_panels; q++) a.add(part[(int64_t)q * n_rows + r]);
+    store_target(y, r, a.template reduce<1>(), acc != 0);
 }
 
 template <typename T>
@@ -349,25 +357,29 @@ int launch_rate_t(const ahip_net* net, const Proj& p, const T* x, T* y, hipStrea
             return (int)hipGetLastError();
         }
         case AHIP_FMT_CSR_PANEL: {
-            const int32_t rpb = 256;
-            const int64_t nb = (p.n_rows + rpb - 1) / rpb;
-            if (nb == 0) return 0;
+            const int64_t ntiles = (int64_t)L.n_units * L.width;
+            if (ntiles == 0 || p.n_rows == 0) return 0;
             const size_t lds = (size_t)L.panel_cols * sizeof(T);
+            T* part = L.perm.as<T>();
             switch (L.lanes) {
-#define AHIP_PANEL_CASE(N)                                                                    \
-    case N:                                                                                   \
-        hipLaunchKernelGGL((csr_panel<T, N>), dim3(nb), dim3(kBlock), lds, s, p.n_rows,       \
-                           p.n_cols, L.panel_cols, L.n_units, rpb, L.aux.as<int64_t>(),       \
-                           L.idx.as<int32_t>(), L.val.as<T>(), x, y, acc);                    \
+#define AHIP_TILE_CASE(N)                                                                      \
+    case N:                                                                                    \
+        hipLaunchKernelGGL((tile_psp<T, N>), dim3(ntiles), dim3(kBlock), lds, s, p.n_rows,      \
+                           p.n_cols, L.panel_cols, L.chunk, L.width, L.ptr.as<int64_t>(),       \
+                           L.aux.as<int32_t>(), L.idx.as<int32_t>(), L.val.as<T>(), x, part);   \
         break;
-                AHIP_PANEL_CASE(8)
-                AHIP_PANEL_CASE(16)
-                AHIP_PANEL_CASE(32)
-                AHIP_PANEL_CASE(64)
-#undef AHIP_PANEL_CASE
+                AHIP_TILE_CASE(4)
+                AHIP_TILE_CASE(8)
+                AHIP_TILE_CASE(16)
+                AHIP_TILE_CASE(32)
+#undef AHIP_TILE_CASE
                 default:
                     return (int)hipErrorInvalidValue;
             }
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return (int)e;
+            hipLaunchKernelGGL((tile_reduce<T>), dim3((p.n_rows + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                               p.n_rows, L.n_units, part, y, acc);
             return (int)hipGetLastError();
         }
         default:

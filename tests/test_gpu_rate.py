@@ -32,6 +32,7 @@ FORMATS = [
     dict(storage_format="bsr", chunk=8),
     dict(storage_format="dense"),
     dict(storage_format="csr_panel", lanes=16, panel_cols=512),
+    dict(storage_format="csr_panel", lanes=4, panel_cols=256, chunk=100),
 ]
 
 
