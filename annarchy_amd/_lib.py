@@ -18,7 +18,7 @@ from .build import CORE_LIB
 AHIP_F64 = 0
 AHIP_F32 = 1
 
-FORMATS = {"csr": 0, "ell": 1, "ellr": 2, "sell": 3, "bsr": 4, "dense": 5, "csr_panel": 6}
+FORMATS = {"csr": 0, "ell": 1, "ellr": 2, "sell": 3, "bsr": 4, "dense": 5, "csr_panel": 6, "csr_vec": 7}
 FORMAT_NAMES = {v: k for k, v in FORMATS.items()}
 
 
@@ -101,6 +101,7 @@ _SIGS = {
     "ahip_batch_run": ([_P, _I64, _I32], _I32),
     "ahip_batch_time": ([_P, _I32, _I32, _PD], _I32),
     "ahip_batch_info": ([_P, _PI64, _PI32, _PD, _PD], _I32),
+    "ahip_batch_set_mode": ([_P, _I32], _I32),
     "ahip_mlp_train": ([_PF, _PF, _I32, _PF, _I32, C.c_float, C.c_float, _PF, _PF], _I32),
     "ahip_mlp_predict": ([_PF, _I32, _PF, _PI32, _PF], _I32),
 }

@@ -152,6 +152,10 @@ class BatchedSweep:
         if rc != 0:
             raise _lib.AhipError(_lib.lib().ahip_batch_last_error().decode())
 
+    def set_mode(self, mode: int):
+        """1: instance rates staged in LDS (default when they fit); 0: global gathers."""
+        self._check(_lib.lib().ahip_batch_set_mode(self._h, int(mode)))
+
     def run(self, steps: int, graph_batch: int = 50):
         self._check(_lib.lib().ahip_batch_run(self._h, int(steps), graph_batch))
 

@@ -186,7 +186,37 @@ int ahip_batch_set_recurrent(ahip_batch* b, const int64_t* row_ptr, const int32_
     if ((rc = balloc(B.w, (size_t)nnz * 4, wf.data()))) return rc;
     B.lanes = lanes;
     b->nnz = nnz;
+    // work chunks for the LDS kernel: rows of one instance, ~32k synapses each
+    std::vector<int32_t> ch;
+    int32_t max_n = 0;
+    for (int32_t i = 0; i < B.n_inst; i++) {
+        max_n = std::max<int32_t>(max_n, b->sizes[i]);
+        int64_t r = b->offsets[i];
+        while (r < b->offsets[i + 1]) {
+            int64_t e = r;
+            while (e < b->offsets[i + 1] && (e == r || row_ptr[e + 1] - row_ptr[r] <= 32768)) e++;
+            ch.push_back(i);
+            ch.push_back((int32_t)r);
+            ch.push_back((int32_t)e);
+            r = e;
+        }
+    }
+    B.max_n = max_n;
+    B.n_chunks = (int32_t)(ch.size() / 3);
+    if ((rc = balloc(B.chunks, ch.size() * 4, ch.data()))) return rc;
+    if ((rc = balloc(B.inst_base, b->offsets.size() * 8, b->offsets.data()))) return rc;
+    B.mode = (size_t)max_n * sizeof(float) <= 64 * 1024 ? 1 : 0;
     b->ready_sparse = true;
+    return 0;
+}
+
+int ahip_batch_set_mode(ahip_batch* b, int32_t mode) {
+    if (!b || (mode != 0 && mode != 1)) return bfail(AHIP_EINVAL, "mode must be 0 (global gathers) or 1 (LDS)");
+    if (mode == 1 && (size_t)b->B.max_n * sizeof(float) > 64 * 1024)
+        return bfail(AHIP_EINVAL, "an instance is too large for the LDS kernel");
+    BCHK(hipStreamSynchronize(b->stream));
+    drop_graph(b);
+    b->B.mode = mode;
     return 0;
 }
 

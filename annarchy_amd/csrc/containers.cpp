@@ -75,9 +75,11 @@ int build_layout(const Proj& p, int32_t precision, const ahip_format_opts& o, Ra
     L.format = o.format;
     int rc;
     switch (o.format) {
-        case AHIP_FMT_CSR: {
+        case AHIP_FMT_CSR:
+        case AHIP_FMT_CSR_VEC: {
             int lanes = o.lanes_per_row > 0 ? o.lanes_per_row : 64;
             if (!pow2(lanes) || lanes > 64) { err = "CSR lanes_per_row must be a power of two <= 64"; return AHIP_EINVAL; }
+            if (o.format == AHIP_FMT_CSR_VEC && lanes < 4) { err = "csr_vec needs at least 4 lanes per row"; return AHIP_EINVAL; }
             L.lanes = lanes;
             std::vector<double> w(p.nnz);
             for (int64_t j = 0; j < p.nnz; j++) w[j] = wval(p, j);

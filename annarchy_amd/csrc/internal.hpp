@@ -163,6 +163,10 @@ struct BatchDev {
     DevBuf phase;                    // double [n_in][n_inst]
     DevBuf y;                        // float [m_tiles*32][n_tiles*32]
     DevBuf t_dev;                    // int64
+    int32_t mode = 1;                // 0: gathers from global memory, 1: instance rates staged in LDS
+    int32_t n_chunks = 0, max_n = 0;
+    DevBuf chunks;                   // int32 [n_chunks][3] = (instance, first row, end row)
+    DevBuf inst_base;                // int64 [n_inst + 1]
 };
 int batch_launch_input(const BatchDev& B, int32_t step, hipStream_t s);
 int batch_launch_gemm(const BatchDev& B, hipStream_t s);

@@ -135,6 +135,60 @@ __global__ __launch_bounds__(256) void batch_spmv_update(
     }
 }
 
+// Same step, one workgroup per chunk of rows of one instance: the instance's
+// rates (N_i <= 16384 floats) are staged in LDS once, the chunk's synapses
+// are read with 16-byte loads (4 per lane per chunk, head/tail scalar) and
+// every gather is served from LDS.  chunks[3*c] = (instance, first row, end row).
+typedef int v4i_b __attribute__((ext_vector_type(4)));
+typedef float v4f_b __attribute__((ext_vector_type(4)));
+
+template <int L>
+__global__ __launch_bounds__(256) void batch_spmv_lds(
+    const int32_t* __restrict__ chunks, const int64_t* __restrict__ inst_base,
+    const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col, const float* __restrict__ w,
+    const float* __restrict__ gain, const float* __restrict__ y, int32_t ld_y, int32_t n_in,
+    const float* __restrict__ r_in, float* __restrict__ r_out, float* __restrict__ sum_out, float dt,
+    float tau) {
+    extern __shared__ float xs[];
+    const int32_t inst = chunks[3 * blockIdx.x];
+    const int32_t r0 = chunks[3 * blockIdx.x + 1], r1 = chunks[3 * blockIdx.x + 2];
+    const int64_t base = inst_base[inst];
+    const int32_t n = (int32_t)(inst_base[inst + 1] - base);
+    for (int32_t k = threadIdx.x; k < n; k += 256) xs[k] = r_in[base + k];
+    __syncthreads();
+    const int lane = threadIdx.x & (L - 1);
+    const int grp = threadIdx.x / L;
+    constexpr int GROUPS = 256 / L;
+    for (int32_t row = r0 + grp; row < r1; row += GROUPS) {
+        const int64_t beg = row_ptr[row], end = row_ptr[row + 1];
+        Kahan k0{0.f, 0.f}, k1{0.f, 0.f};
+        const int64_t body = min(end, (beg + 3) & ~(int64_t)3);
+        if (beg + lane < body) k0.add(w[beg + lane] * xs[col[beg + lane] - base]);
+        int64_t j = body + 4 * lane;
+        for (; j + 4 <= end; j += 4 * L) {
+            const v4i_b c = __builtin_nontemporal_load(reinterpret_cast<const v4i_b*>(col + j));
+            const v4f_b v = __builtin_nontemporal_load(reinterpret_cast<const v4f_b*>(w + j));
+            k0.add(v.x * xs[c.x - base]);
+            k1.add(v.y * xs[c.y - base]);
+            k0.add(v.z * xs[c.z - base]);
+            k1.add(v.w * xs[c.w - base]);
+        }
+        for (; j < end; j++) k0.add(w[j] * xs[col[j] - base]);
+        k0.add(k1.value());
+        const float s = group_sum_kahan<L>(k0.s, k0.c);
+        if (lane == 0) {
+            const int32_t nl = (int32_t)(row - base);
+            const float s_in = nl < n_in ? y[(int64_t)nl * ld_y + inst] : 0.0f;
+            const float r = xs[nl];
+            const float _r = (s + gain[inst] * s_in - r) / tau;
+            float rn = r + dt * _r;
+            if (rn < 0.0f) rn = 0.0f;
+            r_out[row] = rn;
+            sum_out[row] = s;
+        }
+    }
+}
+
 __global__ void batch_advance(int64_t* t, int64_t n) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *t += n;
 }
@@ -164,6 +218,28 @@ int batch_launch_spmv(const BatchDev& B, int32_t parity, hipStream_t s) {
     const int64_t nb = (threads + 255) / 256;
     const float* rin = B.r[parity].as<float>();
     float* rout = B.r[parity ^ 1].as<float>();
+    if (B.mode == 1) {
+        const size_t lds = (size_t)B.max_n * sizeof(float);
+#define AHIP_BLDS_CASE(N)                                                                         \
+    case N:                                                                                       \
+        hipLaunchKernelGGL(batch_spmv_lds<N>, dim3(B.n_chunks), dim3(256), lds, s,                \
+                           B.chunks.as<int32_t>(), B.inst_base.as<int64_t>(), B.row_ptr.as<int64_t>(), \
+                           B.col.as<int32_t>(), B.w.as<float>(), B.gain.as<float>(), B.y.as<float>(), \
+                           B.n_tiles * 32, B.n_in, rin, rout, B.sum_exc.as<float>(), (float)B.dt,     \
+                           (float)B.tau);                                                          \
+        break;
+        switch (B.lanes) {
+            AHIP_BLDS_CASE(4)
+            AHIP_BLDS_CASE(8)
+            AHIP_BLDS_CASE(16)
+            AHIP_BLDS_CASE(32)
+            AHIP_BLDS_CASE(64)
+            default:
+                return (int)hipErrorInvalidValue;
+        }
+#undef AHIP_BLDS_CASE
+        return (int)hipGetLastError();
+    }
 #define AHIP_BATCH_CASE(N)                                                                         \
     case N:                                                                                        \
         hipLaunchKernelGGL(batch_spmv_update<N>, dim3(nb), dim3(256), 0, s, B.total,               \

@@ -77,6 +77,78 @@ __global__ __launch_bounds__(kBlock) void csr_rows(int32_t n_rows, const int64_t
     if (lane == 0) store_target(y, row, s, acc != 0);
 }
 
+// CSR with 16-byte loads: every lane takes 4 consecutive synapses (one int4
+// of column indices, 32 or 16 bytes of weights) and keeps two chunks in
+// flight, so a wave has 2 x 64 x (16 + 4*sizeof(T)) bytes outstanding.  The
+// row head up to the next multiple-of-4 index is done with scalar loads,
+// the straddling tail chunk by the one lane that owns it.
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Quad;
+template <>
+struct Quad<double> {
+    double v[4];
+    __device__ inline void load(const double* p) {
+        const v2d a = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(p));
+        const v2d b = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(p) + 1);
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    }
+};
+template <>
+struct Quad<float> {
+    float v[4];
+    __device__ inline void load(const float* p) {
+        const v4f a = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(p));
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    }
+};
+
+template <typename T, int L>
+__global__ __launch_bounds__(kBlock) void csr_vec(int32_t n_rows, const int64_t* __restrict__ row_ptr,
+                                                  const int32_t* __restrict__ col,
+                                                  const T* __restrict__ w, const T* __restrict__ x,
+                                                  T* __restrict__ y, int32_t acc) {
+    const int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / L;
+    const int lane = threadIdx.x & (L - 1);
+    if (row >= n_rows) return;
+    const int64_t beg = row_ptr[row], end = row_ptr[row + 1];
+    Acc<T> a0, a1;
+    const int64_t body = min(end, (beg + 3) & ~(int64_t)3);
+    if (beg + lane < body) a0.add(w[beg + lane] * x[col[beg + lane]]);
+    int64_t j = body + 4 * lane;
+    for (; j + 4 * L + 4 <= end; j += 8 * L) {
+        const v4i c0 = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(col + j));
+        const v4i c1 = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(col + j + 4 * L));
+        Quad<T> w0, w1;
+        w0.load(w + j);
+        w1.load(w + j + 4 * L);
+        a0.add(w0.v[0] * x[c0.x]);
+        a1.add(w0.v[1] * x[c0.y]);
+        a0.add(w0.v[2] * x[c0.z]);
+        a1.add(w0.v[3] * x[c0.w]);
+        a0.add(w1.v[0] * x[c1.x]);
+        a1.add(w1.v[1] * x[c1.y]);
+        a0.add(w1.v[2] * x[c1.z]);
+        a1.add(w1.v[3] * x[c1.w]);
+    }
+    for (; j + 4 <= end; j += 4 * L) {
+        const v4i c0 = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(col + j));
+        Quad<T> w0;
+        w0.load(w + j);
+        a0.add(w0.v[0] * x[c0.x]);
+        a1.add(w0.v[1] * x[c0.y]);
+        a0.add(w0.v[2] * x[c0.z]);
+        a1.add(w0.v[3] * x[c0.w]);
+    }
+    for (; j < end; j++) a0.add(w[j] * x[col[j]]);
+    a0.add(a1.template reduce<1>());
+    const T s = a0.template reduce<L>();
+    if (lane == 0) store_target(y, row, s, acc != 0);
+}
+
 // ELLPACK, column-major over the stored rows (rows of the LIL); padding column
 // = INT32_MAX ends a row (CUDA/ELL.py:240-262).  ELLR reads the row length.
 template <typename T, bool R>
@@ -283,6 +355,30 @@ int launch_rate_t(const ahip_net* net, const Proj& p, const T* x, T* y, hipStrea
                 AHIP_CSR_CASE(32)
                 AHIP_CSR_CASE(64)
 #undef AHIP_CSR_CASE
+                default:
+                    return (int)hipErrorInvalidValue;
+            }
+            return (int)hipGetLastError();
+        }
+        case AHIP_FMT_CSR_VEC: {
+            const int64_t threads = (int64_t)p.n_rows * L.lanes;
+            const int64_t nb = (threads + kBlock - 1) / kBlock;
+            if (nb == 0) return 0;
+            const int64_t* rp = L.ptr.as<int64_t>();
+            const int32_t* ci = L.idx.as<int32_t>();
+            const T* w = L.val.as<T>();
+            switch (L.lanes) {
+#define AHIP_VEC_CASE(N)                                                                    \
+    case N:                                                                                 \
+        hipLaunchKernelGGL((csr_vec<T, N>), dim3(nb), dim3(kBlock), 0, s, p.n_rows, rp, ci, \
+                           w, x, y, acc);                                                   \
+        break;
+                AHIP_VEC_CASE(4)
+                AHIP_VEC_CASE(8)
+                AHIP_VEC_CASE(16)
+                AHIP_VEC_CASE(32)
+                AHIP_VEC_CASE(64)
+#undef AHIP_VEC_CASE
                 default:
                     return (int)hipErrorInvalidValue;
             }

@@ -492,10 +492,15 @@ class Network:
         _lib.check(_lib.lib().ahip_set_graph_enabled(self._h, int(enabled)))
         _lib.check(_lib.lib().ahip_set_graph_batch(self._h, int(batch)))
 
+    def destroy(self):
+        """Release the device network now (instead of at garbage collection)."""
+        if self._h is not None and _lib._lib is not None:
+            _lib.lib().ahip_net_destroy(self._h)
+        self._h = None
+        self.compiled = False
+
     def __del__(self):
         try:
-            if self._h is not None and _lib._lib is not None:
-                _lib.lib().ahip_net_destroy(self._h)
-                self._h = None
+            self.destroy()
         except Exception:
             pass

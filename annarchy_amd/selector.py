@@ -227,7 +227,7 @@ def autotune(csr: conn.CSR, iters: int = 10, repeats: int = 2, classes=None) -> 
             continue
         proj.set_format(**CLASSES[c])
         times[c] = min(proj.time_psp(iters) for _ in range(repeats))
-    del net
+    net.destroy()
     return times
 
 
@@ -343,6 +343,8 @@ def select(csr: conn.CSR, mode: str = "mlp", selector: MLPSelector | None = None
 # ---------------------------------------------------------------------------
 def _gen_task(spec):
     csr = generate(spec)
+    h = heuristic_class(csr)
+    spec = dict(spec, heuristic=-1 if h == "dense" else int(h), nnz=int(csr.nnz))
     return spec, csr, features(csr, spec["seed"] & 0xFFFF)
 
 

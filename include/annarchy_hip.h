@@ -39,7 +39,8 @@ extern "C" {
 #define AHIP_FMT_SELL 3   /* SELL-C-sigma with row sorting inside sigma windows (CUDA/SELL.py:230-246 has no sigma) */
 #define AHIP_FMT_BSR 4    /* block sparse rows, square tiles (CUDA/BSR.py:229-252) */
 #define AHIP_FMT_DENSE 5  /* row-major dense (CUDA/Dense.py:287-316) */
-#define AHIP_FMT_CSR_PANEL 6 /* CSR split in column panels, pre-synaptic panel staged in LDS */
+#define AHIP_FMT_CSR_PANEL 6 /* row-block x column-panel tiles, pre-synaptic panel staged in LDS */
+#define AHIP_FMT_CSR_VEC 7   /* CSR read with 16-byte loads, 4 synapses per lane per chunk */
 
 /* Precision of a network (core/CTypes.py:30-74, ConfigManagement.py:25). */
 #define AHIP_F64 0
@@ -193,6 +194,9 @@ int ahip_batch_get_input_sums(ahip_batch* b, double* y);
 int ahip_batch_run(ahip_batch* b, int64_t steps, int32_t graph_batch);
 /* which: 0 input update, 1 GEMM, 2 SpMV + update; mean launch time (ms) */
 int ahip_batch_time(ahip_batch* b, int32_t which, int32_t iters, double* avg_ms);
+/* 1 (default when every instance fits): each workgroup stages its instance's
+ * rates in LDS; 0: rates are gathered from global memory. */
+int ahip_batch_set_mode(ahip_batch* b, int32_t mode);
 int ahip_batch_info(ahip_batch* b, int64_t* nnz, int32_t* total, double* sparse_bytes,
                     double* dense_flops);
 

@@ -51,11 +51,13 @@ def test_plain_instances_match_oracle(gpu):
         assert np.max(np.abs(got - ref)) <= 1e-9 * max(1.0, np.max(np.abs(ref))), i
 
 
+@pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("lanes", [8, 32])
-def test_batched_matches_oracle(gpu, lanes):
+def test_batched_matches_oracle(gpu, lanes, mode):
     sw = _sweep(2)
     steps = 300
     b = sw.batched(lanes=lanes)
+    b.set_mode(mode)
     b.run(steps, graph_batch=20)
     got = b.rates()
     worst = 0.0

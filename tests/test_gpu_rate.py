@@ -22,6 +22,9 @@ FORMATS = [
     dict(storage_format="csr", lanes=8),
     dict(storage_format="csr", lanes=32),
     dict(storage_format="csr", lanes=64),
+    dict(storage_format="csr_vec", lanes=4),
+    dict(storage_format="csr_vec", lanes=16),
+    dict(storage_format="csr_vec", lanes=64),
     dict(storage_format="ell"),
     dict(storage_format="ellr"),
     dict(storage_format="sell", lanes=1, sigma=1),
