@@ -303,14 +303,32 @@ __global__ __launch_bounds__(kBlock) void tile_psp(int32_t n_rows, int32_t n_col
     for (int32_t lr = grp; lr < nr; lr += GROUPS) {
         const int64_t beg = b0 + ro[lr], end = b0 + ro[lr + 1];
         Acc<T> a0, a1;
-        int64_t j = beg + lane;
-        for (; j + L < end; j += 2 * L) {
-            const int32_t cA = ldnt(col + j), cB = ldnt(col + j + L);
-            const T wA = ldnt(w + j), wB = ldnt(w + j + L);
-            a0.add(wA * xs[cA - c0]);
-            a1.add(wB * xs[cB - c0]);
+        if (L >= 4) {
+            // 16-byte loads: scalar head up to a multiple-of-4 index, then 4
+            // synapses per lane per chunk, scalar tail in the straddling lane
+            const int64_t body = min(end, (beg + 3) & ~(int64_t)3);
+            if (beg + lane < body) a0.add(ldnt(w + beg + lane) * xs[ldnt(col + beg + lane) - c0]);
+            int64_t j = body + 4 * lane;
+            for (; j + 4 <= end; j += 4 * L) {
+                const v4i c = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(col + j));
+                Quad<T> q4;
+                q4.load(w + j);
+                a0.add(q4.v[0] * xs[c.x - c0]);
+                a1.add(q4.v[1] * xs[c.y - c0]);
+                a0.add(q4.v[2] * xs[c.z - c0]);
+                a1.add(q4.v[3] * xs[c.w - c0]);
+            }
+            for (; j < end; j++) a0.add(ldnt(w + j) * xs[ldnt(col + j) - c0]);
+        } else {
+            int64_t j = beg + lane;
+            for (; j + L < end; j += 2 * L) {
+                const int32_t cA = ldnt(col + j), cB = ldnt(col + j + L);
+                const T wA = ldnt(w + j), wB = ldnt(w + j + L);
+                a0.add(wA * xs[cA - c0]);
+                a1.add(wB * xs[cB - c0]);
+            }
+            if (j < end) a0.add(ldnt(w + j) * xs[ldnt(col + j) - c0]);
         }
-        if (j < end) a0.add(ldnt(w + j) * xs[ldnt(col + j) - c0]);
         a0.add(a1.template reduce<1>());
         const T s = a0.template reduce<L>();
         if (lane == 0) pq[lr] = s;
