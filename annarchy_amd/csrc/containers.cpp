@@ -47,7 +47,13 @@ int upload(DevBuf& b, const void* src, size_t bytes, std::string& err) {
             return AHIP_EHIP;
         }
     } else {
-        hipMemset(b.p, 0, bytes);
+        // wait for the clear: later kernels run on a non-blocking stream
+        e = hipMemset(b.p, 0, bytes);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e != hipSuccess) {
+            err = std::string("hipMemset failed: ") + hipGetErrorString(e);
+            return AHIP_EHIP;
+        }
     }
     return 0;
 }

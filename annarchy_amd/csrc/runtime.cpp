@@ -62,7 +62,11 @@ int dev_alloc(DevBuf& b, size_t bytes) {
         return fail(AHIP_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
     }
     b.bytes = bytes;
+    // The network's work runs on a non-blocking stream, which is not ordered
+    // against the null stream a plain memset uses: wait for the clear here so
+    // a later kernel on the network stream cannot be overtaken by it.
     HIPCHK(hipMemset(b.p, 0, bytes));
+    HIPCHK(hipDeviceSynchronize());
     return 0;
 }
 

@@ -348,20 +348,27 @@ def _gen_task(spec):
     return spec, csr, features(csr, spec["seed"] & 0xFFFF)
 
 
-def build_corpus(count: int, out_path, workers: int = 8, seed: int = 2024, log=print):
+def build_corpus(count: int, out_path, workers: int = 8, seed: int = 2024, log=print, tuner=None):
     """Autotune `count` corpus matrices; writes specs, features and times to
     `out_path` (npz) every 100 matrices and at the end."""
     import json
-    import multiprocessing as mp
     import time
+    from concurrent.futures import ThreadPoolExecutor
 
     specs = corpus_specs(count, seed)
     feats, times, kept = [], [], []
     t0 = time.perf_counter()
-    ctx = mp.get_context("spawn")
-    with ctx.Pool(workers) as pool:
-        for i, (spec, csr, f) in enumerate(pool.imap(_gen_task, specs, chunksize=2)):
-            t = autotune(csr)
+    # generation runs ahead in threads (numpy releases the GIL in its heavy
+    # parts); the device timing stays in this thread
+    def windows():
+        # a bounded window of matrices is generated ahead of the tuner
+        with ThreadPoolExecutor(max_workers=workers) as pool:
+            for w0 in range(0, len(specs), 2 * workers):
+                yield from pool.map(_gen_task, specs[w0:w0 + 2 * workers])
+
+    if True:
+        for i, (spec, csr, f) in enumerate(windows()):
+            t = tuner(csr) if tuner is not None else autotune(csr)
             feats.append(f)
             times.append(t)
             kept.append(spec)
@@ -378,3 +385,38 @@ if __name__ == "__main__":
     n = int(sys.argv[1]) if len(sys.argv) > 1 else 2000
     out = sys.argv[2] if len(sys.argv) > 2 else "selector_corpus.npz"
     build_corpus(n, out, log=lambda m: print(m, flush=True))
+
+
+def evaluate(X, T, heuristic, save_path=None, seeds=(0, 1, 2), epochs=3000, log=print):
+    """80/20 split: regret of the MLP (per seed), of the reference heuristic and
+    of the best single configuration on the held-out matrices."""
+    X, T, heuristic = np.asarray(X), np.asarray(T), np.asarray(heuristic)
+    ok = np.isfinite(T).any(axis=1) & (heuristic >= 0)
+    X, T, heuristic = X[ok], T[ok], heuristic[ok]
+    perm = np.random.default_rng(0).permutation(len(X))
+    ntr = int(0.8 * len(X))
+    tr, te = perm[:ntr], perm[ntr:]
+    Tt = T[te]
+    best = Tt.min(axis=1)
+    rows = np.arange(len(te))
+    res = {"n_train": int(ntr), "n_test": int(len(te))}
+    res["heuristic"] = dict(mean_regret=float(np.mean(Tt[rows, heuristic[te]] / best - 1)),
+                            total_time_ratio=float(Tt[rows, heuristic[te]].sum() / best.sum()))
+    rel = np.where(np.isfinite(T[tr]), T[tr] / T[tr].min(axis=1, keepdims=True), np.nan)
+    single = int(np.nanargmin(np.nanmean(rel, axis=0)))
+    res["best_single_class"] = dict(cls=single, mean_regret=float(np.mean(Tt[:, single] / best - 1)))
+    for seed in seeds:
+        sel = MLPSelector()
+        curve = sel.fit(X[tr], T[tr], epochs=epochs, lr=3e-3, seed=seed)
+        pr = sel.predict(X[te])
+        reg = regret(Tt, pr)
+        res[f"mlp_seed{seed}"] = dict(mean_regret=float(np.mean(reg)), median=float(np.median(reg)),
+                                      p90=float(np.quantile(reg, 0.9)), loss0=float(curve[0]),
+                                      loss=float(curve[-1]),
+                                      total_time_ratio=float(Tt[rows, pr].sum() / best.sum()),
+                                      train_regret=float(np.mean(regret(T[tr], sel.predict(X[tr])))))
+        if seed == seeds[0] and save_path is not None:
+            sel.save(save_path)
+        log(f"seed {seed}: {res[f'mlp_seed{seed}']}")
+    res["label_histogram"] = np.bincount(np.argmin(T, axis=1), minlength=len(CLASSES)).tolist()
+    return res

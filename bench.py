@@ -47,7 +47,7 @@ def build_c1(n, k, seed, lognormal=False):
     return conn.random_in_degree(n, n, rng, deg, w)
 
 
-KERNEL_OF_FORMAT = {"csr": "csr_rows", "sell": "sell_slices", "csr_panel": "csr_panel",
+KERNEL_OF_FORMAT = {"csr": "csr_rows", "sell": "sell_slices", "csr_panel": "tile_psp", "csr_vec": "csr_vec",
                     "ell": "ell_rows", "ellr": "ell_rows", "bsr": "bsr_rows", "dense": "dense_rows",
                     "tiles": "tile_psp"}
 
@@ -90,9 +90,14 @@ def main():
     ap.add_argument("--degree", type=int, default=1000)
     ap.add_argument("--lognormal", action="store_true")
     ap.add_argument("--precision", default="double", choices=["double", "float"])
-    ap.add_argument("--format", default="csr")
-    ap.add_argument("--lanes", type=int, default=64)
+    # default: row-block x column-panel tiles, pre-synaptic panel in LDS (the
+    # fastest FP64 layout of the sweep: 16 lanes per row, 4096-column panels,
+    # 256-row blocks)
+    ap.add_argument("--format", default="csr_panel")
+    ap.add_argument("--lanes", type=int, default=16)
     ap.add_argument("--sigma", type=int, default=1)
+    ap.add_argument("--panel-cols", type=int, default=4096)
+    ap.add_argument("--chunk", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2", "c3"],
@@ -118,7 +123,8 @@ def main():
     net = ah.Network(dt=1.0, seed=42, precision=args.precision)
     pop = net.create(args.neurons, ah.RateNoisy)
     proj = net.connect(pop, pop, "exc")
-    proj.from_csr(csr, storage_format=args.format, lanes=args.lanes, sigma=args.sigma)
+    proj.from_csr(csr, storage_format=args.format, lanes=args.lanes, sigma=args.sigma,
+                  panel_cols=args.panel_cols, chunk=args.chunk)
     net.compile(allow_build=False)
     pop.r = np.random.default_rng(7).random(args.neurons)
     log("network uploaded")

@@ -93,8 +93,12 @@ def _oracle_coba(P, ce, ci, v0, steps):
 
 
 def test_coba_bit_exact_vs_oracle(gpu):
-    steps = 1000  # 100 ms
-    net, P, ce, ci, mon = _coba_net(graph_batch=50)
+    steps = 3000  # 300 ms
+    net, P, ce, ci, mon = _coba_net(graph_batch=50, steps_cap=steps)
+    # regression: population-wide values written at set-up (zeroed allocations
+    # followed by the fill of the initial values) must all have landed
+    assert P.Vr == -60.0 and P.El == -60.0 and P.Vt == -50.0
+    P.I = 20.0
     v0 = P.v.reshape(-1).copy()
     assert np.all(P.refractory == 50)
     net.run(steps)
