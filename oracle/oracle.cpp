@@ -492,6 +492,21 @@ static void dense_fwd(int n, int in, int out, const double* W, const double* b, 
         }
 }
 
+// Logits (n x d_out) of the MLP in FP64.
+void orc_mlp_forward(int32_t n, const double* X, const double* params, double* logits, int32_t d_in,
+                     int32_t h1, int32_t h2, int32_t d_out) {
+    const double* W1 = params;
+    const double* b1 = W1 + h1 * d_in;
+    const double* W2 = b1 + h1;
+    const double* b2 = W2 + h2 * h1;
+    const double* W3 = b2 + h2;
+    const double* b3 = W3 + d_out * h2;
+    std::vector<double> a1(n * h1), a2(n * h2);
+    dense_fwd(n, d_in, h1, W1, b1, X, a1.data(), 1);
+    dense_fwd(n, h1, h2, W2, b2, a1.data(), a2.data(), 1);
+    dense_fwd(n, h2, d_out, W3, b3, a2.data(), logits, 0);
+}
+
 // Returns the mean cross-entropy against target distributions T (n x d_out,
 // one-hot for hard labels); grads (same layout as params) accumulate the mean
 // gradient when non-NULL.  params = [W1 b1 W2 b2 W3 b3].

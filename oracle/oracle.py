@@ -88,6 +88,8 @@ def lib():
         L.orc_sweep_instance_run.argtypes = [_I32, _PI64, _PI32, _PD, _I32, _PD, _PD, _D, _D, _D, _D,
                                              _PD, _I32, _I64, _PD]
         L.orc_sweep_instance_run.restype = None
+        L.orc_mlp_forward.argtypes = [_I32, _PD, _PD, _PD, _I32, _I32, _I32, _I32]
+        L.orc_mlp_forward.restype = None
         L.orc_mlp_loss_grad.argtypes = [_I32, _PD, _PD, _PD, _PD, _I32, _I32, _I32, _I32]
         L.orc_mlp_loss_grad.restype = _D
         _lib = L
@@ -329,3 +331,12 @@ def mlp_loss_grad(X, targets, params, dims=(16, 64, 64, 12), grad=True):
     G = np.zeros_like(P) if grad else None
     loss = lib().orc_mlp_loss_grad(X.shape[0], _p(X, _D), _p(y, _D), _p(P, _D), _p(G, _D), *dims)
     return loss, G
+
+
+def mlp_forward(X, params, dims=(16, 64, 64, 12)):
+    """FP64 logits of the selector MLP."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    P = np.ascontiguousarray(params, dtype=np.float64)
+    out = np.zeros((X.shape[0], dims[-1]))
+    lib().orc_mlp_forward(X.shape[0], _p(X, _D), _p(P, _D), _p(out, _D), *dims)
+    return out

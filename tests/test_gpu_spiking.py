@@ -8,18 +8,20 @@ bit-exact agreement with the CPU oracle.
 * tests_python/SpikeSynapse/test_SpikingTransmission.py:14-78 -- 5 emitters
   firing at t=1, all-to-all w=1 -> g_exc == 5 at step 2 only;
 * the COBA network (4000 neurons, p=0.02, constant weights) against the
-  oracle over 1000 steps: spike lists, v and g_exc bit for bit (FMA
+  oracle over 3000 steps: spike lists, v and g_exc bit for bit (FMA
   contraction off on both sides; one weight per projection makes the
   atomic scatter order-independent);
 * Poisson drive with injected uniforms, bit for bit;
 * compaction: ascending, complete, no duplicates.
 """
 
+import ctypes as C
+
 import numpy as np
 import pytest
 
 import annarchy_amd as ah
-from annarchy_amd import models
+from annarchy_amd import _lib, models
 from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
@@ -112,10 +114,7 @@ def test_coba_bit_exact_vs_oracle(gpu):
     assert np.array_equal(v, ref["trace_v"])
     assert np.array_equal(ge, ref["trace_gexc"])
     assert np.array_equal(P.g_inh.reshape(-1), ref["g_inh"])
-    rr = P.get("refractory") if False else None
     last = np.empty(P.size, dtype=np.int64)
-    import ctypes as C
-    from annarchy_amd import _lib
     _lib.check(_lib.lib().ahip_pop_get_last_spike(net._h, P.id, _lib.ptr(last, C.c_int64), P.size))
     np.testing.assert_array_equal(last, ref["last_spike"])
 
