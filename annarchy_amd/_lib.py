@@ -18,7 +18,8 @@ from .build import CORE_LIB
 AHIP_F64 = 0
 AHIP_F32 = 1
 
-FORMATS = {"csr": 0, "ell": 1, "ellr": 2, "sell": 3, "bsr": 4, "dense": 5, "csr_panel": 6, "csr_vec": 7}
+FORMATS = {"csr": 0, "ell": 1, "ellr": 2, "sell": 3, "bsr": 4, "dense": 5, "csr_panel": 6, "csr_vec": 7,
+           "sell_panel": 8}
 FORMAT_NAMES = {v: k for k, v in FORMATS.items()}
 
 

@@ -15,6 +15,8 @@
 //   DENSE     row-major n_rows x n_cols
 //   CSR_PANEL CSR whose rows are split in column panels: panel-major order,
 //             aux[q][r] = first entry of row r in panel q
+//   SELL_PANEL the tiles of CSR_PANEL, each stored as sorted 64-row slices
+//             with uint16 panel-relative columns
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -272,6 +274,84 @@ int build_layout(const Proj& p, int32_t precision, const ahip_format_opts& o, Ra
             if ((rc = upload(L.ptr, base.data(), base.size() * sizeof(int64_t), err))) return rc;
             if ((rc = upload(L.aux, roff.data(), roff.size() * sizeof(int32_t), err))) return rc;
             if ((rc = upload(L.idx, col.data(), col.size() * sizeof(int32_t), err))) return rc;
+            if ((rc = upload_vals(L.val, val, precision, err))) return rc;
+            if ((rc = upload(L.perm, nullptr, (size_t)np * R * es, err))) return rc;  // partial sums
+            return 0;
+        }
+        case AHIP_FMT_SELL_PANEL: {
+            // tiles as for CSR_PANEL; inside a tile the RB rows are sorted by
+            // their entry count in the tile (descending, stable) and cut into
+            // slices of 64 rows, one lane per row.  Slice width = longest row
+            // of the slice rounded up to 4; entry k of slice lane l sits at
+            // off + (k/4)*256 + 4*l + k%4 so that a wave reads 4 synapses per
+            // lane with one 8-byte and one 16/32-byte load.  Columns are
+            // stored relative to the panel as uint16; padding points at the
+            // zero slot P of the staged panel with weight 0.
+            const size_t es = precision == AHIP_F64 ? 8 : 4;
+            int32_t pc = o.panel_cols > 0 ? o.panel_cols : (int32_t)(32768 / es);
+            if (pc > 65535 || (size_t)(pc + 1) * es > 64 * 1024) { err = "panel exceeds 64 KiB of LDS or 65535 columns"; return AHIP_EINVAL; }
+            const int32_t rb_rows = o.chunk > 0 ? o.chunk : 256;
+            if (rb_rows < 64 || rb_rows > 8192 || rb_rows % 64) { err = "tile rows must be a multiple of 64 in 64..8192"; return AHIP_EINVAL; }
+            const int32_t steps = o.lanes_per_row > 0 ? o.lanes_per_row : 2;
+            if (steps != 1 && steps != 2 && steps != 4) { err = "sliced tiles: steps in flight must be 1, 2 or 4"; return AHIP_EINVAL; }
+            const int32_t np = std::max<int32_t>(1, (p.n_cols + pc - 1) / pc);
+            const int32_t nrb = (R + rb_rows - 1) / rb_rows;
+            const int64_t ntiles = (int64_t)np * nrb;
+            const int32_t spt = rb_rows / 64;  // slices per tile
+            std::vector<int32_t> cnt((size_t)ntiles * rb_rows, 0);
+            for (int32_t r = 0; r < R; r++) {
+                const int32_t rb = r / rb_rows, lr = r % rb_rows;
+                for (int64_t j = rp[r]; j < rp[r + 1]; j++)
+                    cnt[((size_t)(ci[j] / pc) * nrb + rb) * rb_rows + lr]++;
+            }
+            // slot[(t, lr)] = slice-in-tile * 64 + lane; rowof[(t, slice, lane)] = lr or -1
+            std::vector<int32_t> slot((size_t)ntiles * rb_rows, -1), rowof((size_t)ntiles * rb_rows, -1);
+            std::vector<int64_t> off((size_t)ntiles * spt + 1, 0);
+            std::vector<int32_t> order(rb_rows);
+            for (int64_t t = 0; t < ntiles; t++) {
+                const int32_t rb = (int32_t)(t % nrb);
+                const int32_t nr = std::min(rb_rows, R - rb * rb_rows);
+                const int32_t* c = cnt.data() + (size_t)t * rb_rows;
+                std::iota(order.begin(), order.begin() + nr, 0);
+                std::stable_sort(order.begin(), order.begin() + nr, [&](int32_t a, int32_t b) { return c[a] > c[b]; });
+                for (int32_t s = 0; s < spt; s++) {
+                    int32_t wmax = 0;
+                    for (int32_t l = 0; l < 64; l++) {
+                        const int32_t k = s * 64 + l;
+                        if (k >= nr) break;
+                        slot[(size_t)t * rb_rows + order[k]] = k;
+                        rowof[(size_t)t * rb_rows + k] = order[k];
+                        wmax = std::max(wmax, c[order[k]]);
+                    }
+                    const int64_t gs = t * spt + s;
+                    off[gs + 1] = off[gs] + (int64_t)((wmax + 3) / 4 * 4) * 64;
+                }
+            }
+            const int64_t total = off.back();
+            std::vector<uint16_t> col(total, (uint16_t)pc);
+            std::vector<double> val(total, 0.0);
+            std::fill(cnt.begin(), cnt.end(), 0);
+            for (int32_t r = 0; r < R; r++) {
+                const int32_t rb = r / rb_rows, lr = r % rb_rows;
+                for (int64_t j = rp[r]; j < rp[r + 1]; j++) {
+                    const int32_t q = ci[j] / pc;
+                    const int64_t t = (int64_t)q * nrb + rb;
+                    const int32_t k = cnt[(size_t)t * rb_rows + lr]++;
+                    const int32_t sl = slot[(size_t)t * rb_rows + lr];
+                    const int64_t pos = off[t * spt + sl / 64] + (int64_t)(k / 4) * 256 + 4 * (sl % 64) + k % 4;
+                    col[pos] = (uint16_t)(ci[j] - q * pc);
+                    val[pos] = wval(p, j);
+                }
+            }
+            L.lanes = steps;
+            L.panel_cols = pc;
+            L.chunk = rb_rows;
+            L.n_units = np;
+            L.width = nrb;
+            L.stored = total;
+            if ((rc = upload(L.ptr, off.data(), off.size() * sizeof(int64_t), err))) return rc;
+            if ((rc = upload(L.aux, rowof.data(), rowof.size() * sizeof(int32_t), err))) return rc;
+            if ((rc = upload(L.idx, col.data(), col.size() * sizeof(uint16_t), err))) return rc;
             if ((rc = upload_vals(L.val, val, precision, err))) return rc;
             if ((rc = upload(L.perm, nullptr, (size_t)np * R * es, err))) return rc;  // partial sums
             return 0;

@@ -335,6 +335,73 @@ __global__ __launch_bounds__(kBlock) void tile_psp(int32_t n_rows, int32_t n_col
     }
 }
 
+// Sliced tiles: the tiles of tile_psp with their rows sorted by length and
+// laid out as SELL slices of 64 rows (one lane per row, 4 synapses per lane
+// per step, uint16 panel-relative columns).  There is no cross-lane reduction
+// and no per-row set-up: a wave streams its slice with U steps (U x 64 x
+// (8 + 4*sizeof(T)) bytes) in flight.  Padding reads the zero slot xs[P].
+typedef unsigned short v4u16 __attribute__((ext_vector_type(4)));
+
+template <typename T, int U>
+__global__ __launch_bounds__(kBlock) void sell_panel_psp(int32_t n_rows, int32_t n_cols, int32_t panel_cols,
+                                                         int32_t rb_rows, int32_t n_rb,
+                                                         const int64_t* __restrict__ off,
+                                                         const int32_t* __restrict__ rowof,
+                                                         const uint16_t* __restrict__ col,
+                                                         const T* __restrict__ w,
+                                                         const T* __restrict__ x, T* __restrict__ part) {
+    extern __shared__ unsigned char lds_raw[];
+    T* xs = reinterpret_cast<T*>(lds_raw);
+    const int32_t t = blockIdx.x;
+    const int32_t q = t / n_rb, rb = t % n_rb;
+    const int32_t c0 = q * panel_cols;
+    const int32_t nc = min(panel_cols, n_cols - c0);
+    for (int32_t k = threadIdx.x; k < nc; k += kBlock) xs[k] = x[c0 + k];
+    if (threadIdx.x == 0) xs[panel_cols] = T(0);
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int32_t spt = rb_rows / kWave;
+    T* pq = part + (int64_t)q * n_rows + (int64_t)rb * rb_rows;
+    for (int32_t s = wave; s < spt; s += kBlock / kWave) {
+        const int64_t gs = (int64_t)t * spt + s;
+        const int64_t b = off[gs];
+        const int64_t nsteps = (off[gs + 1] - b) >> 8;
+        const uint16_t* cp = col + b + 4 * lane;
+        const T* wp = w + b + 4 * lane;
+        Acc<T> a0, a1;
+        int64_t m = 0;
+        for (; m + U <= nsteps; m += U) {
+            v4u16 c[U];
+            Quad<T> v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                c[u] = __builtin_nontemporal_load(reinterpret_cast<const v4u16*>(cp + (m + u) * 256));
+                v[u].load(wp + (m + u) * 256);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                a0.add(v[u].v[0] * xs[c[u].x]);
+                a1.add(v[u].v[1] * xs[c[u].y]);
+                a0.add(v[u].v[2] * xs[c[u].z]);
+                a1.add(v[u].v[3] * xs[c[u].w]);
+            }
+        }
+        for (; m < nsteps; m++) {
+            const v4u16 c = __builtin_nontemporal_load(reinterpret_cast<const v4u16*>(cp + m * 256));
+            Quad<T> v;
+            v.load(wp + m * 256);
+            a0.add(v.v[0] * xs[c.x]);
+            a1.add(v.v[1] * xs[c.y]);
+            a0.add(v.v[2] * xs[c.z]);
+            a1.add(v.v[3] * xs[c.w]);
+        }
+        a0.add(a1.template reduce<1>());
+        const int32_t lr = rowof[gs * kWave + lane];
+        if (lr >= 0) pq[lr] = a0.template reduce<1>();
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void tile_reduce(int32_t n_rows, int32_t n_panels,
                                                       const T* __restrict__ part, T* __restrict__ y,
@@ -487,6 +554,31 @@ int launch_rate_t(const ahip_net* net, const Proj& p, const T* x, T* y, hipStrea
                 AHIP_TILE_CASE(16)
                 AHIP_TILE_CASE(32)
 #undef AHIP_TILE_CASE
+                default:
+                    return (int)hipErrorInvalidValue;
+            }
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return (int)e;
+            hipLaunchKernelGGL((tile_reduce<T>), dim3((p.n_rows + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                               p.n_rows, L.n_units, part, y, acc);
+            return (int)hipGetLastError();
+        }
+        case AHIP_FMT_SELL_PANEL: {
+            const int64_t ntiles = (int64_t)L.n_units * L.width;
+            if (ntiles == 0 || p.n_rows == 0) return 0;
+            const size_t lds = (size_t)(L.panel_cols + 1) * sizeof(T);
+            T* part = L.perm.as<T>();
+            switch (L.lanes) {
+#define AHIP_SP_CASE(N)                                                                             \
+    case N:                                                                                         \
+        hipLaunchKernelGGL((sell_panel_psp<T, N>), dim3(ntiles), dim3(kBlock), lds, s, p.n_rows,    \
+                           p.n_cols, L.panel_cols, L.chunk, L.width, L.ptr.as<int64_t>(),          \
+                           L.aux.as<int32_t>(), L.idx.as<uint16_t>(), L.val.as<T>(), x, part);     \
+        break;
+                AHIP_SP_CASE(1)
+                AHIP_SP_CASE(2)
+                AHIP_SP_CASE(4)
+#undef AHIP_SP_CASE
                 default:
                     return (int)hipErrorInvalidValue;
             }

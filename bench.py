@@ -49,7 +49,7 @@ def build_c1(n, k, seed, lognormal=False):
 
 KERNEL_OF_FORMAT = {"csr": "csr_rows", "sell": "sell_slices", "csr_panel": "tile_psp", "csr_vec": "csr_vec",
                     "ell": "ell_rows", "ellr": "ell_rows", "bsr": "bsr_rows", "dense": "dense_rows",
-                    "tiles": "tile_psp"}
+                    "tiles": "tile_psp", "sell_panel": "sell_panel_psp"}
 
 
 def traffic_from_profiles(name):

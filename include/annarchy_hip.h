@@ -41,6 +41,7 @@ extern "C" {
 #define AHIP_FMT_DENSE 5  /* row-major dense (CUDA/Dense.py:287-316) */
 #define AHIP_FMT_CSR_PANEL 6 /* row-block x column-panel tiles, pre-synaptic panel staged in LDS */
 #define AHIP_FMT_CSR_VEC 7   /* CSR read with 16-byte loads, 4 synapses per lane per chunk */
+#define AHIP_FMT_SELL_PANEL 8 /* CSR_PANEL tiles stored as sorted 64-row slices, uint16 panel columns */
 
 /* Precision of a network (core/CTypes.py:30-74, ConfigManagement.py:25). */
 #define AHIP_F64 0
@@ -50,10 +51,11 @@ typedef struct ahip_net ahip_net;
 
 typedef struct ahip_format_opts {
     int32_t format;           /* AHIP_FMT_* */
-    int32_t lanes_per_row;    /* CSR: 1..64 (power of two); SELL: threads per row */
-    int32_t chunk;            /* SELL: rows per slice (C); BSR: tile size */
+    int32_t lanes_per_row;    /* CSR: 1..64 (power of two); SELL: threads per row; CSR_PANEL: 4..32;
+                                 SELL_PANEL: 4-synapse steps in flight per lane (1, 2 or 4) */
+    int32_t chunk;            /* SELL: rows per slice (C); BSR: tile size; CSR_PANEL / SELL_PANEL: rows per tile */
     int32_t sigma;            /* SELL: sorting window in rows (1 = no sorting) */
-    int32_t panel_cols;       /* CSR_PANEL: columns per panel (0 = automatic) */
+    int32_t panel_cols;       /* CSR_PANEL / SELL_PANEL: columns per panel (0 = automatic) */
 } ahip_format_opts;
 
 /* ---- errors / device ------------------------------------------------------ */

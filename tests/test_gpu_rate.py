@@ -36,6 +36,9 @@ FORMATS = [
     dict(storage_format="dense"),
     dict(storage_format="csr_panel", lanes=16, panel_cols=512),
     dict(storage_format="csr_panel", lanes=4, panel_cols=256, chunk=100),
+    dict(storage_format="sell_panel", lanes=1, panel_cols=300, chunk=64),
+    dict(storage_format="sell_panel", lanes=2, panel_cols=512, chunk=128),
+    dict(storage_format="sell_panel", lanes=4, panel_cols=4096, chunk=256),
 ]
 
 
