@@ -79,6 +79,13 @@ __device__ inline T group_sum(T v) {
     return v;
 }
 
+// The same butterfly over aligned groups of w lanes, w a power of two known
+// only at run time (uniform across the wave).
+__device__ inline double group_sum_rt(double v, int w) {
+    for (int o = w / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
 // Kahan-Babuska (Neumaier) compensated accumulator for FP32 psp sums.
 struct Kahan {
     float s, c;
@@ -101,6 +108,19 @@ __device__ inline float group_sum_kahan(float s, float c) {
     for (int o = W / 2; o > 0; o >>= 1) {
         float so = __shfl_xor(s, o, W);
         float co = __shfl_xor(c, o, W);
+        float t = s + so;
+        float bp = t - s;
+        float err = (s - (t - bp)) + (so - bp);
+        s = t;
+        c = c + co + err;
+    }
+    return s + c;
+}
+
+__device__ inline float group_sum_kahan_rt(float s, float c, int w) {
+    for (int o = w / 2; o > 0; o >>= 1) {
+        float so = __shfl_xor(s, o, kWave);
+        float co = __shfl_xor(c, o, kWave);
         float t = s + so;
         float bp = t - s;
         float err = (s - (t - bp)) + (so - bp);

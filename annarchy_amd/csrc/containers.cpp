@@ -280,52 +280,75 @@ int build_layout(const Proj& p, int32_t precision, const ahip_format_opts& o, Ra
         }
         case AHIP_FMT_SELL_PANEL: {
             // tiles as for CSR_PANEL; inside a tile the RB rows are sorted by
-            // their entry count in the tile (descending, stable) and cut into
-            // slices of 64 rows, one lane per row.  Slice width = longest row
-            // of the slice rounded up to 4; entry k of slice lane l sits at
-            // off + (k/4)*256 + 4*l + k%4 so that a wave reads 4 synapses per
-            // lane with one 8-byte and one 16/32-byte load.  Columns are
-            // stored relative to the panel as uint16; padding points at the
-            // zero slot P of the staged panel with weight 0.
+            // their entry count in the tile (descending, stable) and packed
+            // into slices of 64 lanes.  A slice has W steps of 4 entries per
+            // lane; entry k of a row that owns lanes [b, b+g) of slice s sits
+            // at off[s] + (e/g)*256 + 4*(b + e%g) + k%4 with e = k/4, so a
+            // wave reads 4 synapses per lane with one 8-byte and one
+            // 16/32-byte load.  Rows up to `cap` entries (twice the tile's
+            // mean, at least 64) get one lane and W = longest row / 4; a
+            // longer row gets g = ceil(len/cap) lanes (at most 64), W follows
+            // from it and the rows after it in the slice get just enough
+            // lanes to fit in W.  Columns are uint16, relative to the panel;
+            // padding points at the zero slot P of the staged panel with
+            // weight 0.  rowof[s][lane] = (g << 16) | row for the first lane
+            // of a row, -1 for the others.
             const size_t es = precision == AHIP_F64 ? 8 : 4;
             int32_t pc = o.panel_cols > 0 ? o.panel_cols : (int32_t)(32768 / es);
             if (pc > 65535 || (size_t)(pc + 1) * es > 64 * 1024) { err = "panel exceeds 64 KiB of LDS or 65535 columns"; return AHIP_EINVAL; }
             const int32_t rb_rows = o.chunk > 0 ? o.chunk : 256;
             if (rb_rows < 64 || rb_rows > 8192 || rb_rows % 64) { err = "tile rows must be a multiple of 64 in 64..8192"; return AHIP_EINVAL; }
-            const int32_t steps = o.lanes_per_row > 0 ? o.lanes_per_row : 2;
-            if (steps != 1 && steps != 2 && steps != 4) { err = "sliced tiles: steps in flight must be 1, 2 or 4"; return AHIP_EINVAL; }
+            const int32_t steps = o.lanes_per_row > 0 ? o.lanes_per_row : 4;
+            const int32_t waves = o.sigma > 1 ? o.sigma : 4;
+            if (waves != 4 && waves != 8) { err = "sliced tiles: waves per workgroup must be 4 or 8"; return AHIP_EINVAL; }
+            if (!(steps == 2 || steps == 4 || steps == 8 || (steps == 1 && waves == 4))) {
+                err = "sliced tiles: steps in flight must be 2, 4 or 8 (1 with 4 waves)";
+                return AHIP_EINVAL;
+            }
             const int32_t np = std::max<int32_t>(1, (p.n_cols + pc - 1) / pc);
             const int32_t nrb = (R + rb_rows - 1) / rb_rows;
             const int64_t ntiles = (int64_t)np * nrb;
-            const int32_t spt = rb_rows / 64;  // slices per tile
             std::vector<int32_t> cnt((size_t)ntiles * rb_rows, 0);
             for (int32_t r = 0; r < R; r++) {
                 const int32_t rb = r / rb_rows, lr = r % rb_rows;
                 for (int64_t j = rp[r]; j < rp[r + 1]; j++)
                     cnt[((size_t)(ci[j] / pc) * nrb + rb) * rb_rows + lr]++;
             }
-            // slot[(t, lr)] = slice-in-tile * 64 + lane; rowof[(t, slice, lane)] = lr or -1
-            std::vector<int32_t> slot((size_t)ntiles * rb_rows, -1), rowof((size_t)ntiles * rb_rows, -1);
-            std::vector<int64_t> off((size_t)ntiles * spt + 1, 0);
+            // slot[(t, lr)] = slice-in-tile << 12 | first lane << 6 | (lanes - 1)
+            std::vector<int32_t> slot((size_t)ntiles * rb_rows, 0);
+            std::vector<int32_t> tsl(ntiles + 1, 0);
+            std::vector<int64_t> off(1, 0);
+            std::vector<int32_t> rowof;
             std::vector<int32_t> order(rb_rows);
             for (int64_t t = 0; t < ntiles; t++) {
                 const int32_t rb = (int32_t)(t % nrb);
                 const int32_t nr = std::min(rb_rows, R - rb * rb_rows);
                 const int32_t* c = cnt.data() + (size_t)t * rb_rows;
+                int64_t tile_nnz = 0;
+                for (int32_t lr = 0; lr < nr; lr++) tile_nnz += c[lr];
+                const int64_t cap = std::max<int64_t>(64, (2 * tile_nnz / nr + 3) / 4 * 4);
                 std::iota(order.begin(), order.begin() + nr, 0);
                 std::stable_sort(order.begin(), order.begin() + nr, [&](int32_t a, int32_t b) { return c[a] > c[b]; });
-                for (int32_t s = 0; s < spt; s++) {
-                    int32_t wmax = 0;
-                    for (int32_t l = 0; l < 64; l++) {
-                        const int32_t k = s * 64 + l;
-                        if (k >= nr) break;
-                        slot[(size_t)t * rb_rows + order[k]] = k;
-                        rowof[(size_t)t * rb_rows + k] = order[k];
-                        wmax = std::max(wmax, c[order[k]]);
+                int32_t ns = 0;
+                for (int32_t i = 0; i < nr; ns++) {
+                    const int64_t len0 = c[order[i]];
+                    const int64_t g0 = len0 > cap ? std::min<int64_t>(64, (len0 + cap - 1) / cap) : 1;
+                    const int64_t W = (len0 + 4 * g0 - 1) / (4 * g0);
+                    int32_t used = 0;
+                    rowof.resize(rowof.size() + 64, -1);
+                    int32_t* ro = rowof.data() + rowof.size() - 64;
+                    while (i < nr) {
+                        const int64_t len = c[order[i]];
+                        const int64_t g = used == 0 ? g0 : std::max<int64_t>(1, W > 0 ? (len + 4 * W - 1) / (4 * W) : 1);
+                        if (used + g > 64) break;
+                        ro[used] = (int32_t)((g << 16) | order[i]);
+                        slot[(size_t)t * rb_rows + order[i]] = (ns << 12) | (used << 6) | (int32_t)(g - 1);
+                        used += (int32_t)g;
+                        i++;
                     }
-                    const int64_t gs = t * spt + s;
-                    off[gs + 1] = off[gs] + (int64_t)((wmax + 3) / 4 * 4) * 64;
+                    off.push_back(off.back() + W * 256);
                 }
+                tsl[t + 1] = tsl[t] + ns;
             }
             const int64_t total = off.back();
             std::vector<uint16_t> col(total, (uint16_t)pc);
@@ -336,14 +359,18 @@ int build_layout(const Proj& p, int32_t precision, const ahip_format_opts& o, Ra
                 for (int64_t j = rp[r]; j < rp[r + 1]; j++) {
                     const int32_t q = ci[j] / pc;
                     const int64_t t = (int64_t)q * nrb + rb;
-                    const int32_t k = cnt[(size_t)t * rb_rows + lr]++;
-                    const int32_t sl = slot[(size_t)t * rb_rows + lr];
-                    const int64_t pos = off[t * spt + sl / 64] + (int64_t)(k / 4) * 256 + 4 * (sl % 64) + k % 4;
+                    const int64_t k = cnt[(size_t)t * rb_rows + lr]++;
+                    const int32_t sv = slot[(size_t)t * rb_rows + lr];
+                    const int64_t gs = tsl[t] + (sv >> 12);
+                    const int64_t b = (sv >> 6) & 63, g = (sv & 63) + 1;
+                    const int64_t e = k / 4;
+                    const int64_t pos = off[gs] + (e / g) * 256 + 4 * (b + e % g) + k % 4;
                     col[pos] = (uint16_t)(ci[j] - q * pc);
                     val[pos] = wval(p, j);
                 }
             }
             L.lanes = steps;
+            L.sigma = waves;
             L.panel_cols = pc;
             L.chunk = rb_rows;
             L.n_units = np;
@@ -351,6 +378,7 @@ int build_layout(const Proj& p, int32_t precision, const ahip_format_opts& o, Ra
             L.stored = total;
             if ((rc = upload(L.ptr, off.data(), off.size() * sizeof(int64_t), err))) return rc;
             if ((rc = upload(L.aux, rowof.data(), rowof.size() * sizeof(int32_t), err))) return rc;
+            if ((rc = upload(L.tiles, tsl.data(), tsl.size() * sizeof(int32_t), err))) return rc;
             if ((rc = upload(L.idx, col.data(), col.size() * sizeof(uint16_t), err))) return rc;
             if ((rc = upload_vals(L.val, val, precision, err))) return rc;
             if ((rc = upload(L.perm, nullptr, (size_t)np * R * es, err))) return rc;  // partial sums

@@ -43,9 +43,10 @@ struct RateLayout {
     DevBuf ptr;               // CSR row_ptr (int64) / SELL slice offsets (int64) / BSR block row ptr (int32)
     DevBuf idx;               // column indices (int32), padded formats included
     DevBuf val;               // weights (double or float)
-    DevBuf rl;                // ELLR row lengths / SELL slice widths (int32)
+    DevBuf rl;                // ELLR row lengths (int32)
     DevBuf perm;              // SELL: slice row -> post rank (int32)
-    DevBuf aux;               // CSR_PANEL: per-row panel offsets (int64)
+    DevBuf aux;               // CSR_PANEL: per-row offsets in a tile (int32) / SELL_PANEL: slice lane map (int32)
+    DevBuf tiles;             // SELL_PANEL: first slice of each tile (int32)
 };
 
 struct Pop {

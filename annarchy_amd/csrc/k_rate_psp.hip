@@ -336,17 +336,23 @@ __global__ __launch_bounds__(kBlock) void tile_psp(int32_t n_rows, int32_t n_col
 }
 
 // Sliced tiles: the tiles of tile_psp with their rows sorted by length and
-// laid out as SELL slices of 64 rows (one lane per row, 4 synapses per lane
-// per step, uint16 panel-relative columns).  There is no cross-lane reduction
-// and no per-row set-up: a wave streams its slice with U steps (U x 64 x
-// (8 + 4*sizeof(T)) bytes) in flight.  Padding reads the zero slot xs[P].
+// packed into slices of 64 lanes (uint16 panel-relative columns, 4 synapses
+// per lane per step).  A wave streams one slice with U steps (U x 64 x
+// (8 + 4*sizeof(T)) bytes) in flight and no per-row set-up; a row of a
+// slice owns g consecutive lanes (g = 1 unless the row is long) and its
+// first lane adds the g lane sums in lane order.  Padding reads the zero slot
+// xs[P].  Tile t = q*n_rb + rb writes part[q][row]; tile_reduce adds the
+// panels in ascending order.  (Adding them in the last tile of a row block
+// to finish needs a device-scope release per tile, which on this chip writes
+// back the L2 of the tile's XCD: three times slower in total.)
 typedef unsigned short v4u16 __attribute__((ext_vector_type(4)));
 
-template <typename T, int U>
-__global__ __launch_bounds__(kBlock) void sell_panel_psp(int32_t n_rows, int32_t n_cols, int32_t panel_cols,
+template <typename T, int U, int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void sell_panel_psp(int32_t n_rows, int32_t n_cols, int32_t panel_cols,
                                                          int32_t rb_rows, int32_t n_rb,
                                                          const int64_t* __restrict__ off,
                                                          const int32_t* __restrict__ rowof,
+                                                         const int32_t* __restrict__ tile_slices,
                                                          const uint16_t* __restrict__ col,
                                                          const T* __restrict__ w,
                                                          const T* __restrict__ x, T* __restrict__ part) {
@@ -356,15 +362,14 @@ __global__ __launch_bounds__(kBlock) void sell_panel_psp(int32_t n_rows, int32_t
     const int32_t q = t / n_rb, rb = t % n_rb;
     const int32_t c0 = q * panel_cols;
     const int32_t nc = min(panel_cols, n_cols - c0);
-    for (int32_t k = threadIdx.x; k < nc; k += kBlock) xs[k] = x[c0 + k];
+    for (int32_t k = threadIdx.x; k < nc; k += WAVES * kWave) xs[k] = x[c0 + k];
     if (threadIdx.x == 0) xs[panel_cols] = T(0);
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
-    const int32_t spt = rb_rows / kWave;
     T* pq = part + (int64_t)q * n_rows + (int64_t)rb * rb_rows;
-    for (int32_t s = wave; s < spt; s += kBlock / kWave) {
-        const int64_t gs = (int64_t)t * spt + s;
+    const int32_t s_end = tile_slices[t + 1];
+    for (int32_t gs = tile_slices[t] + wave; gs < s_end; gs += WAVES) {
         const int64_t b = off[gs];
         const int64_t nsteps = (off[gs + 1] - b) >> 8;
         const uint16_t* cp = col + b + 4 * lane;
@@ -397,8 +402,20 @@ __global__ __launch_bounds__(kBlock) void sell_panel_psp(int32_t n_rows, int32_t
             a1.add(v.v[3] * xs[c.w]);
         }
         a0.add(a1.template reduce<1>());
-        const int32_t lr = rowof[gs * kWave + lane];
-        if (lr >= 0) pq[lr] = a0.template reduce<1>();
+        T v = a0.template reduce<1>();
+        const int32_t ro = rowof[(int64_t)gs * kWave + lane];
+        const int g = ro >= 0 ? (ro >> 16) : 0;
+        if (__any(g > 1)) {
+            Acc<T> r;
+            r.add(v);
+            for (int i = 1; i < kWave; i++) {
+                const T o = __shfl(v, (lane + i) & (kWave - 1), kWave);
+                if (i < g) r.add(o);
+                if (!__any(i + 1 < g)) break;
+            }
+            v = r.template reduce<1>();
+        }
+        if (ro >= 0) pq[ro & 0xFFFF] = v;
     }
 }
 
@@ -568,16 +585,21 @@ int launch_rate_t(const ahip_net* net, const Proj& p, const T* x, T* y, hipStrea
             if (ntiles == 0 || p.n_rows == 0) return 0;
             const size_t lds = (size_t)(L.panel_cols + 1) * sizeof(T);
             T* part = L.perm.as<T>();
-            switch (L.lanes) {
-#define AHIP_SP_CASE(N)                                                                             \
-    case N:                                                                                         \
-        hipLaunchKernelGGL((sell_panel_psp<T, N>), dim3(ntiles), dim3(kBlock), lds, s, p.n_rows,    \
-                           p.n_cols, L.panel_cols, L.chunk, L.width, L.ptr.as<int64_t>(),          \
-                           L.aux.as<int32_t>(), L.idx.as<uint16_t>(), L.val.as<T>(), x, part);     \
+            switch (L.lanes * 16 + L.sigma) {
+#define AHIP_SP_CASE(N, W)                                                                          \
+    case N * 16 + W:                                                                                \
+        hipLaunchKernelGGL((sell_panel_psp<T, N, W>), dim3(ntiles), dim3(W * kWave), lds, s,        \
+                           p.n_rows, p.n_cols, L.panel_cols, L.chunk, L.width,                      \
+                           L.ptr.as<int64_t>(), L.aux.as<int32_t>(), L.tiles.as<int32_t>(),         \
+                           L.idx.as<uint16_t>(), L.val.as<T>(), x, part);                           \
         break;
-                AHIP_SP_CASE(1)
-                AHIP_SP_CASE(2)
-                AHIP_SP_CASE(4)
+                AHIP_SP_CASE(1, 4)
+                AHIP_SP_CASE(2, 4)
+                AHIP_SP_CASE(4, 4)
+                AHIP_SP_CASE(8, 4)
+                AHIP_SP_CASE(2, 8)
+                AHIP_SP_CASE(4, 8)
+                AHIP_SP_CASE(8, 8)
 #undef AHIP_SP_CASE
                 default:
                     return (int)hipErrorInvalidValue;

@@ -26,7 +26,7 @@ import numpy as np
 from . import connectivity as conn
 
 CLASSES = [
-    dict(fmt="csr", lanes=2),
+    dict(fmt="sell_panel", lanes=4, sigma=4, chunk=256),
     dict(fmt="csr", lanes=4),
     dict(fmt="csr", lanes=8),
     dict(fmt="csr", lanes=16),
@@ -127,6 +127,12 @@ def feasible(csr: conn.CSR, cls: int, max_blowup: float = 8.0) -> bool:
         t = cfg["chunk"]
         dens = _tile_stats(csr, np.random.default_rng(0), t)
         return 1.0 / max(dens, 1e-9) <= max_blowup
+    if cfg["fmt"] == "sell_panel":
+        # one partial sum per (panel, row): worth it only with several
+        # synapses per row and panel; bounds the host-side tile tables too
+        panels = -(-csr.n_cols // 4096)
+        cells = panels * csr.n_rows
+        return cells <= 3e7 and nnz >= 4 * cells
     return True
 
 
@@ -348,43 +354,45 @@ def _gen_task(spec):
     return spec, csr, features(csr, spec["seed"] & 0xFFFF)
 
 
-def build_corpus(count: int, out_path, workers: int = 8, seed: int = 2024, log=print, tuner=None):
-    """Autotune `count` corpus matrices; writes specs, features and times to
-    `out_path` (npz) every 100 matrices and at the end."""
+def build_corpus(count: int, out_path, workers: int = 8, seed: int = 2024, log=print, tuner=None,
+                 start: int = 0, stop: int | None = None, time_limit: float | None = None):
+    """Autotune entries [start, stop) of the `count`-matrix corpus; writes
+    specs, features and times to `out_path` (npz) every 50 matrices and at
+    the end.  With `time_limit` (seconds) it stops early after the matrix
+    that crosses it; the file then holds the prefix done so far."""
     import json
     import time
     from concurrent.futures import ThreadPoolExecutor
 
-    specs = corpus_specs(count, seed)
+    specs = corpus_specs(count, seed)[start:stop]
     feats, times, kept = [], [], []
     t0 = time.perf_counter()
-    # generation runs ahead in threads (numpy releases the GIL in its heavy
-    # parts); the device timing stays in this thread
+
     def windows():
-        # a bounded window of matrices is generated ahead of the tuner
+        # generation runs ahead in a bounded window of threads (numpy
+        # releases the GIL in its heavy parts); device timing stays here
         with ThreadPoolExecutor(max_workers=workers) as pool:
             for w0 in range(0, len(specs), 2 * workers):
                 yield from pool.map(_gen_task, specs[w0:w0 + 2 * workers])
 
-    if True:
-        for i, (spec, csr, f) in enumerate(windows()):
-            t = tuner(csr) if tuner is not None else autotune(csr)
-            feats.append(f)
-            times.append(t)
-            kept.append(spec)
-            if (i + 1) % 100 == 0 or i + 1 == count:
-                np.savez(out_path, features=np.array(feats), times=np.array(times),
-                         specs=np.array([json.dumps(s) for s in kept]))
-                log(f"corpus {i + 1}/{count}: {time.perf_counter() - t0:.0f} s")
+    def save():
+        np.savez(out_path, features=np.array(feats), times=np.array(times),
+                 specs=np.array([json.dumps(s) for s in kept]))
+
+    for i, (spec, csr, f) in enumerate(windows()):
+        t = tuner(csr) if tuner is not None else autotune(csr)
+        feats.append(f)
+        times.append(t)
+        kept.append(spec)
+        elapsed = time.perf_counter() - t0
+        if (i + 1) % 50 == 0 or i + 1 == len(specs):
+            save()
+            log(f"corpus {start + i + 1}/{start + len(specs)}: {elapsed:.0f} s")
+        if time_limit is not None and elapsed > time_limit:
+            save()
+            log(f"corpus stopped at {start + i + 1} after {elapsed:.0f} s")
+            break
     return np.array(feats), np.array(times), kept
-
-
-if __name__ == "__main__":
-    import sys
-
-    n = int(sys.argv[1]) if len(sys.argv) > 1 else 2000
-    out = sys.argv[2] if len(sys.argv) > 2 else "selector_corpus.npz"
-    build_corpus(n, out, log=lambda m: print(m, flush=True))
 
 
 def evaluate(X, T, heuristic, save_path=None, seeds=(0, 1, 2), epochs=3000, log=print):

@@ -90,14 +90,16 @@ def main():
     ap.add_argument("--degree", type=int, default=1000)
     ap.add_argument("--lognormal", action="store_true")
     ap.add_argument("--precision", default="double", choices=["double", "float"])
-    # default: row-block x column-panel tiles, pre-synaptic panel in LDS (the
-    # fastest FP64 layout of the sweep: 16 lanes per row, 4096-column panels,
-    # 256-row blocks)
-    ap.add_argument("--format", default="csr_panel")
-    ap.add_argument("--lanes", type=int, default=16)
-    ap.add_argument("--sigma", type=int, default=1)
-    ap.add_argument("--panel-cols", type=int, default=4096)
-    ap.add_argument("--chunk", type=int, default=256)
+    # default: sliced tiles (1024-row blocks x 8191-column panels staged in
+    # LDS, sorted 64-row slices, one lane per row, 2 steps of 4 synapses in
+    # flight, 8 waves per workgroup): the fastest FP64 layout of the sweep
+    ap.add_argument("--format", default="sell_panel")
+    ap.add_argument("--lanes", type=int, default=2,
+                    help="csr/csr_vec/csr_panel: lanes per row; sell: threads per row; sell_panel: steps in flight")
+    ap.add_argument("--sigma", type=int, default=1, help="sell: sorting window")
+    ap.add_argument("--waves", type=int, default=8, help="sell_panel: waves per workgroup")
+    ap.add_argument("--panel-cols", type=int, default=8191)
+    ap.add_argument("--chunk", type=int, default=1024)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2", "c3"],
@@ -123,7 +125,8 @@ def main():
     net = ah.Network(dt=1.0, seed=42, precision=args.precision)
     pop = net.create(args.neurons, ah.RateNoisy)
     proj = net.connect(pop, pop, "exc")
-    proj.from_csr(csr, storage_format=args.format, lanes=args.lanes, sigma=args.sigma,
+    proj.from_csr(csr, storage_format=args.format, lanes=args.lanes,
+                  sigma=args.waves if args.format == "sell_panel" else args.sigma,
                   panel_cols=args.panel_cols, chunk=args.chunk)
     net.compile(allow_build=False)
     pop.r = np.random.default_rng(7).random(args.neurons)
@@ -181,8 +184,7 @@ def main():
                         f"{args.degree}, nnz={csr.nnz}, dt=1ms, {args.format} {dtype}",
             "neurons": args.neurons,
             "synapses": csr.nnz,
-            "format": args.format,
-            "lanes_per_row": args.lanes,
+            "format": proj.get_format(),
             "parallelism": "single GPU",
         },
         "roofline": {
@@ -192,7 +194,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": f"{kname} (compute_psp, {args.format})",
+            "kernel": f"{kname}{' + tile_reduce' if args.format in ('csr_panel', 'sell_panel') else ''}"
+                      f" (compute_psp, {args.format})",
             "kernel_ms": psp_ms,
             "algorithmic_bytes": psp_bytes,
         },

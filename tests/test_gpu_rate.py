@@ -39,6 +39,8 @@ FORMATS = [
     dict(storage_format="sell_panel", lanes=1, panel_cols=300, chunk=64),
     dict(storage_format="sell_panel", lanes=2, panel_cols=512, chunk=128),
     dict(storage_format="sell_panel", lanes=4, panel_cols=4096, chunk=256),
+    dict(storage_format="sell_panel", lanes=8, sigma=8, panel_cols=1000, chunk=192),
+    dict(storage_format="sell_panel", lanes=4, sigma=8, panel_cols=2048, chunk=1024),
 ]
 
 
@@ -120,6 +122,61 @@ def test_rate_run_matches_oracle_100_steps(gpu):
     assert trace.shape == (steps, n)
     for s in range(steps):
         assert _rel_err(trace[s], tr_ref[s]) < 1e-9, s
+    assert _rel_err(pop.r.reshape(-1), r_ref) < 1e-9
+
+
+@pytest.mark.parametrize("fmt", [dict(storage_format="sell_panel", lanes=4, sigma=8, panel_cols=700, chunk=128),
+                                 dict(storage_format="sell_panel", lanes=2, sigma=4, panel_cols=2000, chunk=64),
+                                 dict(storage_format="csr_panel", lanes=8, panel_cols=700, chunk=128)],
+                         ids=["sell_panel-3panels", "sell_panel-1panel", "csr_panel"])
+def test_panel_psp_relaunch_with_new_rates(gpu, fmt):
+    # the partial-sum buffer of the tiles is reused: every launch must read
+    # the current rates and write every row, launch after launch
+    n = 2000
+    csr = _random_csr(n, 80, 31, ragged=True)
+    net, pop, proj = _net_one_step(csr, np.zeros(n), fmt)
+    rng = np.random.default_rng(17)
+    for it in range(4):
+        r = rng.random(n)
+        pop.r = r
+        proj.time_psp(1 + it)
+        got = pop.get("_sum_exc").reshape(-1)
+        ref = orc.spmv_csr(csr.row_ptr, csr.col_idx, csr.weights(), r)
+        np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-13 * np.max(np.abs(ref)))
+
+
+def test_sell_panel_heavy_rows(gpu):
+    # a few rows far longer than the rest get several lanes per row (up to a
+    # whole wave); their lane sums are added by the row's first lane
+    n = 3000
+    rng = np.random.default_rng(23)
+    deg = rng.integers(0, 20, n)
+    deg[[3, 700, 701, 2999]] = [n, 2500, 900, 300]
+    csr = conn.random_in_degree(n, n, rng, deg, conn.Normal(0.0, 0.05, rng=rng))
+    r0 = rng.random(n)
+    ref = orc.spmv_csr(csr.row_ptr, csr.col_idx, csr.weights(), r0)
+    for fmt in (dict(storage_format="sell_panel", lanes=4, sigma=8, panel_cols=1000, chunk=64),
+                dict(storage_format="sell_panel", lanes=2, sigma=4, panel_cols=3000, chunk=1024)):
+        net, pop, proj = _net_one_step(csr, r0, fmt)
+        got = pop.get("_sum_exc").reshape(-1)
+        np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-13 * np.max(np.abs(ref)))
+
+
+def test_sell_panel_graph_run_matches_oracle(gpu):
+    n, steps = 3000, 48
+    csr = _random_csr(n, 120, 6, ragged=True)
+    rng = np.random.default_rng(10)
+    r0 = rng.random(n)
+    noise = rng.normal(0.0, 0.1, (steps, n))
+    net = ah.Network(dt=1.0, seed=1)
+    pop = net.create(n, ah.RateNoisy)
+    net.connect(pop, pop, "exc").from_csr(csr, storage_format="sell_panel", lanes=4, sigma=8,
+                                          panel_cols=1024, chunk=256)
+    net.compile(graph_batch=16)
+    pop.r = r0
+    pop.inject_random(0, noise)
+    net.run(steps)
+    r_ref, _, _ = orc.rate_run(csr.row_ptr, csr.col_idx, csr.w, r0, 10.0, 1.0, noise, steps)
     assert _rel_err(pop.r.reshape(-1), r_ref) < 1e-9
 
 
