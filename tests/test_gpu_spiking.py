@@ -75,23 +75,24 @@ def test_spike_transmission_known_answer(gpu):
     np.testing.assert_array_equal(g, [[0, 0], [0, 0], [5, 5], [0, 0], [0, 0]])
 
 
-def _coba_net(N=4000, seed=7, p=0.02, graph_batch=0, steps_cap=1000):
-    net = ah.Network(dt=0.1, seed=seed)
+def _coba_net(N=4000, seed=7, p=0.02, graph_batch=0, steps_cap=1000, precision="double",
+              record=("v", "g_exc", "spike")):
+    net = ah.Network(dt=0.1, seed=seed, precision=precision)
     P = net.create(N, ah.COBA)
     ne = int(0.8 * N)
     Pe, Pi = P[:ne], P[ne:]
     ce = net.connect(Pe, P, "exc").fixed_probability(p, 0.6)
     ci = net.connect(Pi, P, "inh").fixed_probability(p, 6.7)
     P.v = ah.Uniform(-60.0, -50.0, rng=net.rng)
-    mon = net.monitor(P, ["v", "g_exc", "spike"], capacity=steps_cap)
+    mon = net.monitor(P, list(record), capacity=steps_cap)
     net.compile(graph_batch=graph_batch)
     return net, P, ce, ci, mon
 
 
-def _oracle_coba(P, ce, ci, v0, steps):
+def _oracle_coba(P, ce, ci, v0, steps, trace=True):
     projs = [orc.csrc_projection(ce.csr, 0), orc.csrc_projection(ci.csr, 1)]
     ref = np.asarray(P.refractory, dtype=np.int32)
-    return orc.spiking_run(0, P.size, 0.1, v0, steps, projs, refractory=ref, trace=True)
+    return orc.spiking_run(0, P.size, 0.1, v0, steps, projs, refractory=ref, trace=trace)
 
 
 def test_coba_bit_exact_vs_oracle(gpu):
@@ -158,3 +159,36 @@ def test_poisson_drive_bit_exact(gpu):
     np.testing.assert_array_equal(counts, ref["counts"])
     np.testing.assert_array_equal(idx, ref["idx"])
     assert np.array_equal(P.v.reshape(-1), ref["v"])
+
+
+def _spike_stats(counts, idx, n, dt_ms):
+    t = np.repeat(np.arange(len(counts)), counts)
+    rate = len(idx) / n / (len(counts) * dt_ms * 1e-3)
+    order = np.lexsort((t, idx))
+    ids, ts = idx[order], t[order]
+    cvs = []
+    starts = np.flatnonzero(np.r_[True, ids[1:] != ids[:-1]])
+    for a, b in zip(starts, np.r_[starts[1:], len(ids)]):
+        if b - a > 2:
+            isi = np.diff(ts[a:b]) * dt_ms
+            cvs.append(np.std(isi) / np.mean(isi))
+    return rate, float(np.mean(cvs))
+
+
+def test_coba_fp32_rate_and_isi_cv_vs_fp64(gpu):
+    # long chaotic runs cannot agree spike for spike in FP32; the population
+    # rate and the mean ISI coefficient of variation must stay within 2% of
+    # the FP64 oracle's (core/Monitor.py:1172-1240 define both statistics)
+    steps = 20000  # 2 s
+    net, P, ce, ci, mon = _coba_net(graph_batch=100, steps_cap=steps, precision="float", record=("spike",))
+    P.I = 20.0
+    v0 = P.v.reshape(-1).astype(np.float64)
+    net.run(steps)
+    counts, idx = mon.raw_spikes()
+    ref = _oracle_coba(P, ce, ci, v0, steps, trace=False)
+    r32, cv32 = _spike_stats(counts, idx, P.size, 0.1)
+    r64, cv64 = _spike_stats(ref["counts"], ref["idx"], P.size, 0.1)
+    print(f"rate FP32 {r32:.3f} Hz / FP64 {r64:.3f} Hz, ISI CV {cv32:.4f} / {cv64:.4f}")
+    assert r64 > 5.0
+    assert abs(r32 / r64 - 1.0) < 0.02
+    assert abs(cv32 / cv64 - 1.0) < 0.02
