@@ -88,6 +88,7 @@ struct Proj {
     DevBuf col_ptr;   // int64[n_cols+1]
     DevBuf row_idx;   // int32[nnz]
     DevBuf w_col;     // T[nnz] column order (absent when single_w)
+    int32_t scatter_chunks = 1;   // waves per spike in the scatter (256 synapses each)
     std::vector<int64_t> h_inv_idx;
 };
 

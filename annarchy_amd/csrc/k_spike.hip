@@ -86,54 +86,64 @@ __global__ __launch_bounds__(kCompactThreads) void compact_spikes(int32_t size,
 // Event-driven propagation over the compacted spike list, CSRC backward view
 // (generator/Projection/OpenMP/CSR.py:870-885; CUDA/CSR.py:575-657 uses one
 // block per spike with a host-read spike count).  Here the grid is fixed and
-// reads the count from device memory, so the step needs no host sync: a wave
-// takes spike s, lane 0 reads the column range of the pre-synaptic neuron and
-// broadcasts it, and the lanes add the synapse weights into the post-synaptic
-// conductances with device atomics.  With one weight per projection all
-// addends are equal, so the result does not depend on the order of the atomics.
+// reads the count from device memory, so the step needs no host sync.  Work
+// item (spike s, chunk c) goes to one wave: it reads the column range of the
+// pre-synaptic neuron and, for every 256-synapse chunk c, c + chunks, ... of
+// it, loads 4 post ranks (and weights) per lane before it issues the 4 device
+// atomics into the post-synaptic conductances.  With one weight per
+// projection all addends are equal, so the result does not depend on the
+// order of the atomics.
 template <typename T, bool SINGLE_W>
 __global__ __launch_bounds__(256) void scatter_spikes(const int32_t* __restrict__ spiked,
                                                       const int32_t* __restrict__ count,
-                                                      int32_t n_cols,
+                                                      int32_t n_cols, int32_t chunks,
                                                       const int64_t* __restrict__ col_ptr,
                                                       const int32_t* __restrict__ row_idx,
                                                       const T* __restrict__ w_col, T w_const,
                                                       T* __restrict__ g) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
-    const int32_t n = *count;
+    const int64_t items = (int64_t)(*count) * chunks;
     const int64_t waves = (int64_t)gridDim.x * 4;
-    for (int64_t s = (int64_t)blockIdx.x * 4 + wid; s < n; s += waves) {
-        int64_t b = 0, e = 0;
-        if (lane == 0) {
-            const int32_t pre = spiked[s];
-            if (pre >= 0 && pre < n_cols) { b = col_ptr[pre]; e = col_ptr[pre + 1]; }
-        }
-        b = __shfl(b, 0, kWave);
-        e = __shfl(e, 0, kWave);
-        for (int64_t syn = b + lane; syn < e; syn += kWave) {
-            const int32_t post = row_idx[syn];
-            const T v = SINGLE_W ? w_const : w_col[syn];
-            unsafeAtomicAdd(g + post, v);
+    for (int64_t it = (int64_t)blockIdx.x * 4 + wid; it < items; it += waves) {
+        const int64_t s = it / chunks;
+        const int32_t c = (int32_t)(it - s * chunks);
+        const int32_t pre = spiked[s];
+        if (pre < 0 || pre >= n_cols) continue;
+        const int64_t b = col_ptr[pre], e = col_ptr[pre + 1];
+        for (int64_t base = b + (int64_t)c * 256; base < e; base += (int64_t)chunks * 256) {
+            int32_t post[4];
+            T v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int64_t j = base + lane + u * kWave;
+                post[u] = j < e ? row_idx[j] : -1;
+                v[u] = SINGLE_W ? w_const : (j < e ? w_col[j] : T(0));
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (post[u] >= 0) unsafeAtomicAdd(g + post[u], v[u]);
         }
     }
 }
 
 template <typename T>
 int launch_scatter_t(const Proj& p, const Pop& pre, void* target, hipStream_t s) {
-    // enough waves to cover the pre population's spikes in one pass
-    int64_t waves = pre.size;
-    int64_t nb = (waves + 3) / 4;
-    if (nb > 4096) nb = 4096;
+    // one wave per (spike, chunk of 256 synapses); a wave walks the chunks
+    // c, c + chunks, ... of its spike.  The grid covers every spike of the
+    // pre population in one pass up to two workgroups per CU, then loops.
+    const int32_t ch = p.scatter_chunks;
+    int64_t nb = ((int64_t)pre.size * ch + 3) / 4;
+    if (nb > 512) nb = 512;
     if (nb < 1) nb = 1;
     if (p.single_w)
         hipLaunchKernelGGL((scatter_spikes<T, true>), dim3(nb), dim3(256), 0, s,
-                           pre.spiked.as<int32_t>(), pre.spike_count.as<int32_t>(), p.n_cols,
+                           pre.spiked.as<int32_t>(), pre.spike_count.as<int32_t>(), p.n_cols, ch,
                            p.col_ptr.as<int64_t>(), p.row_idx.as<int32_t>(), (const T*)nullptr,
                            (T)p.w_const, (T*)target);
     else
         hipLaunchKernelGGL((scatter_spikes<T, false>), dim3(nb), dim3(256), 0, s,
-                           pre.spiked.as<int32_t>(), pre.spike_count.as<int32_t>(), p.n_cols,
+                           pre.spiked.as<int32_t>(), pre.spike_count.as<int32_t>(), p.n_cols, ch,
                            p.col_ptr.as<int64_t>(), p.row_idx.as<int32_t>(), p.w_col.as<T>(),
                            (T)0, (T*)target);
     return (int)hipGetLastError();

@@ -12,6 +12,7 @@
 #include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -696,6 +697,9 @@ static int build_backward(ahip_net* net, Proj& p) {
     std::vector<int64_t> col_ptr(p.n_cols + 1, 0);
     for (int64_t j = 0; j < p.nnz; j++) col_ptr[p.h_col[j] + 1]++;
     for (int32_t c = 0; c < p.n_cols; c++) col_ptr[c + 1] += col_ptr[c];
+    int64_t max_len = 0;
+    for (int32_t c = 0; c < p.n_cols; c++) max_len = std::max(max_len, col_ptr[c + 1] - col_ptr[c]);
+    p.scatter_chunks = (int32_t)std::min<int64_t>(16, std::max<int64_t>(1, (max_len + 255) / 256));
     std::vector<int64_t> cur(col_ptr.begin(), col_ptr.end() - 1);
     std::vector<int32_t> row_idx(p.nnz);
     p.h_inv_idx.assign(p.nnz, 0);

@@ -345,7 +345,7 @@ def select(csr: conn.CSR, mode: str = "mlp", selector: MLPSelector | None = None
 
 
 # ---------------------------------------------------------------------------
-# Corpus driver: generate (CPU worker processes) + autotune (device)
+# Corpus driver: generate (CPU threads) + autotune (device)
 # ---------------------------------------------------------------------------
 def _gen_task(spec):
     csr = generate(spec)
@@ -395,19 +395,33 @@ def build_corpus(count: int, out_path, workers: int = 8, seed: int = 2024, log=p
     return np.array(feats), np.array(times), kept
 
 
+def holdout_split(T, heuristic, seed: int = 0):
+    """Indices (train, test) of the 80/20 split over the corpus entries that
+    have a finite timing and a sparse heuristic choice."""
+    T, heuristic = np.asarray(T), np.asarray(heuristic)
+    ok = np.flatnonzero(np.isfinite(T).any(axis=1) & (heuristic >= 0))
+    perm = ok[np.random.default_rng(seed).permutation(len(ok))]
+    ntr = int(0.8 * len(ok))
+    return perm[:ntr], perm[ntr:]
+
+
+def with_fallback(T, choice):
+    """The choice per matrix, with the reference's CSR configuration where the
+    chosen one is infeasible (inf timing), as select() does."""
+    T, choice = np.asarray(T), np.asarray(choice)
+    ok = np.isfinite(T[np.arange(len(T)), choice])
+    return np.where(ok, choice, HEURISTIC_CSR)
+
+
 def evaluate(X, T, heuristic, save_path=None, seeds=(0, 1, 2), epochs=3000, log=print):
     """80/20 split: regret of the MLP (per seed), of the reference heuristic and
     of the best single configuration on the held-out matrices."""
     X, T, heuristic = np.asarray(X), np.asarray(T), np.asarray(heuristic)
-    ok = np.isfinite(T).any(axis=1) & (heuristic >= 0)
-    X, T, heuristic = X[ok], T[ok], heuristic[ok]
-    perm = np.random.default_rng(0).permutation(len(X))
-    ntr = int(0.8 * len(X))
-    tr, te = perm[:ntr], perm[ntr:]
+    tr, te = holdout_split(T, heuristic)
     Tt = T[te]
     best = Tt.min(axis=1)
     rows = np.arange(len(te))
-    res = {"n_train": int(ntr), "n_test": int(len(te))}
+    res = {"n_train": int(len(tr)), "n_test": int(len(te))}
     res["heuristic"] = dict(mean_regret=float(np.mean(Tt[rows, heuristic[te]] / best - 1)),
                             total_time_ratio=float(Tt[rows, heuristic[te]].sum() / best.sum()))
     rel = np.where(np.isfinite(T[tr]), T[tr] / T[tr].min(axis=1, keepdims=True), np.nan)
@@ -416,13 +430,13 @@ def evaluate(X, T, heuristic, save_path=None, seeds=(0, 1, 2), epochs=3000, log=
     for seed in seeds:
         sel = MLPSelector()
         curve = sel.fit(X[tr], T[tr], epochs=epochs, lr=3e-3, seed=seed)
-        pr = sel.predict(X[te])
+        pr = with_fallback(Tt, sel.predict(X[te]))
         reg = regret(Tt, pr)
         res[f"mlp_seed{seed}"] = dict(mean_regret=float(np.mean(reg)), median=float(np.median(reg)),
                                       p90=float(np.quantile(reg, 0.9)), loss0=float(curve[0]),
                                       loss=float(curve[-1]),
                                       total_time_ratio=float(Tt[rows, pr].sum() / best.sum()),
-                                      train_regret=float(np.mean(regret(T[tr], sel.predict(X[tr])))))
+                                      train_regret=float(np.mean(regret(T[tr], with_fallback(T[tr], sel.predict(X[tr]))))))
         if seed == seeds[0] and save_path is not None:
             sel.save(save_path)
         log(f"seed {seed}: {res[f'mlp_seed{seed}']}")

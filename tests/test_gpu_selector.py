@@ -51,3 +51,21 @@ def test_autotuner_vs_heuristic(gpu):
         assert np.isfinite(t[:9]).all()
         h = S.heuristic_class(csr)
         assert t.min() <= t[h]
+
+
+def test_committed_selector_device_predict_matches_oracle(gpu):
+    # the shipped network (annarchy_amd/data) on every corpus matrix: device
+    # FP32 inference against the FP64 oracle forward pass
+    d = np.load(S.DATA_DIR / "selector_corpus.npz")
+    sel = S.MLPSelector.load()
+    dev = sel.predict(d["features"])
+    logits = orc.mlp_forward(sel._norm(d["features"]).astype(np.float64), sel.params.astype(np.float64))
+    ref = np.argmax(logits, axis=1)
+    top2 = np.sort(logits, axis=1)[:, -2:]
+    near_tie = (top2[:, 1] - top2[:, 0]) < 1e-4
+    assert np.all((dev == ref) | near_tie)
+    assert np.mean(dev == ref) > 0.99
+    # select() on a fresh matrix returns a configuration that runs
+    csr = S.generate(dict(n=20000, k=60, kind="lognormal", seed=77))
+    cfg = S.select(csr)
+    assert cfg in S.CLASSES

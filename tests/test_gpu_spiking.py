@@ -179,7 +179,7 @@ def test_coba_fp32_rate_and_isi_cv_vs_fp64(gpu):
     # long chaotic runs cannot agree spike for spike in FP32; the population
     # rate and the mean ISI coefficient of variation must stay within 2% of
     # the FP64 oracle's (core/Monitor.py:1172-1240 define both statistics)
-    steps = 20000  # 2 s
+    steps = 50000  # 5 s
     net, P, ce, ci, mon = _coba_net(graph_batch=100, steps_cap=steps, precision="float", record=("spike",))
     P.I = 20.0
     v0 = P.v.reshape(-1).astype(np.float64)

@@ -197,3 +197,32 @@ def test_mlp_gradient_finite_differences():
         Pm[k] -= h
         fd = (orc.mlp_loss_grad(X, y, Pp, dims, grad=False)[0] - orc.mlp_loss_grad(X, y, Pm, dims, grad=False)[0]) / (2 * h)
         assert abs(fd - G[k]) < 1e-6 + 1e-4 * abs(fd)
+
+
+def test_mlp_forward_closed_form_and_loss():
+    # permutation weights make the logits a closed form that a transposed or
+    # shifted parameter block would break: hidden unit o < 16 copies input
+    # (5*o) % 16, layer 2 is the identity with bias -0.5, logit c reads
+    # hidden unit (c+3) % 16 and has bias c
+    d_in, h1, h2, d_out = 16, 64, 64, 12
+    W1 = np.zeros((h1, d_in))
+    for o in range(16):
+        W1[o, (5 * o) % 16] = 1.0
+    W2 = np.eye(h2)
+    b2 = np.full(h2, -0.5)
+    W3 = np.zeros((d_out, h2))
+    for c in range(d_out):
+        W3[c, (c + 3) % 16] = 1.0
+    b3 = np.arange(d_out, dtype=float)
+    P = np.concatenate([W1.ravel(), np.zeros(h1), W2.ravel(), b2, W3.ravel(), b3])
+    X = np.random.default_rng(4).normal(0, 1, (9, d_in))
+    z = orc.mlp_forward(X, P)
+    for c in range(d_out):
+        src = (5 * ((c + 3) % 16)) % 16
+        np.testing.assert_allclose(z[:, c], np.maximum(np.maximum(X[:, src], 0) - 0.5, 0) + c, rtol=0, atol=1e-15)
+    # the cross-entropy of these logits is the loss of the gradient oracle
+    y = np.random.default_rng(5).integers(0, d_out, 9)
+    zs = z - z.max(axis=1, keepdims=True)
+    ce = np.mean(np.log(np.exp(zs).sum(axis=1)) - zs[np.arange(9), y])
+    loss, _ = orc.mlp_loss_grad(X, y, P, grad=False)
+    assert abs(loss - ce) < 1e-12
