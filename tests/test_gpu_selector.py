@@ -48,7 +48,9 @@ def test_autotuner_vs_heuristic(gpu):
     for spec in S.corpus_specs(12, seed=5)[:6]:
         csr = S.generate(spec)
         t = S.autotune(csr, iters=3, repeats=1)
-        assert np.isfinite(t[:9]).all()
+        # CSR and SELL classes always run; the others exactly when feasible
+        assert np.isfinite(t[1:9]).all()
+        assert all(np.isfinite(t[c]) == S.feasible(csr, c) for c in range(len(S.CLASSES)))
         h = S.heuristic_class(csr)
         assert t.min() <= t[h]
 
