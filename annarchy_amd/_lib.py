@@ -73,6 +73,7 @@ _SIGS = {
     "ahip_pop_get_last_spike": ([_P, _I32, _PI64, _I64], _I32),
     "ahip_pop_spikes": ([_P, _I32, _PI32, _I32, _PI32], _I32),
     "ahip_pop_reset_spiking": ([_P, _I32], _I32),
+    "ahip_pop_reset_delays": ([_P, _I32], _I32),
     "ahip_pop_size": ([_P, _I32, _PI32], _I32),
     "ahip_pop_inject_random": ([_P, _I32, _I32, _PD, _I64], _I32),
     "ahip_proj_create": ([_P, _I32, _I32, C.c_char_p, _I32, _PI32], _I32),
@@ -85,6 +86,9 @@ _SIGS = {
     "ahip_proj_psp_bytes": ([_P, _I32, _PD], _I32),
     "ahip_proj_time_psp": ([_P, _I32, _I32, _PD], _I32),
     "ahip_proj_set_launch": ([_P, _I32, _I32], _I32),
+    "ahip_proj_set_delays": ([_P, _I32, _PI32, _I64], _I32),
+    "ahip_proj_set_max_delay": ([_P, _I32, _I32], _I32),
+    "ahip_proj_get_delays": ([_P, _I32, _PI32, _PI32, _PI32, _I64], _I32),
     "ahip_monitor_create": ([_P, _I32, C.c_char_p, _I64, _PI32], _I32),
     "ahip_monitor_steps": ([_P, _I32, _PI64], _I32),
     "ahip_monitor_get": ([_P, _I32, _PD, _I64], _I32),

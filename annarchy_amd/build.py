@@ -39,6 +39,7 @@ CORE_SOURCES = [
     "k_misc.hip",
     "k_batched.hip",
     "k_mlp.hip",
+    "k_delay.hip",
 ]
 
 COMMON_FLAGS = [

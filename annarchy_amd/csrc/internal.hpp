@@ -59,6 +59,11 @@ struct Pop {
     std::vector<double> slot_host_global;   // host mirror of population-wide slots
     DevBuf refractory, refractory_remaining, last_spike, spike_mask, block_counts;
     DevBuf spiked, spike_count;             // int32[size], int32[1]
+    // delay queue (ring of `depth` steps, slot t mod depth = end of step t):
+    // rate-coded: the psp variable, depth x size; spiking: spike lists,
+    // depth x size int32 with counts int32[depth]
+    int32_t depth = 1;
+    DevBuf hist, hist_count;
     DevBuf rand_inject[AHIP_MAX_RAND];
     int64_t rand_inject_steps[AHIP_MAX_RAND] = {0};
     int64_t rand_t0 = 0;
@@ -89,6 +94,14 @@ struct Proj {
     DevBuf row_idx;   // int32[nnz]
     DevBuf w_col;     // T[nnz] column order (absent when single_w)
     int32_t scatter_chunks = 1;   // waves per spike in the scatter (256 synapses each)
+    // synaptic delays in steps (1 = the value of the previous step)
+    int32_t delay_uniform = 1;    // the delay of every synapse when !delay_per_syn
+    bool delay_per_syn = false;
+    int32_t max_delay = 1;
+    std::vector<int32_t> h_delay; // per synapse, forward order (delay_per_syn)
+    DevBuf delay_dev;             // int16: rate -> forward order, spiking -> column order
+    DevBuf ring;                  // spiking per-synapse delays: max_delay x n_rows events
+    DevBuf xd;                    // rate uniform delay: the delayed pre-synaptic rates
     std::vector<int64_t> h_inv_idx;
 };
 
@@ -109,6 +122,33 @@ struct Monitor {
 
 }  // namespace ahip
 
+namespace ahip {
+// One spiking projection in a fused scatter launch (k_spike.hip:scatter_multi).
+struct ScatterDesc {
+    const int32_t* list;      // spike list of the pre population (ascending ranks)
+    const int32_t* count;     // its length (device)
+    const int64_t* col_ptr;   // CSRC backward view
+    const int32_t* row_idx;
+    const void* w_col;        // weights in column order, or null for w_const
+    void* g;                  // post-synaptic conductance g_<target>
+    double w_const;
+    int64_t list_stride;      // uniform delay >= 2: the pre population's spike-list ring
+    int32_t depth;            // ring depth, 1 = read `list` itself
+    int32_t delay;            // list of step t - delay = ring slot (t - delay) mod depth
+    int32_t n_cols;
+    int32_t chunks;           // waves per spike (256 synapses each)
+};
+constexpr int kMaxScatter = 8;
+// Spiking projections launched together: their targets are distinct, so the
+// atomics of one projection never interleave with another's into the same
+// conductance and every sum keeps the reference's projection order.
+struct ScatterGroup {
+    DevBuf table;             // ScatterDesc[n]
+    int32_t n = 0;
+    int64_t blocks = 1;
+};
+}  // namespace ahip
+
 struct ahip_net {
     int32_t device = 0;
     double dt = 1.0;
@@ -126,6 +166,8 @@ struct ahip_net {
     hipGraphExec_t graph_exec = nullptr;
     int32_t graph_steps = 0;
     bool dirty = true;             // structure changed since the last capture
+    bool scatter_dirty = true;     // the fused scatter tables need rebuilding
+    std::vector<ahip::ScatterGroup> scatter_groups;
     size_t elem() const { return precision == AHIP_F64 ? sizeof(double) : sizeof(float); }
 };
 
@@ -137,6 +179,17 @@ int launch_rate_psp(const ahip_net* net, const Proj& p, const void* pre_r, void*
 int launch_spike_psp(const ahip_net* net, const Proj& p, const Pop& pre, void* target,
                      hipStream_t s);
 int launch_compact(const Pop& p, hipStream_t s);
+int launch_scatter_group(const ahip_net* net, const ScatterGroup& g, int32_t step, hipStream_t s);
+int launch_push_values(const ahip_net* net, const void* src, void* hist, int32_t n, int32_t D, int32_t step,
+                       hipStream_t s);
+int launch_gather_delayed(const ahip_net* net, const void* hist, void* dst, int32_t n, int32_t D, int32_t d,
+                          int32_t step, hipStream_t s);
+int launch_fill_hist(const ahip_net* net, const void* src, void* hist, int32_t n, int32_t D, hipStream_t s);
+int launch_push_spikes(const ahip_net* net, const Pop& p, int32_t step, hipStream_t s);
+int launch_rate_psp_delayed(const ahip_net* net, const Proj& p, const Pop& pre, void* target, int32_t step,
+                            hipStream_t s);
+int launch_spike_ring(const ahip_net* net, const Proj& p, const Pop& pre, void* target, int32_t step,
+                      hipStream_t s);
 int launch_advance_time(int64_t* t, int64_t n, hipStream_t s);
 int launch_fill(void* dst, int64_t n, int32_t elem, double value, hipStream_t s);
 int launch_fill_i32(int32_t* dst, int64_t n, int32_t value, hipStream_t s);

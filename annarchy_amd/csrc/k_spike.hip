@@ -149,6 +149,59 @@ int launch_scatter_t(const Proj& p, const Pop& pre, void* target, hipStream_t s)
     return (int)hipGetLastError();
 }
 
+// All projections of a scatter group in one launch.  Work items of
+// descriptor k are (spike s, chunk c) for s < count_k, c < chunks_k, laid out
+// after those of descriptors 0..k-1; a wave maps its item to (k, s, c) and
+// proceeds as scatter_spikes.  Descriptors of a group have distinct targets.
+__device__ inline int64_t list_slot(const ScatterDesc& d, int64_t t) {
+    if (d.depth <= 1) return 0;
+    const int64_t m = (t - d.delay) % d.depth;
+    return m < 0 ? m + d.depth : m;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void scatter_multi(const ScatterDesc* __restrict__ desc, int32_t nd,
+                                                     const int64_t* __restrict__ t_base, int32_t step) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int64_t t = *t_base + step;
+    int64_t ends[kMaxScatter];
+    int64_t total = 0;
+#pragma unroll
+    for (int k = 0; k < kMaxScatter; k++) {
+        if (k < nd) total += (int64_t)desc[k].count[list_slot(desc[k], t)] * desc[k].chunks;
+        ends[k] = total;
+    }
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    for (int64_t it = (int64_t)blockIdx.x * 4 + wid; it < total; it += waves) {
+        int k = 0;
+        while (k < nd - 1 && it >= ends[k]) k++;
+        const ScatterDesc& d = desc[k];
+        const int64_t local = it - (k > 0 ? ends[k - 1] : 0);
+        const int64_t s = local / d.chunks;
+        const int32_t c = (int32_t)(local - s * d.chunks);
+        const int32_t pre = d.list[list_slot(d, t) * d.list_stride + s];
+        if (pre < 0 || pre >= d.n_cols) continue;
+        const int64_t b = d.col_ptr[pre], e = d.col_ptr[pre + 1];
+        const T* wc = static_cast<const T*>(d.w_col);
+        const T wk = (T)d.w_const;
+        T* g = static_cast<T*>(d.g);
+        for (int64_t base = b + (int64_t)c * 256; base < e; base += (int64_t)d.chunks * 256) {
+            int32_t post[4];
+            T v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int64_t j = base + lane + u * kWave;
+                post[u] = j < e ? d.row_idx[j] : -1;
+                v[u] = wc == nullptr ? wk : (j < e ? wc[j] : T(0));
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (post[u] >= 0) unsafeAtomicAdd(g + post[u], v[u]);
+        }
+    }
+}
+
 // Spike recorder: appends the step's spike list to the monitor buffer.  One
 // workgroup; fill[0] is the number of indices written so far.
 __global__ __launch_bounds__(256) void record_spikes(const int32_t* __restrict__ spiked,
@@ -181,6 +234,17 @@ int launch_compact(const Pop& p, hipStream_t s) {
     hipLaunchKernelGGL(compact_spikes, dim3(nb), dim3(kCompactThreads), 0, s, p.size, p.nblocks(),
                        p.spike_mask.as<uint64_t>(), p.block_counts.as<int32_t>(),
                        p.spiked.as<int32_t>(), p.spike_count.as<int32_t>());
+    return (int)hipGetLastError();
+}
+
+int launch_scatter_group(const ahip_net* net, const ScatterGroup& g, int32_t step, hipStream_t s) {
+    if (g.n == 0) return 0;
+    if (net->precision == AHIP_F64)
+        hipLaunchKernelGGL(scatter_multi<double>, dim3(g.blocks), dim3(256), 0, s,
+                           g.table.as<ScatterDesc>(), g.n, net->t_dev.as<int64_t>(), step);
+    else
+        hipLaunchKernelGGL(scatter_multi<float>, dim3(g.blocks), dim3(256), 0, s,
+                           g.table.as<ScatterDesc>(), g.n, net->t_dev.as<int64_t>(), step);
     return (int)hipGetLastError();
 }
 

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -86,6 +87,7 @@ Monitor* get_mon(ahip_net* net, int32_t id) {
 
 void invalidate(ahip_net* net) {
     net->dirty = true;
+    net->scatter_dirty = true;
     if (net->graph_exec) {
         hipGraphExecDestroy(net->graph_exec);
         net->graph_exec = nullptr;
@@ -105,11 +107,146 @@ struct StepPlan {
 
 // Decide, per rate projection, whether it overwrites its target (first one
 // into that target and its layout writes every row), or adds into it.
+// Re-slot a ring indexed by step (slot k mod D) into a deeper one: the
+// entries of steps [k_from, k_to) move to their new slots, the others of the
+// new ring are zero.  Rings only grow, as the reference's queues
+// (update_max_delay resizes them keeping their content).
+static int regrow_ring(DevBuf& buf, size_t slot_bytes, int32_t d_old, int32_t d_new, int64_t k_from,
+                       int64_t k_to) {
+    DevBuf nb;
+    int rc = dev_alloc(nb, (size_t)d_new * slot_bytes);
+    if (rc) return rc;
+    if (buf.p && d_old > 0) {
+        for (int64_t k = k_from; k < k_to; k++) {
+            const int64_t so = ((k % d_old) + d_old) % d_old, sn = ((k % d_new) + d_new) % d_new;
+            HIPCHK(hipMemcpy((char*)nb.p + sn * slot_bytes, (char*)buf.p + so * slot_bytes, slot_bytes,
+                             hipMemcpyDeviceToDevice));
+        }
+    }
+    buf = std::move(nb);
+    return 0;
+}
+
+// Delay queues: the depth of a population's queue is the largest delay that
+// reads it (rate-coded: every delayed projection; spiking: the uniform
+// delays, per-synapse delays keep their own event ring), and it never
+// shrinks.  A new queue is zero (no spikes); a deeper one keeps the values of
+// the last steps.
+int ensure_delay_buffers(ahip_net* net) {
+    const size_t es = net->elem();
+    std::vector<int32_t> need(net->pops.size(), 1);
+    for (auto& pp : net->projs) {
+        Proj& p = *pp;
+        if (!p.spiking)
+            need[p.pre] = std::max(need[p.pre], p.max_delay);
+        else if (!p.delay_per_syn)
+            need[p.pre] = std::max(need[p.pre], p.max_delay);
+    }
+    for (auto& pp : net->pops) {
+        Pop& p = *pp;
+        const int32_t D = std::max(need[p.id], p.depth);
+        if (D == p.depth && (D == 1 || p.hist.p)) continue;
+        HIPCHK(hipStreamSynchronize(net->stream));
+        const int32_t d_old = p.hist.p ? p.depth : 0;
+        int rc;
+        if (p.model->spiking) {
+            if ((rc = regrow_ring(p.hist, (size_t)p.size * sizeof(int32_t), d_old, D, net->t - d_old, net->t))) return rc;
+            if ((rc = regrow_ring(p.hist_count, sizeof(int32_t), d_old, D, net->t - d_old, net->t))) return rc;
+        } else {
+            if ((rc = regrow_ring(p.hist, (size_t)p.size * es, d_old, D, net->t - d_old, net->t))) return rc;
+        }
+        p.depth = D;
+        invalidate(net);
+    }
+    for (auto& pp : net->projs) {
+        Proj& p = *pp;
+        if (!p.spiking && p.delay_uniform >= 2 && !p.delay_per_syn && !p.xd.p) {
+            int rc = dev_alloc(p.xd, (size_t)p.n_cols * es);
+            if (rc) return rc;
+            invalidate(net);
+        }
+        if (p.spiking && p.delay_per_syn) {
+            // pending events of steps [t, t + old depth) keep their delivery step
+            const size_t slot = (size_t)p.n_rows * (p.single_w ? sizeof(int32_t) : es);
+            const int32_t d_old = p.ring.p ? (int32_t)(p.ring.bytes / std::max<size_t>(slot, 1)) : 0;
+            if (!p.ring.p || d_old < p.max_delay) {
+                HIPCHK(hipStreamSynchronize(net->stream));
+                int rc = regrow_ring(p.ring, slot, d_old, std::max(p.max_delay, d_old), net->t, net->t + d_old);
+                if (rc) return rc;
+                invalidate(net);
+            }
+        }
+    }
+    return 0;
+}
+
 int prepare(ahip_net* net) {
     if (!net->initialized) return fail(AHIP_ESTATE, "network not initialized");
     for (auto& pp : net->projs) {
         Proj& p = *pp;
         if (p.nnz < 0) return fail(AHIP_ESTATE, "projection %d has no connectivity", p.id);
+    }
+    int rc0 = ensure_delay_buffers(net);
+    if (rc0) return rc0;
+    // fused scatter groups: spiking projections in creation order, each in the
+    // first group after the last one that already writes its target
+    if (net->scatter_dirty) {
+        net->scatter_groups.clear();
+        std::vector<std::vector<ScatterDesc>> groups;
+        std::vector<int64_t> blocks;
+        std::vector<std::vector<const void*>> targets;
+        for (auto& pp : net->projs) {
+            Proj& p = *pp;
+            if (!p.spiking || p.nnz == 0 || p.delay_per_syn) continue;
+            Pop& pre = *net->pops[p.pre];
+            void* g = net->pops[p.post]->slots[p.target_slot].p;
+            size_t k = 0;
+            for (size_t q = 0; q < groups.size(); q++)
+                for (const void* t : targets[q])
+                    if (t == g) k = q + 1;
+            while (k < groups.size() && (int)groups[k].size() >= kMaxScatter) k++;
+            if (k == groups.size()) {
+                groups.emplace_back();
+                targets.emplace_back();
+                blocks.push_back(0);
+            }
+            ScatterDesc d;
+            if (p.delay_uniform >= 2) {
+                // the spike list of step t - delay from the pre population's ring
+                d.list = pre.hist.as<int32_t>();
+                d.count = pre.hist_count.as<int32_t>();
+                d.list_stride = pre.size;
+                d.depth = pre.depth;
+                d.delay = p.delay_uniform;
+            } else {
+                d.list = pre.spiked.as<int32_t>();
+                d.count = pre.spike_count.as<int32_t>();
+                d.list_stride = 0;
+                d.depth = 1;
+                d.delay = 0;
+            }
+            d.col_ptr = p.col_ptr.as<int64_t>();
+            d.row_idx = p.row_idx.as<int32_t>();
+            d.w_col = p.single_w ? nullptr : p.w_col.p;
+            d.g = g;
+            d.w_const = p.w_const;
+            d.n_cols = p.n_cols;
+            d.chunks = p.scatter_chunks;
+            groups[k].push_back(d);
+            targets[k].push_back(g);
+            blocks[k] += ((int64_t)pre.size * p.scatter_chunks + 3) / 4;
+        }
+        for (size_t q = 0; q < groups.size(); q++) {
+            ScatterGroup sg;
+            sg.n = (int32_t)groups[q].size();
+            sg.blocks = std::max<int64_t>(1, std::min<int64_t>(512, blocks[q]));
+            int rc = dev_alloc(sg.table, groups[q].size() * sizeof(ScatterDesc));
+            if (rc) return rc;
+            HIPCHK(hipMemcpy(sg.table.p, groups[q].data(), groups[q].size() * sizeof(ScatterDesc),
+                             hipMemcpyHostToDevice));
+            net->scatter_groups.push_back(std::move(sg));
+        }
+        net->scatter_dirty = false;
     }
     // pre-loop values (hoisted step sizes) from the host mirror of the globals
     for (auto& pp : net->pops) {
@@ -184,16 +321,25 @@ ahip_pop_args make_args(ahip_net* net, Pop& p, int32_t step) {
 int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_before) {
     hipStream_t s = net->stream;
     const size_t es = net->elem();
-    // 1. compute_psp
+    // 1. compute_psp: the spiking projections in fused scatter groups, then
+    // the rate-coded ones
+    for (const ScatterGroup& g : net->scatter_groups) LCHK(launch_scatter_group(net, g, step, s));
     for (auto& pp : net->projs) {
         Proj& p = *pp;
         Pop& pre = *net->pops[p.pre];
         Pop& post = *net->pops[p.post];
         void* target = post.slots[p.target_slot].p;
         if (p.spiking) {
-            LCHK(launch_spike_psp(net, p, pre, target, s));
+            if (p.delay_per_syn && p.nnz > 0) LCHK(launch_spike_ring(net, p, pre, target, step, s));
+            continue;
+        }
+        if (zero_before[p.id]) LCHK(launch_fill(target, post.size, (int32_t)es, 0.0, s));
+        if (p.delay_per_syn) {
+            LCHK(launch_rate_psp_delayed(net, p, pre, target, step, s));
+        } else if (p.delay_uniform >= 2) {
+            LCHK(launch_gather_delayed(net, pre.hist.p, p.xd.p, pre.size, pre.depth, p.delay_uniform, step, s));
+            LCHK(launch_rate_psp(net, p, p.xd.p, target, s));
         } else {
-            if (zero_before[p.id]) LCHK(launch_fill(target, post.size, (int32_t)es, 0.0, s));
             LCHK(launch_rate_psp(net, p, pre.slots[p.pre_slot].p, target, s));
         }
     }
@@ -214,6 +360,16 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
         if (rc != 0) return fail(AHIP_EHIP, "update of population %d failed: %s", p.id,
                                  hipGetErrorString((hipError_t)rc));
         if (p.model->spiking) LCHK(launch_compact(p, s));
+        // delay queues: push this step's values (after the update, as the
+        // reference's update_delay)
+        if (p.depth > 1) {
+            if (p.model->spiking) {
+                LCHK(launch_push_spikes(net, p, step, s));
+            } else {
+                const int32_t rs = p.slot_index("r");
+                if (rs >= 0) LCHK(launch_push_values(net, p.slots[rs].p, p.hist.p, p.size, p.depth, step, s));
+            }
+        }
     }
     // 5. record variables and spikes
     for (auto& mp : net->mons) {
@@ -754,6 +910,14 @@ int ahip_proj_init_csr(ahip_net* net, int32_t proj, int32_t n_rows, int32_t n_co
     p->h_col.assign(col_idx, col_idx + nnz);
     p->single_w = (w == nullptr);
     p->w_const = w_const;
+    // new connectivity: no delays until ahip_proj_set_delays
+    p->delay_uniform = 1;
+    p->delay_per_syn = false;
+    p->max_delay = 1;
+    p->h_delay.clear();
+    p->delay_dev.release();
+    p->ring.release();
+    p->xd.release();
     if (w)
         p->h_w.assign(w, w + nnz);
     else
@@ -772,6 +936,8 @@ int ahip_proj_set_format(ahip_net* net, int32_t proj, const ahip_format_opts* op
     Proj* p = get_proj(net, proj);
     if (!p || !opts || p->nnz < 0) return fail(AHIP_EINVAL, "bad projection");
     if (p->spiking) return fail(AHIP_EINVAL, "spiking projections use the CSRC backward view");
+    if (p->delay_per_syn && opts->format != AHIP_FMT_CSR)
+        return fail(AHIP_EINVAL, "per-synapse delays of a rate-coded projection need the csr format");
     HIPCHK(hipStreamSynchronize(net->stream));
     invalidate(net);
     std::string err;
@@ -790,6 +956,92 @@ int ahip_proj_get_format(ahip_net* net, int32_t proj, ahip_format_opts* opts) {
     opts->chunk = p->rate.chunk;
     opts->sigma = p->rate.sigma;
     opts->panel_cols = p->rate.panel_cols;
+    return 0;
+}
+
+int ahip_proj_set_delays(ahip_net* net, int32_t proj, const int32_t* steps, int64_t n) {
+    Proj* p = get_proj(net, proj);
+    if (!p || !steps || p->nnz < 0) return fail(AHIP_EINVAL, "bad projection or delays");
+    if (n != 1 && n != p->nnz)
+        return fail(AHIP_EINVAL, "give one delay or one per synapse (%lld), not %lld", (long long)p->nnz, (long long)n);
+    int32_t lo = INT32_MAX, hi = 0;
+    for (int64_t j = 0; j < n; j++) {
+        lo = std::min(lo, steps[j]);
+        hi = std::max(hi, steps[j]);
+    }
+    if (n > 0 && (lo < 1 || hi > 32767)) return fail(AHIP_EINVAL, "delays must be 1..32767 steps");
+    HIPCHK(hipStreamSynchronize(net->stream));
+    invalidate(net);
+    p->delay_dev.release();
+    p->h_delay.clear();
+    if (n == 1 || lo == hi) {
+        // one value for every synapse (core/Projection.py:653-657)
+        p->delay_per_syn = false;
+        p->delay_uniform = n > 0 ? lo : 1;
+        p->max_delay = std::max(p->max_delay, p->delay_uniform);
+        p->ring.release();
+        return 0;
+    }
+    if (!p->spiking && p->rate.format != AHIP_FMT_CSR)
+        return fail(AHIP_EINVAL, "per-synapse delays of a rate-coded projection need the csr format");
+    p->delay_per_syn = true;
+    p->delay_uniform = -1;
+    p->max_delay = std::max(p->max_delay, hi);
+    p->h_delay.assign(steps, steps + n);
+    std::vector<int16_t> d16(p->nnz);
+    for (int64_t q = 0; q < p->nnz; q++)
+        d16[q] = (int16_t)(p->spiking ? steps[p->h_inv_idx[q]] : steps[q]);
+    int rc = dev_alloc(p->delay_dev, d16.size() * sizeof(int16_t));
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(p->delay_dev.p, d16.data(), d16.size() * sizeof(int16_t), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int ahip_proj_set_max_delay(ahip_net* net, int32_t proj, int32_t max_delay) {
+    Proj* p = get_proj(net, proj);
+    if (!p || max_delay < 1 || max_delay > 32767) return fail(AHIP_EINVAL, "bad projection or max_delay");
+    if (max_delay > p->max_delay) {
+        HIPCHK(hipStreamSynchronize(net->stream));
+        p->max_delay = max_delay;
+        invalidate(net);
+    }
+    return 0;
+}
+
+int ahip_proj_get_delays(ahip_net* net, int32_t proj, int32_t* uniform, int32_t* max_delay, int32_t* steps,
+                         int64_t n) {
+    Proj* p = get_proj(net, proj);
+    if (!p || !uniform || !max_delay) return fail(AHIP_EINVAL, "bad projection or arguments");
+    *uniform = p->delay_per_syn ? -1 : p->delay_uniform;
+    *max_delay = p->max_delay;
+    if (steps) {
+        if (n < p->nnz) return fail(AHIP_EINVAL, "buffer too small (%lld synapses)", (long long)p->nnz);
+        for (int64_t j = 0; j < p->nnz; j++) steps[j] = p->delay_per_syn ? p->h_delay[j] : p->delay_uniform;
+    }
+    return 0;
+}
+
+int ahip_pop_reset_delays(ahip_net* net, int32_t pop) {
+    Pop* p = get_pop(net, pop);
+    if (!p) return fail(AHIP_EINVAL, "bad population");
+    HIPCHK(hipSetDevice(net->device));
+    int rc = ensure_delay_buffers(net);
+    if (rc) return rc;
+    if (p->depth > 1) {
+        if (p->model->spiking) {
+            LCHK(launch_fill_i32(p->hist_count.as<int32_t>(), p->depth, 0, net->stream));
+        } else {
+            const int32_t rs = p->slot_index("r");
+            if (rs >= 0) LCHK(launch_fill_hist(net, p->slots[rs].p, p->hist.p, p->size, p->depth, net->stream));
+        }
+    }
+    // pending events of per-synapse spiking delays from this population
+    for (auto& pp : net->projs) {
+        Proj& q = *pp;
+        if (q.pre == pop && q.spiking && q.delay_per_syn && q.ring.p)
+            LCHK(launch_fill_i32(q.ring.as<int32_t>(), (int64_t)(q.ring.bytes / sizeof(int32_t)), 0, net->stream));
+    }
+    HIPCHK(hipStreamSynchronize(net->stream));
     return 0;
 }
 

@@ -116,6 +116,19 @@ class Population:
                                            _lib.ptr(out, C.c_double), n))
         return out.reshape(self.geometry) if n > 1 else float(out[0])
 
+    def reset(self, attributes=None):
+        """Population.reset (core/Population.py:309-343): parameters and
+        variables back to their initial values (those set before compile or
+        the model's init), then the spiking state and the delay queues
+        (refilled with the current values)."""
+        info = self._slot_info()
+        names = [s[0] for s in info.values() if s[3] in (0, 1)] if attributes is None else list(attributes)
+        for name in names:
+            self.set(name, self._init.get(name, info[name][2]))
+        if self.spiking:
+            _lib.check(_lib.lib().ahip_pop_reset_spiking(self.net._h, self.id))
+        _lib.check(_lib.lib().ahip_pop_reset_delays(self.net._h, self.id))
+
     def _materialize(self, value, local):
         n = self.size if local else 1
         if isinstance(value, conn.RandomDistribution):
@@ -189,6 +202,7 @@ class Projection:
         self.pre, self.post, self.target = pre, post, target
         self.id = -1
         self._connector = None
+        self._delays = 0.0
         self._format = _lib.opts("csr", 64)
         self.csr: conn.CSR | None = None
 
@@ -202,34 +216,77 @@ class Projection:
 
     def fixed_probability(self, probability, weights, delays=0.0, allow_self_connections=False,
                           storage_format=None, **fmt):
-        self._check_delay(delays)
+        self._delays = delays
         return self._store("fixed_probability", probability, weights, allow_self_connections,
                            storage_format=storage_format, **fmt)
 
     def fixed_number_pre(self, number, weights, delays=0.0, allow_self_connections=False,
                          storage_format=None, **fmt):
-        self._check_delay(delays)
+        self._delays = delays
         if number > _sizes(self.pre):
             raise ValueError("fixed_number_pre: the number of pre-synaptic neurons exceeds the population size")
         return self._store("fixed_number_pre", number, weights, allow_self_connections,
                            storage_format=storage_format, **fmt)
 
     def all_to_all(self, weights, delays=0.0, allow_self_connections=False, storage_format=None, **fmt):
-        self._check_delay(delays)
+        self._delays = delays
         return self._store("all_to_all", weights, allow_self_connections,
                            storage_format=storage_format, **fmt)
 
     def one_to_one(self, weights, delays=0.0, storage_format=None, **fmt):
-        self._check_delay(delays)
+        self._delays = delays
         return self._store("one_to_one", weights, storage_format=storage_format, **fmt)
 
-    def from_csr(self, csr: conn.CSR, storage_format=None, **fmt):
+    def from_csr(self, csr: conn.CSR, storage_format=None, delays=0.0, **fmt):
+        self._delays = delays
         return self._store("csr", csr, storage_format=storage_format, **fmt)
 
-    @staticmethod
-    def _check_delay(d):
-        if not (isinstance(d, (int, float)) and d == 0.0):
-            raise NotImplementedError("synaptic delays are not supported by this backend yet")
+    # -- delays (core/Projection.py:640-690, :1280-1412) ------------------------
+    def _delay_steps(self, value) -> np.ndarray:
+        """Delay(s) in ms -> steps, round(d/dt) as the reference, at least 1
+        (one step is the plain transmission)."""
+        dt = self.net.dt
+        if isinstance(value, conn.RandomDistribution):
+            value = value.get_values(self.nb_synapses)
+        elif isinstance(value, (list, tuple)) and value and isinstance(value[0], (list, tuple, np.ndarray)):
+            value = np.concatenate([np.asarray(v, dtype=np.float64) for v in value])
+        d = np.rint(np.asarray(value, dtype=np.float64).reshape(-1) / dt).astype(np.int64)
+        return np.ascontiguousarray(np.maximum(d, 1), dtype=np.int32)
+
+    def _upload_delays(self, value):
+        steps = self._delay_steps(value)
+        if steps.size not in (1, self.nb_synapses):
+            raise ValueError(f"expected one delay or {self.nb_synapses}")
+        if isinstance(value, conn.RandomDistribution) and getattr(value, "max", None) is not None:
+            # the queue holds the distribution's largest delay (core/Projection.py:661-679)
+            m = max(1, int(round(float(value.max) / self.net.dt)))
+            _lib.check(_lib.lib().ahip_proj_set_max_delay(self.net._h, self.id, m))
+        _lib.check(_lib.lib().ahip_proj_set_delays(self.net._h, self.id, _lib.ptr(steps, C.c_int32), steps.size))
+
+    @property
+    def delay(self):
+        """Uniform delay in ms, or the delay of every synapse (forward order)."""
+        if not self.net.compiled:
+            return self._delays
+        u, m = C.c_int32(), C.c_int32()
+        if self.nb_synapses == 0:
+            return self.net.dt
+        out = np.empty(self.nb_synapses, dtype=np.int32)
+        _lib.check(_lib.lib().ahip_proj_get_delays(self.net._h, self.id, C.byref(u), C.byref(m),
+                                                   _lib.ptr(out, C.c_int32), out.size))
+        return u.value * self.net.dt if u.value > 0 else out * self.net.dt
+
+    @delay.setter
+    def delay(self, value):
+        self._delays = value
+        if self.net.compiled:
+            self._upload_delays(value)
+
+    @property
+    def max_delay(self) -> int:
+        u, m = C.c_int32(), C.c_int32()
+        _lib.check(_lib.lib().ahip_proj_get_delays(self.net._h, self.id, C.byref(u), C.byref(m), None, 0))
+        return m.value
 
     def _build_csr(self) -> conn.CSR:
         kind, args = self._connector
@@ -268,6 +325,8 @@ class Projection:
                                         _lib.ptr(c.row_ptr, C.c_int64), _lib.ptr(c.col_idx, C.c_int32),
                                         _lib.ptr(w, C.c_double) if w is not None else None,
                                         c.w_const, C.byref(self._format)))
+        if not (np.isscalar(self._delays) and float(self._delays) <= self.net.dt):
+            self._upload_delays(self._delays)
 
     # -- accessors ------------------------------------------------------------
     @property
@@ -330,6 +389,11 @@ class Monitor:
             _lib.check(_lib.lib().ahip_monitor_create(self.net._h, pop.id, v.encode(), self.capacity,
                                                       C.byref(mid)))
             self._ids[v] = mid.value
+
+    def clear(self):
+        """Empty every recorded buffer; recording restarts at the current step."""
+        for mid in self._ids.values():
+            _lib.check(_lib.lib().ahip_monitor_clear(self.net._h, mid))
 
     def _steps(self, mid) -> int:
         s = C.c_int64()
@@ -487,6 +551,19 @@ class Network:
 
     def get_time(self) -> float:
         return self.time * self.dt
+
+    def reset(self, populations: bool = True, projections: bool = False, monitors: bool = True):
+        """Network.reset (core/Network.py:628-670): time 0, populations back
+        to their initial values with their spiking state and delay queues,
+        monitors emptied.  Projections keep their weights (no synaptic
+        variables besides w exist in this backend)."""
+        _lib.check(_lib.lib().ahip_set_time(self._h, 0))
+        if populations:
+            for p in self.populations:
+                p.reset()
+        if monitors:
+            for m in self.monitors:
+                m.clear()
 
     def set_graph(self, enabled: bool = True, batch: int = 0):
         _lib.check(_lib.lib().ahip_set_graph_enabled(self._h, int(enabled)))

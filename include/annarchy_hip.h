@@ -115,6 +115,13 @@ int ahip_pop_get_last_spike(ahip_net* net, int32_t pop, int64_t* values, int64_t
 int ahip_pop_spikes(ahip_net* net, int32_t pop, int32_t* out, int32_t cap, int32_t* n);
 /* reset(): refractory_remaining = 0, last_spike = -10000, spiked cleared. */
 int ahip_pop_reset_spiking(ahip_net* net, int32_t pop);
+/* reset() of the delay queues (Population/OpenMPTemplates.py:193-197, the
+ * "reset" code of attribute_delayed; Projection reset_ring_buffer,
+ * core/Network.py:652-655): every slot of a rate-coded population's queue
+ * takes the current rates, a spiking population's queue of spike lists is
+ * emptied, and so are the pending events of per-synapse delays from it.  A
+ * queue is zero (no spikes) until then, as after the reference's init. */
+int ahip_pop_reset_delays(ahip_net* net, int32_t pop);
 int ahip_pop_size(ahip_net* net, int32_t pop, int32_t* size);
 /* Injected random streams: buf holds nsteps*size draws of random variable k
  * (rand_k), row 0 used at the current time step; replaces the on-device
@@ -151,6 +158,23 @@ int ahip_proj_psp_bytes(ahip_net* net, int32_t proj, double* bytes);
 int ahip_proj_time_psp(ahip_net* net, int32_t proj, int32_t iters, double* avg_ms);
 /* update_launch_config(nb, tpb) equivalent (Projection.py:2125-2150). */
 int ahip_proj_set_launch(ahip_net* net, int32_t proj, int32_t lanes_per_row);
+/* Synaptic delays in steps: set_delay(int) / set_delay(vector<vector<int>>)
+ * of the uniform and per-synapse delay templates (Projection/OpenMP/CSR.py:
+ * 67-101).  n = 1: one delay for every synapse; n = nnz: one per synapse in
+ * the forward order of ahip_proj_init_csr.  1 <= delay <= 32767; a delay of 1
+ * is the plain transmission (values of the previous step).  A rate-coded
+ * projection with differing delays needs the csr format.  Re-initialising
+ * the connectivity clears the delays. */
+int ahip_proj_set_delays(ahip_net* net, int32_t proj, const int32_t* steps, int64_t n);
+/* set_max_delay / update_max_delay (Projection/OpenMP/CSR.py:85-101): the
+ * queues read by this projection hold at least max_delay steps (delays drawn
+ * from a distribution size them by its maximum, core/Projection.py:661-679).
+ * max_delay only grows; growing keeps the queued values. */
+int ahip_proj_set_max_delay(ahip_net* net, int32_t proj, int32_t max_delay);
+/* *uniform = the common delay or -1 if the delays differ, *max_delay; steps
+ * (optional, n >= nnz) receives the delay of every synapse (forward order). */
+int ahip_proj_get_delays(ahip_net* net, int32_t proj, int32_t* uniform, int32_t* max_delay, int32_t* steps,
+                         int64_t n);
 
 /* ---- monitors (device-side recording) ------------------------------------ */
 /* Record attr of pop each step into a device buffer of `capacity` steps

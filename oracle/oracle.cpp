@@ -291,6 +291,68 @@ void orc_rate_run(int32_t N, const int64_t* row_begin, const int32_t* col_idx,
 }
 
 // ---------------------------------------------------------------------------
+// Rate-coded delays (Population/OpenMPTemplates.py:181-220 and
+// Projection/OpenMP/CSR.py:67-101, :962-963).  The pre population keeps a
+// queue of its last max_delay rate vectors, hist[k] = slot k of
+// `_delayed_r` (k = 0 is the newest); compute_psp reads hist[d-1][col] with
+// d the uniform delay or the synapse's delay, and after the population
+// updates the new rates are pushed to the front (push_front + pop_back).
+// The queue is zero at initialisation and holds the current rates after a
+// reset; the caller passes its content in hist (max_delay * n_pre, updated).
+//
+// orc_delayed_psp_run: psp of one projection for a given sequence of pre
+// rates (pre_seq[s] = rates at the end of step s), out[s] = psp of step s.
+// orc_rate_run_delayed: orc_rate_run's recurrent network with a delayed
+// projection.  delay == NULL means the uniform delay d_uniform (>= 1).
+// ---------------------------------------------------------------------------
+static void delayed_psp(int32_t n_post, const int64_t* row_begin, const int32_t* col_idx,
+                        const double* w, const int32_t* delay, int32_t d_uniform, int32_t n_pre,
+                        const double* hist, double* out) {
+    for (int32_t i = 0; i < n_post; i++) {
+        double sum = 0.0;
+        for (int64_t j = row_begin[i]; j < row_begin[i + 1]; j++) {
+            const int32_t d = delay ? delay[j] : d_uniform;
+            sum += w[j] * hist[(int64_t)(d - 1) * n_pre + col_idx[j]];
+        }
+        out[i] = sum;
+    }
+}
+
+static void push_front(double* hist, int32_t max_delay, int32_t n, const double* r) {
+    for (int32_t k = max_delay - 1; k > 0; k--)
+        std::memcpy(hist + (int64_t)k * n, hist + (int64_t)(k - 1) * n, sizeof(double) * n);
+    std::memcpy(hist, r, sizeof(double) * n);
+}
+
+void orc_delayed_psp_run(int32_t n_post, const int64_t* row_begin, const int32_t* col_idx,
+                         const double* w, const int32_t* delay, int32_t d_uniform, int32_t max_delay,
+                         int32_t n_pre, const double* pre_seq, double* hist, int32_t nsteps,
+                         double* out) {
+    for (int32_t s = 0; s < nsteps; s++) {
+        delayed_psp(n_post, row_begin, col_idx, w, delay, d_uniform, n_pre, hist, out + (int64_t)s * n_post);
+        push_front(hist, max_delay, n_pre, pre_seq + (int64_t)s * n_pre);
+    }
+}
+
+void orc_rate_run_delayed(int32_t N, const int64_t* row_begin, const int32_t* col_idx,
+                          const double* w, const int32_t* delay, int32_t d_uniform,
+                          int32_t max_delay, double* hist, double* r, double* sum_exc, double tau,
+                          double dt, const double* noise, int32_t nsteps, double* trace) {
+    for (int32_t s = 0; s < nsteps; s++) {
+        delayed_psp(N, row_begin, col_idx, w, delay, d_uniform, N, hist, sum_exc);
+        const double* rand_0 = noise ? noise + (int64_t)s * N : nullptr;
+        for (int32_t i = 0; i < N; i++) {
+            double rd = rand_0 ? rand_0[i] : 0.0;
+            double _r = (sum_exc[i] - r[i] + rd) / tau;
+            r[i] += dt * _r;
+            if (r[i] < 0.0) r[i] = 0.0;
+        }
+        push_front(hist, max_delay, N, r);
+        if (trace) std::memcpy(trace + (int64_t)s * N, r, sizeof(double) * N);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Spiking network step loop.
 //
 // One spiking population P ("main") with model selected by `model`, plus an
@@ -334,6 +396,8 @@ struct orc_proj {
     const int32_t* row_idx;  // post rank per synapse (column order)
     const double* w_col;     // weight per synapse in column order, or NULL
     double w_const;          // used when w_col == NULL
+    int32_t delay;           // uniform delay in steps (<= 1: none)
+    const int32_t* delay_col;  // per-synapse delays in steps (column order), or NULL
 };
 
 struct orc_spiking_cfg {
@@ -364,22 +428,62 @@ int64_t orc_spiking_run(const orc_spiking_cfg* c, double* v, double* g_exc, doub
     const int32_t N = c->N;
     std::vector<int32_t> spiked, in_spiked;
     std::vector<int16_t> in_ref(N, 1);
+    // Delays.  Uniform: the pre population keeps its last D spike lists,
+    // hist[0] = the list of the previous step, and the projection reads
+    // hist[d-1] (Population/OpenMPGenerator.py:433-450, `_delayed_spike`;
+    // Projection/OpenMPGenerator.py:1238-1249).  Per synapse: a ring of
+    // per-post event lists indexed by idx_delay, filled with (idx_delay + d-1)
+    // % max_delay and drained in post order, then idx_delay advances
+    // (Projection/OpenMP/LIL.py:946-1017).
+    int32_t Dp = 1, Dq = 1;
+    std::vector<int32_t> maxd(c->n_proj, 1), idx_delay(c->n_proj, 0);
+    for (int32_t q = 0; q < c->n_proj; q++) {
+        const orc_proj& pr = c->proj[q];
+        int32_t d = std::max(1, pr.delay);
+        if (pr.delay_col)
+            for (int64_t syn = 0; syn < pr.col_ptr[pr.num_cols]; syn++) d = std::max(d, pr.delay_col[syn]);
+        maxd[q] = d;
+        if (pr.pre_is_input) Dq = std::max(Dq, d); else Dp = std::max(Dp, d);
+    }
+    std::vector<std::vector<int32_t>> hist_p(Dp), hist_q(Dq);
+    std::vector<std::vector<std::vector<std::vector<int64_t>>>> ring(c->n_proj);
+    for (int32_t q = 0; q < c->n_proj; q++)
+        if (c->proj[q].delay_col) ring[q].assign(maxd[q], std::vector<std::vector<int64_t>>(N));
     std::vector<double> p(c->M > 0 ? c->M : 1, 0.0);
     int64_t t = c->t0;
     int64_t rec = 0;
     bool overflow = false;
     const double dt = c->dt;
     for (int32_t s = 0; s < nsteps; s++) {
-        // compute_psp: spikes emitted at the previous step
+        // compute_psp: spikes emitted at the previous step (or d-1 steps before)
         for (int32_t q = 0; q < c->n_proj; q++) {
             const orc_proj& pr = c->proj[q];
-            const std::vector<int32_t>& pre = pr.pre_is_input ? in_spiked : spiked;
+            const std::vector<std::vector<int32_t>>& hist = pr.pre_is_input ? hist_q : hist_p;
             double* g = pr.target == 0 ? g_exc : g_inh;
-            for (size_t k = 0; k < pre.size(); k++) {
-                int32_t _pre = pre[k];
-                if (_pre >= pr.num_cols) continue;
-                for (int64_t syn = pr.col_ptr[_pre]; syn < pr.col_ptr[_pre + 1]; syn++)
-                    g[pr.row_idx[syn]] += pr.w_col ? pr.w_col[syn] : pr.w_const;
+            if (!pr.delay_col) {
+                const std::vector<int32_t>& pre = hist[std::max(1, pr.delay) - 1];
+                for (size_t k = 0; k < pre.size(); k++) {
+                    int32_t _pre = pre[k];
+                    if (_pre >= pr.num_cols) continue;
+                    for (int64_t syn = pr.col_ptr[_pre]; syn < pr.col_ptr[_pre + 1]; syn++)
+                        g[pr.row_idx[syn]] += pr.w_col ? pr.w_col[syn] : pr.w_const;
+                }
+            } else {
+                const std::vector<int32_t>& pre = hist[0];
+                for (size_t k = 0; k < pre.size(); k++) {
+                    int32_t _pre = pre[k];
+                    if (_pre >= pr.num_cols) continue;
+                    for (int64_t syn = pr.col_ptr[_pre]; syn < pr.col_ptr[_pre + 1]; syn++) {
+                        const int32_t slot = (idx_delay[q] + pr.delay_col[syn] - 1) % maxd[q];
+                        ring[q][slot][pr.row_idx[syn]].push_back(syn);
+                    }
+                }
+                std::vector<std::vector<int64_t>>& now = ring[q][idx_delay[q]];
+                for (int32_t i = 0; i < N; i++) {
+                    for (int64_t syn : now[i]) g[i] += pr.w_col ? pr.w_col[syn] : pr.w_const;
+                    now[i].clear();
+                }
+                idx_delay[q] = (idx_delay[q] + 1) % maxd[q];
             }
         }
         if (trace_gexc) std::memcpy(trace_gexc + (int64_t)s * N, g_exc, sizeof(double) * N);
@@ -432,6 +536,11 @@ int64_t orc_spiking_run(const orc_spiking_cfg* c, double* v, double* g_exc, doub
             rec++;
         }
         if (trace_v) std::memcpy(trace_v + (int64_t)s * N, v, sizeof(double) * N);
+        // delay queues: push the lists of this step to the front
+        std::rotate(hist_p.rbegin(), hist_p.rbegin() + 1, hist_p.rend());
+        hist_p[0] = spiked;
+        std::rotate(hist_q.rbegin(), hist_q.rbegin() + 1, hist_q.rend());
+        hist_q[0] = in_spiked;
         t++;
     }
     return overflow ? -1 : rec;

@@ -29,6 +29,8 @@ class OrcProj(C.Structure):
         ("row_idx", _PI32),
         ("w_col", _PD),
         ("w_const", _D),
+        ("delay", _I32),
+        ("delay_col", _PI32),
     ]
 
 
@@ -63,6 +65,12 @@ def lib():
         if not LIB.exists():
             build()
         L = C.CDLL(str(LIB))
+        L.orc_delayed_psp_run.argtypes = [_I32, _PI64, _PI32, _PD, _PI32, _I32, _I32, _I32, _PD, _PD,
+                                          _I32, _PD]
+        L.orc_delayed_psp_run.restype = None
+        L.orc_rate_run_delayed.argtypes = [_I32, _PI64, _PI32, _PD, _PI32, _I32, _I32, _PD, _PD, _PD,
+                                           _D, _D, _PD, _I32, _PD]
+        L.orc_rate_run_delayed.restype = None
         L.orc_csr_from_lil.argtypes = [_I32, _I32, _PI32, _PI64, _PI32, _PI64, _PI32]
         L.orc_csr_from_lil.restype = _I64
         L.orc_csrc_inverse.argtypes = [_I32, _I32, _PI64, _PI32, _PI64, _PI32, _PI64]
@@ -235,6 +243,39 @@ def rate_run(row_ptr, col_idx, w, r0, tau, dt, noise, nsteps, trace=False, nthre
     return r, s, (tr.reshape(nsteps, n) if trace else None)
 
 
+def delayed_psp_run(row_ptr, col_idx, w, pre_seq, max_delay, delays=None, delay=1, hist=None):
+    """psp of one projection per step for rates pre_seq[s] (end of step s);
+    `hist` is the initial queue (max_delay x n_pre; zeros by default)."""
+    n_post = len(row_ptr) - 1
+    pre_seq = np.ascontiguousarray(pre_seq, dtype=np.float64)
+    nsteps, n_pre = pre_seq.shape
+    h = np.zeros((max_delay, n_pre)) if hist is None else np.array(hist, dtype=np.float64)
+    dl = None if delays is None else np.ascontiguousarray(delays, dtype=np.int32)
+    out = np.zeros((nsteps, n_post))
+    lib().orc_delayed_psp_run(n_post, _p(np.ascontiguousarray(row_ptr, dtype=np.int64), _I64),
+                              _p(np.ascontiguousarray(col_idx, dtype=np.int32), _I32),
+                              _p(np.ascontiguousarray(w, dtype=np.float64), _D), _p(dl, _I32), int(delay),
+                              int(max_delay), n_pre, _p(pre_seq, _D), _p(h, _D), nsteps, _p(out, _D))
+    return out, h
+
+
+def rate_run_delayed(row_ptr, col_idx, w, r0, tau, dt, noise, nsteps, max_delay, delays=None, delay=1,
+                     hist=None, trace=False):
+    n = len(row_ptr) - 1
+    h = np.zeros((max_delay, n)) if hist is None else np.array(hist, dtype=np.float64)
+    dl = None if delays is None else np.ascontiguousarray(delays, dtype=np.int32)
+    r = np.array(r0, dtype=np.float64)
+    s = np.zeros(n)
+    nz = None if noise is None else np.ascontiguousarray(noise, dtype=np.float64)
+    tr = np.zeros(nsteps * n) if trace else None
+    lib().orc_rate_run_delayed(n, _p(np.ascontiguousarray(row_ptr, dtype=np.int64), _I64),
+                               _p(np.ascontiguousarray(col_idx, dtype=np.int32), _I32),
+                               _p(np.ascontiguousarray(w, dtype=np.float64), _D), _p(dl, _I32), int(delay),
+                               int(max_delay), _p(h, _D), _p(r, _D), _p(s, _D), tau, dt, _p(nz, _D), nsteps,
+                               _p(tr, _D))
+    return r, s, (tr.reshape(nsteps, n) if trace else None)
+
+
 # ---- spiking path ----------------------------------------------------------------
 COBA_DEFAULTS = dict(El=-60.0, Vr=-60.0, Erev_exc=0.0, Erev_inh=-80.0, Vt=-50.0, tau=20.0,
                      tau_exc=5.0, tau_inh=10.0, I=20.0)
@@ -254,9 +295,11 @@ def spiking_run(model, N, dt, v, nsteps, projections=(), refractory=None, g_exc=
         cp = np.ascontiguousarray(pr["col_ptr"], dtype=np.int64)
         ri = np.ascontiguousarray(pr["row_idx"], dtype=np.int32)
         wc = None if pr.get("w_col") is None else np.ascontiguousarray(pr["w_col"], dtype=np.float64)
-        keep += [cp, ri, wc]
+        dc = None if pr.get("delay_col") is None else np.ascontiguousarray(pr["delay_col"], dtype=np.int32)
+        keep += [cp, ri, wc, dc]
         projs[k] = OrcProj(int(pr.get("pre_is_input", 0)), int(pr["target"]), int(pr["num_cols"]),
-                           _p(cp, _I64), _p(ri, _I32), _p(wc, _D), float(pr.get("w_const", 0.0)))
+                           _p(cp, _I64), _p(ri, _I32), _p(wc, _D), float(pr.get("w_const", 0.0)),
+                           int(pr.get("delay", 1)), _p(dc, _I32))
     cfg = OrcSpikingCfg()
     cfg.model = model
     cfg.N = N
@@ -295,12 +338,14 @@ def spiking_run(model, N, dt, v, nsteps, projections=(), refractory=None, g_exc=
                 trace_gexc=None if tg is None else tg.reshape(nsteps, N), input_counts=in_counts)
 
 
-def csrc_projection(csr, target, pre_is_input=False):
-    """Backward view of a connectivity.CSR as an oracle projection dict."""
+def csrc_projection(csr, target, pre_is_input=False, delay=1, delays=None):
+    """Backward view of a connectivity.CSR as an oracle projection dict;
+    `delays` (steps per synapse, forward CSR order) or a uniform `delay`."""
     col_ptr, row_idx, inv = csrc_inverse(csr.n_rows, csr.n_cols, csr.row_ptr, csr.col_idx)
     w_col = None if csr.w is None else np.asarray(csr.w)[inv]
+    d_col = None if delays is None else np.asarray(delays, dtype=np.int32)[inv]
     return dict(pre_is_input=int(pre_is_input), target=target, num_cols=csr.n_cols, col_ptr=col_ptr,
-                row_idx=row_idx, w_col=w_col, w_const=csr.w_const)
+                row_idx=row_idx, w_col=w_col, w_const=csr.w_const, delay=int(delay), delay_col=d_col)
 
 
 def sweep_instance_run(csr, w_in, phase, omega, gain, tau, dt, r0, nsteps, trace=False):

@@ -226,3 +226,103 @@ def test_mlp_forward_closed_form_and_loss():
     ce = np.mean(np.log(np.exp(zs).sum(axis=1)) - zs[np.arange(9), y])
     loss, _ = orc.mlp_loss_grad(X, y, P, grad=False)
     assert abs(loss - ce) < 1e-12
+
+
+# ---- delays -------------------------------------------------------------------
+def _kat_input_sequence(nsteps):
+    # input neuron `r = r + t : init = -1` (dt = 1): r at the end of step s
+    r, out = -1.0, []
+    for s in range(nsteps):
+        r = r + s
+        out.append(r)
+    return np.array(out)
+
+
+def test_rate_uniform_delay_kat():
+    # tests_python/RateSynapse/test_RateDelays.py:180-320: three posts, one
+    # pre each (w = 1), after reset() the queue holds the initial value -1
+    seq = np.repeat(_kat_input_sequence(20)[:, None], 3, axis=1)
+    rp, ci, w = np.array([0, 1, 2, 3]), np.array([2, 0, 1]), np.ones(3)
+    for d, (a, b, c) in ((10, (9, 10, 14)), (5, (4, 5, 9))):
+        out, _ = orc.delayed_psp_run(rp, ci, w, seq, d, delay=d, hist=np.full((d, 3), -1.0))
+        np.testing.assert_array_equal(out[a], -1.0)   # "the first steps: initial value"
+        np.testing.assert_array_equal(out[b], -1.0)   # "first queue value, t = 0"
+        np.testing.assert_array_equal(out[c], 9.0)    # "t = 4"
+
+
+def test_rate_nonuniform_delay_kat():
+    # test_RateDelays.py:330-400: delays [[3], [5], [2]] -> [0, -1, 2] after 5
+    # steps; [[3], [3], [3]] -> 20 after 10 steps
+    seq = np.repeat(_kat_input_sequence(20)[:, None], 3, axis=1)
+    rp, ci, w = np.array([0, 1, 2, 3]), np.array([0, 1, 2]), np.ones(3)
+    out, _ = orc.delayed_psp_run(rp, ci, w, seq, 5, delays=[3, 5, 2], hist=np.full((5, 3), -1.0))
+    np.testing.assert_array_equal(out[4], [0.0, -1.0, 2.0])
+    out, _ = orc.delayed_psp_run(rp, ci, w, seq, 5, delays=[3, 3, 3], hist=np.full((5, 3), -1.0))
+    np.testing.assert_array_equal(out[9], [20.0, 20.0, 20.0])
+
+
+def test_rate_delay_one_is_no_delay_and_queue_is_a_shift():
+    rng = np.random.default_rng(1)
+    csr = conn.random_in_degree(50, 40, rng, np.full(40, 7), conn.Normal(0, 1, rng=rng))
+    r0 = rng.random(50)
+    seq = rng.random((12, 50))
+    # delay 1 on every synapse reads the previous step's rates
+    h0 = np.zeros((4, 50)); h0[0] = r0
+    out, _ = orc.delayed_psp_run(csr.row_ptr, csr.col_idx, csr.w, seq, 4, delays=np.ones(csr.nnz, int), hist=h0)
+    prev = np.vstack([r0, seq[:-1]])
+    for s in range(12):
+        np.testing.assert_allclose(out[s], csr.dense() @ prev[s], rtol=1e-12, atol=1e-12)
+    # mixed delays: each synapse reads the rates d steps back
+    d = rng.integers(1, 5, csr.nnz)
+    out, _ = orc.delayed_psp_run(csr.row_ptr, csr.col_idx, csr.w, seq, 4, delays=d, hist=np.zeros((4, 50)))
+    rows = np.repeat(np.arange(40), np.diff(csr.row_ptr))
+    for s in range(4, 12):
+        ref = np.zeros(40)
+        np.add.at(ref, rows, csr.w * seq[s - d, csr.col_idx])
+        np.testing.assert_allclose(out[s], ref, rtol=1e-12, atol=1e-12)
+    # the recurrent run: delay 1 with the queue holding r0 is orc_rate_run;
+    # per-synapse delays all equal to 3 are the uniform delay 3
+    noise = rng.normal(0, 0.1, (30, 40))
+    sq = conn.random_in_degree(40, 40, rng, np.full(40, 6), conn.Normal(0, 0.3, rng=rng))
+    r0 = rng.random(40)
+    h = np.zeros((1, 40)); h[0] = r0
+    a = orc.rate_run_delayed(sq.row_ptr, sq.col_idx, sq.w, r0, 10.0, 1.0, noise, 30, 1, delay=1, hist=h)[0]
+    b = orc.rate_run(sq.row_ptr, sq.col_idx, sq.w, r0, 10.0, 1.0, noise, 30)[0]
+    np.testing.assert_array_equal(a, b)
+    c = orc.rate_run_delayed(sq.row_ptr, sq.col_idx, sq.w, r0, 10.0, 1.0, noise, 30, 3,
+                             delays=np.full(sq.nnz, 3))[0]
+    e = orc.rate_run_delayed(sq.row_ptr, sq.col_idx, sq.w, r0, 10.0, 1.0, noise, 30, 3, delay=3)[0]
+    np.testing.assert_array_equal(c, e)
+    assert not np.array_equal(c, b)
+
+
+def _emitter_projection(n_pre, n_post, w, delays=None, delay=1):
+    csr = conn.all_to_all(np.arange(n_pre), np.arange(n_post), n_pre, n_post, w, True)
+    return csr, orc.csrc_projection(csr, 0, pre_is_input=True, delay=delay, delays=delays)
+
+
+def _emit_at_step1(steps, n):
+    # Poisson rule p = 1000*u/dt < rates with rates = 1: u = 0 spikes, u = 1 does not
+    u = np.ones((steps, n))
+    u[1] = 0.0
+    return dict(M=n, rates=1.0, uniforms=u)
+
+
+def test_spiking_uniform_delay_kat():
+    # test_SpikingTransmission.py:86-163: 5 emitters fire at t == 1, all-to-all
+    # w = 1 with delay 3 -> g_exc == 5 at step 4 only.  Receivers: COBA with
+    # tau_exc = dt, so g_exc returns to 0 in every update (`g_exc = 0`).
+    csr, pr = _emitter_projection(5, 2, 1.0, delay=3)
+    ref = orc.spiking_run(0, 2, 1.0, np.full(2, -60.0), 6, [pr], params=dict(tau_exc=1.0, I=0.0),
+                          poisson=_emit_at_step1(6, 5), trace=True)
+    np.testing.assert_array_equal(ref["trace_gexc"], [[0, 0], [0, 0], [0, 0], [0, 0], [5, 5], [0, 0]])
+
+
+def test_spiking_nonuniform_delay_kat():
+    # test_SpikingTransmission.py:166-253: delays [[1,2,3,2,1],[3,2,1,2,3]]
+    csr, _ = _emitter_projection(5, 2, 1.0)
+    d = np.array([[1, 2, 3, 2, 1], [3, 2, 1, 2, 3]]).ravel()   # forward CSR order (post-major)
+    pr = orc.csrc_projection(csr, 0, pre_is_input=True, delays=d)
+    ref = orc.spiking_run(0, 2, 1.0, np.full(2, -60.0), 5, [pr], params=dict(tau_exc=1.0, I=0.0),
+                          poisson=_emit_at_step1(5, 5), trace=True)
+    np.testing.assert_array_equal(ref["trace_gexc"], [[0, 0], [0, 0], [2, 1], [2, 2], [1, 2]])
