@@ -23,6 +23,14 @@ FORMATS = {"csr": 0, "ell": 1, "ellr": 2, "sell": 3, "bsr": 4, "dense": 5, "csr_
 FORMAT_NAMES = {v: k for k, v in FORMATS.items()}
 
 
+class ConnectSpec(C.Structure):
+    """ahip_connect_spec (include/annarchy_hip.h)."""
+    _fields_ = [("kind", C.c_int32), ("number", C.c_int32), ("probability", C.c_double),
+                ("pre_begin", C.c_int32), ("pre_end", C.c_int32), ("post_begin", C.c_int32),
+                ("post_end", C.c_int32), ("allow_self", C.c_int32), ("w_kind", C.c_int32),
+                ("w_a", C.c_double), ("w_b", C.c_double), ("seed", C.c_uint64)]
+
+
 class AhipError(RuntimeError):
     pass
 
@@ -88,6 +96,8 @@ _SIGS = {
     "ahip_proj_set_launch": ([_P, _I32, _I32], _I32),
     "ahip_proj_set_delays": ([_P, _I32, _PI32, _I64], _I32),
     "ahip_proj_set_max_delay": ([_P, _I32, _I32], _I32),
+    "ahip_proj_generate": ([_P, _I32, C.c_void_p, C.c_void_p], _I32),
+    "ahip_proj_get_csr": ([_P, _I32, _PI64, _PI32, _I64], _I32),
     "ahip_proj_get_delays": ([_P, _I32, _PI32, _PI32, _PI32, _I64], _I32),
     "ahip_monitor_create": ([_P, _I32, C.c_char_p, _I64, _PI32], _I32),
     "ahip_monitor_steps": ([_P, _I32, _PI64], _I32),

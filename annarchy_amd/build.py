@@ -40,6 +40,7 @@ CORE_SOURCES = [
     "k_batched.hip",
     "k_mlp.hip",
     "k_delay.hip",
+    "k_connect.hip",
 ]
 
 COMMON_FLAGS = [

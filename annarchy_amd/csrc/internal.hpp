@@ -190,6 +190,13 @@ int launch_rate_psp_delayed(const ahip_net* net, const Proj& p, const Pop& pre, 
                             hipStream_t s);
 int launch_spike_ring(const ahip_net* net, const Proj& p, const Pop& pre, void* target, int32_t step,
                       hipStream_t s);
+int launch_gen_fixed_number_pre(int32_t post_begin, int32_t post_end, int32_t pre_begin, int32_t n_pre,
+                                int32_t number, int32_t no_self, uint64_t seed, const int64_t* row_ptr,
+                                int32_t* col, double* w, int32_t w_kind, double wa, double wb, hipStream_t s);
+int launch_gen_fixed_probability(int32_t post_begin, int32_t post_end, int32_t pre_begin, int32_t n_pre,
+                                 double p, int32_t no_self, uint64_t seed, int32_t pass, int64_t* counts,
+                                 const int64_t* row_ptr, int32_t* col, double* w, int32_t w_kind, double wa,
+                                 double wb, hipStream_t s);
 int launch_advance_time(int64_t* t, int64_t n, hipStream_t s);
 int launch_fill(void* dst, int64_t n, int32_t elem, double value, hipStream_t s);
 int launch_fill_i32(int32_t* dst, int64_t n, int32_t value, hipStream_t s);

@@ -932,6 +932,77 @@ int ahip_proj_init_csr(ahip_net* net, int32_t proj, int32_t n_rows, int32_t n_co
     return 0;
 }
 
+int ahip_proj_generate(ahip_net* net, int32_t proj, const ahip_connect_spec* c, const ahip_format_opts* opts) {
+    Proj* p = get_proj(net, proj);
+    if (!p || !c) return fail(AHIP_EINVAL, "bad projection or spec");
+    Pop& pre = *net->pops[p->pre];
+    Pop& post = *net->pops[p->post];
+    if (c->pre_begin < 0 || c->pre_end > pre.size || c->pre_begin > c->pre_end || c->post_begin < 0 ||
+        c->post_end > post.size || c->post_begin > c->post_end)
+        return fail(AHIP_EINVAL, "rank ranges outside the populations");
+    if (c->w_kind < 0 || c->w_kind > 2) return fail(AHIP_EINVAL, "bad weight kind");
+    const int32_t n_pre = c->pre_end - c->pre_begin;
+    const int32_t no_self = (p->pre == p->post && !c->allow_self) ? 1 : 0;
+    const int32_t n_rows = post.size;
+    std::vector<int64_t> row_ptr(n_rows + 1, 0);
+    HIPCHK(hipSetDevice(net->device));
+    HIPCHK(hipStreamSynchronize(net->stream));
+    DevBuf d_rp, d_counts, d_col, d_w;
+    int rc;
+    if (c->kind == AHIP_CONN_FIXED_NUMBER_PRE) {
+        if (c->number < 0 || c->number > 2016) return fail(AHIP_EINVAL, "number must be 0..2016");
+        if (c->number > n_pre - no_self)
+            return fail(AHIP_EINVAL, "number (%d) exceeds the %d pre-synaptic candidates", c->number, n_pre - no_self);
+        for (int32_t r = 0; r < n_rows; r++)
+            row_ptr[r + 1] = row_ptr[r] + ((r >= c->post_begin && r < c->post_end) ? c->number : 0);
+    } else if (c->kind == AHIP_CONN_FIXED_PROBABILITY) {
+        if (!(c->probability >= 0.0 && c->probability <= 1.0)) return fail(AHIP_EINVAL, "probability outside [0, 1]");
+        if ((rc = dev_alloc(d_counts, (size_t)n_rows * sizeof(int64_t)))) return rc;
+        LCHK(launch_gen_fixed_probability(c->post_begin, c->post_end, c->pre_begin, n_pre, c->probability, no_self,
+                                          c->seed, 0, d_counts.as<int64_t>(), nullptr, nullptr, nullptr, 0, 0.0,
+                                          0.0, net->stream));
+        std::vector<int64_t> counts(n_rows);
+        HIPCHK(hipStreamSynchronize(net->stream));
+        HIPCHK(hipMemcpy(counts.data(), d_counts.p, counts.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+        for (int32_t r = 0; r < n_rows; r++) row_ptr[r + 1] = row_ptr[r] + counts[r];
+    } else {
+        return fail(AHIP_EINVAL, "unknown connector kind");
+    }
+    const int64_t nnz = row_ptr[n_rows];
+    if ((rc = dev_alloc(d_rp, row_ptr.size() * sizeof(int64_t)))) return rc;
+    HIPCHK(hipMemcpy(d_rp.p, row_ptr.data(), row_ptr.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    if ((rc = dev_alloc(d_col, (size_t)std::max<int64_t>(nnz, 1) * sizeof(int32_t)))) return rc;
+    if (c->w_kind != 0 && (rc = dev_alloc(d_w, (size_t)std::max<int64_t>(nnz, 1) * sizeof(double)))) return rc;
+    if (c->kind == AHIP_CONN_FIXED_NUMBER_PRE)
+        LCHK(launch_gen_fixed_number_pre(c->post_begin, c->post_end, c->pre_begin, n_pre, c->number, no_self, c->seed,
+                                         d_rp.as<int64_t>(), d_col.as<int32_t>(), d_w.as<double>(), c->w_kind, c->w_a,
+                                         c->w_b, net->stream));
+    else
+        LCHK(launch_gen_fixed_probability(c->post_begin, c->post_end, c->pre_begin, n_pre, c->probability, no_self,
+                                          c->seed, 1, nullptr, d_rp.as<int64_t>(), d_col.as<int32_t>(),
+                                          d_w.as<double>(), c->w_kind, c->w_a, c->w_b, net->stream));
+    HIPCHK(hipStreamSynchronize(net->stream));
+    std::vector<int32_t> col(nnz);
+    std::vector<double> w(c->w_kind != 0 ? nnz : 0);
+    if (nnz) HIPCHK(hipMemcpy(col.data(), d_col.p, nnz * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (nnz && c->w_kind != 0) HIPCHK(hipMemcpy(w.data(), d_w.p, nnz * sizeof(double), hipMemcpyDeviceToHost));
+    d_col.release();
+    d_w.release();
+    return ahip_proj_init_csr(net, proj, n_rows, pre.size, row_ptr.data(), col.data(),
+                              c->w_kind != 0 ? w.data() : nullptr, c->w_a, opts);
+}
+
+int ahip_proj_get_csr(ahip_net* net, int32_t proj, int64_t* row_ptr, int32_t* col, int64_t nnz) {
+    Proj* p = get_proj(net, proj);
+    if (!p || p->nnz < 0) return fail(AHIP_EINVAL, "bad projection");
+    if (row_ptr) std::memcpy(row_ptr, p->h_row_ptr.data(), p->h_row_ptr.size() * sizeof(int64_t));
+    if (col) {
+        if (nnz < p->nnz) return fail(AHIP_EINVAL, "buffer too small (%lld synapses)", (long long)p->nnz);
+        std::memcpy(col, p->h_col.data(), p->nnz * sizeof(int32_t));
+    }
+    return 0;
+}
+
 int ahip_proj_set_format(ahip_net* net, int32_t proj, const ahip_format_opts* opts) {
     Proj* p = get_proj(net, proj);
     if (!p || !opts || p->nnz < 0) return fail(AHIP_EINVAL, "bad projection");

@@ -202,6 +202,7 @@ class Projection:
         self.pre, self.post, self.target = pre, post, target
         self.id = -1
         self._connector = None
+        self._generator = "numpy"
         self._delays = 0.0
         self._format = _lib.opts("csr", 64)
         self.csr: conn.CSR | None = None
@@ -215,14 +216,20 @@ class Projection:
         return self
 
     def fixed_probability(self, probability, weights, delays=0.0, allow_self_connections=False,
-                          storage_format=None, **fmt):
+                          storage_format=None, generator="numpy", **fmt):
+        """generator="numpy" replays the reference's numpy calls
+        (Connector.pyx:306-345); "device" draws the same distribution on the
+        GPU from a counter-based stream (ahip_proj_generate)."""
         self._delays = delays
+        self._generator = generator
         return self._store("fixed_probability", probability, weights, allow_self_connections,
                            storage_format=storage_format, **fmt)
 
     def fixed_number_pre(self, number, weights, delays=0.0, allow_self_connections=False,
-                         storage_format=None, **fmt):
+                         storage_format=None, generator="numpy", **fmt):
+        """generator as for fixed_probability (Connector.pyx:347-380)."""
         self._delays = delays
+        self._generator = generator
         if number > _sizes(self.pre):
             raise ValueError("fixed_number_pre: the number of pre-synaptic neurons exceeds the population size")
         return self._store("fixed_number_pre", number, weights, allow_self_connections,
@@ -310,6 +317,51 @@ class Projection:
             return args[0]
         raise ValueError(kind)
 
+    @staticmethod
+    def _rank_range(obj):
+        r = np.asarray(obj.ranks)
+        if r.size and not np.array_equal(r, np.arange(r[0], r[0] + r.size)):
+            raise ValueError("device-generated connectivity needs contiguous rank ranges")
+        return (int(r[0]), int(r[0] + r.size)) if r.size else (0, 0)
+
+    def _generate_on_device(self):
+        kind, args = self._connector
+        if kind not in ("fixed_number_pre", "fixed_probability"):
+            raise ValueError(f"{kind} has no device generator")
+        spec = _lib.ConnectSpec()
+        value, w, allow = args
+        if kind == "fixed_number_pre":
+            spec.kind, spec.number = 0, int(value)
+        else:
+            spec.kind, spec.probability = 1, float(value)
+        spec.pre_begin, spec.pre_end = self._rank_range(self.pre)
+        spec.post_begin, spec.post_end = self._rank_range(self.post)
+        spec.allow_self = int(bool(allow) or self.pre is not self.post)
+        if isinstance(w, conn.Uniform):
+            spec.w_kind, spec.w_a, spec.w_b = 1, float(w.min), float(w.max)
+        elif isinstance(w, conn.Normal) and w.min is None and w.max is None:
+            spec.w_kind, spec.w_a, spec.w_b = 2, float(w.mu), float(w.sigma)
+        elif isinstance(w, conn.RandomDistribution):
+            raise ValueError("device-generated weights: a constant, Uniform or Normal without bounds")
+        else:
+            spec.w_kind, spec.w_a = 0, float(w)
+        spec.seed = int(self.net.rng.integers(1 << 63))
+        L = _lib.lib()
+        _lib.check(L.ahip_proj_generate(self.net._h, self.id, C.byref(spec), C.byref(self._format)))
+        nnz = C.c_int64()
+        _lib.check(L.ahip_proj_nb_synapses(self.net._h, self.id, C.byref(nnz)))
+        post_b, pre_b = _base(self.post), _base(self.pre)
+        rp = np.zeros(post_b.size + 1, dtype=np.int64)
+        col = np.zeros(max(1, nnz.value), dtype=np.int32)
+        _lib.check(L.ahip_proj_get_csr(self.net._h, self.id, _lib.ptr(rp, C.c_int64), _lib.ptr(col, C.c_int32),
+                                       col.size))
+        if spec.w_kind == 0:
+            self.csr = conn.CSR(post_b.size, pre_b.size, rp, col[:nnz.value], None, w_const=spec.w_a)
+        else:
+            wv = np.empty(nnz.value)
+            _lib.check(L.ahip_proj_get_w(self.net._h, self.id, _lib.ptr(wv, C.c_double), wv.size))
+            self.csr = conn.CSR(post_b.size, pre_b.size, rp, col[:nnz.value], wv)
+
     def _instantiate(self):
         L = _lib.lib()
         pre_b, post_b = _base(self.pre), _base(self.post)
@@ -318,13 +370,16 @@ class Projection:
         _lib.check(L.ahip_proj_create(self.net._h, pre_b.id, post_b.id, self.target.encode(),
                                       int(spiking), C.byref(pid)))
         self.id = pid.value
-        self.csr = self._build_csr()
-        c = self.csr
-        w = c.w
-        _lib.check(L.ahip_proj_init_csr(self.net._h, self.id, c.n_rows, c.n_cols,
-                                        _lib.ptr(c.row_ptr, C.c_int64), _lib.ptr(c.col_idx, C.c_int32),
-                                        _lib.ptr(w, C.c_double) if w is not None else None,
-                                        c.w_const, C.byref(self._format)))
+        if self._generator == "device":
+            self._generate_on_device()
+        else:
+            self.csr = self._build_csr()
+            c = self.csr
+            w = c.w
+            _lib.check(L.ahip_proj_init_csr(self.net._h, self.id, c.n_rows, c.n_cols,
+                                            _lib.ptr(c.row_ptr, C.c_int64), _lib.ptr(c.col_idx, C.c_int32),
+                                            _lib.ptr(w, C.c_double) if w is not None else None,
+                                            c.w_const, C.byref(self._format)))
         if not (np.isscalar(self._delays) and float(self._delays) <= self.net.dt):
             self._upload_delays(self._delays)
 

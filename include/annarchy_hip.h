@@ -143,6 +143,34 @@ int ahip_proj_create(ahip_net* net, int32_t pre, int32_t post, const char* targe
 int ahip_proj_init_csr(ahip_net* net, int32_t proj, int32_t n_rows, int32_t n_cols,
                        const int64_t* row_ptr, const int32_t* col_idx, const double* w,
                        double w_const, const ahip_format_opts* opts);
+/* Connectivity drawn on the device (no counterpart call in the reference,
+ * whose connectors draw on the host, cython_ext/Connector.pyx:306-380; same
+ * distributions, a counter-based random stream instead of numpy's).
+ * fixed_number_pre: `number` distinct pre ranks per post neuron (at most
+ * 2016); fixed_probability: each pre rank with probability p.  Pre ranks
+ * [pre_begin, pre_end) of the pre population connect to post ranks
+ * [post_begin, post_end); when pre and post are the same population and
+ * allow_self is 0, rank i does not connect to itself.  Weights: w_kind 0 =
+ * the constant w_a, 1 = Uniform(w_a, w_b), 2 = Normal(w_a, w_b).  The result
+ * replaces the projection's connectivity as ahip_proj_init_csr would. */
+#define AHIP_CONN_FIXED_NUMBER_PRE 0
+#define AHIP_CONN_FIXED_PROBABILITY 1
+typedef struct ahip_connect_spec {
+    int32_t kind;
+    int32_t number;
+    double probability;
+    int32_t pre_begin, pre_end;
+    int32_t post_begin, post_end;
+    int32_t allow_self;
+    int32_t w_kind;
+    double w_a, w_b;
+    uint64_t seed;
+} ahip_connect_spec;
+int ahip_proj_generate(ahip_net* net, int32_t proj, const ahip_connect_spec* spec,
+                       const ahip_format_opts* opts);
+/* The forward CSR view of a projection: row_ptr (n_rows + 1) and col (nnz);
+ * either may be NULL. */
+int ahip_proj_get_csr(ahip_net* net, int32_t proj, int64_t* row_ptr, int32_t* col, int64_t nnz);
 /* Re-layout an initialised rate projection in another format. */
 int ahip_proj_set_format(ahip_net* net, int32_t proj, const ahip_format_opts* opts);
 int ahip_proj_get_format(ahip_net* net, int32_t proj, ahip_format_opts* opts);
