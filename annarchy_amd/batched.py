@@ -153,7 +153,9 @@ class BatchedSweep:
             raise _lib.AhipError(_lib.lib().ahip_batch_last_error().decode())
 
     def set_mode(self, mode: int):
-        """1: instance rates staged in LDS (default when they fit); 0: global gathers."""
+        """2: sliced chunks, one lane per row (default when every instance has
+        fewer than 16384 neurons); 1: instance rates staged in LDS, L lanes per
+        row; 0: gathers from global memory."""
         self._check(_lib.lib().ahip_batch_set_mode(self._h, int(mode)))
 
     def run(self, steps: int, graph_batch: int = 50):
@@ -177,7 +179,9 @@ class BatchedSweep:
     def info(self):
         nnz, total, sb, df = C.c_int64(), C.c_int32(), C.c_double(), C.c_double()
         self._check(_lib.lib().ahip_batch_info(self._h, C.byref(nnz), C.byref(total), C.byref(sb), C.byref(df)))
-        return dict(nnz=nnz.value, total=total.value, sparse_bytes=sb.value, dense_flops=df.value)
+        mode = C.c_int32()
+        self._check(_lib.lib().ahip_batch_get_mode(self._h, C.byref(mode)))
+        return dict(nnz=nnz.value, total=total.value, sparse_bytes=sb.value, dense_flops=df.value, mode=mode.value)
 
     def __del__(self):
         try:

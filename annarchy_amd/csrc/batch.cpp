@@ -208,17 +208,82 @@ int ahip_batch_set_recurrent(ahip_batch* b, const int64_t* row_ptr, const int32_
     if ((rc = balloc(B.chunks, ch.size() * 4, ch.data()))) return rc;
     if ((rc = balloc(B.inst_base, b->offsets.size() * 8, b->offsets.data()))) return rc;
     B.mode = (size_t)max_n * sizeof(float) <= 64 * 1024 ? 1 : 0;
+    // mode 2: chunks of <= 512 rows of one instance, rows sorted by length
+    // (descending, stable) into 64-row slices, width rounded up to 4
+    if (max_n < 65535 && (size_t)(max_n + 1) * sizeof(float) <= 64 * 1024) {
+        std::vector<int32_t> sch, sch_sl(1, 0), sl_row;
+        std::vector<int64_t> sl_off(1, 0);
+        std::vector<int32_t> order;
+        for (int32_t i = 0; i < B.n_inst; i++) {
+            for (int64_t r0 = b->offsets[i]; r0 < b->offsets[i + 1]; r0 += 512) {
+                const int64_t r1 = std::min<int64_t>(r0 + 512, b->offsets[i + 1]);
+                order.resize(r1 - r0);
+                for (int64_t r = r0; r < r1; r++) order[r - r0] = (int32_t)r;
+                std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t c) {
+                    return row_ptr[a + 1] - row_ptr[a] > row_ptr[c + 1] - row_ptr[c];
+                });
+                for (size_t q = 0; q < order.size(); q += 64) {
+                    const int64_t len0 = row_ptr[order[q] + 1] - row_ptr[order[q]];
+                    for (size_t l = 0; l < 64; l++) sl_row.push_back(q + l < order.size() ? order[q + l] : -1);
+                    sl_off.push_back(sl_off.back() + (len0 + 3) / 4 * 4 * 64);
+                }
+                sch.push_back(i);
+                sch.push_back((int32_t)r0);
+                sch.push_back((int32_t)r1);
+                sch_sl.push_back((int32_t)(sl_off.size() - 1));
+            }
+        }
+        const int64_t stored = sl_off.back();
+        std::vector<uint16_t> scol(stored);
+        std::vector<float> sw(stored, 0.0f);
+        const size_t nsl = sl_off.size() - 1;
+        for (size_t sl = 0; sl < nsl; sl++) {
+            // padding: column = the instance size (the zero slot), weight 0
+            const int32_t r_any = sl_row[sl * 64];
+            const int32_t ii = inst[r_any];
+            const uint16_t pad = (uint16_t)b->sizes[ii];
+            for (int64_t e = sl_off[sl]; e < sl_off[sl + 1]; e++) scol[e] = pad;
+            for (int32_t l = 0; l < 64; l++) {
+                const int32_t row = sl_row[sl * 64 + l];
+                if (row < 0) continue;
+                const int64_t lo = b->offsets[ii];
+                for (int64_t j = row_ptr[row], k = 0; j < row_ptr[row + 1]; j++, k++) {
+                    const int64_t pos = sl_off[sl] + (k / 4) * 256 + 4 * l + k % 4;
+                    scol[pos] = (uint16_t)(col[j] - lo);
+                    sw[pos] = wf[j];
+                }
+            }
+        }
+        B.n_schunks = (int32_t)(sch.size() / 3);
+        B.sl_stored = stored;
+        if ((rc = balloc(B.schunks, sch.size() * 4, sch.data()))) return rc;
+        if ((rc = balloc(B.schunk_slices, sch_sl.size() * 4, sch_sl.data()))) return rc;
+        if ((rc = balloc(B.sl_off, sl_off.size() * 8, sl_off.data()))) return rc;
+        if ((rc = balloc(B.sl_row, sl_row.size() * 4, sl_row.data()))) return rc;
+        if ((rc = balloc(B.sl_col, (size_t)std::max<int64_t>(stored, 1) * 2, stored ? scol.data() : nullptr))) return rc;
+        if ((rc = balloc(B.sl_w, (size_t)std::max<int64_t>(stored, 1) * 4, stored ? sw.data() : nullptr))) return rc;
+        B.mode = 2;
+    }
     b->ready_sparse = true;
     return 0;
 }
 
 int ahip_batch_set_mode(ahip_batch* b, int32_t mode) {
-    if (!b || (mode != 0 && mode != 1)) return bfail(AHIP_EINVAL, "mode must be 0 (global gathers) or 1 (LDS)");
+    if (!b || mode < 0 || mode > 2)
+        return bfail(AHIP_EINVAL, "mode must be 0 (global gathers), 1 (LDS) or 2 (sliced chunks)");
     if (mode == 1 && (size_t)b->B.max_n * sizeof(float) > 64 * 1024)
         return bfail(AHIP_EINVAL, "an instance is too large for the LDS kernel");
+    if (mode == 2 && b->B.n_schunks == 0)
+        return bfail(AHIP_EINVAL, "no sliced layout (an instance is larger than 16383 neurons)");
     BCHK(hipStreamSynchronize(b->stream));
     drop_graph(b);
     b->B.mode = mode;
+    return 0;
+}
+
+int ahip_batch_get_mode(ahip_batch* b, int32_t* mode) {
+    if (!b || !mode) return bfail(AHIP_EINVAL, "null argument");
+    *mode = b->B.mode;
     return 0;
 }
 

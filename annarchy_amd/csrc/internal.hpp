@@ -225,10 +225,17 @@ struct BatchDev {
     DevBuf phase;                    // double [n_in][n_inst]
     DevBuf y;                        // float [m_tiles*32][n_tiles*32]
     DevBuf t_dev;                    // int64
-    int32_t mode = 1;                // 0: gathers from global memory, 1: instance rates staged in LDS
+    int32_t mode = 1;                // 0: gathers from global memory, 1: instance rates staged in LDS,
+                                     // 2: sliced chunks (one lane per row, uint16 columns)
     int32_t n_chunks = 0, max_n = 0;
     DevBuf chunks;                   // int32 [n_chunks][3] = (instance, first row, end row)
     DevBuf inst_base;                // int64 [n_inst + 1]
+    // mode 2: chunks of <= 512 rows, their 64-row slices and the sliced arrays
+    int32_t n_schunks = 0;
+    int64_t sl_stored = 0;
+    DevBuf schunks, schunk_slices;   // int32 [n][3], int32 [n + 1]
+    DevBuf sl_off, sl_row;           // int64 [slices + 1], int32 [slices][64] (-1: no row)
+    DevBuf sl_col, sl_w;             // uint16 / float [sl_stored]
 };
 int batch_launch_input(const BatchDev& B, int32_t step, hipStream_t s);
 int batch_launch_gemm(const BatchDev& B, hipStream_t s);

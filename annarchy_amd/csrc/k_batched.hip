@@ -189,6 +189,78 @@ __global__ __launch_bounds__(256) void batch_spmv_lds(
     }
 }
 
+// Sliced chunks (mode 2): a chunk holds up to 512 rows of one instance,
+// sorted by length and packed into slices of 64 rows, one lane per row.  Step
+// m of lane l is the 4 synapses at off + 256*m + 4*l: columns as uint16
+// relative to the instance (padding points at the zero slot xs[n]), weights
+// as one 16-byte load.  With U steps in flight and no cross-lane reduction,
+// the lane that owns a row finishes its Kahan sum and applies the update.
+typedef unsigned short v4u16_b __attribute__((ext_vector_type(4)));
+
+template <int U>
+__global__ __launch_bounds__(512) void batch_sell(
+    const int32_t* __restrict__ chunks, const int32_t* __restrict__ chunk_slices,
+    const int64_t* __restrict__ inst_base, const int64_t* __restrict__ off, const int32_t* __restrict__ slice_row,
+    const uint16_t* __restrict__ col, const float* __restrict__ w, const float* __restrict__ gain,
+    const float* __restrict__ y, int32_t ld_y, int32_t n_in, const float* __restrict__ r_in,
+    float* __restrict__ r_out, float* __restrict__ sum_out, float dt, float tau) {
+    extern __shared__ float xs[];
+    const int32_t inst = chunks[3 * blockIdx.x];
+    const int64_t base = inst_base[inst];
+    const int32_t n = (int32_t)(inst_base[inst + 1] - base);
+    for (int32_t k = threadIdx.x; k < n; k += 512) xs[k] = r_in[base + k];
+    if (threadIdx.x == 0) xs[n] = 0.0f;
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int32_t s_end = chunk_slices[blockIdx.x + 1];
+    for (int32_t sl = chunk_slices[blockIdx.x] + wave; sl < s_end; sl += 8) {
+        const int64_t b = off[sl];
+        const int64_t nsteps = (off[sl + 1] - b) >> 8;
+        const uint16_t* cp = col + b + 4 * lane;
+        const float* wp = w + b + 4 * lane;
+        Kahan k0{0.f, 0.f}, k1{0.f, 0.f};
+        int64_t m = 0;
+        for (; m + U <= nsteps; m += U) {
+            v4u16_b c[U];
+            v4f_b v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                c[u] = __builtin_nontemporal_load(reinterpret_cast<const v4u16_b*>(cp + (m + u) * 256));
+                v[u] = __builtin_nontemporal_load(reinterpret_cast<const v4f_b*>(wp + (m + u) * 256));
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                k0.add(v[u].x * xs[c[u].x]);
+                k1.add(v[u].y * xs[c[u].y]);
+                k0.add(v[u].z * xs[c[u].z]);
+                k1.add(v[u].w * xs[c[u].w]);
+            }
+        }
+        for (; m < nsteps; m++) {
+            const v4u16_b c = __builtin_nontemporal_load(reinterpret_cast<const v4u16_b*>(cp + m * 256));
+            const v4f_b v = __builtin_nontemporal_load(reinterpret_cast<const v4f_b*>(wp + m * 256));
+            k0.add(v.x * xs[c.x]);
+            k1.add(v.y * xs[c.y]);
+            k0.add(v.z * xs[c.z]);
+            k1.add(v.w * xs[c.w]);
+        }
+        k0.add(k1.value());
+        const int32_t row = slice_row[(int64_t)sl * kWave + lane];
+        if (row >= 0) {
+            const float sum = k0.value();
+            const int32_t nl = (int32_t)(row - base);
+            const float s_in = nl < n_in ? y[(int64_t)nl * ld_y + inst] : 0.0f;
+            const float r = xs[nl];
+            const float _r = (sum + gain[inst] * s_in - r) / tau;
+            float rn = r + dt * _r;
+            if (rn < 0.0f) rn = 0.0f;
+            r_out[row] = rn;
+            sum_out[row] = sum;
+        }
+    }
+}
+
 __global__ void batch_advance(int64_t* t, int64_t n) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *t += n;
 }
@@ -218,6 +290,15 @@ int batch_launch_spmv(const BatchDev& B, int32_t parity, hipStream_t s) {
     const int64_t nb = (threads + 255) / 256;
     const float* rin = B.r[parity].as<float>();
     float* rout = B.r[parity ^ 1].as<float>();
+    if (B.mode == 2) {
+        const size_t lds = (size_t)(B.max_n + 1) * sizeof(float);
+        hipLaunchKernelGGL(batch_sell<2>, dim3(B.n_schunks), dim3(512), lds, s, B.schunks.as<int32_t>(),
+                           B.schunk_slices.as<int32_t>(), B.inst_base.as<int64_t>(), B.sl_off.as<int64_t>(),
+                           B.sl_row.as<int32_t>(), B.sl_col.as<uint16_t>(), B.sl_w.as<float>(), B.gain.as<float>(),
+                           B.y.as<float>(), B.n_tiles * 32, B.n_in, rin, rout, B.sum_exc.as<float>(), (float)B.dt,
+                           (float)B.tau);
+        return (int)hipGetLastError();
+    }
     if (B.mode == 1) {
         const size_t lds = (size_t)B.max_n * sizeof(float);
 #define AHIP_BLDS_CASE(N)                                                                         \

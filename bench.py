@@ -280,30 +280,26 @@ def bench_c0(args):
 
 
 def bench_c2(args):
-    """COBA LIF, 400k neurons (320k E / 80k I), in-degree 1000 (800 E + 200 I),
-    4e8 synapses with per-synapse weights, Poisson 5 Hz into 10%."""
+    """COBA LIF, 400k neurons (320k E / 80k I), fixed_number_pre 1000 (800 E +
+    200 I) drawn on the device, 4e8 synapses with per-synapse weights,
+    Poisson 5 Hz into 10%."""
     from annarchy_amd import connectivity as cn
 
     steps = args.steps if args.steps != 200 else 2000
     N, ne = 400_000, 320_000
     scale = 80.0 / 1000.0  # same mean input conductance as the 4000-neuron benchmark
     prec = args.precision
-    t_b = time.perf_counter()
-    rng = np.random.default_rng(99)
-    we = cn.random_in_degree(ne, N, rng, np.full(N, 800), 0.6 * scale)
-    wi = cn.random_in_degree(N - ne, N, rng, np.full(N, 200), 6.7 * scale)
-    # per-synapse weights (4 bytes each in FP32), equal values
-    we.w = np.full(we.nnz, 0.6 * scale)
-    wi.w = np.full(wi.nnz, 6.7 * scale)
-    # pre columns are ranks of the whole population: shift the inhibitory ones
-    wi = cn.CSR(N, N, wi.row_ptr, (wi.col_idx + ne).astype(np.int32), wi.w)
-    we = cn.CSR(N, N, we.row_ptr, we.col_idx, we.w)
-    log(f"c2 connectivity {we.nnz + wi.nnz} synapses in {time.perf_counter() - t_b:.0f} s")
+    we, wi = 0.6 * scale, 6.7 * scale
 
     def conn(net, P, ne_):
-        return [net.connect(P, P, "exc").from_csr(we), net.connect(P, P, "inh").from_csr(wi)]
+        # one weight array entry per synapse (Uniform(w, w) draws w exactly)
+        return [net.connect(P[:ne_], P, "exc").fixed_number_pre(800, cn.Uniform(we, we), generator="device"),
+                net.connect(P[ne_:], P, "inh").fixed_number_pre(200, cn.Uniform(wi, wi), generator="device")]
+    t_b = time.perf_counter()
     net, P, Q, projs, mon = _coba(N, ne, conn, steps + args.warmup, poisson=(N // 10, 5.0, 0.6),
                                   precision=prec)
+    log(f"c2 network ({sum(p.nb_synapses for p in projs)} synapses, connectivity drawn on the device) "
+        f"set up in {time.perf_counter() - t_b:.1f} s")
     net.set_graph(True, 100)
     net.run(args.warmup)
     t0 = time.perf_counter()
@@ -315,8 +311,9 @@ def bench_c2(args):
     events = _events(projs[:2], P, c_t, idx_t)
     return _line("synaptic events/s", events / el, "synaptic events/s", steps, args.warmup,
                  el * 1e3 / steps, "f64" if prec == "double" else "f32",
-                 {"workload": "COBA LIF N=400000 in-degree 1000 (800E/200I) 4e8 synapses, "
-                              "Poisson 5 Hz into 10%, dt=0.1ms", "precision": prec},
+                 {"workload": "COBA LIF N=400000 fixed_number_pre 1000 (800E/200I) 4e8 synapses, "
+                              "Poisson 5 Hz into 10%, dt=0.1ms", "precision": prec,
+                  "connectivity": "drawn on the device (Philox stream)"},
                  None, None, spikes_per_step=float(c_t.mean()),
                  mean_rate_hz=float(c_t.sum() / N / (steps * 1e-4)))
 
@@ -361,7 +358,8 @@ def bench_c3(args):
                  {"workload": "256 rate instances N~U{500..8000} p=0.05 + shared dense 2000x2000 input",
                   "neurons": sw.total, "synapses": sw.nnz},
                  {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                  "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "batch_spmv_update",
+                  "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                  "kernel": {0: "batch_spmv_update", 1: "batch_spmv_lds", 2: "batch_sell"}[info["mode"]],
                   "kernel_ms": ms_spmv},
                  None, plain_steps_per_s=sps_p, speedup_vs_plain=sps_b / sps_p,
                  max_rel_rate_error_vs_fp64=worst, max_rel_population_rate_error=pop_err,

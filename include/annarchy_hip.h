@@ -249,9 +249,13 @@ int ahip_batch_get_input_sums(ahip_batch* b, double* y);
 int ahip_batch_run(ahip_batch* b, int64_t steps, int32_t graph_batch);
 /* which: 0 input update, 1 GEMM, 2 SpMV + update; mean launch time (ms) */
 int ahip_batch_time(ahip_batch* b, int32_t which, int32_t iters, double* avg_ms);
-/* 1 (default when every instance fits): each workgroup stages its instance's
- * rates in LDS; 0: rates are gathered from global memory. */
+/* 2 (default when every instance has fewer than 16384 neurons): chunks of up
+ * to 512 rows of one instance in sorted 64-row slices, one lane per row,
+ * uint16 instance-local columns, instance rates staged in LDS; 1: the
+ * instance's rates staged in LDS, L lanes per row over the CSR; 0: rates
+ * gathered from global memory. */
 int ahip_batch_set_mode(ahip_batch* b, int32_t mode);
+int ahip_batch_get_mode(ahip_batch* b, int32_t* mode);
 int ahip_batch_info(ahip_batch* b, int64_t* nnz, int32_t* total, double* sparse_bytes,
                     double* dense_flops);
 

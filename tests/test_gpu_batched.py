@@ -51,8 +51,7 @@ def test_plain_instances_match_oracle(gpu):
         assert np.max(np.abs(got - ref)) <= 1e-9 * max(1.0, np.max(np.abs(ref))), i
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-@pytest.mark.parametrize("lanes", [8, 32])
+@pytest.mark.parametrize("mode,lanes", [(0, 8), (0, 32), (1, 8), (1, 32), (2, 16)])
 def test_batched_matches_oracle(gpu, lanes, mode):
     sw = _sweep(2)
     steps = 300
@@ -65,6 +64,20 @@ def test_batched_matches_oracle(gpu, lanes, mode):
         ref = _oracle(sw, i, steps)
         worst = max(worst, np.max(np.abs(got[i] - ref)) / max(1e-3, np.max(np.abs(ref))))
     assert worst <= 1e-4, worst
+
+
+def test_sliced_mode_is_the_default_and_matches_lds_mode(gpu):
+    # mode 2 (sliced chunks) is chosen when every instance fits; it sums each
+    # row in one lane, so it agrees with the LDS kernel to FP32 rounding
+    sw = _sweep(4)
+    a = sw.batched(16)
+    assert a.info()["mode"] == 2
+    a.run(40, graph_batch=8)
+    b = sw.batched(16)
+    b.set_mode(1)
+    b.run(40, graph_batch=8)
+    for x, y in zip(a.rates(), b.rates()):
+        assert np.max(np.abs(x - y)) <= 1e-5 * max(1e-3, np.max(np.abs(y)))
 
 
 def test_batched_graph_remainder_parity(gpu):
