@@ -334,6 +334,13 @@ def bench_c3(args):
     t0 = time.perf_counter()
     b.run(steps)
     el_b = time.perf_counter() - t0
+    # streaming-copy probe: the achievable HBM rate, and the calibration of the
+    # HBM byte counters of a profiled run
+    import ctypes as C
+    from annarchy_amd import _lib
+    copy_ms = C.c_double()
+    _lib.check(_lib.lib().ahip_probe_copy(1 << 31, 10, C.byref(copy_ms)))
+    copy_gbs = 2.0 * (1 << 31) / (copy_ms.value * 1e-3) / 1e9
     info = b.info()
     ms_spmv = b.time_kernel(2, 20)
     ms_gemm = b.time_kernel(1, 50)
@@ -364,7 +371,7 @@ def bench_c3(args):
                  None, plain_steps_per_s=sps_p, speedup_vs_plain=sps_b / sps_p,
                  max_rel_rate_error_vs_fp64=worst, max_rel_population_rate_error=pop_err,
                  gemm={"ms": ms_gemm, "tflops": 3 * info["dense_flops"] / (ms_gemm * 1e-3) / 1e12,
-                       "peak_bf16_tflops": 2500.0})
+                       "peak_bf16_tflops": 2500.0}, hbm_copy_gbs=copy_gbs)
 
 
 if __name__ == "__main__":
