@@ -365,7 +365,9 @@ def bench_c3(args):
                  {"workload": "256 rate instances N~U{500..8000} p=0.05 + shared dense 2000x2000 input",
                   "neurons": sw.total, "synapses": sw.nnz},
                  {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                  "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                  "frac": achieved / HBM_PEAK_GBS,
+                  "traffic": traffic_from_profiles("batch_sell_f32_c3_sweep256_traffic.txt") if info["mode"] == 2
+                  else None,
                   "kernel": {0: "batch_spmv_update", 1: "batch_spmv_lds", 2: "batch_sell"}[info["mode"]],
                   "kernel_ms": ms_spmv},
                  None, plain_steps_per_s=sps_p, speedup_vs_plain=sps_b / sps_p,
