@@ -91,6 +91,10 @@ _SIGS = {
     "ahip_proj_nb_synapses": ([_P, _I32, _PI64], _I32),
     "ahip_proj_get_w": ([_P, _I32, _PD, _I64], _I32),
     "ahip_proj_set_w": ([_P, _I32, _PD, _I64], _I32),
+    "ahip_proj_set_synapse": ([_P, _I32, C.c_char_p], _I32),
+    "ahip_proj_syn_get": ([_P, _I32, C.c_char_p, _PD, _I64], _I32),
+    "ahip_proj_syn_set": ([_P, _I32, C.c_char_p, _PD, _I64], _I32),
+    "ahip_proj_get_last_event": ([_P, _I32, _PI64, _I64], _I32),
     "ahip_proj_psp_bytes": ([_P, _I32, _PD], _I32),
     "ahip_proj_time_psp": ([_P, _I32, _I32, _PD], _I32),
     "ahip_proj_set_launch": ([_P, _I32, _I32], _I32),

@@ -2,7 +2,7 @@
 
 ``build_core()`` compiles the runtime and the projection/spike kernels into
 ``annarchy_amd/lib/libannarchy_hip.so``; ``build_model()`` compiles one
-generated neuron-model source into its own shared object.  Both call hipcc
+generated neuron-model or synapse-model source into its own shared object.  Both call hipcc
 directly (no CMake): every kernel is written for CDNA4 wave64.
 
 Flags common to every device build:
@@ -96,16 +96,17 @@ def build_core(jobs: int = 8) -> Path:
     return CORE_LIB
 
 
-def model_lib_path(name: str, source: str) -> Path:
+def model_lib_path(name: str, source: str, kind: str = "model") -> Path:
     h = hashlib.sha1(source.encode()).hexdigest()[:12]
-    return LIB_DIR / f"libahip_model_{name}_{h}.so"
+    return LIB_DIR / f"libahip_{kind}_{name}_{h}.so"
 
 
-def build_model(name: str, source: str, force: bool = False) -> Path:
-    """Write the generated source and compile it into a model library."""
+def build_model(name: str, source: str, force: bool = False, kind: str = "model") -> Path:
+    """Write the generated source and compile it into a model library
+    (kind "model": a neuron model, "syn": a spiking-synapse model)."""
     GEN_DIR.mkdir(parents=True, exist_ok=True)
     LIB_DIR.mkdir(parents=True, exist_ok=True)
-    out = model_lib_path(name, source)
+    out = model_lib_path(name, source, kind)
     src = GEN_DIR / (out.stem + ".hip")
     src.write_text(source)
     if out.exists() and not force:

@@ -279,3 +279,186 @@ def generate(desc: NeuronDescription) -> ModelSource:
 def generate_from_text(name: str, parameters: str = "", equations: str = "", spike=None,
                        reset=None, refractory=None) -> ModelSource:
     return generate(analyse_neuron(parameters, equations, spike, reset, refractory, name))
+
+
+# ---------------------------------------------------------------------------
+# Spiking synapse models (pre / post events)
+# ---------------------------------------------------------------------------
+
+SYNAPSE_ABI = 1  # AHIP_SYNAPSE_ABI in include/annarchy_hip_synapse.h
+
+_SYN_KERNELS = r"""
+#include "annarchy_hip_synapse.h"
+
+namespace {
+
+// Pre events (compute_psp of step t, spikes of step t-d): one wave per
+// spike, lanes over the synapses of the pre neuron's column.  A synapse
+// belongs to one pre neuron, so no two lanes touch the same state.
+template <typename T>
+__global__ __launch_bounds__(256) void pre_event(ahip_syn_args a) {
+    const int lane = threadIdx.x & (ahip::kWave - 1);
+    const int wid = threadIdx.x / ahip::kWave;
+    const int64_t t = *a.t_base + a.step;
+    const T dt = (T)a.dt;
+    (void)dt;
+    T* __restrict__ w = (T*)a.w;
+    T* __restrict__ _g_post = (T*)a.g;
+    int64_t* __restrict__ _last_event = a.last_event;
+    const int64_t* __restrict__ _pre_last_spike = a.pre_last_spike;
+    const int64_t* __restrict__ _post_last_spike = a.post_last_spike;
+    (void)_last_event; (void)_pre_last_spike; (void)_post_last_spike;
+%(globals)s
+%(locals)s
+    int64_t slot = 0;
+    if (a.pre_depth > 1) {
+        slot = (t - a.pre_delay) %% a.pre_depth;
+        if (slot < 0) slot += a.pre_depth;
+    }
+    const int32_t n = a.pre_count[slot];
+    const int32_t* __restrict__ list = a.pre_spiked + slot * a.pre_stride;
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    for (int64_t s = (int64_t)blockIdx.x * 4 + wid; s < n; s += waves) {
+        const int32_t pre = list[s];
+        if (pre < 0 || pre >= a.n_cols) continue;
+        const int64_t e = a.col_ptr[pre + 1];
+        for (int64_t q = a.col_ptr[pre] + lane; q < e; q += ahip::kWave) {
+            const int32_t post = a.row_idx[q];
+            (void)post;
+%(pre_body)s
+        }
+    }
+}
+
+// Post events (after the neuron updates of step t, spikes of step t): one
+// wave per spiking post neuron, lanes over its row; state by column position.
+template <typename T>
+__global__ __launch_bounds__(256) void post_event(ahip_syn_args a) {
+    const int lane = threadIdx.x & (ahip::kWave - 1);
+    const int wid = threadIdx.x / ahip::kWave;
+    const int64_t t = *a.t_base + a.step;
+    const T dt = (T)a.dt;
+    (void)dt;
+    T* __restrict__ w = (T*)a.w;
+    int64_t* __restrict__ _last_event = a.last_event;
+    const int64_t* __restrict__ _pre_last_spike = a.pre_last_spike;
+    const int64_t* __restrict__ _post_last_spike = a.post_last_spike;
+    (void)_last_event; (void)_pre_last_spike; (void)_post_last_spike;
+%(globals)s
+%(locals)s
+    const int32_t n = *a.post_count;
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    for (int64_t s = (int64_t)blockIdx.x * 4 + wid; s < n; s += waves) {
+        const int32_t i = a.post_spiked[s];
+        if (i < 0 || i >= a.n_rows) continue;
+        const int64_t e = a.row_ptr[i + 1];
+        for (int64_t j = a.row_ptr[i] + lane; j < e; j += ahip::kWave) {
+            const int64_t q = a.fwd2col[j];
+            const int32_t pre = a.col_idx[j];
+            const int32_t post = i;
+            (void)pre; (void)post;
+%(post_body)s
+        }
+    }
+}
+
+template <typename T, bool PRE>
+int launch(const ahip_syn_args* a, void* stream) {
+    const int32_t units = PRE ? a->n_cols : a->n_rows;
+    int nb = (units + 3) / 4;
+    if (nb > 512) nb = 512;
+    if (nb < 1) nb = 1;
+    if (PRE)
+        hipLaunchKernelGGL(pre_event<T>, dim3(nb), dim3(256), 0, (hipStream_t)stream, *a);
+    else
+        hipLaunchKernelGGL(post_event<T>, dim3(nb), dim3(256), 0, (hipStream_t)stream, *a);
+    return (int)hipGetLastError();
+}
+
+int pre_f64(const ahip_syn_args* a, void* s) { return launch<double, true>(a, s); }
+int pre_f32(const ahip_syn_args* a, void* s) { return launch<float, true>(a, s); }
+int post_f64(const ahip_syn_args* a, void* s) { return launch<double, false>(a, s); }
+int post_f32(const ahip_syn_args* a, void* s) { return launch<float, false>(a, s); }
+
+const char* const kAttrNames[] = {%(attr_names)s};
+const int32_t kAttrLocal[] = {%(attr_local)s};
+const double kAttrInit[] = {%(attr_init)s};
+
+const ahip_synapse_desc kDesc = {
+    %(abi)d, "%(name)s", %(n_attr)d, kAttrNames, kAttrLocal, kAttrInit, %(has_pre)d, %(has_post)d, %(pre_g)d,
+    pre_f64, pre_f32, post_f64, post_f32, "%(hash)s"};
+
+}  // namespace
+
+extern "C" const ahip_synapse_desc* ahip_synapse_describe(void) { return &kDesc; }
+"""
+
+
+class SynapseSource:
+    """Generated HIP source of one synapse model plus its attribute table."""
+
+    def __init__(self, desc, source: str, attrs: list):
+        self.desc = desc
+        self.source = source
+        self.attrs = attrs  # (name, local, init)
+
+    @property
+    def name(self) -> str:
+        return self.desc.name
+
+
+def generate_synapse(desc) -> SynapseSource:
+    """Build the HIP source of the pre / post event kernels of a synapse model."""
+    attrs = [(a.name, a.locality == "local", a.init) for a in desc.parameters + desc.variables]
+    if len(attrs) > 16:
+        raise ValueError("at most 16 synapse attributes besides w")
+    slot = {a[0]: k for k, a in enumerate(attrs)}
+    globals_ = "\n".join(f"    const T _s_{_c_name(n)} = (T)a.global[{slot[n]}];" for n, loc, _ in attrs if not loc)
+    locals_ = "\n".join(f"    T* __restrict__ _s_{_c_name(n)} = (T*)a.local[{slot[n]}];" for n, loc, _ in attrs
+                         if loc)
+
+    def subst(code: str, t_text: str) -> str:
+        code = code.replace("%(t)s", t_text)
+        return code.replace("%(local_index)s", "[q]").replace("%(global_index)s", "")
+
+    pre, post = [], []
+    # Projection/OpenMPGenerator.py:1182-1230: event-driven decay to t-1,
+    # _last_event = t, then g_target += w (the default transmission when the
+    # model does not write g_target itself), then the pre_spike statements
+    for _, code in desc.event_driven:
+        pre.append(subst(code, "(t-1)"))
+    if desc.event_driven:
+        pre.append("_last_event[q] = t;")
+    pre.append("unsafeAtomicAdd(_g_post + post, w[q]);")
+    pre += [subst(c, "(t)") for c in desc.pre_code]
+    if desc.post_code:
+        # OpenMPGenerator.py:1575-1600: event-driven decay to t, _last_event = t
+        for _, code in desc.event_driven:
+            post.append(subst(code, "(t)"))
+        if desc.event_driven:
+            post.append("_last_event[q] = t;")
+        post += [subst(c, "(t)") for c in desc.post_code]
+    body = _PRELUDE % {"name": desc.name} + "}  // namespace\n" + _SYN_KERNELS % {
+        "globals": globals_,
+        "locals": locals_,
+        "pre_body": _indent("\n".join(pre), 12),
+        "post_body": _indent("\n".join(post), 12),
+        "attr_names": ", ".join(f'"{n}"' for n, _, _ in attrs) or "nullptr",
+        "attr_local": ", ".join(str(int(loc)) for _, loc, _ in attrs) or "0",
+        "attr_init": ", ".join(repr(float(v)) for _, _, v in attrs) or "0.0",
+        "abi": SYNAPSE_ABI,
+        "name": desc.name,
+        "n_attr": len(attrs),
+        "has_pre": int(bool(pre)),
+        "has_post": int(bool(post)),
+        "pre_g": 1,
+        "hash": "%(hash)s",
+    }
+    h = hashlib.sha1(body.encode()).hexdigest()[:12]
+    return SynapseSource(desc, body.replace("%(hash)s", h), attrs)
+
+
+def generate_synapse_from_text(name: str, parameters: str = "", equations: str = "", pre_spike=None,
+                               post_spike=None) -> SynapseSource:
+    from .equations import analyse_synapse
+    return generate_synapse(analyse_synapse(parameters, equations, pre_spike, post_spike, name))

@@ -10,6 +10,7 @@
 
 #include "annarchy_hip.h"
 #include "annarchy_hip_model.h"
+#include "annarchy_hip_synapse.h"
 
 namespace ahip {
 
@@ -102,6 +103,13 @@ struct Proj {
     DevBuf delay_dev;             // int16: rate -> forward order, spiking -> column order
     DevBuf ring;                  // spiking per-synapse delays: max_delay x n_rows events
     DevBuf xd;                    // rate uniform delay: the delayed pre-synaptic rates
+    // plasticity: a generated synapse model and its per-synapse state (column order)
+    void* syn_dl = nullptr;
+    std::string syn_path;
+    const ahip_synapse_desc* syn = nullptr;
+    std::vector<DevBuf> syn_local;   // per attribute; empty for per-projection ones
+    std::vector<double> syn_global;  // per-projection values
+    DevBuf last_event, fwd2col, row_ptr_dev, col_dev;
     std::vector<int64_t> h_inv_idx;
 };
 

@@ -180,6 +180,89 @@ int ensure_delay_buffers(ahip_net* net) {
     return 0;
 }
 
+// Per-synapse state of a plastic projection, column order as w_col: every
+// attribute at its initial value, _last_event = -10000 (the reference's
+// initial value, Projection/CUDA/CSR.py:280-291), the forward -> column map
+// of the post events and the forward row pointer.
+int init_syn_state(ahip_net* net, Proj& p) {
+    const ahip_synapse_desc* d = p.syn;
+    const size_t es = net->elem();
+    HIPCHK(hipStreamSynchronize(net->stream));
+    p.syn_local.clear();
+    p.syn_local.resize(d->n_attr);
+    p.syn_global.assign(d->n_attr, 0.0);
+    int rc;
+    for (int32_t k = 0; k < d->n_attr; k++) {
+        if (d->attr_local[k]) {
+            if ((rc = dev_alloc(p.syn_local[k], (size_t)p.nnz * es))) return rc;
+            LCHK(launch_fill(p.syn_local[k].p, p.nnz, (int32_t)es, d->attr_init[k], net->stream));
+        } else {
+            p.syn_global[k] = d->attr_init[k];
+        }
+    }
+    if ((rc = dev_alloc(p.last_event, (size_t)p.nnz * sizeof(int64_t)))) return rc;
+    LCHK(launch_fill_i64(p.last_event.as<int64_t>(), p.nnz, -10000, net->stream));
+    std::vector<int64_t> f2c(p.nnz);
+    for (int64_t q = 0; q < p.nnz; q++) f2c[p.h_inv_idx[q]] = q;
+    if ((rc = dev_alloc(p.fwd2col, (size_t)p.nnz * sizeof(int64_t)))) return rc;
+    if (p.nnz) HIPCHK(hipMemcpy(p.fwd2col.p, f2c.data(), f2c.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    if ((rc = dev_alloc(p.row_ptr_dev, p.h_row_ptr.size() * sizeof(int64_t)))) return rc;
+    HIPCHK(hipMemcpy(p.row_ptr_dev.p, p.h_row_ptr.data(), p.h_row_ptr.size() * sizeof(int64_t),
+                     hipMemcpyHostToDevice));
+    if ((rc = dev_alloc(p.col_dev, (size_t)p.nnz * sizeof(int32_t)))) return rc;
+    if (p.nnz) HIPCHK(hipMemcpy(p.col_dev.p, p.h_col.data(), p.nnz * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipStreamSynchronize(net->stream));
+    return 0;
+}
+
+// Pre (compute_psp) or post events of a plastic projection at `step`.
+int launch_syn_events(ahip_net* net, Proj& p, bool pre_events, int32_t step) {
+    Pop& pre = *net->pops[p.pre];
+    Pop& post = *net->pops[p.post];
+    ahip_syn_args a;
+    std::memset(&a, 0, sizeof(a));
+    if (p.delay_uniform >= 2) {
+        a.pre_spiked = pre.hist.as<int32_t>();
+        a.pre_count = pre.hist_count.as<int32_t>();
+        a.pre_stride = pre.size;
+        a.pre_depth = pre.depth;
+        a.pre_delay = p.delay_uniform;
+    } else {
+        a.pre_spiked = pre.spiked.as<int32_t>();
+        a.pre_count = pre.spike_count.as<int32_t>();
+        a.pre_depth = 1;
+    }
+    a.n_cols = p.n_cols;
+    a.col_ptr = p.col_ptr.as<int64_t>();
+    a.row_idx = p.row_idx.as<int32_t>();
+    a.post_spiked = post.spiked.as<int32_t>();
+    a.post_count = post.spike_count.as<int32_t>();
+    a.n_rows = p.n_rows;
+    a.row_ptr = p.row_ptr_dev.as<int64_t>();
+    a.col_idx = p.col_dev.as<int32_t>();
+    a.fwd2col = p.fwd2col.as<int64_t>();
+    a.pre_last_spike = pre.last_spike.as<int64_t>();
+    a.post_last_spike = post.last_spike.as<int64_t>();
+    a.g = post.slots[p.target_slot].p;
+    a.w = p.w_col.p;
+    a.last_event = p.last_event.as<int64_t>();
+    for (int32_t k = 0; k < p.syn->n_attr; k++) {
+        a.local[k] = p.syn_local[k].p;
+        a.global[k] = p.syn_global[k];
+    }
+    a.t_base = net->t_dev.as<int64_t>();
+    a.step = step;
+    a.dt = net->dt;
+    const bool f64 = net->precision == AHIP_F64;
+    ahip_syn_launch_fn fn = pre_events ? (f64 ? p.syn->pre_f64 : p.syn->pre_f32)
+                                       : (f64 ? p.syn->post_f64 : p.syn->post_f32);
+    const int rc = fn(&a, (void*)net->stream);
+    if (rc != 0)
+        return fail(AHIP_EHIP, "%s events of projection %d failed: %s", pre_events ? "pre" : "post", p.id,
+                    hipGetErrorString((hipError_t)rc));
+    return 0;
+}
+
 int prepare(ahip_net* net) {
     if (!net->initialized) return fail(AHIP_ESTATE, "network not initialized");
     for (auto& pp : net->projs) {
@@ -197,7 +280,7 @@ int prepare(ahip_net* net) {
         std::vector<std::vector<const void*>> targets;
         for (auto& pp : net->projs) {
             Proj& p = *pp;
-            if (!p.spiking || p.nnz == 0 || p.delay_per_syn) continue;
+            if (!p.spiking || p.nnz == 0 || p.delay_per_syn || p.syn) continue;
             Pop& pre = *net->pops[p.pre];
             void* g = net->pops[p.post]->slots[p.target_slot].p;
             size_t k = 0;
@@ -330,7 +413,14 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
         Pop& post = *net->pops[p.post];
         void* target = post.slots[p.target_slot].p;
         if (p.spiking) {
-            if (p.delay_per_syn && p.nnz > 0) LCHK(launch_spike_ring(net, p, pre, target, step, s));
+            if (p.syn) {
+                if (p.nnz > 0) {
+                    int rc = launch_syn_events(net, p, true, step);
+                    if (rc) return rc;
+                }
+            } else if (p.delay_per_syn && p.nnz > 0) {
+                LCHK(launch_spike_ring(net, p, pre, target, step, s));
+            }
             continue;
         }
         if (zero_before[p.id]) LCHK(launch_fill(target, post.size, (int32_t)es, 0.0, s));
@@ -370,6 +460,14 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
                 if (rs >= 0) LCHK(launch_push_values(net, p.slots[rs].p, p.hist.p, p.size, p.depth, step, s));
             }
         }
+    }
+    // post events of the plastic projections, after every population update
+    // (the reference's post_event, OpenMPBaseTemplate.py:217-325)
+    for (auto& pp : net->projs) {
+        Proj& p = *pp;
+        if (!p.syn || !p.syn->has_post || p.nnz == 0) continue;
+        int rc = launch_syn_events(net, p, false, step);
+        if (rc) return rc;
     }
     // 5. record variables and spikes
     for (auto& mp : net->mons) {
@@ -922,8 +1020,17 @@ int ahip_proj_init_csr(ahip_net* net, int32_t proj, int32_t n_rows, int32_t n_co
         p->h_w.assign(w, w + nnz);
     else
         p->h_w.clear();
+    if (p->syn && !w) {
+        // a plastic projection has one weight per synapse
+        p->single_w = false;
+        p->h_w.assign(nnz, w_const);
+    }
     invalidate(net);
-    if (p->spiking) return build_backward(net, *p);
+    if (p->spiking) {
+        int rc = build_backward(net, *p);
+        if (rc || !p->syn) return rc;
+        return init_syn_state(net, *p);
+    }
     ahip_format_opts o{AHIP_FMT_CSR, 64, 0, 1, 0};
     if (opts) o = *opts;
     std::string err;
@@ -1041,6 +1148,7 @@ int ahip_proj_set_delays(ahip_net* net, int32_t proj, const int32_t* steps, int6
         hi = std::max(hi, steps[j]);
     }
     if (n > 0 && (lo < 1 || hi > 32767)) return fail(AHIP_EINVAL, "delays must be 1..32767 steps");
+    if (p->syn && !(n == 1 || lo == hi)) return fail(AHIP_EINVAL, "a plastic projection takes one uniform delay");
     HIPCHK(hipStreamSynchronize(net->stream));
     invalidate(net);
     p->delay_dev.release();
@@ -1133,9 +1241,39 @@ int ahip_proj_nb_synapses(ahip_net* net, int32_t proj, int64_t* nnz) {
     return 0;
 }
 
+// Column-order device values of a plastic projection -> forward order.
+static int syn_read(ahip_net* net, Proj& p, const void* dev, double* out) {
+    std::vector<double> col(p.nnz);
+    HIPCHK(hipStreamSynchronize(net->stream));
+    if (net->precision == AHIP_F64) {
+        if (p.nnz) HIPCHK(hipMemcpy(col.data(), dev, p.nnz * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> f(p.nnz);
+        if (p.nnz) HIPCHK(hipMemcpy(f.data(), dev, p.nnz * sizeof(float), hipMemcpyDeviceToHost));
+        for (int64_t q = 0; q < p.nnz; q++) col[q] = f[q];
+    }
+    for (int64_t q = 0; q < p.nnz; q++) out[p.h_inv_idx[q]] = col[q];
+    return 0;
+}
+
+static int syn_write(ahip_net* net, Proj& p, void* dev, const double* in) {
+    HIPCHK(hipStreamSynchronize(net->stream));
+    if (net->precision == AHIP_F64) {
+        std::vector<double> col(p.nnz);
+        for (int64_t q = 0; q < p.nnz; q++) col[q] = in[p.h_inv_idx[q]];
+        if (p.nnz) HIPCHK(hipMemcpy(dev, col.data(), p.nnz * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> col(p.nnz);
+        for (int64_t q = 0; q < p.nnz; q++) col[q] = (float)in[p.h_inv_idx[q]];
+        if (p.nnz) HIPCHK(hipMemcpy(dev, col.data(), p.nnz * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
 int ahip_proj_get_w(ahip_net* net, int32_t proj, double* w, int64_t nnz) {
     Proj* p = get_proj(net, proj);
     if (!p || !w || nnz != p->nnz) return fail(AHIP_EINVAL, "bad projection or size");
+    if (p->syn) return syn_read(net, *p, p->w_col.p, w);
     for (int64_t j = 0; j < nnz; j++) w[j] = p->single_w ? p->w_const : p->h_w[j];
     return 0;
 }
@@ -1147,6 +1285,7 @@ int ahip_proj_set_w(ahip_net* net, int32_t proj, const double* w, int64_t nnz) {
     p->single_w = false;
     p->h_w.assign(w, w + nnz);
     invalidate(net);
+    if (p->syn) return syn_write(net, *p, p->w_col.p, w);
     if (p->spiking) return build_backward(net, *p);
     ahip_format_opts o{p->rate.format, p->rate.lanes, p->rate.chunk, p->rate.sigma, p->rate.panel_cols};
     std::string err;
@@ -1154,6 +1293,91 @@ int ahip_proj_set_w(ahip_net* net, int32_t proj, const double* w, int64_t nnz) {
     int rc = build_layout(*p, net->precision, o, L, err);
     if (rc) return fail(rc, "%s", err.c_str());
     p->rate = std::move(L);
+    return 0;
+}
+
+int ahip_proj_set_synapse(ahip_net* net, int32_t proj, const char* synapse_lib) {
+    Proj* p = get_proj(net, proj);
+    if (!p || !synapse_lib) return fail(AHIP_EINVAL, "bad projection or library");
+    if (!p->spiking) return fail(AHIP_EINVAL, "synapse models apply to spiking projections");
+    if (p->syn) return fail(AHIP_ESTATE, "projection %d already has a synapse model", p->id);
+    if (p->delay_per_syn) return fail(AHIP_EINVAL, "a plastic projection takes one uniform delay");
+    void* dl = dlopen(synapse_lib, RTLD_NOW | RTLD_LOCAL);
+    if (!dl) return fail(AHIP_EMODEL, "cannot load synapse library %s: %s", synapse_lib, dlerror());
+    auto describe = (const ahip_synapse_desc* (*)(void))dlsym(dl, "ahip_synapse_describe");
+    if (!describe) return fail(AHIP_EMODEL, "%s does not export ahip_synapse_describe", synapse_lib);
+    const ahip_synapse_desc* d = describe();
+    if (!d || d->abi != AHIP_SYNAPSE_ABI)
+        return fail(AHIP_EMODEL, "%s: synapse ABI %d, runtime expects %d", synapse_lib, d ? d->abi : -1,
+                    AHIP_SYNAPSE_ABI);
+    if (d->n_attr < 0 || d->n_attr > AHIP_MAX_SYN_ATTRS) return fail(AHIP_EMODEL, "%s: too many attributes", synapse_lib);
+    if (!d->pre_f64 || !d->pre_f32 || !d->post_f64 || !d->post_f32)
+        return fail(AHIP_EMODEL, "%s: missing event launchers", synapse_lib);
+    if (!net->pops[p->post]->model->spiking)
+        return fail(AHIP_EINVAL, "synapse models need a spiking post-synaptic population");
+    HIPCHK(hipSetDevice(net->device));
+    HIPCHK(hipStreamSynchronize(net->stream));
+    p->syn_dl = dl;  // stays mapped, as the neuron model libraries
+    p->syn_path = synapse_lib;
+    p->syn = d;
+    invalidate(net);
+    if (p->nnz < 0) return 0;
+    if (p->single_w) {
+        p->single_w = false;
+        p->h_w.assign(p->nnz, p->w_const);
+        int rc = build_backward(net, *p);
+        if (rc) return rc;
+    }
+    return init_syn_state(net, *p);
+}
+
+static int syn_attr(Proj* p, const char* attr) {
+    if (!p || !p->syn || !attr) return -1;
+    for (int32_t k = 0; k < p->syn->n_attr; k++)
+        if (std::strcmp(p->syn->attr_names[k], attr) == 0) return k;
+    return -1;
+}
+
+int ahip_proj_syn_get(ahip_net* net, int32_t proj, const char* attr, double* values, int64_t n) {
+    Proj* p = get_proj(net, proj);
+    const int k = syn_attr(p, attr);
+    if (k < 0 || !values || p->nnz < 0) return fail(AHIP_EINVAL, "no synaptic attribute %s", attr ? attr : "");
+    if (!p->syn->attr_local[k]) {
+        if (n != 1) return fail(AHIP_EINVAL, "%s is one value per projection", attr);
+        values[0] = p->syn_global[k];
+        return 0;
+    }
+    if (n != p->nnz) return fail(AHIP_EINVAL, "%s holds %lld values", attr, (long long)p->nnz);
+    return syn_read(net, *p, p->syn_local[k].p, values);
+}
+
+int ahip_proj_syn_set(ahip_net* net, int32_t proj, const char* attr, const double* values, int64_t n) {
+    Proj* p = get_proj(net, proj);
+    const int k = syn_attr(p, attr);
+    if (k < 0 || !values || p->nnz < 0) return fail(AHIP_EINVAL, "no synaptic attribute %s", attr ? attr : "");
+    if (!p->syn->attr_local[k]) {
+        if (n != 1) return fail(AHIP_EINVAL, "%s is one value per projection", attr);
+        p->syn_global[k] = values[0];
+        invalidate(net);  // launch arguments are captured by value
+        return 0;
+    }
+    if (n == 1) {
+        HIPCHK(hipStreamSynchronize(net->stream));
+        LCHK(launch_fill(p->syn_local[k].p, p->nnz, (int32_t)net->elem(), values[0], net->stream));
+        HIPCHK(hipStreamSynchronize(net->stream));
+        return 0;
+    }
+    if (n != p->nnz) return fail(AHIP_EINVAL, "%s holds %lld values", attr, (long long)p->nnz);
+    return syn_write(net, *p, p->syn_local[k].p, values);
+}
+
+int ahip_proj_get_last_event(ahip_net* net, int32_t proj, int64_t* values, int64_t n) {
+    Proj* p = get_proj(net, proj);
+    if (!p || !p->syn || !values || n != p->nnz) return fail(AHIP_EINVAL, "bad plastic projection or size");
+    std::vector<int64_t> col(p->nnz);
+    HIPCHK(hipStreamSynchronize(net->stream));
+    if (p->nnz) HIPCHK(hipMemcpy(col.data(), p->last_event.p, p->nnz * sizeof(int64_t), hipMemcpyDeviceToHost));
+    for (int64_t q = 0; q < p->nnz; q++) values[p->h_inv_idx[q]] = col[q];
     return 0;
 }
 

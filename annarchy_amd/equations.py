@@ -367,6 +367,17 @@ def analyse_neuron(parameters: str = "", equations: str = "", spike: str | None 
         for v in variables:
             if v.name.startswith("g_") and v.name[2:] not in targets:
                 targets.append(v.name[2:])
+        # a conductance read but not defined is reset after every step: the
+        # variable g_<target> = 0.0 appended after the others
+        # (parser/AnalyseNeuron.py:186-209, Extraction.py:686-703)
+        for v in list(variables):
+            for m in re.finditer(r"(?:^|[^\w.])g_(\w+)", v.eq):
+                gname = "g_" + m.group(1)
+                if gname not in declared:
+                    declared.add(gname)
+                    targets.append(m.group(1))
+                    variables.append(Variable(name=gname, eq=gname + " = 0.0", locality="local", flags=[],
+                                              bounds={}, init=0.0, ctype=precision))
     # random distributions -> rand_k
     randoms = []
     for v in variables:
@@ -487,3 +498,128 @@ def equation_code(desc: NeuronDescription, locality: str, mask: str | None) -> s
                 if v.bound_code:
                     out.append(v.bound_code)
     return "\n".join(out)
+
+
+# ---------------------------------------------------------------------------
+# Synapse definitions (parser/AnalyseSynapse.py)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class SynapseDescription:
+    """Attributes of a spiking synapse model and the C text of its events.
+
+    Locality: ``: projection`` makes a parameter one value per projection
+    (AnalyseSynapse.py, "projection"); every other attribute has one value per
+    synapse.  ``w`` is always a per-synapse variable.  Only event-driven
+    variables are integrated (Equation.py:606-649); continuous synaptic ODEs
+    are not supported by this backend.
+    """
+    name: str
+    parameters: list
+    variables: list
+    pre_g: bool = False                     # pre_spike holds g_target += w
+    pre_code: list = field(default_factory=list)
+    post_code: list = field(default_factory=list)
+    event_driven: list = field(default_factory=list)   # (name, C text with %(t)s)
+    randoms: list = field(default_factory=list)
+    targets: list = field(default_factory=list)
+
+    def local_names(self):
+        return ["w"] + [a.name for a in self.parameters + self.variables if a.locality == "local"]
+
+    def global_names(self):
+        return [a.name for a in self.parameters + self.variables if a.locality == "global"]
+
+
+def _statement(tr, line, declared_local, declared_global) -> str:
+    eq, constraint = _split_constraint(line)
+    bounds, _ = _flags(constraint)
+    name = _variable_name(eq)
+    if name not in declared_local and name not in declared_global:
+        raise EquationError(f"unknown synaptic variable in: {line}")
+    if name in declared_global:
+        raise EquationError(f"per-projection parameters cannot be modified by events: {line}")
+    code = tr.increment(eq) if any(op in eq for op in ["+=", "-=", "*=", "/="]) else tr.assignment(eq)
+    lhs = tr.ccode(tr.local_dict[name])
+    for b in ("min", "max"):
+        if b in bounds:
+            val = bounds[b]
+            try:
+                float(val)
+            except ValueError:
+                val = tr.expression(val)
+            op = "<" if b == "min" else ">"
+            code += f"\nif({lhs} {op} {val})\n    {lhs} = {val};"
+    return code
+
+
+def analyse_synapse(parameters: str = "", equations: str = "", pre_spike: str | None = None,
+                    post_spike: str | None = None, name: str = "synapse",
+                    precision: str = "double") -> SynapseDescription:
+    """Analyse a spiking synapse definition (AnalyseSynapse.py) into the C text
+    of its pre- and post-synaptic events."""
+    params = []
+    for line in _lines(parameters):
+        eq, constraint = _split_constraint(line)
+        bounds, flags = _flags(constraint)
+        if "=" not in eq:
+            raise EquationError(f"parameter without value: {line}")
+        if "postsynaptic" in flags:
+            raise EquationError("postsynaptic (per post neuron) synaptic attributes are not supported")
+        pname, val = eq.split("=", 1)
+        params.append(Variable(name=pname.strip(), eq=eq, locality="global" if "projection" in flags else "local",
+                               flags=flags, bounds=bounds, init=_parse_value(val), is_parameter=True))
+    variables = []
+    for line in _lines(equations):
+        eq, constraint = _split_constraint(line)
+        bounds, flags = _flags(constraint)
+        vname = _variable_name(eq)
+        init = _parse_value(bounds.pop("init")) if "init" in bounds else 0.0
+        if "event-driven" not in flags or _equation_type(vname, eq) != "ODE":
+            raise EquationError(f"only event-driven synaptic ODEs are supported: {line}")
+        variables.append(Variable(name=vname, eq=eq, locality="local", flags=flags, bounds=bounds, init=init,
+                                  method="event-driven"))
+    desc = SynapseDescription(name=name, parameters=params, variables=variables)
+    for a in params + variables:
+        if a.name == "w":
+            raise EquationError("w is the synaptic weight, not a model attribute")
+    tr = _Translator(desc, precision)
+    # model attributes become _s_<name> in the kernels, so that no attribute
+    # name can collide with a kernel variable
+    for a in params + variables:
+        idx = "%(local_index)s" if a.locality == "local" else "%(global_index)s"
+        tr.local_dict[a.name] = sp.Symbol("_s_" + a.name + idx)
+    # last spike times of the pre / post neuron in ms (ParserTemplate.py:18-19)
+    tr.local_dict["t_pre"] = sp.Symbol("((double)(_pre_last_spike[pre])*dt)")
+    tr.local_dict["t_post"] = sp.Symbol("((double)(_post_last_spike[post])*dt)")
+    local = set(desc.local_names())
+    glob = set(desc.global_names())
+    # event-driven variables: exact decay since the last event (Equation.py:606-649)
+    for v in variables:
+        expr = v.eq.replace("=", "- (", 1) + ")"
+        expr = expr.replace(" ", "")
+        real_tau, _, steady = tr.standardize(v.name, expr)
+        if real_tau is None:
+            raise EquationError(f"the equation is not a linear ODE and cannot be evaluated exactly: {v.eq}")
+        for other in desc.local_names() + desc.global_names():
+            if other != v.name and tr.local_dict[other] in steady.atoms() and other in [x.name for x in variables]:
+                raise EquationError(f"an event-driven variable cannot depend on other variables: {v.eq}")
+        var = tr.ccode(tr.local_dict[v.name])
+        st = tr.ccode(steady)
+        decay = f"exp(dt*(_last_event%(local_index)s - %(t)s)/({tr.ccode(real_tau)}))"
+        if st == "0":
+            v.cpp = f"{var} *= {decay};"
+        else:
+            v.cpp = f"{var} = {st} + ({var} - {st})*{decay};"
+        desc.event_driven.append((v.name, v.cpp))
+    for line in _lines(pre_spike):
+        eq, _ = _split_constraint(line)
+        if eq.replace(" ", "") == "g_target+=w":
+            desc.pre_g = True
+            continue
+        if "g_target" in eq:
+            raise EquationError(f"only g_target += w is supported: {line}")
+        desc.pre_code.append(_statement(tr, line, local, glob))
+    for line in _lines(post_spike):
+        desc.post_code.append(_statement(tr, line, local, glob))
+    return desc

@@ -13,6 +13,48 @@ from dataclasses import dataclass
 
 
 @dataclass(frozen=True)
+class Synapse:
+    """Spiking synapse type (``ANNarchy.core.Synapse``): parameters,
+    event-driven equations and the pre_spike / post_spike statements."""
+    parameters: str = ""
+    equations: str = ""
+    pre_spike: str | None = None
+    post_spike: str | None = None
+    name: str = "synapse"
+    description: str = ""
+
+
+def STDP(tau_plus=20.0, tau_minus=20.0, A_plus=0.01, A_minus=0.01, w_min=0.0, w_max=1.0) -> Synapse:
+    """Spike-timing dependent plasticity with exponential traces, the
+    reference's models/Synapses.py:232-310 (online additive rule, traces x
+    and y integrated exactly between events)."""
+    return Synapse(
+        parameters=f"""
+            tau_plus = {float(tau_plus)!r} : projection
+            tau_minus = {float(tau_minus)!r} : projection
+            A_plus = {float(A_plus)!r} : projection
+            A_minus = {float(A_minus)!r} : projection
+            w_min = {float(w_min)!r} : projection
+            w_max = {float(w_max)!r} : projection
+        """,
+        equations="""
+            tau_plus  * dx/dt = -x : event-driven
+            tau_minus * dy/dt = -y : event-driven
+        """,
+        pre_spike="""
+            g_target += w
+            x += A_plus * w_max
+            w = clip(w + y, w_min , w_max)
+        """,
+        post_spike="""
+            y -= A_minus * w_max
+            w = clip(w + x, w_min , w_max)
+        """,
+        name="STDP",
+    )
+
+
+@dataclass(frozen=True)
 class Neuron:
     parameters: str = ""
     equations: str = ""

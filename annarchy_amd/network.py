@@ -28,7 +28,7 @@ from . import _lib
 from . import build as _build
 from . import codegen
 from . import connectivity as conn
-from .models import Neuron
+from .models import Neuron, Synapse
 
 
 def model_library(neuron: Neuron, allow_build: bool = True) -> Path:
@@ -42,6 +42,18 @@ def model_library(neuron: Neuron, allow_build: bool = True) -> Path:
     if not allow_build or os.environ.get("AHIP_NO_BUILD"):
         raise _lib.AhipError(f"model library {path.name} for {neuron.name} is not built")
     return _build.build_model(neuron.name, src.source)
+
+
+def synapse_library(synapse: Synapse, allow_build: bool = True) -> Path:
+    """Path of the compiled event-kernel library of a spiking synapse type."""
+    src = codegen.generate_synapse_from_text(synapse.name, synapse.parameters, synapse.equations,
+                                             synapse.pre_spike, synapse.post_spike)
+    path = _build.model_lib_path(synapse.name, src.source, "syn")
+    if path.exists():
+        return path
+    if not allow_build or os.environ.get("AHIP_NO_BUILD"):
+        raise _lib.AhipError(f"synapse library {path.name} for {synapse.name} is not built")
+    return _build.build_model(synapse.name, src.source, kind="syn")
 
 
 class Population:
@@ -197,15 +209,89 @@ def _base(obj):
 
 
 class Projection:
-    def __init__(self, net: "Network", pre, post, target: str):
+    _OWN = ("net", "pre", "post", "target", "id", "synapse", "csr")
+
+    def __init__(self, net: "Network", pre, post, target: str, synapse: Synapse | None = None):
         self.net = net
         self.pre, self.post, self.target = pre, post, target
         self.id = -1
+        self.synapse = synapse
         self._connector = None
         self._generator = "numpy"
         self._delays = 0.0
         self._format = _lib.opts("csr", 64)
+        self._syn_init = {}
+        self._syn_attrs = None
         self.csr: conn.CSR | None = None
+
+    # -- synaptic attributes of a plastic projection (Projection.py:1000-1200)
+    def _syn_info(self) -> dict:
+        if self._syn_attrs is None:
+            if self.synapse is None:
+                self._syn_attrs = {}
+            else:
+                src = codegen.generate_synapse_from_text(self.synapse.name, self.synapse.parameters,
+                                                         self.synapse.equations, self.synapse.pre_spike,
+                                                         self.synapse.post_spike)
+                self._syn_attrs = {a[0]: a for a in src.attrs}
+        return self._syn_attrs
+
+    def __getattr__(self, name):
+        if name.startswith("_") or name in Projection._OWN:
+            raise AttributeError(name)
+        if name in self._syn_info():
+            return self.get(name)
+        raise AttributeError(name)
+
+    def __setattr__(self, name, value):
+        if not name.startswith("_") and name not in Projection._OWN and name in self._syn_info():
+            self.set(name, value)
+        else:
+            object.__setattr__(self, name, value)
+
+    def get(self, name):
+        """A synaptic attribute: one value per synapse (forward order) or,
+        for a ``: projection`` parameter, one float."""
+        if name == "w":
+            return self.w
+        info = self._syn_info()
+        if name not in info:
+            raise AttributeError(f"the projection has no synaptic attribute {name}")
+        if not self.net.compiled:
+            return self._syn_init.get(name, info[name][2])
+        n = self.nb_synapses if info[name][1] else 1
+        out = np.empty(max(n, 1))
+        if n == 0:
+            return out[:0]
+        _lib.check(_lib.lib().ahip_proj_syn_get(self.net._h, self.id, name.encode(), _lib.ptr(out, C.c_double), n))
+        return out if info[name][1] else float(out[0])
+
+    def set(self, name, value):
+        if name == "w":
+            self.w = value
+            return
+        info = self._syn_info()
+        if name not in info:
+            raise AttributeError(f"the projection has no synaptic attribute {name}")
+        if not self.net.compiled:
+            self._syn_init[name] = value
+            return
+        if isinstance(value, conn.RandomDistribution):
+            value = value.get_values(self.nb_synapses)
+        v = np.ascontiguousarray(np.asarray(value, dtype=np.float64).reshape(-1))
+        if not info[name][1] and v.size != 1:
+            raise ValueError(f"{name} is one value per projection")
+        if self.nb_synapses == 0 and info[name][1]:
+            return
+        _lib.check(_lib.lib().ahip_proj_syn_set(self.net._h, self.id, name.encode(), _lib.ptr(v, C.c_double),
+                                                v.size))
+
+    @property
+    def last_event(self) -> np.ndarray:
+        """Step of the last pre or post event of every synapse (forward order)."""
+        out = np.empty(self.nb_synapses, dtype=np.int64)
+        _lib.check(_lib.lib().ahip_proj_get_last_event(self.net._h, self.id, _lib.ptr(out, C.c_int64), out.size))
+        return out
 
     # -- connectors (core/ConnectorMethods.py) ---------------------------------
     def _store(self, kind, *args, storage_format=None, **fmt):
@@ -370,6 +456,11 @@ class Projection:
         _lib.check(L.ahip_proj_create(self.net._h, pre_b.id, post_b.id, self.target.encode(),
                                       int(spiking), C.byref(pid)))
         self.id = pid.value
+        if self.synapse is not None:
+            if not spiking:
+                raise ValueError("synapse types apply to spiking projections")
+            lib_path = synapse_library(self.synapse, self.net._allow_build)
+            _lib.check(L.ahip_proj_set_synapse(self.net._h, self.id, str(lib_path).encode()))
         if self._generator == "device":
             self._generate_on_device()
         else:
@@ -382,6 +473,8 @@ class Projection:
                                             c.w_const, C.byref(self._format)))
         if not (np.isscalar(self._delays) and float(self._delays) <= self.net.dt):
             self._upload_delays(self._delays)
+        for name, value in self._syn_init.items():
+            self.set(name, value)
 
     # -- accessors ------------------------------------------------------------
     @property
@@ -538,10 +631,12 @@ class Network:
         self.populations.append(p)
         return p
 
-    def connect(self, pre, post, target: str) -> Projection:
+    def connect(self, pre, post, target: str, synapse: Synapse | None = None) -> Projection:
+        """Projection(pre, post, target, synapse): `synapse` is a spiking
+        synapse type with pre_spike / post_spike events (plasticity)."""
         if self.compiled:
             raise RuntimeError("the network is already compiled")
-        p = Projection(self, pre, post, target)
+        p = Projection(self, pre, post, target, synapse)
         self.projections.append(p)
         return p
 
@@ -557,6 +652,7 @@ class Network:
         run the connectors and upload everything (Compiler._instantiate,
         generator/Compiler.py:796-1000)."""
         L = _lib.lib()
+        self._allow_build = allow_build
         h = C.c_void_p()
         _lib.check(L.ahip_net_create(C.byref(h)))
         self._h = h
@@ -610,8 +706,8 @@ class Network:
     def reset(self, populations: bool = True, projections: bool = False, monitors: bool = True):
         """Network.reset (core/Network.py:628-670): time 0, populations back
         to their initial values with their spiking state and delay queues,
-        monitors emptied.  Projections keep their weights (no synaptic
-        variables besides w exist in this backend)."""
+        monitors emptied.  Projections keep their weights and synaptic
+        variables (the reference's default projections=False)."""
         _lib.check(_lib.lib().ahip_set_time(self._h, 0))
         if populations:
             for p in self.populations:

@@ -176,9 +176,29 @@ int ahip_proj_set_format(ahip_net* net, int32_t proj, const ahip_format_opts* op
 int ahip_proj_get_format(ahip_net* net, int32_t proj, ahip_format_opts* opts);
 /* nb_synapses(), nb_dendrites() */
 int ahip_proj_nb_synapses(ahip_net* net, int32_t proj, int64_t* nnz);
-/* get/set_local_attribute_all("w") in forward (row) order. */
+/* get/set_local_attribute_all("w") in forward (row) order (a plastic
+ * projection reads and writes its device weights). */
 int ahip_proj_get_w(ahip_net* net, int32_t proj, double* w, int64_t nnz);
 int ahip_proj_set_w(ahip_net* net, int32_t proj, const double* w, int64_t nnz);
+/* Plasticity: attach a generated spiking-synapse model (a shared library
+ * exporting ahip_synapse_describe(), include/annarchy_hip_synapse.h) to a
+ * spiking projection, as the reference's Projection(synapse=...) with the
+ * generated compute_psp / post_event code (Projection/OpenMPGenerator.py:
+ * 1057-1230, 1482-1815).  The projection then keeps one weight per synapse
+ * and the model's per-synapse attributes at their initial values, with
+ * _last_event = -10000; re-initialising the connectivity re-initialises them.
+ * Pre events replace the projection's plain spike scatter; post events run
+ * after every population update of the step.  Only a uniform delay is
+ * accepted with a synapse model. */
+int ahip_proj_set_synapse(ahip_net* net, int32_t proj, const char* synapse_lib);
+/* get/set_local_attribute_all / get/set_global_attribute of a synaptic
+ * attribute: n = nnz values in forward order for a per-synapse attribute
+ * (set also takes n = 1, one value for all), n = 1 for a per-projection
+ * one (Projection/OpenMP/CSR.py accessor templates). */
+int ahip_proj_syn_get(ahip_net* net, int32_t proj, const char* attr, double* values, int64_t n);
+int ahip_proj_syn_set(ahip_net* net, int32_t proj, const char* attr, const double* values, int64_t n);
+/* _last_event of every synapse (forward order), in steps. */
+int ahip_proj_get_last_event(ahip_net* net, int32_t proj, int64_t* values, int64_t n);
 /* Bytes the psp kernel must move per step (algorithmic, no padding). */
 int ahip_proj_psp_bytes(ahip_net* net, int32_t proj, double* bytes);
 /* Times `iters` back-to-back launches of the projection's psp kernel with HIP

@@ -26,6 +26,11 @@
 //                         test_SpikingTransmission.py:14-78 (5 spikes x w=1),
 //                         COBA: free-membrane exponential relaxation invariant
 //   orc_mlp_*             finite differences on the CPU (numpy)
+//   orc_plastic_run       hand-derived closed forms of single-synapse spike
+//                         pairings (trace decays between events, the order
+//                         g += w before the weight change, clipping, t_pre /
+//                         t_post) and the pre/post-spike known answers of
+//                         tests_python/SpikeSynapse/test_SpikingSynapse.py
 
 #include <algorithm>
 #include <climits>
@@ -544,6 +549,101 @@ int64_t orc_spiking_run(const orc_spiking_cfg* c, double* v, double* g_exc, doub
         t++;
     }
     return overflow ? -1 : rec;
+}
+
+// ---------------------------------------------------------------------------
+// Plastic spiking projection driven by given spike trains of its pre and post
+// populations (steps t0 .. t0+steps-1; no spike before t0).
+//   rule 0  STDP, models/Synapses.py:232-310; prm = tau_plus, tau_minus,
+//           A_plus, A_minus, w_min, w_max; x, y event-driven traces.
+//   rule 1  the t_pre / t_post rule of tests_python/SpikeSynapse/
+//           test_SpikingSynapse.py (test_TimeDependentUpdate); prm = tau_pre,
+//           tau_post, cApre, cApost, wmax; no event-driven variable.
+// Step t:
+//   compute_psp (Projection/OpenMP/CSR.py:870-885, OpenMPGenerator.py:
+//     1182-1230): for every pre spike of step t-1 in ascending rank, every
+//     synapse of its column in ascending post rank: event-driven decay to
+//     t-1, _last_event = t, g[post] += w, the pre_spike statements;
+//   neuron updates: last_spike of the neurons spiking at t becomes t
+//     (t_pre = last_spike[pre]*dt, t_post = last_spike[post]*dt,
+//     parser/ParserTemplate.py:18-19; t = t*dt);
+//   post_event (Projection/OpenMP/CSR.py:903-918, OpenMPGenerator.py:
+//     1575-1600): for every post spike of step t, its row: event-driven decay
+//     to t, _last_event = t, the post_spike statements.
+// State arrays (w, x, y, last_event) in forward (row) order, updated in
+// place; pre_last / post_last in place.  g_out[s*n_rows + i]: conductance
+// added to post neuron i by the pre events of step t0+s.  With a uniform
+// delay d the pre events of step t are the spikes of step t-d.
+// ---------------------------------------------------------------------------
+static inline double orc_clip(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+void orc_plastic_run(int32_t rule, int32_t n_rows, int32_t n_cols, const int64_t* row_ptr, const int32_t* col,
+                     double* w, double* x, double* y, int64_t* last_event, const double* prm, double dt,
+                     int64_t t0, int32_t steps, const int64_t* pre_ptr, const int32_t* pre_idx,
+                     const int64_t* post_ptr, const int32_t* post_idx, int64_t* pre_last, int64_t* post_last,
+                     int32_t delay, double* g_out) {
+    // CSRC view: per pre neuron, its synapses in ascending post rank
+    // (include/CSRCMatrix.hpp:164-210)
+    const int64_t nnz = row_ptr[n_rows];
+    std::vector<int64_t> col_ptr(n_cols + 1, 0), cur;
+    for (int64_t j = 0; j < nnz; j++) col_ptr[col[j] + 1]++;
+    for (int32_t c = 0; c < n_cols; c++) col_ptr[c + 1] += col_ptr[c];
+    cur.assign(col_ptr.begin(), col_ptr.end() - 1);
+    std::vector<int64_t> syn_of(nnz);
+    std::vector<int32_t> post_of(nnz);
+    for (int32_t r = 0; r < n_rows; r++)
+        for (int64_t j = row_ptr[r]; j < row_ptr[r + 1]; j++) {
+            const int64_t q = cur[col[j]]++;
+            syn_of[q] = j;
+            post_of[q] = r;
+        }
+    for (int32_t s = 0; s < steps; s++) {
+        const int64_t t = t0 + s;
+        double* g = g_out + (int64_t)s * n_rows;
+        for (int32_t i = 0; i < n_rows; i++) g[i] = 0.0;
+        // compute_psp: spikes of step t-d (uniform delay d >= 1; the
+        // event-driven decay still runs to t-1, OpenMPGenerator.py:1207-1211)
+        if (s >= delay) {
+            for (int64_t k = pre_ptr[s - delay]; k < pre_ptr[s - delay + 1]; k++) {
+                const int32_t pre = pre_idx[k];
+                for (int64_t q = col_ptr[pre]; q < col_ptr[pre + 1]; q++) {
+                    const int64_t j = syn_of[q];
+                    const int32_t post = post_of[q];
+                    if (rule == 0) {
+                        x[j] *= std::exp(dt * (double)(last_event[j] - (t - 1)) / (prm[0]));
+                        y[j] *= std::exp(dt * (double)(last_event[j] - (t - 1)) / (prm[1]));
+                        last_event[j] = t;
+                        g[post] += w[j];
+                        x[j] += prm[2] * prm[5];
+                        w[j] = orc_clip(w[j] + y[j], prm[4], prm[5]);
+                    } else {
+                        g[post] += w[j];
+                        const double t_post = (double)post_last[post] * dt;
+                        w[j] = orc_clip(w[j] - prm[3] * std::exp((t_post - (double)t * dt) / prm[1]), 0.0, prm[4]);
+                    }
+                }
+            }
+        }
+        // neuron updates: the spikes of step t
+        for (int64_t k = pre_ptr[s]; k < pre_ptr[s + 1]; k++) pre_last[pre_idx[k]] = t;
+        for (int64_t k = post_ptr[s]; k < post_ptr[s + 1]; k++) post_last[post_idx[k]] = t;
+        // post_event: spikes of step t
+        for (int64_t k = post_ptr[s]; k < post_ptr[s + 1]; k++) {
+            const int32_t i = post_idx[k];
+            for (int64_t j = row_ptr[i]; j < row_ptr[i + 1]; j++) {
+                if (rule == 0) {
+                    x[j] *= std::exp(dt * (double)(last_event[j] - t) / (prm[0]));
+                    y[j] *= std::exp(dt * (double)(last_event[j] - t) / (prm[1]));
+                    last_event[j] = t;
+                    y[j] -= prm[3] * prm[5];
+                    w[j] = orc_clip(w[j] + x[j], prm[4], prm[5]);
+                } else {
+                    const double t_pre = (double)pre_last[col[j]] * dt;
+                    w[j] = orc_clip(w[j] + prm[2] * std::exp((t_pre - (double)t * dt) / prm[0]), 0.0, prm[4]);
+                }
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------

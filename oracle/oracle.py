@@ -96,6 +96,9 @@ def lib():
         L.orc_sweep_instance_run.argtypes = [_I32, _PI64, _PI32, _PD, _I32, _PD, _PD, _D, _D, _D, _D,
                                              _PD, _I32, _I64, _PD]
         L.orc_sweep_instance_run.restype = None
+        L.orc_plastic_run.argtypes = [_I32, _I32, _I32, _PI64, _PI32, _PD, _PD, _PD, _PI64, _PD, _D, _I64,
+                                      _I32, _PI64, _PI32, _PI64, _PI32, _PI64, _PI64, _I32, _PD]
+        L.orc_plastic_run.restype = None
         L.orc_mlp_forward.argtypes = [_I32, _PD, _PD, _PD, _I32, _I32, _I32, _I32]
         L.orc_mlp_forward.restype = None
         L.orc_mlp_loss_grad.argtypes = [_I32, _PD, _PD, _PD, _PD, _I32, _I32, _I32, _I32]
@@ -385,3 +388,39 @@ def mlp_forward(X, params, dims=(16, 64, 64, 12)):
     out = np.zeros((X.shape[0], dims[-1]))
     lib().orc_mlp_forward(X.shape[0], _p(X, _D), _p(P, _D), _p(out, _D), *dims)
     return out
+
+
+def plastic_run(rule, row_ptr, col_idx, n_cols, w0, params, dt, pre_counts, pre_idx, post_counts, post_idx,
+                t0=0, x0=None, y0=None, last_event=None, pre_last=None, post_last=None, delay=1):
+    """orc_plastic_run: a plastic projection driven by the spike trains
+    (per-step counts and concatenated ascending ranks) of its pre and post
+    populations.  rule 0 STDP (params tau_plus, tau_minus, A_plus, A_minus,
+    w_min, w_max), rule 1 the t_pre / t_post rule (tau_pre, tau_post, cApre,
+    cApost, wmax); `delay` is the uniform delay in steps.  Returns w, x, y,
+    last_event (forward order), the last spikes and g[steps, n_rows]
+    (conductance added by each step's pre events)."""
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int64)
+    col_idx = np.ascontiguousarray(col_idx, dtype=np.int32)
+    n_rows = row_ptr.size - 1
+    nnz = int(row_ptr[-1])
+    steps = len(pre_counts)
+    assert len(post_counts) == steps
+    w = np.array(np.broadcast_to(np.asarray(w0, dtype=np.float64), (nnz,)))
+    x = np.zeros(nnz) if x0 is None else np.array(x0, dtype=np.float64)
+    y = np.zeros(nnz) if y0 is None else np.array(y0, dtype=np.float64)
+    le = np.full(nnz, -10000, dtype=np.int64) if last_event is None else np.array(last_event, dtype=np.int64)
+    pl = np.full(n_cols, -10000, dtype=np.int64) if pre_last is None else np.array(pre_last, dtype=np.int64)
+    ql = np.full(n_rows, -10000, dtype=np.int64) if post_last is None else np.array(post_last, dtype=np.int64)
+    pre_ptr = np.concatenate([[0], np.cumsum(pre_counts)]).astype(np.int64)
+    post_ptr = np.concatenate([[0], np.cumsum(post_counts)]).astype(np.int64)
+    pre_idx = np.ascontiguousarray(pre_idx, dtype=np.int32)
+    post_idx = np.ascontiguousarray(post_idx, dtype=np.int32)
+    prm = np.ascontiguousarray(params, dtype=np.float64)
+    g = np.zeros((steps, n_rows))
+    e32 = np.zeros(1, dtype=np.int32)
+    lib().orc_plastic_run(rule, n_rows, n_cols, _p(row_ptr, C.c_int64), _p(col_idx, C.c_int32), _p(w, C.c_double),
+                          _p(x, C.c_double), _p(y, C.c_double), _p(le, C.c_int64), _p(prm, C.c_double), dt, t0,
+                          steps, _p(pre_ptr, C.c_int64), _p(pre_idx if pre_idx.size else e32, C.c_int32),
+                          _p(post_ptr, C.c_int64), _p(post_idx if post_idx.size else e32, C.c_int32),
+                          _p(pl, C.c_int64), _p(ql, C.c_int64), int(delay), _p(g, C.c_double))
+    return {"w": w, "x": x, "y": y, "last_event": le, "pre_last": pl, "post_last": ql, "g": g}

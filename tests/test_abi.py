@@ -16,7 +16,7 @@ ROOT = Path(__file__).resolve().parent.parent
 
 def _declared(header: str) -> list[str]:
     text = (ROOT / "include" / header).read_text()
-    return sorted(set(re.findall(r"\b(ahip_\w+)\s*\(", text)) - {"ahip_model_describe"})
+    return sorted(set(re.findall(r"\b(ahip_\w+)\s*\(", text)) - {"ahip_model_describe", "ahip_synapse_describe"})
 
 
 def test_core_library_exports_every_declared_symbol():
@@ -37,6 +37,28 @@ def test_model_libraries_export_descriptor():
     for p in libs:
         L = C.CDLL(str(p))
         assert hasattr(L, "ahip_model_describe")
+
+
+class _SynDesc(C.Structure):
+    """Leading fields of ahip_synapse_desc (include/annarchy_hip_synapse.h)."""
+    _fields_ = [("abi", C.c_int32), ("name", C.c_char_p), ("n_attr", C.c_int32),
+                ("attr_names", C.POINTER(C.c_char_p)), ("attr_local", C.POINTER(C.c_int32)),
+                ("attr_init", C.POINTER(C.c_double)), ("has_pre", C.c_int32), ("has_post", C.c_int32),
+                ("pre_g", C.c_int32)]
+
+
+def test_synapse_library_descriptor():
+    import annarchy_amd as ah
+    from annarchy_amd.network import synapse_library
+    path = synapse_library(ah.STDP())
+    L = C.CDLL(str(path))
+    L.ahip_synapse_describe.restype = C.POINTER(_SynDesc)
+    d = L.ahip_synapse_describe().contents
+    assert d.abi == 1 and d.name == b"STDP" and d.n_attr == 8
+    names = [d.attr_names[k].decode() for k in range(d.n_attr)]
+    assert names == ["tau_plus", "tau_minus", "A_plus", "A_minus", "w_min", "w_max", "x", "y"]
+    assert [d.attr_local[k] for k in range(d.n_attr)] == [0] * 6 + [1, 1]
+    assert d.attr_init[0] == 20.0 and d.has_pre == 1 and d.has_post == 1 and d.pre_g == 1
 
 
 def test_error_path_without_device_calls():
