@@ -282,15 +282,21 @@ def generate_from_text(name: str, parameters: str = "", equations: str = "", spi
 
 
 # ---------------------------------------------------------------------------
-# Spiking synapse models (pre / post events)
+# Synapse models (pre / post events, continuous updates)
 # ---------------------------------------------------------------------------
 
-SYNAPSE_ABI = 1  # AHIP_SYNAPSE_ABI in include/annarchy_hip_synapse.h
+SYNAPSE_ABI = 2  # AHIP_SYNAPSE_ABI in include/annarchy_hip_synapse.h
 
 _SYN_KERNELS = r"""
 #include "annarchy_hip_synapse.h"
 
 namespace {
+
+__device__ inline bool update_due(const ahip_syn_args& a, int64_t t) {
+    if (a.update_period <= 1) return true;
+    int64_t m = (t - a.update_offset) %% a.update_period;
+    return m == 0;
+}
 
 // Pre events (compute_psp of step t, spikes of step t-d): one wave per
 // spike, lanes over the synapses of the pre neuron's column.  A synapse
@@ -301,15 +307,15 @@ __global__ __launch_bounds__(256) void pre_event(ahip_syn_args a) {
     const int wid = threadIdx.x / ahip::kWave;
     const int64_t t = *a.t_base + a.step;
     const T dt = (T)a.dt;
-    (void)dt;
+    const int32_t _plasticity = a.plasticity;
     T* __restrict__ w = (T*)a.w;
     T* __restrict__ _g_post = (T*)a.g;
     int64_t* __restrict__ _last_event = a.last_event;
     const int64_t* __restrict__ _pre_last_spike = a.pre_last_spike;
     const int64_t* __restrict__ _post_last_spike = a.post_last_spike;
-    (void)_last_event; (void)_pre_last_spike; (void)_post_last_spike;
-%(globals)s
-%(locals)s
+    (void)dt; (void)_plasticity; (void)_last_event; (void)_pre_last_spike; (void)_post_last_spike;
+%(ev_attrs)s
+%(deps)s
     int64_t slot = 0;
     if (a.pre_depth > 1) {
         slot = (t - a.pre_delay) %% a.pre_depth;
@@ -324,7 +330,6 @@ __global__ __launch_bounds__(256) void pre_event(ahip_syn_args a) {
         const int64_t e = a.col_ptr[pre + 1];
         for (int64_t q = a.col_ptr[pre] + lane; q < e; q += ahip::kWave) {
             const int32_t post = a.row_idx[q];
-            (void)post;
 %(pre_body)s
         }
     }
@@ -338,14 +343,14 @@ __global__ __launch_bounds__(256) void post_event(ahip_syn_args a) {
     const int wid = threadIdx.x / ahip::kWave;
     const int64_t t = *a.t_base + a.step;
     const T dt = (T)a.dt;
-    (void)dt;
+    const int32_t _plasticity = a.plasticity;
     T* __restrict__ w = (T*)a.w;
     int64_t* __restrict__ _last_event = a.last_event;
     const int64_t* __restrict__ _pre_last_spike = a.pre_last_spike;
     const int64_t* __restrict__ _post_last_spike = a.post_last_spike;
-    (void)_last_event; (void)_pre_last_spike; (void)_post_last_spike;
-%(globals)s
-%(locals)s
+    (void)dt; (void)_plasticity; (void)_last_event; (void)_pre_last_spike; (void)_post_last_spike;
+%(ev_attrs)s
+%(deps)s
     const int32_t n = *a.post_count;
     const int64_t waves = (int64_t)gridDim.x * 4;
     for (int64_t s = (int64_t)blockIdx.x * 4 + wid; s < n; s += waves) {
@@ -353,7 +358,7 @@ __global__ __launch_bounds__(256) void post_event(ahip_syn_args a) {
         if (i < 0 || i >= a.n_rows) continue;
         const int64_t e = a.row_ptr[i + 1];
         for (int64_t j = a.row_ptr[i] + lane; j < e; j += ahip::kWave) {
-            const int64_t q = a.fwd2col[j];
+            const int64_t q = a.fwd2col ? a.fwd2col[j] : j;
             const int32_t pre = a.col_idx[j];
             const int32_t post = i;
             (void)pre; (void)post;
@@ -362,31 +367,98 @@ __global__ __launch_bounds__(256) void post_event(ahip_syn_args a) {
     }
 }
 
-template <typename T, bool PRE>
-int launch(const ahip_syn_args* a, void* stream) {
-    const int32_t units = PRE ? a->n_cols : a->n_rows;
+// update_synapse (Projection/OpenMP/CSR.py:833-865), once every
+// update_period steps with dt scaled by the period
+// (OpenMPGenerator.py:1722-1725, :1805).  Per-projection equations run in a
+// one-thread kernel first.
+template <typename T>
+__global__ void update_global(ahip_syn_args a) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int64_t t = *a.t_base + a.step;
+    if (!update_due(a, t)) return;
+    const T dt = (T)(a.dt * (double)(a.update_period > 1 ? a.update_period : 1));
+    const int32_t _plasticity = a.plasticity;
+    (void)dt; (void)_plasticity;
+%(glob_regs)s
+%(preloop)s
+%(global_eqs)s
+%(glob_store)s
+}
+
+// Then one wave per post row: the per-post-neuron equations (evaluated in
+// every lane on the same values, stored by lane 0), then the per-synapse
+// equations, one lane per synapse.
+template <typename T>
+__global__ __launch_bounds__(256) void update_local(ahip_syn_args a) {
+    const int lane = threadIdx.x & (ahip::kWave - 1);
+    const int wid = threadIdx.x / ahip::kWave;
+    const int64_t t = *a.t_base + a.step;
+    if (!update_due(a, t)) return;
+    const T dt = (T)(a.dt * (double)(a.update_period > 1 ? a.update_period : 1));
+    const int32_t _plasticity = a.plasticity;
+    T* __restrict__ w = (T*)a.w;
+    const int64_t* __restrict__ _pre_last_spike = a.pre_last_spike;
+    const int64_t* __restrict__ _post_last_spike = a.post_last_spike;
+    (void)dt; (void)_plasticity; (void)w; (void)_pre_last_spike; (void)_post_last_spike;
+%(up_attrs)s
+%(deps)s
+%(preloop)s
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    for (int64_t row = (int64_t)blockIdx.x * 4 + wid; row < a.n_rows; row += waves) {
+        const int32_t post = (int32_t)row;
+        (void)post;
+%(sg_load)s
+%(semiglobal_eqs)s
+%(sg_store)s
+        const int64_t e = a.row_ptr[row + 1];
+        for (int64_t j = a.row_ptr[row] + lane; j < e; j += ahip::kWave) {
+            const int64_t q = a.fwd2col ? a.fwd2col[j] : j;
+            const int32_t pre = a.col_idx[j];
+            (void)q; (void)pre;
+%(local_eqs)s
+        }
+    }
+    (void)lane;
+}
+
+template <typename T>
+int launch_events(const ahip_syn_args* a, void* stream, bool pre) {
+    const int32_t units = pre ? a->n_cols : a->n_rows;
     int nb = (units + 3) / 4;
     if (nb > 512) nb = 512;
     if (nb < 1) nb = 1;
-    if (PRE)
+    if (pre)
         hipLaunchKernelGGL(pre_event<T>, dim3(nb), dim3(256), 0, (hipStream_t)stream, *a);
     else
         hipLaunchKernelGGL(post_event<T>, dim3(nb), dim3(256), 0, (hipStream_t)stream, *a);
     return (int)hipGetLastError();
 }
 
-int pre_f64(const ahip_syn_args* a, void* s) { return launch<double, true>(a, s); }
-int pre_f32(const ahip_syn_args* a, void* s) { return launch<float, true>(a, s); }
-int post_f64(const ahip_syn_args* a, void* s) { return launch<double, false>(a, s); }
-int post_f32(const ahip_syn_args* a, void* s) { return launch<float, false>(a, s); }
+template <typename T>
+int launch_update(const ahip_syn_args* a, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+%(launch_global)s
+%(launch_local)s
+    return (int)hipGetLastError();
+}
+
+int pre_f64(const ahip_syn_args* a, void* s) { return launch_events<double>(a, s, true); }
+int pre_f32(const ahip_syn_args* a, void* s) { return launch_events<float>(a, s, true); }
+int post_f64(const ahip_syn_args* a, void* s) { return launch_events<double>(a, s, false); }
+int post_f32(const ahip_syn_args* a, void* s) { return launch_events<float>(a, s, false); }
+int update_f64(const ahip_syn_args* a, void* s) { return launch_update<double>(a, s); }
+int update_f32(const ahip_syn_args* a, void* s) { return launch_update<float>(a, s); }
 
 const char* const kAttrNames[] = {%(attr_names)s};
-const int32_t kAttrLocal[] = {%(attr_local)s};
+const int32_t kAttrLocality[] = {%(attr_loc)s};
 const double kAttrInit[] = {%(attr_init)s};
+const char* const kPreDeps[] = {%(pre_deps)s};
+const char* const kPostDeps[] = {%(post_deps)s};
 
 const ahip_synapse_desc kDesc = {
-    %(abi)d, "%(name)s", %(n_attr)d, kAttrNames, kAttrLocal, kAttrInit, %(has_pre)d, %(has_post)d, %(pre_g)d,
-    pre_f64, pre_f32, post_f64, post_f32, "%(hash)s"};
+    %(abi)d, "%(name)s", %(spiking)d, %(n_attr)d, kAttrNames, kAttrLocality, kAttrInit,
+    %(n_pre_deps)d, kPreDeps, %(n_post_deps)d, kPostDeps, %(has_pre)d, %(has_post)d, %(has_update)d,
+    pre_f64, pre_f32, post_f64, post_f32, update_f64, update_f32, "%(hash)s"};
 
 }  // namespace
 
@@ -400,58 +472,123 @@ class SynapseSource:
     def __init__(self, desc, source: str, attrs: list):
         self.desc = desc
         self.source = source
-        self.attrs = attrs  # (name, local, init)
+        self.attrs = attrs  # (name, locality, init)
 
     @property
     def name(self) -> str:
         return self.desc.name
 
 
+def _syn_subst(code: str, t_text: str, sg_index: str) -> str:
+    code = code.replace("%(t)s", t_text)
+    return (code.replace("%(local_index)s", "[q]").replace("%(global_index)s", "")
+            .replace("%(semiglobal_index)s", sg_index))
+
+
 def generate_synapse(desc) -> SynapseSource:
-    """Build the HIP source of the pre / post event kernels of a synapse model."""
-    attrs = [(a.name, a.locality == "local", a.init) for a in desc.parameters + desc.variables]
+    """Build the HIP source of the event and update kernels of a synapse model."""
+    from .equations import SYN_LOCALITY, synapse_update_code
+
+    attrs = desc.attrs()
     if len(attrs) > 16:
         raise ValueError("at most 16 synapse attributes besides w")
+    if len(desc.pre_deps) > 8 or len(desc.post_deps) > 8:
+        raise ValueError("at most 8 pre- and 8 post-synaptic dependencies")
     slot = {a[0]: k for k, a in enumerate(attrs)}
-    globals_ = "\n".join(f"    const T _s_{_c_name(n)} = (T)a.global[{slot[n]}];" for n, loc, _ in attrs if not loc)
-    locals_ = "\n".join(f"    T* __restrict__ _s_{_c_name(n)} = (T*)a.local[{slot[n]}];" for n, loc, _ in attrs
-                         if loc)
-
-    def subst(code: str, t_text: str) -> str:
-        code = code.replace("%(t)s", t_text)
-        return code.replace("%(local_index)s", "[q]").replace("%(global_index)s", "")
+    ptr = "    T* __restrict__ {p}_{n} = (T*)a.attr[{k}];"
+    load = "    const T _s_{n} = ((const T*)a.attr[{k}])[0];"
+    # event kernels: per-synapse and per-post-neuron attributes by pointer, per-projection by value
+    ev = []
+    for n, loc, _ in attrs:
+        ev.append(load.format(n=_c_name(n), k=slot[n]) if loc == "global"
+                  else ptr.format(p="_s", n=_c_name(n), k=slot[n]))
+    # update kernel: semiglobal attributes in registers per row
+    up, sg_load, sg_store = [], [], []
+    for n, loc, _ in attrs:
+        c = _c_name(n)
+        if loc == "global":
+            up.append(load.format(n=c, k=slot[n]))
+        elif loc == "local":
+            up.append(ptr.format(p="_s", n=c, k=slot[n]))
+        else:
+            up.append(ptr.format(p="_sp", n=c, k=slot[n]))
+            sg_load.append(f"        T _s_{c} = _sp_{c}[post];")
+            if any(v.name == n and v.cpp for v in desc.variables):
+                sg_store.append(f"        if (lane == 0) _sp_{c}[post] = _s_{c};")
+    glob_regs, glob_store = [], []
+    for n, loc, _ in attrs:
+        if loc != "global":
+            continue
+        c = _c_name(n)
+        glob_regs.append(f"    T _s_{c} = ((T*)a.attr[{slot[n]}])[0];")
+        if any(v.name == n and v.cpp for v in desc.variables):
+            glob_store.append(f"    ((T*)a.attr[{slot[n]}])[0] = _s_{c};")
+    deps = []
+    for side, lst in (("pre", desc.pre_deps), ("post", desc.post_deps)):
+        for k, n in enumerate(lst):
+            deps.append(f"    const T* __restrict__ _d{side}_{n} = (const T*)a.{side}_attr[{k}];")
+            deps.append(f"    const int32_t _st_{side}_{n} = a.{side}_attr_stride[{k}];")
+            deps.append(f"    (void)_d{side}_{n}; (void)_st_{side}_{n};")
+    preloop = "\n".join(f"    const T {n} = (T)({_syn_subst(v, '(t)', '')});" for n, v in desc.preloop)
 
     pre, post = [], []
     # Projection/OpenMPGenerator.py:1182-1230: event-driven decay to t-1,
-    # _last_event = t, then g_target += w (the default transmission when the
-    # model does not write g_target itself), then the pre_spike statements
+    # _last_event = t, then g_target += <expr> (w when the model does not
+    # write g_target), then the pre_spike statements
     for _, code in desc.event_driven:
-        pre.append(subst(code, "(t-1)"))
+        pre.append(_syn_subst(code, "(t-1)", "[post]"))
     if desc.event_driven:
         pre.append("_last_event[q] = t;")
-    pre.append("unsafeAtomicAdd(_g_post + post, w[q]);")
-    pre += [subst(c, "(t)") for c in desc.pre_code]
+    g = _syn_subst(desc.g_code, "(t)", "[post]") if desc.g_code else "w[q]"
+    pre.append(f"unsafeAtomicAdd(_g_post + post, (T)({g}));")
+    pre += [_syn_subst(c, "(t)", "[post]") for c in desc.pre_code]
     if desc.post_code:
         # OpenMPGenerator.py:1575-1600: event-driven decay to t, _last_event = t
         for _, code in desc.event_driven:
-            post.append(subst(code, "(t)"))
+            post.append(_syn_subst(code, "(t)", "[post]"))
         if desc.event_driven:
             post.append("_last_event[q] = t;")
-        post += [subst(c, "(t)") for c in desc.post_code]
+        post += [_syn_subst(c, "(t)", "[post]") for c in desc.post_code]
+    global_eqs = _syn_subst(synapse_update_code(desc, "global"), "(t)", "")
+    sg_eqs = _syn_subst(synapse_update_code(desc, "semiglobal"), "(t)", "")
+    local_eqs = _syn_subst(synapse_update_code(desc, "local"), "(t)", "")
+    has_global = bool(global_eqs.strip())
+    has_rows = bool(sg_eqs.strip() or local_eqs.strip())
+    launch_global = ("    hipLaunchKernelGGL(update_global<T>, dim3(1), dim3(64), 0, s, *a);"
+                     if has_global else "")
+    launch_local = ("    {\n        int nb = (a->n_rows + 3) / 4;\n        if (nb > 1024) nb = 1024;\n"
+                    "        if (nb > 0) hipLaunchKernelGGL(update_local<T>, dim3(nb), dim3(256), 0, s, *a);\n    }"
+                    if has_rows else "")
     body = _PRELUDE % {"name": desc.name} + "}  // namespace\n" + _SYN_KERNELS % {
-        "globals": globals_,
-        "locals": locals_,
+        "ev_attrs": "\n".join(ev),
+        "up_attrs": "\n".join(up),
+        "deps": "\n".join(deps),
+        "preloop": preloop,
+        "sg_load": "\n".join(sg_load),
+        "sg_store": "\n".join(sg_store),
+        "glob_regs": "\n".join(glob_regs),
+        "glob_store": "\n".join(glob_store),
+        "global_eqs": _indent(global_eqs, 4),
+        "semiglobal_eqs": _indent(sg_eqs, 8),
+        "local_eqs": _indent(local_eqs, 12),
         "pre_body": _indent("\n".join(pre), 12),
         "post_body": _indent("\n".join(post), 12),
+        "launch_global": launch_global,
+        "launch_local": launch_local,
         "attr_names": ", ".join(f'"{n}"' for n, _, _ in attrs) or "nullptr",
-        "attr_local": ", ".join(str(int(loc)) for _, loc, _ in attrs) or "0",
+        "attr_loc": ", ".join(str(SYN_LOCALITY[loc]) for _, loc, _ in attrs) or "0",
         "attr_init": ", ".join(repr(float(v)) for _, _, v in attrs) or "0.0",
+        "pre_deps": ", ".join(f'"{n}"' for n in desc.pre_deps) or "nullptr",
+        "post_deps": ", ".join(f'"{n}"' for n in desc.post_deps) or "nullptr",
+        "n_pre_deps": len(desc.pre_deps),
+        "n_post_deps": len(desc.post_deps),
         "abi": SYNAPSE_ABI,
         "name": desc.name,
+        "spiking": int(desc.spiking),
         "n_attr": len(attrs),
-        "has_pre": int(bool(pre)),
+        "has_pre": int(desc.spiking),
         "has_post": int(bool(post)),
-        "pre_g": 1,
+        "has_update": int(has_global or has_rows),
         "hash": "%(hash)s",
     }
     h = hashlib.sha1(body.encode()).hexdigest()[:12]

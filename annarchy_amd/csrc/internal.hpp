@@ -103,12 +103,14 @@ struct Proj {
     DevBuf delay_dev;             // int16: rate -> forward order, spiking -> column order
     DevBuf ring;                  // spiking per-synapse delays: max_delay x n_rows events
     DevBuf xd;                    // rate uniform delay: the delayed pre-synaptic rates
-    // plasticity: a generated synapse model and its per-synapse state (column order)
+    // plasticity: a generated synapse model and its state.  State order:
+    // spiking -> column order (as w_col), rate -> forward order (CSR val).
     void* syn_dl = nullptr;
     std::string syn_path;
     const ahip_synapse_desc* syn = nullptr;
-    std::vector<DevBuf> syn_local;   // per attribute; empty for per-projection ones
-    std::vector<double> syn_global;  // per-projection values
+    std::vector<DevBuf> syn_attr;            // per attribute: nnz / n_rows / 1 values
+    std::vector<int32_t> dep_pre, dep_post;  // slots of the pre.<x> / post.<x> attributes
+    int32_t syn_update = 1, syn_plasticity = 1, syn_period = 1, syn_offset = 0;
     DevBuf last_event, fwd2col, row_ptr_dev, col_dev;
     std::vector<int64_t> h_inv_idx;
 };

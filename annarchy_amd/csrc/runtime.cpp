@@ -25,6 +25,8 @@
 
 using namespace ahip;
 
+static int syn_read(ahip_net* net, Proj& p, const void* dev, double* out, int64_t n, bool per_synapse);
+
 namespace {
 
 thread_local std::string g_err;
@@ -180,32 +182,34 @@ int ensure_delay_buffers(ahip_net* net) {
     return 0;
 }
 
-// Per-synapse state of a plastic projection, column order as w_col: every
-// attribute at its initial value, _last_event = -10000 (the reference's
-// initial value, Projection/CUDA/CSR.py:280-291), the forward -> column map
-// of the post events and the forward row pointer.
+// Synaptic state of a plastic projection (spiking: column order as w_col;
+// rate-coded: forward order as the CSR weights): every attribute at its
+// initial value, one per synapse / post neuron / projection by locality;
+// _last_event = -10000 (the reference's initial value,
+// Projection/CUDA/CSR.py:280-291); the forward row pointer and columns, and
+// for spiking projections the forward -> column map.
 int init_syn_state(ahip_net* net, Proj& p) {
     const ahip_synapse_desc* d = p.syn;
     const size_t es = net->elem();
     HIPCHK(hipStreamSynchronize(net->stream));
-    p.syn_local.clear();
-    p.syn_local.resize(d->n_attr);
-    p.syn_global.assign(d->n_attr, 0.0);
+    p.syn_attr.clear();
+    p.syn_attr.resize(d->n_attr);
     int rc;
     for (int32_t k = 0; k < d->n_attr; k++) {
-        if (d->attr_local[k]) {
-            if ((rc = dev_alloc(p.syn_local[k], (size_t)p.nnz * es))) return rc;
-            LCHK(launch_fill(p.syn_local[k].p, p.nnz, (int32_t)es, d->attr_init[k], net->stream));
-        } else {
-            p.syn_global[k] = d->attr_init[k];
-        }
+        const int64_t n = d->attr_locality[k] == AHIP_SYN_LOCAL ? p.nnz
+                          : d->attr_locality[k] == AHIP_SYN_SEMIGLOBAL ? p.n_rows : 1;
+        if ((rc = dev_alloc(p.syn_attr[k], (size_t)n * es))) return rc;
+        LCHK(launch_fill(p.syn_attr[k].p, n, (int32_t)es, d->attr_init[k], net->stream));
     }
     if ((rc = dev_alloc(p.last_event, (size_t)p.nnz * sizeof(int64_t)))) return rc;
     LCHK(launch_fill_i64(p.last_event.as<int64_t>(), p.nnz, -10000, net->stream));
-    std::vector<int64_t> f2c(p.nnz);
-    for (int64_t q = 0; q < p.nnz; q++) f2c[p.h_inv_idx[q]] = q;
-    if ((rc = dev_alloc(p.fwd2col, (size_t)p.nnz * sizeof(int64_t)))) return rc;
-    if (p.nnz) HIPCHK(hipMemcpy(p.fwd2col.p, f2c.data(), f2c.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    p.fwd2col.release();
+    if (p.spiking) {
+        std::vector<int64_t> f2c(p.nnz);
+        for (int64_t q = 0; q < p.nnz; q++) f2c[p.h_inv_idx[q]] = q;
+        if ((rc = dev_alloc(p.fwd2col, (size_t)p.nnz * sizeof(int64_t)))) return rc;
+        if (p.nnz) HIPCHK(hipMemcpy(p.fwd2col.p, f2c.data(), f2c.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    }
     if ((rc = dev_alloc(p.row_ptr_dev, p.h_row_ptr.size() * sizeof(int64_t)))) return rc;
     HIPCHK(hipMemcpy(p.row_ptr_dev.p, p.h_row_ptr.data(), p.h_row_ptr.size() * sizeof(int64_t),
                      hipMemcpyHostToDevice));
@@ -215,50 +219,67 @@ int init_syn_state(ahip_net* net, Proj& p) {
     return 0;
 }
 
-// Pre (compute_psp) or post events of a plastic projection at `step`.
-int launch_syn_events(ahip_net* net, Proj& p, bool pre_events, int32_t step) {
+// Weights of a plastic projection on the device (state order).
+void* syn_w(Proj& p) { return p.spiking ? p.w_col.p : p.rate.val.p; }
+
+// Pre events (kind 0), update_synapse (1) or post events (2) of a plastic
+// projection at `step`.
+int launch_syn(ahip_net* net, Proj& p, int kind, int32_t step) {
     Pop& pre = *net->pops[p.pre];
     Pop& post = *net->pops[p.post];
     ahip_syn_args a;
     std::memset(&a, 0, sizeof(a));
-    if (p.delay_uniform >= 2) {
-        a.pre_spiked = pre.hist.as<int32_t>();
-        a.pre_count = pre.hist_count.as<int32_t>();
-        a.pre_stride = pre.size;
-        a.pre_depth = pre.depth;
-        a.pre_delay = p.delay_uniform;
-    } else {
-        a.pre_spiked = pre.spiked.as<int32_t>();
-        a.pre_count = pre.spike_count.as<int32_t>();
-        a.pre_depth = 1;
+    if (p.spiking) {
+        if (p.delay_uniform >= 2) {
+            a.pre_spiked = pre.hist.as<int32_t>();
+            a.pre_count = pre.hist_count.as<int32_t>();
+            a.pre_stride = pre.size;
+            a.pre_depth = pre.depth;
+            a.pre_delay = p.delay_uniform;
+        } else {
+            a.pre_spiked = pre.spiked.as<int32_t>();
+            a.pre_count = pre.spike_count.as<int32_t>();
+            a.pre_depth = 1;
+        }
+        a.col_ptr = p.col_ptr.as<int64_t>();
+        a.row_idx = p.row_idx.as<int32_t>();
+        a.post_spiked = post.spiked.as<int32_t>();
+        a.post_count = post.spike_count.as<int32_t>();
+        a.fwd2col = p.fwd2col.as<int64_t>();
+        a.g = post.slots[p.target_slot].p;
     }
     a.n_cols = p.n_cols;
-    a.col_ptr = p.col_ptr.as<int64_t>();
-    a.row_idx = p.row_idx.as<int32_t>();
-    a.post_spiked = post.spiked.as<int32_t>();
-    a.post_count = post.spike_count.as<int32_t>();
     a.n_rows = p.n_rows;
     a.row_ptr = p.row_ptr_dev.as<int64_t>();
     a.col_idx = p.col_dev.as<int32_t>();
-    a.fwd2col = p.fwd2col.as<int64_t>();
     a.pre_last_spike = pre.last_spike.as<int64_t>();
     a.post_last_spike = post.last_spike.as<int64_t>();
-    a.g = post.slots[p.target_slot].p;
-    a.w = p.w_col.p;
+    a.w = syn_w(p);
     a.last_event = p.last_event.as<int64_t>();
-    for (int32_t k = 0; k < p.syn->n_attr; k++) {
-        a.local[k] = p.syn_local[k].p;
-        a.global[k] = p.syn_global[k];
+    for (int32_t k = 0; k < p.syn->n_attr; k++) a.attr[k] = p.syn_attr[k].p;
+    for (size_t k = 0; k < p.dep_pre.size(); k++) {
+        a.pre_attr[k] = pre.slots[p.dep_pre[k]].p;
+        a.pre_attr_stride[k] = pre.model->slot_local[p.dep_pre[k]] ? 1 : 0;
+    }
+    for (size_t k = 0; k < p.dep_post.size(); k++) {
+        a.post_attr[k] = post.slots[p.dep_post[k]].p;
+        a.post_attr_stride[k] = post.model->slot_local[p.dep_post[k]] ? 1 : 0;
     }
     a.t_base = net->t_dev.as<int64_t>();
     a.step = step;
     a.dt = net->dt;
+    a.plasticity = p.syn_plasticity;
+    a.update_period = p.syn_period;
+    a.update_offset = p.syn_offset;
     const bool f64 = net->precision == AHIP_F64;
-    ahip_syn_launch_fn fn = pre_events ? (f64 ? p.syn->pre_f64 : p.syn->pre_f32)
-                                       : (f64 ? p.syn->post_f64 : p.syn->post_f32);
+    const ahip_synapse_desc* d = p.syn;
+    ahip_syn_launch_fn fn = kind == 0 ? (f64 ? d->pre_f64 : d->pre_f32)
+                            : kind == 1 ? (f64 ? d->update_f64 : d->update_f32)
+                                        : (f64 ? d->post_f64 : d->post_f32);
     const int rc = fn(&a, (void*)net->stream);
     if (rc != 0)
-        return fail(AHIP_EHIP, "%s events of projection %d failed: %s", pre_events ? "pre" : "post", p.id,
+        return fail(AHIP_EHIP, "%s of projection %d failed: %s",
+                    kind == 0 ? "pre events" : kind == 1 ? "update_synapse" : "post events", p.id,
                     hipGetErrorString((hipError_t)rc));
     return 0;
 }
@@ -415,7 +436,7 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
         if (p.spiking) {
             if (p.syn) {
                 if (p.nnz > 0) {
-                    int rc = launch_syn_events(net, p, true, step);
+                    int rc = launch_syn(net, p, 0, step);
                     if (rc) return rc;
                 }
             } else if (p.delay_per_syn && p.nnz > 0) {
@@ -461,13 +482,19 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
             }
         }
     }
-    // post events of the plastic projections, after every population update
-    // (the reference's post_event, OpenMPBaseTemplate.py:217-325)
+    // plastic projections, after every population update: update_synapse,
+    // then the post events (OpenMPBaseTemplate.py:217-325)
     for (auto& pp : net->projs) {
         Proj& p = *pp;
-        if (!p.syn || !p.syn->has_post || p.nnz == 0) continue;
-        int rc = launch_syn_events(net, p, false, step);
-        if (rc) return rc;
+        if (!p.syn || p.nnz == 0) continue;
+        if (p.syn->has_update && p.syn_update) {
+            int rc = launch_syn(net, p, 1, step);
+            if (rc) return rc;
+        }
+        if (p.spiking && p.syn->has_post) {
+            int rc = launch_syn(net, p, 2, step);
+            if (rc) return rc;
+        }
     }
     // 5. record variables and spikes
     for (auto& mp : net->mons) {
@@ -1033,10 +1060,14 @@ int ahip_proj_init_csr(ahip_net* net, int32_t proj, int32_t n_rows, int32_t n_co
     }
     ahip_format_opts o{AHIP_FMT_CSR, 64, 0, 1, 0};
     if (opts) o = *opts;
+    if (p->syn && o.format != AHIP_FMT_CSR && o.format != AHIP_FMT_CSR_VEC) {
+        // a plastic rate-coded projection updates its forward-order weights in place
+        o = ahip_format_opts{AHIP_FMT_CSR, 64, 0, 1, 0};
+    }
     std::string err;
     int rc = build_layout(*p, net->precision, o, p->rate, err);
     if (rc) return fail(rc, "%s", err.c_str());
-    return 0;
+    return p->syn ? init_syn_state(net, *p) : 0;
 }
 
 int ahip_proj_generate(ahip_net* net, int32_t proj, const ahip_connect_spec* c, const ahip_format_opts* opts) {
@@ -1114,10 +1145,17 @@ int ahip_proj_set_format(ahip_net* net, int32_t proj, const ahip_format_opts* op
     Proj* p = get_proj(net, proj);
     if (!p || !opts || p->nnz < 0) return fail(AHIP_EINVAL, "bad projection");
     if (p->spiking) return fail(AHIP_EINVAL, "spiking projections use the CSRC backward view");
+    if (p->syn && opts->format != AHIP_FMT_CSR && opts->format != AHIP_FMT_CSR_VEC)
+        return fail(AHIP_EINVAL, "a plastic rate-coded projection keeps a csr format");
     if (p->delay_per_syn && opts->format != AHIP_FMT_CSR)
         return fail(AHIP_EINVAL, "per-synapse delays of a rate-coded projection need the csr format");
     HIPCHK(hipStreamSynchronize(net->stream));
     invalidate(net);
+    if (p->syn) {
+        // the learned weights are on the device: refresh the host copy first
+        int rc = syn_read(net, *p, p->rate.val.p, p->h_w.data(), p->nnz, true);
+        if (rc) return rc;
+    }
     std::string err;
     RateLayout L;
     int rc = build_layout(*p, net->precision, *opts, L, err);
@@ -1149,6 +1187,9 @@ int ahip_proj_set_delays(ahip_net* net, int32_t proj, const int32_t* steps, int6
     }
     if (n > 0 && (lo < 1 || hi > 32767)) return fail(AHIP_EINVAL, "delays must be 1..32767 steps");
     if (p->syn && !(n == 1 || lo == hi)) return fail(AHIP_EINVAL, "a plastic projection takes one uniform delay");
+    if (p->syn && (!p->spiking || !p->dep_pre.empty()) && n > 0 && hi > 1)
+        return fail(AHIP_EINVAL, "delays are not supported for plastic projections reading pre-synaptic "
+                                 "attributes or rate-coded plastic projections");
     HIPCHK(hipStreamSynchronize(net->stream));
     invalidate(net);
     p->delay_dev.release();
@@ -1241,31 +1282,35 @@ int ahip_proj_nb_synapses(ahip_net* net, int32_t proj, int64_t* nnz) {
     return 0;
 }
 
-// Column-order device values of a plastic projection -> forward order.
-static int syn_read(ahip_net* net, Proj& p, const void* dev, double* out) {
-    std::vector<double> col(p.nnz);
+// Device values of a plastic projection (n of them; state order) -> host,
+// forward order for per-synapse values of a spiking projection.
+static int syn_read(ahip_net* net, Proj& p, const void* dev, double* out, int64_t n, bool per_synapse) {
+    std::vector<double> v(n);
     HIPCHK(hipStreamSynchronize(net->stream));
     if (net->precision == AHIP_F64) {
-        if (p.nnz) HIPCHK(hipMemcpy(col.data(), dev, p.nnz * sizeof(double), hipMemcpyDeviceToHost));
+        if (n) HIPCHK(hipMemcpy(v.data(), dev, n * sizeof(double), hipMemcpyDeviceToHost));
     } else {
-        std::vector<float> f(p.nnz);
-        if (p.nnz) HIPCHK(hipMemcpy(f.data(), dev, p.nnz * sizeof(float), hipMemcpyDeviceToHost));
-        for (int64_t q = 0; q < p.nnz; q++) col[q] = f[q];
+        std::vector<float> f(n);
+        if (n) HIPCHK(hipMemcpy(f.data(), dev, n * sizeof(float), hipMemcpyDeviceToHost));
+        for (int64_t q = 0; q < n; q++) v[q] = f[q];
     }
-    for (int64_t q = 0; q < p.nnz; q++) out[p.h_inv_idx[q]] = col[q];
+    if (per_synapse && p.spiking)
+        for (int64_t q = 0; q < n; q++) out[p.h_inv_idx[q]] = v[q];
+    else
+        std::copy(v.begin(), v.end(), out);
     return 0;
 }
 
-static int syn_write(ahip_net* net, Proj& p, void* dev, const double* in) {
+static int syn_write(ahip_net* net, Proj& p, void* dev, const double* in, int64_t n, bool per_synapse) {
     HIPCHK(hipStreamSynchronize(net->stream));
+    std::vector<double> v(in, in + n);
+    if (per_synapse && p.spiking)
+        for (int64_t q = 0; q < n; q++) v[q] = in[p.h_inv_idx[q]];
     if (net->precision == AHIP_F64) {
-        std::vector<double> col(p.nnz);
-        for (int64_t q = 0; q < p.nnz; q++) col[q] = in[p.h_inv_idx[q]];
-        if (p.nnz) HIPCHK(hipMemcpy(dev, col.data(), p.nnz * sizeof(double), hipMemcpyHostToDevice));
+        if (n) HIPCHK(hipMemcpy(dev, v.data(), n * sizeof(double), hipMemcpyHostToDevice));
     } else {
-        std::vector<float> col(p.nnz);
-        for (int64_t q = 0; q < p.nnz; q++) col[q] = (float)in[p.h_inv_idx[q]];
-        if (p.nnz) HIPCHK(hipMemcpy(dev, col.data(), p.nnz * sizeof(float), hipMemcpyHostToDevice));
+        std::vector<float> f(v.begin(), v.end());
+        if (n) HIPCHK(hipMemcpy(dev, f.data(), n * sizeof(float), hipMemcpyHostToDevice));
     }
     return 0;
 }
@@ -1273,7 +1318,7 @@ static int syn_write(ahip_net* net, Proj& p, void* dev, const double* in) {
 int ahip_proj_get_w(ahip_net* net, int32_t proj, double* w, int64_t nnz) {
     Proj* p = get_proj(net, proj);
     if (!p || !w || nnz != p->nnz) return fail(AHIP_EINVAL, "bad projection or size");
-    if (p->syn) return syn_read(net, *p, p->w_col.p, w);
+    if (p->syn) return syn_read(net, *p, syn_w(*p), w, nnz, true);
     for (int64_t j = 0; j < nnz; j++) w[j] = p->single_w ? p->w_const : p->h_w[j];
     return 0;
 }
@@ -1285,7 +1330,7 @@ int ahip_proj_set_w(ahip_net* net, int32_t proj, const double* w, int64_t nnz) {
     p->single_w = false;
     p->h_w.assign(w, w + nnz);
     invalidate(net);
-    if (p->syn) return syn_write(net, *p, p->w_col.p, w);
+    if (p->syn) return syn_write(net, *p, syn_w(*p), w, nnz, true);
     if (p->spiking) return build_backward(net, *p);
     ahip_format_opts o{p->rate.format, p->rate.lanes, p->rate.chunk, p->rate.sigma, p->rate.panel_cols};
     std::string err;
@@ -1299,9 +1344,7 @@ int ahip_proj_set_w(ahip_net* net, int32_t proj, const double* w, int64_t nnz) {
 int ahip_proj_set_synapse(ahip_net* net, int32_t proj, const char* synapse_lib) {
     Proj* p = get_proj(net, proj);
     if (!p || !synapse_lib) return fail(AHIP_EINVAL, "bad projection or library");
-    if (!p->spiking) return fail(AHIP_EINVAL, "synapse models apply to spiking projections");
     if (p->syn) return fail(AHIP_ESTATE, "projection %d already has a synapse model", p->id);
-    if (p->delay_per_syn) return fail(AHIP_EINVAL, "a plastic projection takes one uniform delay");
     void* dl = dlopen(synapse_lib, RTLD_NOW | RTLD_LOCAL);
     if (!dl) return fail(AHIP_EMODEL, "cannot load synapse library %s: %s", synapse_lib, dlerror());
     auto describe = (const ahip_synapse_desc* (*)(void))dlsym(dl, "ahip_synapse_describe");
@@ -1310,23 +1353,54 @@ int ahip_proj_set_synapse(ahip_net* net, int32_t proj, const char* synapse_lib) 
     if (!d || d->abi != AHIP_SYNAPSE_ABI)
         return fail(AHIP_EMODEL, "%s: synapse ABI %d, runtime expects %d", synapse_lib, d ? d->abi : -1,
                     AHIP_SYNAPSE_ABI);
-    if (d->n_attr < 0 || d->n_attr > AHIP_MAX_SYN_ATTRS) return fail(AHIP_EMODEL, "%s: too many attributes", synapse_lib);
-    if (!d->pre_f64 || !d->pre_f32 || !d->post_f64 || !d->post_f32)
-        return fail(AHIP_EMODEL, "%s: missing event launchers", synapse_lib);
-    if (!net->pops[p->post]->model->spiking)
-        return fail(AHIP_EINVAL, "synapse models need a spiking post-synaptic population");
+    if (d->n_attr < 0 || d->n_attr > AHIP_MAX_SYN_ATTRS || d->n_pre_deps < 0 || d->n_pre_deps > AHIP_MAX_SYN_DEPS ||
+        d->n_post_deps < 0 || d->n_post_deps > AHIP_MAX_SYN_DEPS)
+        return fail(AHIP_EMODEL, "%s: too many attributes", synapse_lib);
+    if (!d->pre_f64 || !d->pre_f32 || !d->post_f64 || !d->post_f32 || !d->update_f64 || !d->update_f32)
+        return fail(AHIP_EMODEL, "%s: missing launchers", synapse_lib);
+    if ((d->spiking != 0) != p->spiking)
+        return fail(AHIP_EINVAL, "%s synapse model on a %s projection", d->spiking ? "a spiking" : "a rate-coded",
+                    p->spiking ? "spiking" : "rate-coded");
+    Pop& pre = *net->pops[p->pre];
+    Pop& post = *net->pops[p->post];
+    if (p->spiking && !post.model->spiking)
+        return fail(AHIP_EINVAL, "spiking synapse models need a spiking post-synaptic population");
+    std::vector<int32_t> dep_pre, dep_post;
+    for (int32_t k = 0; k < d->n_pre_deps; k++) {
+        const int s = pre.slot_index(d->pre_deps[k]);
+        if (s < 0) return fail(AHIP_EINVAL, "the pre-synaptic neuron has no attribute %s", d->pre_deps[k]);
+        dep_pre.push_back(s);
+    }
+    for (int32_t k = 0; k < d->n_post_deps; k++) {
+        const int s = post.slot_index(d->post_deps[k]);
+        if (s < 0) return fail(AHIP_EINVAL, "the post-synaptic neuron has no attribute %s", d->post_deps[k]);
+        dep_post.push_back(s);
+    }
+    if (p->delay_per_syn || (p->delay_uniform > 1 && (!p->spiking || !dep_pre.empty())))
+        return fail(AHIP_EINVAL, "this plastic projection cannot have delays");
     HIPCHK(hipSetDevice(net->device));
     HIPCHK(hipStreamSynchronize(net->stream));
     p->syn_dl = dl;  // stays mapped, as the neuron model libraries
     p->syn_path = synapse_lib;
     p->syn = d;
+    p->dep_pre = dep_pre;
+    p->dep_post = dep_post;
     invalidate(net);
     if (p->nnz < 0) return 0;
     if (p->single_w) {
         p->single_w = false;
         p->h_w.assign(p->nnz, p->w_const);
-        int rc = build_backward(net, *p);
-        if (rc) return rc;
+    }
+    int rc;
+    if (p->spiking) {
+        if ((rc = build_backward(net, *p))) return rc;
+    } else {
+        const bool csr = p->rate.format == AHIP_FMT_CSR || p->rate.format == AHIP_FMT_CSR_VEC;
+        ahip_format_opts o{csr ? p->rate.format : AHIP_FMT_CSR, csr ? p->rate.lanes : 64, 0, 1, 0};
+        std::string err;
+        RateLayout L;
+        if ((rc = build_layout(*p, net->precision, o, L, err))) return fail(rc, "%s", err.c_str());
+        p->rate = std::move(L);
     }
     return init_syn_state(net, *p);
 }
@@ -1338,46 +1412,56 @@ static int syn_attr(Proj* p, const char* attr) {
     return -1;
 }
 
+static int64_t syn_attr_size(const Proj& p, int k) {
+    const int32_t loc = p.syn->attr_locality[k];
+    return loc == AHIP_SYN_LOCAL ? p.nnz : loc == AHIP_SYN_SEMIGLOBAL ? p.n_rows : 1;
+}
+
 int ahip_proj_syn_get(ahip_net* net, int32_t proj, const char* attr, double* values, int64_t n) {
     Proj* p = get_proj(net, proj);
     const int k = syn_attr(p, attr);
     if (k < 0 || !values || p->nnz < 0) return fail(AHIP_EINVAL, "no synaptic attribute %s", attr ? attr : "");
-    if (!p->syn->attr_local[k]) {
-        if (n != 1) return fail(AHIP_EINVAL, "%s is one value per projection", attr);
-        values[0] = p->syn_global[k];
-        return 0;
-    }
-    if (n != p->nnz) return fail(AHIP_EINVAL, "%s holds %lld values", attr, (long long)p->nnz);
-    return syn_read(net, *p, p->syn_local[k].p, values);
+    const int64_t size = syn_attr_size(*p, k);
+    if (n != size) return fail(AHIP_EINVAL, "%s holds %lld values", attr, (long long)size);
+    return syn_read(net, *p, p->syn_attr[k].p, values, n, p->syn->attr_locality[k] == AHIP_SYN_LOCAL);
 }
 
 int ahip_proj_syn_set(ahip_net* net, int32_t proj, const char* attr, const double* values, int64_t n) {
     Proj* p = get_proj(net, proj);
     const int k = syn_attr(p, attr);
     if (k < 0 || !values || p->nnz < 0) return fail(AHIP_EINVAL, "no synaptic attribute %s", attr ? attr : "");
-    if (!p->syn->attr_local[k]) {
-        if (n != 1) return fail(AHIP_EINVAL, "%s is one value per projection", attr);
-        p->syn_global[k] = values[0];
-        invalidate(net);  // launch arguments are captured by value
-        return 0;
-    }
-    if (n == 1) {
+    const int64_t size = syn_attr_size(*p, k);
+    if (n == 1 && size != 1) {
         HIPCHK(hipStreamSynchronize(net->stream));
-        LCHK(launch_fill(p->syn_local[k].p, p->nnz, (int32_t)net->elem(), values[0], net->stream));
+        LCHK(launch_fill(p->syn_attr[k].p, size, (int32_t)net->elem(), values[0], net->stream));
         HIPCHK(hipStreamSynchronize(net->stream));
         return 0;
     }
-    if (n != p->nnz) return fail(AHIP_EINVAL, "%s holds %lld values", attr, (long long)p->nnz);
-    return syn_write(net, *p, p->syn_local[k].p, values);
+    if (n != size) return fail(AHIP_EINVAL, "%s holds %lld values", attr, (long long)size);
+    return syn_write(net, *p, p->syn_attr[k].p, values, n, p->syn->attr_locality[k] == AHIP_SYN_LOCAL);
 }
 
 int ahip_proj_get_last_event(ahip_net* net, int32_t proj, int64_t* values, int64_t n) {
     Proj* p = get_proj(net, proj);
     if (!p || !p->syn || !values || n != p->nnz) return fail(AHIP_EINVAL, "bad plastic projection or size");
-    std::vector<int64_t> col(p->nnz);
+    std::vector<int64_t> v(p->nnz);
     HIPCHK(hipStreamSynchronize(net->stream));
-    if (p->nnz) HIPCHK(hipMemcpy(col.data(), p->last_event.p, p->nnz * sizeof(int64_t), hipMemcpyDeviceToHost));
-    for (int64_t q = 0; q < p->nnz; q++) values[p->h_inv_idx[q]] = col[q];
+    if (p->nnz) HIPCHK(hipMemcpy(v.data(), p->last_event.p, p->nnz * sizeof(int64_t), hipMemcpyDeviceToHost));
+    for (int64_t q = 0; q < p->nnz; q++) values[p->spiking ? p->h_inv_idx[q] : q] = v[q];
+    return 0;
+}
+
+int ahip_proj_set_learning(ahip_net* net, int32_t proj, int32_t update, int32_t plasticity, int32_t period,
+                           int32_t offset) {
+    Proj* p = get_proj(net, proj);
+    if (!p || !p->syn) return fail(AHIP_EINVAL, "bad plastic projection");
+    if (period < 1 || offset < 0 || offset >= period) return fail(AHIP_EINVAL, "need period >= 1 and 0 <= offset < period");
+    HIPCHK(hipStreamSynchronize(net->stream));
+    p->syn_update = update != 0;
+    p->syn_plasticity = plasticity != 0;
+    p->syn_period = period;
+    p->syn_offset = offset;
+    invalidate(net);  // the flags are launch arguments of the captured step
     return 0;
 }
 

@@ -504,122 +504,230 @@ def equation_code(desc: NeuronDescription, locality: str, mask: str | None) -> s
 # Synapse definitions (parser/AnalyseSynapse.py)
 # ---------------------------------------------------------------------------
 
+SYN_LOCALITY = {"local": 0, "semiglobal": 1, "global": 2}
+_DEP = re.compile(r"(?<![\w.])(pre|post)\.([A-Za-z_]\w*)")
+
+
 @dataclass
 class SynapseDescription:
-    """Attributes of a spiking synapse model and the C text of its events.
+    """Attributes of a synapse model and the C text of its events and
+    continuous updates.
 
-    Locality: ``: projection`` makes a parameter one value per projection
-    (AnalyseSynapse.py, "projection"); every other attribute has one value per
-    synapse.  ``w`` is always a per-synapse variable.  Only event-driven
-    variables are integrated (Equation.py:606-649); continuous synaptic ODEs
-    are not supported by this backend.
+    Locality (AnalyseSynapse.py): ``: projection`` = one value per projection
+    ("global"), ``: postsynaptic`` = one value per post-synaptic neuron
+    ("semiglobal"), otherwise one value per synapse ("local").  ``w`` is the
+    per-synapse weight; it is a variable when an equation defines it.
+    Event-driven variables are integrated exactly between events
+    (Equation.py:606-649); the other equations run every step in
+    update_synapse (explicit or exponential Euler, Equation.py:276-307,
+    :559-604).  ``pre.x`` / ``post.x`` read attribute x of the pre / post
+    neuron (ParserTemplate.py:14-21).
     """
     name: str
     parameters: list
     variables: list
-    pre_g: bool = False                     # pre_spike holds g_target += w
+    spiking: bool = False
+    g_code: str | None = None               # transmission of g_target += <expr>; None = w
     pre_code: list = field(default_factory=list)
     post_code: list = field(default_factory=list)
     event_driven: list = field(default_factory=list)   # (name, C text with %(t)s)
+    preloop: list = field(default_factory=list)        # (name, C value) hoisted step sizes
+    pre_deps: list = field(default_factory=list)
+    post_deps: list = field(default_factory=list)
     randoms: list = field(default_factory=list)
     targets: list = field(default_factory=list)
 
+    def attrs(self):
+        """(name, locality, init) of every attribute besides w."""
+        return [(a.name, a.locality, a.init) for a in self.parameters + self.variables if a.name != "w"]
+
     def local_names(self):
-        return ["w"] + [a.name for a in self.parameters + self.variables if a.locality == "local"]
+        return ["w"] + [n for n, loc, _ in self.attrs() if loc == "local"]
 
     def global_names(self):
-        return [a.name for a in self.parameters + self.variables if a.locality == "global"]
+        return [n for n, loc, _ in self.attrs() if loc != "local"]
+
+    def has_update(self, locality: str) -> bool:
+        return any(v.cpp and v.locality == locality for v in self.variables)
 
 
-def _statement(tr, line, declared_local, declared_global) -> str:
-    eq, constraint = _split_constraint(line)
-    bounds, _ = _flags(constraint)
-    name = _variable_name(eq)
-    if name not in declared_local and name not in declared_global:
-        raise EquationError(f"unknown synaptic variable in: {line}")
-    if name in declared_global:
-        raise EquationError(f"per-projection parameters cannot be modified by events: {line}")
-    code = tr.increment(eq) if any(op in eq for op in ["+=", "-=", "*=", "/="]) else tr.assignment(eq)
+def _deps_to_symbols(text: str, desc: SynapseDescription) -> str:
+    def repl(m):
+        side, name = m.group(1), m.group(2)
+        lst = desc.pre_deps if side == "pre" else desc.post_deps
+        if name not in lst:
+            lst.append(name)
+        return f"_{side}_{name}_"
+    return _DEP.sub(repl, text)
+
+
+def _syn_locality(flags) -> str:
+    if "projection" in flags:
+        return "global"
+    if "postsynaptic" in flags:
+        return "semiglobal"
+    return "local"
+
+
+def _bound_code(tr, name, bounds) -> str:
     lhs = tr.ccode(tr.local_dict[name])
+    code = ""
     for b in ("min", "max"):
         if b in bounds:
             val = bounds[b]
             try:
                 float(val)
             except ValueError:
-                val = tr.expression(val)
+                val = tr.expression(_DEP.sub(lambda m: f"_{m.group(1)}_{m.group(2)}_", val))
             op = "<" if b == "min" else ">"
-            code += f"\nif({lhs} {op} {val})\n    {lhs} = {val};"
+            code += f"if({lhs} {op} {val})\n    {lhs} = {val};\n"
     return code
+
+
+def _wrap_w(name: str, code: str) -> str:
+    # updates of w are skipped while learning is disabled (OpenMPGenerator.py:
+    # 1137-1139, :1600-1603; Utils.py:105-116 wrap_w)
+    return "if (_plasticity) {\n" + code.rstrip("\n") + "\n}" if name == "w" else code
+
+
+def _statement(tr, line, desc) -> str:
+    eq, constraint = _split_constraint(line)
+    bounds, _ = _flags(constraint)
+    name = _variable_name(eq)
+    loc = {n: l for n, l, _ in desc.attrs()}
+    if name != "w" and name not in loc:
+        raise EquationError(f"unknown synaptic variable in: {line}")
+    if name != "w" and loc[name] != "local":
+        raise EquationError(f"events can only modify per-synapse variables: {line}")
+    code = tr.increment(eq) if any(op in eq for op in ["+=", "-=", "*=", "/="]) else tr.assignment(eq)
+    code += "\n" + _bound_code(tr, name, bounds)
+    return _wrap_w(name, code)
 
 
 def analyse_synapse(parameters: str = "", equations: str = "", pre_spike: str | None = None,
                     post_spike: str | None = None, name: str = "synapse",
                     precision: str = "double") -> SynapseDescription:
-    """Analyse a spiking synapse definition (AnalyseSynapse.py) into the C text
-    of its pre- and post-synaptic events."""
+    """Analyse a synapse definition (AnalyseSynapse.py) into the C text of its
+    pre- and post-synaptic events and of its continuous updates."""
     params = []
     for line in _lines(parameters):
         eq, constraint = _split_constraint(line)
         bounds, flags = _flags(constraint)
         if "=" not in eq:
             raise EquationError(f"parameter without value: {line}")
-        if "postsynaptic" in flags:
-            raise EquationError("postsynaptic (per post neuron) synaptic attributes are not supported")
         pname, val = eq.split("=", 1)
-        params.append(Variable(name=pname.strip(), eq=eq, locality="global" if "projection" in flags else "local",
-                               flags=flags, bounds=bounds, init=_parse_value(val), is_parameter=True))
-    variables = []
+        params.append(Variable(name=pname.strip(), eq=eq, locality=_syn_locality(flags), flags=flags,
+                               bounds=bounds, init=_parse_value(val), is_parameter=True))
+    desc = SynapseDescription(name=name, parameters=params, variables=[],
+                              spiking=bool(pre_spike) or bool(post_spike))
+    if not desc.spiking and re.search(r"\bt_(pre|post)\b", (equations or "") + (parameters or "")):
+        raise EquationError("t_pre / t_post are spike times: only spiking synapses use them")
     for line in _lines(equations):
         eq, constraint = _split_constraint(line)
         bounds, flags = _flags(constraint)
+        eq = _deps_to_symbols(eq, desc)
         vname = _variable_name(eq)
         init = _parse_value(bounds.pop("init")) if "init" in bounds else 0.0
-        if "event-driven" not in flags or _equation_type(vname, eq) != "ODE":
-            raise EquationError(f"only event-driven synaptic ODEs are supported: {line}")
-        variables.append(Variable(name=vname, eq=eq, locality="local", flags=flags, bounds=bounds, init=init,
-                                  method="event-driven"))
-    desc = SynapseDescription(name=name, parameters=params, variables=variables)
-    for a in params + variables:
+        method = "explicit"
+        for m in ("event-driven", "exponential", "explicit"):
+            if m in flags:
+                method = m
+        for m in ("implicit", "semiimplicit", "midpoint", "rk4"):
+            if m in flags:
+                raise EquationError(f"numerical method '{m}' is not supported by this backend")
+        loc = _syn_locality(flags)
+        if vname == "w" and loc != "local":
+            raise EquationError("w is a per-synapse variable")
+        if method == "event-driven" and (loc != "local" or not desc.spiking):
+            raise EquationError(f"event-driven variables are per-synapse variables of spiking synapses: {line}")
+        desc.variables.append(Variable(name=vname, eq=eq, locality=loc, flags=flags, bounds=bounds, init=init,
+                                       method=method))
+    for a in params:
         if a.name == "w":
-            raise EquationError("w is the synaptic weight, not a model attribute")
+            raise EquationError("w is the synaptic weight: define it with an equation, not as a parameter")
+    declared = {a.name for a in params + desc.variables}
     tr = _Translator(desc, precision)
-    # model attributes become _s_<name> in the kernels, so that no attribute
-    # name can collide with a kernel variable
-    for a in params + variables:
-        idx = "%(local_index)s" if a.locality == "local" else "%(global_index)s"
-        tr.local_dict[a.name] = sp.Symbol("_s_" + a.name + idx)
+    # attributes become _s_<name> in the kernels so that no attribute name
+    # can collide with a kernel variable
+    index = {"local": "%(local_index)s", "semiglobal": "%(semiglobal_index)s", "global": "%(global_index)s"}
+    for n, loc, _ in desc.attrs():
+        tr.local_dict[n] = sp.Symbol("_s_" + n + index[loc])
     # last spike times of the pre / post neuron in ms (ParserTemplate.py:18-19)
     tr.local_dict["t_pre"] = sp.Symbol("((double)(_pre_last_spike[pre])*dt)")
     tr.local_dict["t_post"] = sp.Symbol("((double)(_post_last_spike[post])*dt)")
-    local = set(desc.local_names())
-    glob = set(desc.global_names())
-    # event-driven variables: exact decay since the last event (Equation.py:606-649)
-    for v in variables:
-        expr = v.eq.replace("=", "- (", 1) + ")"
-        expr = expr.replace(" ", "")
-        real_tau, _, steady = tr.standardize(v.name, expr)
-        if real_tau is None:
-            raise EquationError(f"the equation is not a linear ODE and cannot be evaluated exactly: {v.eq}")
-        for other in desc.local_names() + desc.global_names():
-            if other != v.name and tr.local_dict[other] in steady.atoms() and other in [x.name for x in variables]:
-                raise EquationError(f"an event-driven variable cannot depend on other variables: {v.eq}")
-        var = tr.ccode(tr.local_dict[v.name])
-        st = tr.ccode(steady)
-        decay = f"exp(dt*(_last_event%(local_index)s - %(t)s)/({tr.ccode(real_tau)}))"
-        if st == "0":
-            v.cpp = f"{var} *= {decay};"
+    texts = [pre_spike or "", post_spike or ""]
+    texts = [_deps_to_symbols(t, desc) for t in texts]
+    for side, lst in (("pre", desc.pre_deps), ("post", desc.post_deps)):
+        for n in lst:
+            idx = "pre" if side == "pre" else "post"
+            tr.local_dict[f"_{side}_{n}_"] = sp.Symbol(f"_d{side}_{n}[{idx}*_st_{side}_{n}]")
+    local_names = [n for n in desc.local_names()]
+    for v in desc.variables:
+        if v.method == "event-driven":
+            # exact decay since the last event (Equation.py:606-649)
+            expr = v.eq.replace("=", "- (", 1) + ")"
+            expr = expr.replace(" ", "")
+            real_tau, _, steady = tr.standardize(v.name, expr)
+            if real_tau is None:
+                raise EquationError(f"the equation is not a linear ODE and cannot be evaluated exactly: {v.eq}")
+            for other in desc.variables:
+                if other.name != v.name and tr.local_dict[other.name] in steady.atoms():
+                    raise EquationError(f"an event-driven variable cannot depend on other variables: {v.eq}")
+            var = tr.ccode(tr.local_dict[v.name])
+            st = tr.ccode(steady)
+            decay = f"exp(dt*(_last_event%(local_index)s - %(t)s)/({tr.ccode(real_tau)}))"
+            code = f"{var} *= {decay};" if st == "0" else f"{var} = {st} + ({var} - {st})*{decay};"
+            desc.event_driven.append((v.name, code))
+            v.cpp = ""
+            continue
+        kind = _equation_type(v.name, v.eq)
+        if kind == "ODE":
+            expr = v.eq.replace("=", "- (", 1) + ")"
+            expr = expr.replace(" ", "")
+            if v.method == "exponential":
+                v.pre_loop, v.cpp, v.switch = tr.exponential(v.name, expr, local_names)
+            else:
+                v.pre_loop, v.cpp, v.switch = tr.explicit(v.name, expr)
+            if v.pre_loop:
+                desc.preloop.append((v.pre_loop["name"], v.pre_loop["value"]))
+        elif kind == "inc":
+            v.cpp = tr.increment(v.eq)
+        elif kind == "simple":
+            v.cpp = tr.assignment(v.eq)
         else:
-            v.cpp = f"{var} = {st} + ({var} - {st})*{decay};"
-        desc.event_driven.append((v.name, v.cpp))
-    for line in _lines(pre_spike):
+            raise EquationError(f"not an equation: {v.eq}")
+        v.bound_code = _bound_code(tr, v.name, v.bounds)
+    for v in desc.variables:
+        if v.name not in declared:
+            raise EquationError(f"cannot identify the variable of: {v.eq}")
+    for line in _lines(texts[0]):
         eq, _ = _split_constraint(line)
-        if eq.replace(" ", "") == "g_target+=w":
-            desc.pre_g = True
+        e = eq.replace(" ", "")
+        if e.startswith("g_target"):
+            if not e.startswith("g_target+="):
+                raise EquationError(f"only g_target += <expression> is supported: {line}")
+            desc.g_code = tr.expression(eq.split("+=", 1)[1])
             continue
         if "g_target" in eq:
-            raise EquationError(f"only g_target += w is supported: {line}")
-        desc.pre_code.append(_statement(tr, line, local, glob))
-    for line in _lines(post_spike):
-        desc.post_code.append(_statement(tr, line, local, glob))
+            raise EquationError(f"g_target can only be incremented: {line}")
+        desc.pre_code.append(_statement(tr, line, desc))
+    for line in _lines(texts[1]):
+        desc.post_code.append(_statement(tr, line, desc))
     return desc
+
+
+def synapse_update_code(desc: SynapseDescription, locality: str) -> str:
+    """Statements of update_synapse for one locality (Utils.py:139-265): ODE
+    blocks compute every increment, then apply every switch; updates of w
+    are wrapped in the learning flag."""
+    out = []
+    for kind, block in sort_odes(desc, locality):
+        if kind == "ode":
+            for v in block:
+                out.append(v.cpp)
+            for v in block:
+                out.append(_wrap_w(v.name, v.switch + "\n" + v.bound_code))
+        else:
+            for v in block:
+                out.append(_wrap_w(v.name, v.cpp + "\n" + v.bound_code))
+    return "\n".join(out)

@@ -24,6 +24,49 @@ class Synapse:
     description: str = ""
 
 
+def Hebb(eta=0.01) -> Synapse:
+    """Hebbian rule of the reference's models/Synapses.py:33-61."""
+    return Synapse(parameters=f"eta = {float(eta)!r} : projection",
+                   equations="dw/dt = eta * pre.r * post.r : min=0.0, explicit", name="Hebb")
+
+
+def Oja(eta=0.01, alpha=1.0) -> Synapse:
+    """Oja's rule, models/Synapses.py:63-94."""
+    return Synapse(parameters=f"eta = {float(eta)!r} : projection\nalpha = {float(alpha)!r} : projection",
+                   equations="dw/dt = eta * ( pre.r * post.r - alpha * post.r^2 * w ) : min=0.0, explicit",
+                   name="Oja")
+
+
+def IBCM(eta=0.01, tau=2000.0) -> Synapse:
+    """Intrator and Cooper (1992) rule, models/Synapses.py:96-129."""
+    return Synapse(parameters=f"eta = {float(eta)!r} : projection\ntau = {float(tau)!r} : projection",
+                   equations="""
+                       tau * dtheta/dt + theta = post.r^2 : postsynaptic, exponential
+                       dw/dt = eta * post.r * (post.r - theta) * pre.r : min=0.0, explicit
+                   """, name="IBCM")
+
+
+def STP(tau_rec=100.0, tau_facil=0.01, U=0.5) -> Synapse:
+    """Short-term facilitation and depression (Tsodyks, Uziel and Markram
+    2000), models/Synapses.py:131-230."""
+    if tau_facil <= 0.0:
+        raise ValueError("STP: tau_facil must be positive")
+    return Synapse(
+        parameters=f"tau_rec = {float(tau_rec)!r} : projection\ntau_facil = {float(tau_facil)!r} : projection\n"
+                   f"U = {float(U)!r}",
+        equations=f"""
+            dx/dt = (1 - x)/tau_rec : init = 1.0, event-driven
+            du/dt = (U - u)/tau_facil : init = {float(U)!r}, event-driven
+        """,
+        pre_spike="""
+            g_target += w * u * x
+            x *= (1 - u)
+            u += U * (1 - u)
+        """,
+        name="STP",
+    )
+
+
 def STDP(tau_plus=20.0, tau_minus=20.0, A_plus=0.01, A_minus=0.01, w_min=0.0, w_max=1.0) -> Synapse:
     """Spike-timing dependent plasticity with exponential traces, the
     reference's models/Synapses.py:232-310 (online additive rule, traces x

@@ -249,9 +249,15 @@ class Projection:
         else:
             object.__setattr__(self, name, value)
 
+    def _syn_size(self, locality: str) -> int:
+        if locality == "local":
+            return self.nb_synapses
+        return _base(self.post).size if locality == "semiglobal" else 1
+
     def get(self, name):
-        """A synaptic attribute: one value per synapse (forward order) or,
-        for a ``: projection`` parameter, one float."""
+        """A synaptic attribute: one value per synapse (forward order), per
+        post-synaptic neuron (``: postsynaptic``) or, for ``: projection``,
+        one float."""
         if name == "w":
             return self.w
         info = self._syn_info()
@@ -259,12 +265,13 @@ class Projection:
             raise AttributeError(f"the projection has no synaptic attribute {name}")
         if not self.net.compiled:
             return self._syn_init.get(name, info[name][2])
-        n = self.nb_synapses if info[name][1] else 1
+        loc = info[name][1]
+        n = self._syn_size(loc)
         out = np.empty(max(n, 1))
         if n == 0:
             return out[:0]
         _lib.check(_lib.lib().ahip_proj_syn_get(self.net._h, self.id, name.encode(), _lib.ptr(out, C.c_double), n))
-        return out if info[name][1] else float(out[0])
+        return float(out[0]) if loc == "global" else out
 
     def set(self, name, value):
         if name == "w":
@@ -276,15 +283,45 @@ class Projection:
         if not self.net.compiled:
             self._syn_init[name] = value
             return
+        loc = info[name][1]
         if isinstance(value, conn.RandomDistribution):
-            value = value.get_values(self.nb_synapses)
+            value = value.get_values(self._syn_size(loc))
         v = np.ascontiguousarray(np.asarray(value, dtype=np.float64).reshape(-1))
-        if not info[name][1] and v.size != 1:
+        if loc == "global" and v.size != 1:
             raise ValueError(f"{name} is one value per projection")
-        if self.nb_synapses == 0 and info[name][1]:
+        if self._syn_size(loc) == 0:
             return
         _lib.check(_lib.lib().ahip_proj_syn_set(self.net._h, self.id, name.encode(), _lib.ptr(v, C.c_double),
                                                 v.size))
+
+    # -- learning flags (core/Projection.py:1440-1510) ---------------------------
+    def enable_learning(self, period: float | None = None, offset: float | None = None):
+        """update_synapse every `period` ms (default every step) at `offset`
+        ms into the period; weights change again."""
+        dt = self.net.dt
+        if offset is not None and period is None:
+            raise ValueError("enable_learning(): an offset needs a period")
+        if period is not None and offset is not None and offset >= period:
+            raise ValueError("enable_learning(): the offset must be smaller than the period")
+        n = int(period / dt) if period is not None else 1
+        off = 0
+        if offset is not None:
+            rel = (self.net.time * dt) % period + offset
+            off = int(int(rel % period) / dt)
+        self._learning = (1, 1, max(n, 1), off)
+        self._upload_learning()
+
+    def disable_learning(self):
+        """Rate-coded: no update_synapse at all; spiking: the weights stay
+        fixed while the other synaptic variables keep evolving."""
+        u, p, n, off = getattr(self, "_learning", (1, 1, 1, 0))
+        self._learning = (0, p, n, off) if not _base(self.pre).spiking else (u, 0, n, off)
+        self._upload_learning()
+
+    def _upload_learning(self):
+        if self.net.compiled and self.synapse is not None:
+            u, p, n, off = self._learning
+            _lib.check(_lib.lib().ahip_proj_set_learning(self.net._h, self.id, u, p, n, off))
 
     @property
     def last_event(self) -> np.ndarray:
@@ -457,8 +494,6 @@ class Projection:
                                       int(spiking), C.byref(pid)))
         self.id = pid.value
         if self.synapse is not None:
-            if not spiking:
-                raise ValueError("synapse types apply to spiking projections")
             lib_path = synapse_library(self.synapse, self.net._allow_build)
             _lib.check(L.ahip_proj_set_synapse(self.net._h, self.id, str(lib_path).encode()))
         if self._generator == "device":
@@ -475,6 +510,8 @@ class Projection:
             self._upload_delays(self._delays)
         for name, value in self._syn_init.items():
             self.set(name, value)
+        if hasattr(self, "_learning"):
+            self._upload_learning()
 
     # -- accessors ------------------------------------------------------------
     @property
@@ -715,6 +752,17 @@ class Network:
         if monitors:
             for m in self.monitors:
                 m.clear()
+
+    def enable_learning(self, projections=None, period=None, offset=None):
+        """Network.enable_learning: every (or the given) plastic projection."""
+        for p in projections or self.projections:
+            if p.synapse is not None:
+                p.enable_learning(period, offset)
+
+    def disable_learning(self, projections=None):
+        for p in projections or self.projections:
+            if p.synapse is not None:
+                p.disable_learning()
 
     def set_graph(self, enabled: bool = True, batch: int = 0):
         _lib.check(_lib.lib().ahip_set_graph_enabled(self._h, int(enabled)))

@@ -180,25 +180,34 @@ int ahip_proj_nb_synapses(ahip_net* net, int32_t proj, int64_t* nnz);
  * projection reads and writes its device weights). */
 int ahip_proj_get_w(ahip_net* net, int32_t proj, double* w, int64_t nnz);
 int ahip_proj_set_w(ahip_net* net, int32_t proj, const double* w, int64_t nnz);
-/* Plasticity: attach a generated spiking-synapse model (a shared library
- * exporting ahip_synapse_describe(), include/annarchy_hip_synapse.h) to a
- * spiking projection, as the reference's Projection(synapse=...) with the
- * generated compute_psp / post_event code (Projection/OpenMPGenerator.py:
+/* Plasticity: attach a generated synapse model (a shared library exporting
+ * ahip_synapse_describe(), include/annarchy_hip_synapse.h) to a projection,
+ * as the reference's Projection(synapse=...) with the generated compute_psp,
+ * update_synapse and post_event code (Projection/OpenMPGenerator.py:
  * 1057-1230, 1482-1815).  The projection then keeps one weight per synapse
- * and the model's per-synapse attributes at their initial values, with
- * _last_event = -10000; re-initialising the connectivity re-initialises them.
- * Pre events replace the projection's plain spike scatter; post events run
- * after every population update of the step.  Only a uniform delay is
- * accepted with a synapse model. */
+ * (a rate-coded one in a csr format, updated in place) and the model's
+ * attributes at their initial values, with _last_event = -10000;
+ * re-initialising the connectivity re-initialises them.  Spiking: pre events
+ * replace the plain spike scatter.  After every population update of the
+ * step: update_synapse, then (spiking) the post events.  The model's
+ * pre.<x> / post.<x> must be attributes of the populations.  Delays: a
+ * spiking projection takes one uniform delay when it reads no pre.<x>. */
 int ahip_proj_set_synapse(ahip_net* net, int32_t proj, const char* synapse_lib);
-/* get/set_local_attribute_all / get/set_global_attribute of a synaptic
- * attribute: n = nnz values in forward order for a per-synapse attribute
- * (set also takes n = 1, one value for all), n = 1 for a per-projection
- * one (Projection/OpenMP/CSR.py accessor templates). */
+/* get/set_local_attribute_all / get/set_semiglobal_attribute_all /
+ * get/set_global_attribute of a synaptic attribute: n = nnz values in
+ * forward order per synapse, n_rows per post neuron, 1 per projection (set
+ * also takes n = 1, one value for all) (Projection/OpenMP/CSR.py accessor
+ * templates). */
 int ahip_proj_syn_get(ahip_net* net, int32_t proj, const char* attr, double* values, int64_t n);
 int ahip_proj_syn_set(ahip_net* net, int32_t proj, const char* attr, const double* values, int64_t n);
 /* _last_event of every synapse (forward order), in steps. */
 int ahip_proj_get_last_event(ahip_net* net, int32_t proj, int64_t* values, int64_t n);
+/* enable_learning / disable_learning (core/Projection.py:1440-1510): update
+ * = _update (0 skips update_synapse), plasticity = _plasticity (0 leaves w
+ * unchanged in events and updates), update_synapse every `period` steps when
+ * (t - offset) mod period == 0, with dt scaled by the period. */
+int ahip_proj_set_learning(ahip_net* net, int32_t proj, int32_t update, int32_t plasticity, int32_t period,
+                           int32_t offset);
 /* Bytes the psp kernel must move per step (algorithmic, no padding). */
 int ahip_proj_psp_bytes(ahip_net* net, int32_t proj, double* bytes);
 /* Times `iters` back-to-back launches of the projection's psp kernel with HIP

@@ -26,6 +26,11 @@
 //                         test_SpikingTransmission.py:14-78 (5 spikes x w=1),
 //                         COBA: free-membrane exponential relaxation invariant
 //   orc_mlp_*             finite differences on the CPU (numpy)
+//   orc_rate_plastic_run  closed forms: geometric Hebbian growth of one
+//                         synapse at a constant rate, Oja's fixed point
+//                         1/(r*sqrt(alpha)), the exponential relaxation of
+//                         IBCM's threshold at a fixed post rate, the update
+//                         period (fewer, larger steps)
 //   orc_plastic_run       hand-derived closed forms of single-synapse spike
 //                         pairings (trace decays between events, the order
 //                         g += w before the weight change, clipping, t_pre /
@@ -559,6 +564,8 @@ int64_t orc_spiking_run(const orc_spiking_cfg* c, double* v, double* g_exc, doub
 //   rule 1  the t_pre / t_post rule of tests_python/SpikeSynapse/
 //           test_SpikingSynapse.py (test_TimeDependentUpdate); prm = tau_pre,
 //           tau_post, cApre, cApost, wmax; no event-driven variable.
+//   rule 2  STP, models/Synapses.py:131-230; prm = tau_rec, tau_facil, U;
+//           x (in x) and u (in y) event-driven, g_target += w*u*x.
 // Step t:
 //   compute_psp (Projection/OpenMP/CSR.py:870-885, OpenMPGenerator.py:
 //     1182-1230): for every pre spike of step t-1 in ascending rank, every
@@ -616,6 +623,14 @@ void orc_plastic_run(int32_t rule, int32_t n_rows, int32_t n_cols, const int64_t
                         g[post] += w[j];
                         x[j] += prm[2] * prm[5];
                         w[j] = orc_clip(w[j] + y[j], prm[4], prm[5]);
+                    } else if (rule == 2) {
+                        // STP: x -> 1 with tau_rec, u -> U with tau_facil between events
+                        x[j] = 1.0 + (x[j] - 1.0) * std::exp(dt * (double)(last_event[j] - (t - 1)) / (prm[0]));
+                        y[j] = prm[2] + (y[j] - prm[2]) * std::exp(dt * (double)(last_event[j] - (t - 1)) / (prm[1]));
+                        last_event[j] = t;
+                        g[post] += w[j] * y[j] * x[j];
+                        x[j] *= 1.0 - y[j];
+                        y[j] += prm[2] * (1.0 - y[j]);
                     } else {
                         g[post] += w[j];
                         const double t_post = (double)post_last[post] * dt;
@@ -631,6 +646,7 @@ void orc_plastic_run(int32_t rule, int32_t n_rows, int32_t n_cols, const int64_t
         for (int64_t k = post_ptr[s]; k < post_ptr[s + 1]; k++) {
             const int32_t i = post_idx[k];
             for (int64_t j = row_ptr[i]; j < row_ptr[i + 1]; j++) {
+                if (rule == 2) continue;
                 if (rule == 0) {
                     x[j] *= std::exp(dt * (double)(last_event[j] - t) / (prm[0]));
                     y[j] *= std::exp(dt * (double)(last_event[j] - t) / (prm[1]));
@@ -641,6 +657,62 @@ void orc_plastic_run(int32_t rule, int32_t n_rows, int32_t n_cols, const int64_t
                     const double t_pre = (double)pre_last[col[j]] * dt;
                     w[j] = orc_clip(w[j] + prm[2] * std::exp((t_pre - (double)t * dt) / prm[0]), 0.0, prm[4]);
                 }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Rate-coded plastic projection pre -> post with the post neuron r = sum(exc)
+// and given pre-synaptic rates (r_pre0 before step 0, r_pre_seq[t] after the
+// update of step t).  Step t (OpenMPBaseTemplate.py:217-325):
+//   compute_psp (Projection/OpenMP/CSR.py:178-196): sum[i] = sum over the
+//     row, j ascending, of w[j]*r_pre[col[j]] with the rates of step t-1;
+//   neuron updates: r_post[i] = sum[i]; r_pre = r_pre_seq[t];
+//   update_synapse (Projection/OpenMP/CSR.py:833-865), when t % period == 0,
+//     with _dt = dt*period (OpenMPGenerator.py:1722-1725): per row the
+//     per-post-neuron equation, then per synapse the weight equation
+//     (explicit Euler, Equation.py:276-307, min = 0):
+//   rule 0 Hebb (models/Synapses.py:33-61)  dw/dt = eta*pre.r*post.r
+//   rule 1 Oja  (:63-94)   dw/dt = eta*(pre.r*post.r - alpha*post.r^2*w)
+//   rule 2 IBCM (:96-129)  tau*dtheta/dt + theta = post.r^2 (exponential
+//          Euler, Equation.py:559-604), dw/dt = eta*post.r*(post.r-theta)*pre.r
+// prm = eta, alpha (Oja) / tau (IBCM).  sum_out[t*n_rows + i] = sum of step t.
+// ---------------------------------------------------------------------------
+void orc_rate_plastic_run(int32_t rule, int32_t n_rows, int32_t n_cols, const int64_t* row_ptr,
+                          const int32_t* col, double* w, double* theta, const double* prm, double dt,
+                          int32_t period, int32_t steps, const double* r_pre0, const double* r_pre_seq,
+                          double* r_post, double* sum_out) {
+    std::vector<double> r_pre(r_pre0, r_pre0 + n_cols);
+    const double eta = prm[0];
+    for (int32_t t = 0; t < steps; t++) {
+        double* sum = sum_out + (int64_t)t * n_rows;
+        for (int32_t i = 0; i < n_rows; i++) {
+            double s = 0.0;
+            for (int64_t j = row_ptr[i]; j < row_ptr[i + 1]; j++) s += w[j] * r_pre[col[j]];
+            sum[i] = s;
+        }
+        for (int32_t i = 0; i < n_rows; i++) r_post[i] = sum[i];
+        for (int32_t c = 0; c < n_cols; c++) r_pre[c] = r_pre_seq[(int64_t)t * n_cols + c];
+        if (t % period != 0) continue;
+        const double _dt = dt * period;
+        for (int32_t i = 0; i < n_rows; i++) {
+            const double post_r = r_post[i];
+            if (rule == 2) {
+                const double stepsize = 1.0 - std::exp(-_dt / prm[1]);
+                theta[i] += stepsize * (std::pow(post_r, 2) - theta[i]);
+            }
+            for (int64_t j = row_ptr[i]; j < row_ptr[i + 1]; j++) {
+                const double pre_r = r_pre[col[j]];
+                double dw;
+                if (rule == 0)
+                    dw = eta * pre_r * post_r;
+                else if (rule == 1)
+                    dw = eta * (pre_r * post_r - prm[1] * std::pow(post_r, 2) * w[j]);
+                else
+                    dw = eta * post_r * (post_r - theta[i]) * pre_r;
+                w[j] += _dt * dw;
+                if (w[j] < 0.0) w[j] = 0.0;
             }
         }
     }

@@ -99,6 +99,9 @@ def lib():
         L.orc_plastic_run.argtypes = [_I32, _I32, _I32, _PI64, _PI32, _PD, _PD, _PD, _PI64, _PD, _D, _I64,
                                       _I32, _PI64, _PI32, _PI64, _PI32, _PI64, _PI64, _I32, _PD]
         L.orc_plastic_run.restype = None
+        L.orc_rate_plastic_run.argtypes = [_I32, _I32, _I32, _PI64, _PI32, _PD, _PD, _PD, _D, _I32, _I32, _PD,
+                                           _PD, _PD, _PD]
+        L.orc_rate_plastic_run.restype = None
         L.orc_mlp_forward.argtypes = [_I32, _PD, _PD, _PD, _I32, _I32, _I32, _I32]
         L.orc_mlp_forward.restype = None
         L.orc_mlp_loss_grad.argtypes = [_I32, _PD, _PD, _PD, _PD, _I32, _I32, _I32, _I32]
@@ -424,3 +427,26 @@ def plastic_run(rule, row_ptr, col_idx, n_cols, w0, params, dt, pre_counts, pre_
                           _p(post_ptr, C.c_int64), _p(post_idx if post_idx.size else e32, C.c_int32),
                           _p(pl, C.c_int64), _p(ql, C.c_int64), int(delay), _p(g, C.c_double))
     return {"w": w, "x": x, "y": y, "last_event": le, "pre_last": pl, "post_last": ql, "g": g}
+
+
+def rate_plastic_run(rule, row_ptr, col_idx, w0, params, dt, r_pre0, r_pre_seq, period=1, theta0=None):
+    """orc_rate_plastic_run: rule 0 Hebb (eta), 1 Oja (eta, alpha), 2 IBCM
+    (eta, tau) on a rate projection into post neurons r = sum(exc), driven
+    by the pre rates r_pre_seq[steps, n_cols] (r_pre0 before step 0).
+    Returns w, theta, r_post and sum[steps, n_rows]."""
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int64)
+    col_idx = np.ascontiguousarray(col_idx, dtype=np.int32)
+    n_rows = row_ptr.size - 1
+    r_pre_seq = np.ascontiguousarray(r_pre_seq, dtype=np.float64)
+    steps, n_cols = r_pre_seq.shape
+    w = np.array(np.broadcast_to(np.asarray(w0, dtype=np.float64), (int(row_ptr[-1]),)))
+    theta = np.zeros(n_rows) if theta0 is None else np.array(theta0, dtype=np.float64)
+    r0 = np.ascontiguousarray(np.broadcast_to(np.asarray(r_pre0, dtype=np.float64), (n_cols,)))
+    prm = np.ascontiguousarray(params, dtype=np.float64)
+    r_post = np.zeros(n_rows)
+    sums = np.zeros((steps, n_rows))
+    lib().orc_rate_plastic_run(rule, n_rows, n_cols, _p(row_ptr, C.c_int64), _p(col_idx, C.c_int32),
+                               _p(w, C.c_double), _p(theta, C.c_double), _p(prm, C.c_double), dt, int(period),
+                               steps, _p(r0, C.c_double), _p(r_pre_seq, C.c_double), _p(r_post, C.c_double),
+                               _p(sums, C.c_double))
+    return {"w": w, "theta": theta, "r_post": r_post, "sum": sums}

@@ -41,24 +41,33 @@ def test_model_libraries_export_descriptor():
 
 class _SynDesc(C.Structure):
     """Leading fields of ahip_synapse_desc (include/annarchy_hip_synapse.h)."""
-    _fields_ = [("abi", C.c_int32), ("name", C.c_char_p), ("n_attr", C.c_int32),
-                ("attr_names", C.POINTER(C.c_char_p)), ("attr_local", C.POINTER(C.c_int32)),
-                ("attr_init", C.POINTER(C.c_double)), ("has_pre", C.c_int32), ("has_post", C.c_int32),
-                ("pre_g", C.c_int32)]
+    _fields_ = [("abi", C.c_int32), ("name", C.c_char_p), ("spiking", C.c_int32), ("n_attr", C.c_int32),
+                ("attr_names", C.POINTER(C.c_char_p)), ("attr_locality", C.POINTER(C.c_int32)),
+                ("attr_init", C.POINTER(C.c_double)), ("n_pre_deps", C.c_int32),
+                ("pre_deps", C.POINTER(C.c_char_p)), ("n_post_deps", C.c_int32),
+                ("post_deps", C.POINTER(C.c_char_p)), ("has_pre", C.c_int32), ("has_post", C.c_int32),
+                ("has_update", C.c_int32)]
+
+
+def _describe(syn):
+    from annarchy_amd.network import synapse_library
+    L = C.CDLL(str(synapse_library(syn)))
+    L.ahip_synapse_describe.restype = C.POINTER(_SynDesc)
+    return L.ahip_synapse_describe().contents
 
 
 def test_synapse_library_descriptor():
     import annarchy_amd as ah
-    from annarchy_amd.network import synapse_library
-    path = synapse_library(ah.STDP())
-    L = C.CDLL(str(path))
-    L.ahip_synapse_describe.restype = C.POINTER(_SynDesc)
-    d = L.ahip_synapse_describe().contents
-    assert d.abi == 1 and d.name == b"STDP" and d.n_attr == 8
+    d = _describe(ah.STDP())
+    assert d.abi == 2 and d.name == b"STDP" and d.spiking == 1 and d.n_attr == 8
     names = [d.attr_names[k].decode() for k in range(d.n_attr)]
     assert names == ["tau_plus", "tau_minus", "A_plus", "A_minus", "w_min", "w_max", "x", "y"]
-    assert [d.attr_local[k] for k in range(d.n_attr)] == [0] * 6 + [1, 1]
-    assert d.attr_init[0] == 20.0 and d.has_pre == 1 and d.has_post == 1 and d.pre_g == 1
+    assert [d.attr_locality[k] for k in range(d.n_attr)] == [2] * 6 + [0, 0]
+    assert d.attr_init[0] == 20.0 and d.has_pre == 1 and d.has_post == 1 and d.has_update == 0
+    d = _describe(ah.IBCM())
+    assert d.spiking == 0 and d.has_update == 1 and d.has_pre == 0 and d.has_post == 0
+    assert [d.attr_locality[k] for k in range(d.n_attr)] == [2, 2, 1]
+    assert d.n_pre_deps == 1 and d.pre_deps[0] == b"r" and d.n_post_deps == 1 and d.post_deps[0] == b"r"
 
 
 def test_error_path_without_device_calls():

@@ -130,3 +130,75 @@ def test_stdp_uniform_delay_closed_form():
     out = orc.plastic_run(0, [0, 1], [0], 1, 0.5, p, 0.5, pc, pi, qc, qi, delay=4)
     assert np.flatnonzero(out["g"][:, 0]).tolist() == [6]
     np.testing.assert_allclose(out["w"], [0.5 + 0.01 * math.exp(-3 * 0.5 / 20.0)], rtol=0, atol=1e-15)
+
+
+# -- rate-coded update_synapse (orc_rate_plastic_run) ---------------------------
+
+def _one(rule, w0, a, steps, prm, dt=1.0, period=1, theta0=None):
+    seq = np.full((steps, 1), a)
+    return orc.rate_plastic_run(rule, [0, 1], [0], w0, prm, dt, a, seq, period=period, theta0=theta0)
+
+
+def test_hebb_geometric_growth():
+    """One synapse at a constant pre rate a: the psp of step t is w*a, the
+    post rate of the same step, so every update multiplies w by
+    1 + dt*eta*a^2."""
+    a, w0, eta, dt, steps = 0.7, 0.2, 0.01, 0.5, 40
+    out = _one(0, w0, a, steps, [eta], dt=dt)
+    g = 1.0 + dt * eta * a * a
+    np.testing.assert_allclose(out["w"], [w0 * g ** steps], rtol=1e-13)
+    np.testing.assert_allclose(out["sum"][:, 0], w0 * g ** np.arange(steps) * a, rtol=1e-13)
+
+
+def test_hebb_update_period():
+    """period 3: updates at steps 0, 3, 6 with dt scaled by 3."""
+    a, w0, eta = 0.7, 0.2, 0.01
+    out = _one(0, w0, a, 9, [eta], period=3)
+    np.testing.assert_allclose(out["w"], [w0 * (1.0 + 3 * eta * a * a) ** 3], rtol=1e-13)
+    s = out["sum"][:, 0]
+    assert s[1] == s[2] == s[3] and s[4] == s[6] and s[3] < s[4]
+
+
+def test_oja_fixed_point():
+    """Oja's rule: dw = eta a^2 w (1 - alpha w^2) for one synapse, so w
+    settles at 1 / sqrt(alpha) from either side, whatever the rate a (for
+    Euler steps small enough not to overshoot below 0, where w stays)."""
+    alpha = 4.0
+    for a in (0.5, 1.3):
+        for w0 in (0.2, 0.9):
+            out = _one(1, w0, a, 20000, [0.05, alpha])
+            np.testing.assert_allclose(out["w"], [1.0 / np.sqrt(alpha)], rtol=1e-9)
+
+
+def test_ibcm_threshold_relaxation():
+    """eta = 0 keeps w and the post rate b = w*a fixed; theta relaxes as
+    b^2 (1 - exp(-dt/tau)^(t+1))."""
+    a, w0, tau, dt, steps = 0.5, 0.8, 25.0, 1.0, 60
+    out = _one(2, w0, a, steps, [0.0, tau], dt=dt)
+    b = w0 * a
+    np.testing.assert_allclose(out["theta"], [b * b * (1.0 - np.exp(-dt / tau) ** steps)], rtol=1e-12)
+    np.testing.assert_array_equal(out["w"], [w0])
+    # with learning: below threshold (theta > post rate^2 is impossible at start) w grows while post.r > theta
+    out = _one(2, w0, a, 5, [0.1, tau], dt=dt)
+    assert out["w"][0] > w0
+
+
+def test_stp_closed_form():
+    """STP (Tsodyks-Markram): the first spike transmits w*U and leaves
+    x = 1 - U, u = U + U(1 - U); between spikes x relaxes to 1 with tau_rec
+    and u to U with tau_facil (from the handled step, the decay ending one
+    step before the next handled spike)."""
+    tau_rec, tau_facil, U, dt, w = 100.0, 50.0, 0.3, 1.0, 2.0
+    pc, pi = _trains(30, 1, {4: [0], 14: [0]})
+    qc, qi = _trains(30, 1, {})
+    out = orc.plastic_run(2, [0, 1], [0], 1, w, [tau_rec, tau_facil, U], dt, pc, pi, qc, qi,
+                          x0=[1.0], y0=[U])
+    x1, u1 = 1.0 - U, U + U * (1.0 - U)
+    # second spike handled at 15: decay from 5 to 14
+    x2 = 1.0 + (x1 - 1.0) * math.exp(-9 * dt / tau_rec)
+    u2 = U + (u1 - U) * math.exp(-9 * dt / tau_facil)
+    g = out["g"][:, 0]
+    assert np.flatnonzero(g).tolist() == [5, 15]
+    np.testing.assert_allclose(g[[5, 15]], [w * U, w * u2 * x2], rtol=1e-15)
+    np.testing.assert_allclose(out["x"], [x2 * (1.0 - u2)], rtol=1e-15)
+    np.testing.assert_allclose(out["y"], [u2 + U * (1.0 - u2)], rtol=1e-15)
