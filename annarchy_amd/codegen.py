@@ -298,9 +298,10 @@ __device__ inline bool update_due(const ahip_syn_args& a, int64_t t) {
     return m == 0;
 }
 
-// Pre events (compute_psp of step t, spikes of step t-d): one wave per
-// spike, lanes over the synapses of the pre neuron's column.  A synapse
-// belongs to one pre neuron, so no two lanes touch the same state.
+// Pre events (compute_psp of step t, spikes of step t-d): work item (spike,
+// chunk c) goes to one wave, whose lanes take synapses c*64 + lane,
+// (c + chunks)*64 + lane, ... of the pre neuron's column.  A synapse belongs
+// to one pre neuron and one chunk, so no two lanes touch the same state.
 template <typename T>
 __global__ __launch_bounds__(256) void pre_event(ahip_syn_args a) {
     const int lane = threadIdx.x & (ahip::kWave - 1);
@@ -308,6 +309,7 @@ __global__ __launch_bounds__(256) void pre_event(ahip_syn_args a) {
     const int64_t t = *a.t_base + a.step;
     const T dt = (T)a.dt;
     const int32_t _plasticity = a.plasticity;
+    const int64_t _rs = a.state_stride, _rs8 = a.state_stride8;
     T* __restrict__ w = (T*)a.w;
     T* __restrict__ _g_post = (T*)a.g;
     int64_t* __restrict__ _last_event = a.last_event;
@@ -323,20 +325,25 @@ __global__ __launch_bounds__(256) void pre_event(ahip_syn_args a) {
     }
     const int32_t n = a.pre_count[slot];
     const int32_t* __restrict__ list = a.pre_spiked + slot * a.pre_stride;
+    const int32_t ch = a.pre_chunks;
+    const int64_t items = (int64_t)n * ch;
     const int64_t waves = (int64_t)gridDim.x * 4;
-    for (int64_t s = (int64_t)blockIdx.x * 4 + wid; s < n; s += waves) {
+    for (int64_t it = (int64_t)blockIdx.x * 4 + wid; it < items; it += waves) {
+        const int64_t s = it / ch;
+        const int32_t c = (int32_t)(it - s * ch);
         const int32_t pre = list[s];
         if (pre < 0 || pre >= a.n_cols) continue;
         const int64_t e = a.col_ptr[pre + 1];
-        for (int64_t q = a.col_ptr[pre] + lane; q < e; q += ahip::kWave) {
+        for (int64_t q = a.col_ptr[pre] + (int64_t)c * ahip::kWave + lane; q < e; q += (int64_t)ch * ahip::kWave) {
             const int32_t post = a.row_idx[q];
 %(pre_body)s
         }
     }
 }
 
-// Post events (after the neuron updates of step t, spikes of step t): one
-// wave per spiking post neuron, lanes over its row; state by column position.
+// Post events (after the neuron updates of step t, spikes of step t): work
+// items (spike, chunk) over the spiking post neuron's row as for the pre
+// events; state by column position.
 template <typename T>
 __global__ __launch_bounds__(256) void post_event(ahip_syn_args a) {
     const int lane = threadIdx.x & (ahip::kWave - 1);
@@ -344,6 +351,7 @@ __global__ __launch_bounds__(256) void post_event(ahip_syn_args a) {
     const int64_t t = *a.t_base + a.step;
     const T dt = (T)a.dt;
     const int32_t _plasticity = a.plasticity;
+    const int64_t _rs = a.state_stride, _rs8 = a.state_stride8;
     T* __restrict__ w = (T*)a.w;
     int64_t* __restrict__ _last_event = a.last_event;
     const int64_t* __restrict__ _pre_last_spike = a.pre_last_spike;
@@ -352,12 +360,16 @@ __global__ __launch_bounds__(256) void post_event(ahip_syn_args a) {
 %(ev_attrs)s
 %(deps)s
     const int32_t n = *a.post_count;
+    const int32_t ch = a.post_chunks;
+    const int64_t items = (int64_t)n * ch;
     const int64_t waves = (int64_t)gridDim.x * 4;
-    for (int64_t s = (int64_t)blockIdx.x * 4 + wid; s < n; s += waves) {
+    for (int64_t it = (int64_t)blockIdx.x * 4 + wid; it < items; it += waves) {
+        const int64_t s = it / ch;
+        const int32_t c = (int32_t)(it - s * ch);
         const int32_t i = a.post_spiked[s];
         if (i < 0 || i >= a.n_rows) continue;
         const int64_t e = a.row_ptr[i + 1];
-        for (int64_t j = a.row_ptr[i] + lane; j < e; j += ahip::kWave) {
+        for (int64_t j = a.row_ptr[i] + (int64_t)c * ahip::kWave + lane; j < e; j += (int64_t)ch * ahip::kWave) {
             const int64_t q = a.fwd2col ? a.fwd2col[j] : j;
             const int32_t pre = a.col_idx[j];
             const int32_t post = i;
@@ -396,6 +408,7 @@ __global__ __launch_bounds__(256) void update_local(ahip_syn_args a) {
     if (!update_due(a, t)) return;
     const T dt = (T)(a.dt * (double)(a.update_period > 1 ? a.update_period : 1));
     const int32_t _plasticity = a.plasticity;
+    const int64_t _rs = a.state_stride;
     T* __restrict__ w = (T*)a.w;
     const int64_t* __restrict__ _pre_last_spike = a.pre_last_spike;
     const int64_t* __restrict__ _post_last_spike = a.post_last_spike;
@@ -423,9 +436,11 @@ __global__ __launch_bounds__(256) void update_local(ahip_syn_args a) {
 
 template <typename T>
 int launch_events(const ahip_syn_args* a, void* stream, bool pre) {
-    const int32_t units = pre ? a->n_cols : a->n_rows;
-    int nb = (units + 3) / 4;
-    if (nb > 512) nb = 512;
+    // work items (spike, chunk of 64 synapses): one pass over every neuron
+    // of the population spiking, up to 4 workgroups per CU, then loops
+    const int64_t units = (int64_t)(pre ? a->n_cols : a->n_rows) * (pre ? a->pre_chunks : a->post_chunks);
+    int64_t nb = (units + 3) / 4;
+    if (nb > 1024) nb = 1024;
     if (nb < 1) nb = 1;
     if (pre)
         hipLaunchKernelGGL(pre_event<T>, dim3(nb), dim3(256), 0, (hipStream_t)stream, *a);
@@ -480,8 +495,10 @@ class SynapseSource:
 
 
 def _syn_subst(code: str, t_text: str, sg_index: str) -> str:
-    code = code.replace("%(t)s", t_text)
-    return (code.replace("%(local_index)s", "[q]").replace("%(global_index)s", "")
+    """Per-synapse values of synapse q sit at [q * _rs] (_last_event at
+    [q * _rs8]): one record per synapse, or separate arrays with stride 1."""
+    code = code.replace("%(t)s", t_text).replace("_last_event%(local_index)s", "_last_event[q*_rs8]")
+    return (code.replace("%(local_index)s", "[q*_rs]").replace("%(global_index)s", "")
             .replace("%(semiglobal_index)s", sg_index))
 
 
@@ -538,8 +555,8 @@ def generate_synapse(desc) -> SynapseSource:
     for _, code in desc.event_driven:
         pre.append(_syn_subst(code, "(t-1)", "[post]"))
     if desc.event_driven:
-        pre.append("_last_event[q] = t;")
-    g = _syn_subst(desc.g_code, "(t)", "[post]") if desc.g_code else "w[q]"
+        pre.append("_last_event[q*_rs8] = t;")
+    g = _syn_subst(desc.g_code, "(t)", "[post]") if desc.g_code else "w[q*_rs]"
     pre.append(f"unsafeAtomicAdd(_g_post + post, (T)({g}));")
     pre += [_syn_subst(c, "(t)", "[post]") for c in desc.pre_code]
     if desc.post_code:
@@ -547,7 +564,7 @@ def generate_synapse(desc) -> SynapseSource:
         for _, code in desc.event_driven:
             post.append(_syn_subst(code, "(t)", "[post]"))
         if desc.event_driven:
-            post.append("_last_event[q] = t;")
+            post.append("_last_event[q*_rs8] = t;")
         post += [_syn_subst(c, "(t)", "[post]") for c in desc.post_code]
     global_eqs = _syn_subst(synapse_update_code(desc, "global"), "(t)", "")
     sg_eqs = _syn_subst(synapse_update_code(desc, "semiglobal"), "(t)", "")

@@ -108,9 +108,16 @@ struct Proj {
     void* syn_dl = nullptr;
     std::string syn_path;
     const ahip_synapse_desc* syn = nullptr;
-    std::vector<DevBuf> syn_attr;            // per attribute: nnz / n_rows / 1 values
+    std::vector<DevBuf> syn_attr;            // per attribute: nnz / n_rows / 1 values (a per-synapse
+                                             // attribute of a spiking projection lives in syn_rec)
+    // spiking: one record per synapse, column order: w, the per-synapse
+    // attributes, _last_event (8-aligned); rec_off = byte offset of each field
+    DevBuf syn_rec;
+    int32_t rec_bytes = 0, rec_off_w = 0, rec_off_last = 0;
+    std::vector<int32_t> rec_off;            // per attribute, -1 if not per synapse
     std::vector<int32_t> dep_pre, dep_post;  // slots of the pre.<x> / post.<x> attributes
     int32_t syn_update = 1, syn_plasticity = 1, syn_period = 1, syn_offset = 0;
+    int32_t syn_pre_chunks = 1, syn_post_chunks = 1;   // waves per spike in the event kernels
     DevBuf last_event, fwd2col, row_ptr_dev, col_dev;
     std::vector<int64_t> h_inv_idx;
 };
@@ -177,6 +184,9 @@ struct ahip_net {
     int32_t graph_steps = 0;
     bool dirty = true;             // structure changed since the last capture
     bool scatter_dirty = true;     // the fused scatter tables need rebuilding
+    // kernel-time probe: HIP events around one plastic kernel inside eager steps
+    int32_t probe_proj = -1, probe_kind = -1;
+    hipEvent_t probe_ev[2] = {nullptr, nullptr};
     std::vector<ahip::ScatterGroup> scatter_groups;
     size_t elem() const { return precision == AHIP_F64 ? sizeof(double) : sizeof(float); }
 };

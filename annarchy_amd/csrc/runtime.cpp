@@ -25,7 +25,7 @@
 
 using namespace ahip;
 
-static int syn_read(ahip_net* net, Proj& p, const void* dev, double* out, int64_t n, bool per_synapse);
+static int syn_field_read(ahip_net* net, Proj& p, int field, double* out);
 
 namespace {
 
@@ -194,15 +194,56 @@ int init_syn_state(ahip_net* net, Proj& p) {
     HIPCHK(hipStreamSynchronize(net->stream));
     p.syn_attr.clear();
     p.syn_attr.resize(d->n_attr);
+    p.rec_off.assign(d->n_attr, -1);
     int rc;
     for (int32_t k = 0; k < d->n_attr; k++) {
         const int64_t n = d->attr_locality[k] == AHIP_SYN_LOCAL ? p.nnz
                           : d->attr_locality[k] == AHIP_SYN_SEMIGLOBAL ? p.n_rows : 1;
+        if (p.spiking && d->attr_locality[k] == AHIP_SYN_LOCAL) continue;   // in the record
         if ((rc = dev_alloc(p.syn_attr[k], (size_t)n * es))) return rc;
         LCHK(launch_fill(p.syn_attr[k].p, n, (int32_t)es, d->attr_init[k], net->stream));
     }
-    if ((rc = dev_alloc(p.last_event, (size_t)p.nnz * sizeof(int64_t)))) return rc;
-    LCHK(launch_fill_i64(p.last_event.as<int64_t>(), p.nnz, -10000, net->stream));
+    p.syn_rec.release();
+    p.last_event.release();
+    if (p.spiking) {
+        // record of a synapse: w, the per-synapse attributes, _last_event
+        int32_t off = (int32_t)es;
+        p.rec_off_w = 0;
+        for (int32_t k = 0; k < d->n_attr; k++)
+            if (d->attr_locality[k] == AHIP_SYN_LOCAL) {
+                p.rec_off[k] = off;
+                off += (int32_t)es;
+            }
+        off = (off + 7) & ~7;
+        p.rec_off_last = off;
+        p.rec_bytes = off + 8;
+        if ((rc = dev_alloc(p.syn_rec, (size_t)p.nnz * p.rec_bytes))) return rc;
+        const int64_t chunk = 1 << 20;
+        std::vector<char> buf((size_t)std::min<int64_t>(chunk, std::max<int64_t>(p.nnz, 1)) * p.rec_bytes);
+        for (int64_t q0 = 0; q0 < p.nnz; q0 += chunk) {
+            const int64_t m = std::min(chunk, p.nnz - q0);
+            for (int64_t i = 0; i < m; i++) {
+                char* r = buf.data() + (size_t)i * p.rec_bytes;
+                const double wv = p.h_w[p.h_inv_idx[q0 + i]];
+                auto put = [&](int32_t o, double v) {
+                    if (es == 8) std::memcpy(r + o, &v, 8);
+                    else { const float f = (float)v; std::memcpy(r + o, &f, 4); }
+                };
+                std::memset(r, 0, p.rec_bytes);
+                put(p.rec_off_w, wv);
+                for (int32_t k = 0; k < d->n_attr; k++)
+                    if (p.rec_off[k] >= 0) put(p.rec_off[k], d->attr_init[k]);
+                const int64_t le = -10000;
+                std::memcpy(r + p.rec_off_last, &le, 8);
+            }
+            HIPCHK(hipMemcpy((char*)p.syn_rec.p + (size_t)q0 * p.rec_bytes, buf.data(), (size_t)m * p.rec_bytes,
+                             hipMemcpyHostToDevice));
+        }
+        p.w_col.release();   // the weights live in the records
+    } else {
+        if ((rc = dev_alloc(p.last_event, (size_t)p.nnz * sizeof(int64_t)))) return rc;
+        LCHK(launch_fill_i64(p.last_event.as<int64_t>(), p.nnz, -10000, net->stream));
+    }
     p.fwd2col.release();
     if (p.spiking) {
         std::vector<int64_t> f2c(p.nnz);
@@ -210,6 +251,14 @@ int init_syn_state(ahip_net* net, Proj& p) {
         if ((rc = dev_alloc(p.fwd2col, (size_t)p.nnz * sizeof(int64_t)))) return rc;
         if (p.nnz) HIPCHK(hipMemcpy(p.fwd2col.p, f2c.data(), f2c.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     }
+    // waves per spike: 64-synapse chunks of the longest column / row, at most 32
+    int64_t max_row = 0;
+    std::vector<int64_t> col_len(p.n_cols, 0);
+    for (int32_t r = 0; r < p.n_rows; r++) max_row = std::max(max_row, p.h_row_ptr[r + 1] - p.h_row_ptr[r]);
+    for (int64_t j = 0; j < p.nnz; j++) col_len[p.h_col[j]]++;
+    const int64_t max_col = col_len.empty() ? 0 : *std::max_element(col_len.begin(), col_len.end());
+    p.syn_pre_chunks = (int32_t)std::min<int64_t>(32, std::max<int64_t>(1, (max_col + 63) / 64));
+    p.syn_post_chunks = (int32_t)std::min<int64_t>(32, std::max<int64_t>(1, (max_row + 63) / 64));
     if ((rc = dev_alloc(p.row_ptr_dev, p.h_row_ptr.size() * sizeof(int64_t)))) return rc;
     HIPCHK(hipMemcpy(p.row_ptr_dev.p, p.h_row_ptr.data(), p.h_row_ptr.size() * sizeof(int64_t),
                      hipMemcpyHostToDevice));
@@ -219,12 +268,21 @@ int init_syn_state(ahip_net* net, Proj& p) {
     return 0;
 }
 
-// Weights of a plastic projection on the device (state order).
-void* syn_w(Proj& p) { return p.spiking ? p.w_col.p : p.rate.val.p; }
+
+int launch_syn_now(ahip_net* net, Proj& p, int kind, int32_t step);
 
 // Pre events (kind 0), update_synapse (1) or post events (2) of a plastic
-// projection at `step`.
+// projection at `step`, between the probe events when this kernel is probed.
 int launch_syn(ahip_net* net, Proj& p, int kind, int32_t step) {
+    const bool probe = net->probe_proj == p.id && net->probe_kind == kind;
+    if (probe) HIPCHK(hipEventRecord(net->probe_ev[0], net->stream));
+    int rc = launch_syn_now(net, p, kind, step);
+    if (rc) return rc;
+    if (probe) HIPCHK(hipEventRecord(net->probe_ev[1], net->stream));
+    return 0;
+}
+
+int launch_syn_now(ahip_net* net, Proj& p, int kind, int32_t step) {
     Pop& pre = *net->pops[p.pre];
     Pop& post = *net->pops[p.post];
     ahip_syn_args a;
@@ -254,9 +312,21 @@ int launch_syn(ahip_net* net, Proj& p, int kind, int32_t step) {
     a.col_idx = p.col_dev.as<int32_t>();
     a.pre_last_spike = pre.last_spike.as<int64_t>();
     a.post_last_spike = post.last_spike.as<int64_t>();
-    a.w = syn_w(p);
-    a.last_event = p.last_event.as<int64_t>();
-    for (int32_t k = 0; k < p.syn->n_attr; k++) a.attr[k] = p.syn_attr[k].p;
+    if (p.spiking) {
+        char* base = (char*)p.syn_rec.p;
+        a.w = base + p.rec_off_w;
+        a.last_event = (int64_t*)(base + p.rec_off_last);
+        a.state_stride = p.rec_bytes / (int32_t)net->elem();
+        a.state_stride8 = p.rec_bytes / 8;
+        for (int32_t k = 0; k < p.syn->n_attr; k++)
+            a.attr[k] = p.rec_off[k] >= 0 ? (void*)(base + p.rec_off[k]) : p.syn_attr[k].p;
+    } else {
+        a.w = p.rate.val.p;
+        a.last_event = p.last_event.as<int64_t>();
+        a.state_stride = 1;
+        a.state_stride8 = 1;
+        for (int32_t k = 0; k < p.syn->n_attr; k++) a.attr[k] = p.syn_attr[k].p;
+    }
     for (size_t k = 0; k < p.dep_pre.size(); k++) {
         a.pre_attr[k] = pre.slots[p.dep_pre[k]].p;
         a.pre_attr_stride[k] = pre.model->slot_local[p.dep_pre[k]] ? 1 : 0;
@@ -268,6 +338,8 @@ int launch_syn(ahip_net* net, Proj& p, int kind, int32_t step) {
     a.t_base = net->t_dev.as<int64_t>();
     a.step = step;
     a.dt = net->dt;
+    a.pre_chunks = p.syn_pre_chunks;
+    a.post_chunks = p.syn_post_chunks;
     a.plasticity = p.syn_plasticity;
     a.update_period = p.syn_period;
     a.update_offset = p.syn_offset;
@@ -1153,7 +1225,7 @@ int ahip_proj_set_format(ahip_net* net, int32_t proj, const ahip_format_opts* op
     invalidate(net);
     if (p->syn) {
         // the learned weights are on the device: refresh the host copy first
-        int rc = syn_read(net, *p, p->rate.val.p, p->h_w.data(), p->nnz, true);
+        int rc = syn_field_read(net, *p, -1, p->h_w.data());
         if (rc) return rc;
     }
     std::string err;
@@ -1282,35 +1354,78 @@ int ahip_proj_nb_synapses(ahip_net* net, int32_t proj, int64_t* nnz) {
     return 0;
 }
 
-// Device values of a plastic projection (n of them; state order) -> host,
-// forward order for per-synapse values of a spiking projection.
-static int syn_read(ahip_net* net, Proj& p, const void* dev, double* out, int64_t n, bool per_synapse) {
-    std::vector<double> v(n);
+// Per-synapse field of a plastic projection (field -1 = w, k = attribute k,
+// -2 = _last_event) between the device and a host array in forward order.
+// Spiking: fields of the column-order records, copied in chunks of records;
+// rate-coded: forward-order arrays.
+static int syn_field_io(ahip_net* net, Proj& p, int field, double* dbl, int64_t* i64, bool write) {
+    const size_t es = net->elem();
     HIPCHK(hipStreamSynchronize(net->stream));
-    if (net->precision == AHIP_F64) {
-        if (n) HIPCHK(hipMemcpy(v.data(), dev, n * sizeof(double), hipMemcpyDeviceToHost));
-    } else {
-        std::vector<float> f(n);
-        if (n) HIPCHK(hipMemcpy(f.data(), dev, n * sizeof(float), hipMemcpyDeviceToHost));
-        for (int64_t q = 0; q < n; q++) v[q] = f[q];
+    if (!p.spiking) {
+        void* dev = field == -1 ? p.rate.val.p : field == -2 ? p.last_event.p : p.syn_attr[field].p;
+        const size_t vb = field == -2 ? 8 : es;
+        std::vector<char> buf((size_t)p.nnz * vb);
+        if (write) {
+            for (int64_t q = 0; q < p.nnz; q++) {
+                if (field == -2) std::memcpy(&buf[q * 8], &i64[q], 8);
+                else if (es == 8) std::memcpy(&buf[q * 8], &dbl[q], 8);
+                else { const float f = (float)dbl[q]; std::memcpy(&buf[q * 4], &f, 4); }
+            }
+            if (p.nnz) HIPCHK(hipMemcpy(dev, buf.data(), buf.size(), hipMemcpyHostToDevice));
+        } else {
+            if (p.nnz) HIPCHK(hipMemcpy(buf.data(), dev, buf.size(), hipMemcpyDeviceToHost));
+            for (int64_t q = 0; q < p.nnz; q++) {
+                if (field == -2) std::memcpy(&i64[q], &buf[q * 8], 8);
+                else if (es == 8) std::memcpy(&dbl[q], &buf[q * 8], 8);
+                else { float f; std::memcpy(&f, &buf[q * 4], 4); dbl[q] = f; }
+            }
+        }
+        return 0;
     }
-    if (per_synapse && p.spiking)
-        for (int64_t q = 0; q < n; q++) out[p.h_inv_idx[q]] = v[q];
-    else
-        std::copy(v.begin(), v.end(), out);
+    const int32_t off = field == -1 ? p.rec_off_w : field == -2 ? p.rec_off_last : p.rec_off[field];
+    const int64_t chunk = 1 << 20;
+    std::vector<char> buf((size_t)std::min<int64_t>(chunk, std::max<int64_t>(p.nnz, 1)) * p.rec_bytes);
+    for (int64_t q0 = 0; q0 < p.nnz; q0 += chunk) {
+        const int64_t m = std::min(chunk, p.nnz - q0);
+        char* dev = (char*)p.syn_rec.p + (size_t)q0 * p.rec_bytes;
+        HIPCHK(hipMemcpy(buf.data(), dev, (size_t)m * p.rec_bytes, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < m; i++) {
+            char* r = buf.data() + (size_t)i * p.rec_bytes + off;
+            const int64_t j = p.h_inv_idx[q0 + i];
+            if (field == -2) {
+                if (write) std::memcpy(r, &i64[j], 8); else std::memcpy(&i64[j], r, 8);
+            } else if (es == 8) {
+                if (write) std::memcpy(r, &dbl[j], 8); else std::memcpy(&dbl[j], r, 8);
+            } else {
+                float f;
+                if (write) { f = (float)dbl[j]; std::memcpy(r, &f, 4); }
+                else { std::memcpy(&f, r, 4); dbl[j] = f; }
+            }
+        }
+        if (write) HIPCHK(hipMemcpy(dev, buf.data(), (size_t)m * p.rec_bytes, hipMemcpyHostToDevice));
+    }
     return 0;
 }
 
-static int syn_write(ahip_net* net, Proj& p, void* dev, const double* in, int64_t n, bool per_synapse) {
+static int syn_field_read(ahip_net* net, Proj& p, int field, double* out) {
+    return syn_field_io(net, p, field, out, nullptr, false);
+}
+
+// Per-post-neuron / per-projection attribute (n values) between device and host.
+static int syn_plain_io(ahip_net* net, void* dev, double* v, int64_t n, bool write) {
     HIPCHK(hipStreamSynchronize(net->stream));
-    std::vector<double> v(in, in + n);
-    if (per_synapse && p.spiking)
-        for (int64_t q = 0; q < n; q++) v[q] = in[p.h_inv_idx[q]];
     if (net->precision == AHIP_F64) {
-        if (n) HIPCHK(hipMemcpy(dev, v.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        if (n) HIPCHK(hipMemcpy(write ? dev : (void*)v, write ? (void*)v : dev, n * 8,
+                                write ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost));
     } else {
-        std::vector<float> f(v.begin(), v.end());
-        if (n) HIPCHK(hipMemcpy(dev, f.data(), n * sizeof(float), hipMemcpyHostToDevice));
+        std::vector<float> f(n);
+        if (write) {
+            for (int64_t q = 0; q < n; q++) f[q] = (float)v[q];
+            if (n) HIPCHK(hipMemcpy(dev, f.data(), n * 4, hipMemcpyHostToDevice));
+        } else {
+            if (n) HIPCHK(hipMemcpy(f.data(), dev, n * 4, hipMemcpyDeviceToHost));
+            for (int64_t q = 0; q < n; q++) v[q] = f[q];
+        }
     }
     return 0;
 }
@@ -1318,7 +1433,7 @@ static int syn_write(ahip_net* net, Proj& p, void* dev, const double* in, int64_
 int ahip_proj_get_w(ahip_net* net, int32_t proj, double* w, int64_t nnz) {
     Proj* p = get_proj(net, proj);
     if (!p || !w || nnz != p->nnz) return fail(AHIP_EINVAL, "bad projection or size");
-    if (p->syn) return syn_read(net, *p, syn_w(*p), w, nnz, true);
+    if (p->syn) return syn_field_read(net, *p, -1, w);
     for (int64_t j = 0; j < nnz; j++) w[j] = p->single_w ? p->w_const : p->h_w[j];
     return 0;
 }
@@ -1330,7 +1445,7 @@ int ahip_proj_set_w(ahip_net* net, int32_t proj, const double* w, int64_t nnz) {
     p->single_w = false;
     p->h_w.assign(w, w + nnz);
     invalidate(net);
-    if (p->syn) return syn_write(net, *p, syn_w(*p), w, nnz, true);
+    if (p->syn) return syn_field_io(net, *p, -1, const_cast<double*>(w), nullptr, true);
     if (p->spiking) return build_backward(net, *p);
     ahip_format_opts o{p->rate.format, p->rate.lanes, p->rate.chunk, p->rate.sigma, p->rate.panel_cols};
     std::string err;
@@ -1423,7 +1538,8 @@ int ahip_proj_syn_get(ahip_net* net, int32_t proj, const char* attr, double* val
     if (k < 0 || !values || p->nnz < 0) return fail(AHIP_EINVAL, "no synaptic attribute %s", attr ? attr : "");
     const int64_t size = syn_attr_size(*p, k);
     if (n != size) return fail(AHIP_EINVAL, "%s holds %lld values", attr, (long long)size);
-    return syn_read(net, *p, p->syn_attr[k].p, values, n, p->syn->attr_locality[k] == AHIP_SYN_LOCAL);
+    if (p->syn->attr_locality[k] == AHIP_SYN_LOCAL) return syn_field_read(net, *p, k, values);
+    return syn_plain_io(net, p->syn_attr[k].p, values, n, false);
 }
 
 int ahip_proj_syn_set(ahip_net* net, int32_t proj, const char* attr, const double* values, int64_t n) {
@@ -1431,24 +1547,26 @@ int ahip_proj_syn_set(ahip_net* net, int32_t proj, const char* attr, const doubl
     const int k = syn_attr(p, attr);
     if (k < 0 || !values || p->nnz < 0) return fail(AHIP_EINVAL, "no synaptic attribute %s", attr ? attr : "");
     const int64_t size = syn_attr_size(*p, k);
+    const bool local = p->syn->attr_locality[k] == AHIP_SYN_LOCAL;
     if (n == 1 && size != 1) {
+        if (local) {
+            std::vector<double> v(size, values[0]);
+            return syn_field_io(net, *p, k, v.data(), nullptr, true);
+        }
         HIPCHK(hipStreamSynchronize(net->stream));
         LCHK(launch_fill(p->syn_attr[k].p, size, (int32_t)net->elem(), values[0], net->stream));
         HIPCHK(hipStreamSynchronize(net->stream));
         return 0;
     }
     if (n != size) return fail(AHIP_EINVAL, "%s holds %lld values", attr, (long long)size);
-    return syn_write(net, *p, p->syn_attr[k].p, values, n, p->syn->attr_locality[k] == AHIP_SYN_LOCAL);
+    if (local) return syn_field_io(net, *p, k, const_cast<double*>(values), nullptr, true);
+    return syn_plain_io(net, p->syn_attr[k].p, const_cast<double*>(values), n, true);
 }
 
 int ahip_proj_get_last_event(ahip_net* net, int32_t proj, int64_t* values, int64_t n) {
     Proj* p = get_proj(net, proj);
     if (!p || !p->syn || !values || n != p->nnz) return fail(AHIP_EINVAL, "bad plastic projection or size");
-    std::vector<int64_t> v(p->nnz);
-    HIPCHK(hipStreamSynchronize(net->stream));
-    if (p->nnz) HIPCHK(hipMemcpy(v.data(), p->last_event.p, p->nnz * sizeof(int64_t), hipMemcpyDeviceToHost));
-    for (int64_t q = 0; q < p->nnz; q++) values[p->spiking ? p->h_inv_idx[q] : q] = v[q];
-    return 0;
+    return syn_field_io(net, *p, -2, nullptr, values, false);
 }
 
 int ahip_proj_set_learning(ahip_net* net, int32_t proj, int32_t update, int32_t plasticity, int32_t period,
@@ -1475,6 +1593,7 @@ int ahip_proj_psp_bytes(ahip_net* net, int32_t proj, double* bytes) {
 int ahip_proj_time_psp(ahip_net* net, int32_t proj, int32_t iters, double* avg_ms) {
     Proj* p = get_proj(net, proj);
     if (!p || !avg_ms || iters <= 0 || p->nnz < 0) return fail(AHIP_EINVAL, "bad projection or iteration count");
+    if (p->syn && p->spiking) return fail(AHIP_EINVAL, "a plastic spiking projection: use ahip_proj_time_syn");
     HIPCHK(hipSetDevice(net->device));
     int rc = prepare(net);
     if (rc) return rc;
@@ -1501,6 +1620,46 @@ int ahip_proj_time_psp(ahip_net* net, int32_t proj, int32_t iters, double* avg_m
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     *avg_ms = (double)ms / iters;
+    return 0;
+}
+
+int ahip_proj_time_syn(ahip_net* net, int32_t proj, int32_t kind, int32_t iters, double* avg_ms) {
+    Proj* p = get_proj(net, proj);
+    if (!p || !p->syn || !avg_ms || iters <= 0 || p->nnz < 0) return fail(AHIP_EINVAL, "bad plastic projection");
+    if (kind < 0 || kind > 2 || (kind != 1 && !p->spiking) || (kind == 1 && !p->syn->has_update) ||
+        (kind == 2 && !p->syn->has_post))
+        return fail(AHIP_EINVAL, "the projection has no kernel of kind %d", kind);
+    if (kind == 1 && !p->syn_update) return fail(AHIP_ESTATE, "update_synapse is disabled");
+    HIPCHK(hipSetDevice(net->device));
+    int rc = prepare(net);
+    if (rc) return rc;
+    std::vector<int32_t> zb;
+    assign_accumulate(net, zb);
+    HIPCHK(hipEventCreate(&net->probe_ev[0]));
+    HIPCHK(hipEventCreate(&net->probe_ev[1]));
+    net->probe_proj = proj;
+    net->probe_kind = kind;
+    // `iters` ordinary steps, launched eagerly, with the events around the
+    // probed kernel: its time in the context of a real step
+    double total = 0.0;
+    for (int32_t i = 0; i < iters && rc == 0; i++) {
+        rc = enqueue_batch(net, 1, zb);
+        if (rc) break;
+        hipError_t e = hipStreamSynchronize(net->stream);
+        if (e != hipSuccess) { rc = fail(AHIP_EHIP, "step failed: %s", hipGetErrorString(e)); break; }
+        net->t += 1;
+        float ms = 0.f;
+        e = hipEventElapsedTime(&ms, net->probe_ev[0], net->probe_ev[1]);
+        if (e != hipSuccess) { rc = fail(AHIP_EHIP, "probe events: %s", hipGetErrorString(e)); break; }
+        total += ms;
+    }
+    net->probe_proj = -1;
+    net->probe_kind = -1;
+    hipEventDestroy(net->probe_ev[0]);
+    hipEventDestroy(net->probe_ev[1]);
+    net->probe_ev[0] = net->probe_ev[1] = nullptr;
+    if (rc) return rc;
+    *avg_ms = total / iters;
     return 0;
 }
 

@@ -545,6 +545,15 @@ class Projection:
         _lib.check(_lib.lib().ahip_proj_psp_bytes(self.net._h, self.id, C.byref(b)))
         return b.value
 
+    def time_events(self, kind="pre", iters=20) -> float:
+        """Mean time (ms) of the plastic projection's pre-event, update or
+        post-event kernel inside `iters` ordinary steps, which advance the
+        simulation."""
+        k = {"pre": 0, "update": 1, "post": 2}[kind]
+        ms = C.c_double()
+        _lib.check(_lib.lib().ahip_proj_time_syn(self.net._h, self.id, k, iters, C.byref(ms)))
+        return ms.value
+
     def time_psp(self, iters=20) -> float:
         ms = C.c_double()
         _lib.check(_lib.lib().ahip_proj_time_psp(self.net._h, self.id, iters, C.byref(ms)))

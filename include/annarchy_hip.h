@@ -208,6 +208,12 @@ int ahip_proj_get_last_event(ahip_net* net, int32_t proj, int64_t* values, int64
  * (t - offset) mod period == 0, with dt scaled by the period. */
 int ahip_proj_set_learning(ahip_net* net, int32_t proj, int32_t update, int32_t plasticity, int32_t period,
                            int32_t offset);
+/* Measurement helper with no counterpart in the reference: runs `iters`
+ * ordinary simulation steps (they advance the simulation, eagerly launched)
+ * with HIP events on the network stream around a plastic projection's
+ * pre-event (kind 0), update_synapse (1) or post-event (2) kernel inside
+ * each step; *avg_ms = mean time of that kernel within real steps. */
+int ahip_proj_time_syn(ahip_net* net, int32_t proj, int32_t kind, int32_t iters, double* avg_ms);
 /* Bytes the psp kernel must move per step (algorithmic, no padding). */
 int ahip_proj_psp_bytes(ahip_net* net, int32_t proj, double* bytes);
 /* Times `iters` back-to-back launches of the projection's psp kernel with HIP

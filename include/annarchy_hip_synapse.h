@@ -61,9 +61,16 @@ typedef struct ahip_syn_args {
     const int64_t* pre_last_spike;
     const int64_t* post_last_spike;
     void* g;                  /* g_<target> of the post population (spiking) */
-    void* w;                  /* weights, state order (network precision) */
-    int64_t* last_event;      /* state order, -10000 initially */
-    void* attr[AHIP_MAX_SYN_ATTRS];   /* attribute k: nnz, n_rows or 1 values by locality */
+    /* Per-synapse values: value of synapse q at ptr[q * state_stride]
+     * (state_stride8 for _last_event, in int64 units).  A spiking projection
+     * keeps one record per synapse (w, the per-synapse attributes,
+     * _last_event), so these pointers point into the same records; a
+     * rate-coded one keeps separate arrays (strides 1). */
+    void* w;                  /* weights (network precision) */
+    int64_t* last_event;      /* -10000 initially */
+    int32_t state_stride;     /* in values of the network precision */
+    int32_t state_stride8;    /* in int64 */
+    void* attr[AHIP_MAX_SYN_ATTRS];   /* attribute k: per synapse as above, n_rows or 1 values */
     /* neuron attributes read as pre.<name> / post.<name>: element i of
      * neuron n is ptr[n * stride] (stride 0 for a population-wide value) */
     const void* pre_attr[AHIP_MAX_SYN_DEPS];
@@ -73,6 +80,8 @@ typedef struct ahip_syn_args {
     const int64_t* t_base;    /* simulation step at the start of the batch */
     int32_t step;
     double dt;
+    int32_t pre_chunks;       /* waves per pre spike (64 synapses each, strided) */
+    int32_t post_chunks;      /* waves per post spike */
     int32_t plasticity;       /* 0: w is left unchanged (_plasticity) */
     int32_t update_period;    /* update every period steps ... */
     int32_t update_offset;    /* ... when (t - offset) mod period == 0 */
