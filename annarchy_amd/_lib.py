@@ -96,7 +96,8 @@ _SIGS = {
     "ahip_proj_syn_set": ([_P, _I32, C.c_char_p, _PD, _I64], _I32),
     "ahip_proj_get_last_event": ([_P, _I32, _PI64, _I64], _I32),
     "ahip_proj_set_learning": ([_P, _I32, _I32, _I32, _I32, _I32], _I32),
-    "ahip_proj_time_syn": ([_P, _I32, _I32, _I32, _PD], _I32),
+    "ahip_time_kernel": ([_P, _I32, _I32, _I32, _PD], _I32),
+    "ahip_scatter_groups": ([_P, _PI32], _I32),
     "ahip_proj_psp_bytes": ([_P, _I32, _PD], _I32),
     "ahip_proj_time_psp": ([_P, _I32, _I32, _PD], _I32),
     "ahip_proj_set_launch": ([_P, _I32, _I32], _I32),

@@ -368,8 +368,15 @@ __global__ __launch_bounds__(WAVES * kWave) void sell_panel_psp(int32_t n_rows, 
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     T* pq = part + (int64_t)q * n_rows + (int64_t)rb * rb_rows;
-    const int32_t s_end = tile_slices[t + 1];
-    for (int32_t gs = tile_slices[t] + wave; gs < s_end; gs += WAVES) {
+    const int32_t s0 = tile_slices[t], s_end = tile_slices[t + 1];
+    // slices are sorted by length (descending); in every round of 2*WAVES
+    // slices wave w takes the w-th and the (2*WAVES-1-w)-th, so each wave's
+    // pair of a long and a short slice is about as long as any other's
+    for (int32_t k = 0;; k++) {
+        const int32_t r = k >> 1;
+        const int32_t gs = s0 + r * 2 * WAVES + ((k & 1) ? 2 * WAVES - 1 - wave : wave);
+        if (s0 + r * 2 * WAVES >= s_end) break;
+        if (gs >= s_end) continue;
         const int64_t b = off[gs];
         const int64_t nsteps = (off[gs + 1] - b) >> 8;
         const uint16_t* cp = col + b + 4 * lane;

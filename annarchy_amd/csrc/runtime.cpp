@@ -499,7 +499,12 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
     const size_t es = net->elem();
     // 1. compute_psp: the spiking projections in fused scatter groups, then
     // the rate-coded ones
-    for (const ScatterGroup& g : net->scatter_groups) LCHK(launch_scatter_group(net, g, step, s));
+    for (size_t gi = 0; gi < net->scatter_groups.size(); gi++) {
+        const bool probe = net->probe_kind == AHIP_PROBE_SCATTER && net->probe_proj == (int32_t)gi;
+        if (probe) HIPCHK(hipEventRecord(net->probe_ev[0], s));
+        LCHK(launch_scatter_group(net, net->scatter_groups[gi], step, s));
+        if (probe) HIPCHK(hipEventRecord(net->probe_ev[1], s));
+    }
     for (auto& pp : net->projs) {
         Proj& p = *pp;
         Pop& pre = *net->pops[p.pre];
@@ -539,10 +544,18 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
         Pop& p = *pp;
         ahip_pop_args a = make_args(net, p, step);
         ahip_launch_fn fn = net->precision == AHIP_F64 ? p.model->launch_f64 : p.model->launch_f32;
+        const bool probe_u = net->probe_kind == AHIP_PROBE_UPDATE && net->probe_proj == p.id;
+        if (probe_u) HIPCHK(hipEventRecord(net->probe_ev[0], s));
         int rc = fn(&a, (void*)s);
         if (rc != 0) return fail(AHIP_EHIP, "update of population %d failed: %s", p.id,
                                  hipGetErrorString((hipError_t)rc));
-        if (p.model->spiking) LCHK(launch_compact(p, s));
+        if (probe_u) HIPCHK(hipEventRecord(net->probe_ev[1], s));
+        if (p.model->spiking) {
+            const bool probe_c = net->probe_kind == AHIP_PROBE_COMPACT && net->probe_proj == p.id;
+            if (probe_c) HIPCHK(hipEventRecord(net->probe_ev[0], s));
+            LCHK(launch_compact(p, s));
+            if (probe_c) HIPCHK(hipEventRecord(net->probe_ev[1], s));
+        }
         // delay queues: push this step's values (after the update, as the
         // reference's update_delay)
         if (p.depth > 1) {
@@ -1623,22 +1636,41 @@ int ahip_proj_time_psp(ahip_net* net, int32_t proj, int32_t iters, double* avg_m
     return 0;
 }
 
-int ahip_proj_time_syn(ahip_net* net, int32_t proj, int32_t kind, int32_t iters, double* avg_ms) {
-    Proj* p = get_proj(net, proj);
-    if (!p || !p->syn || !avg_ms || iters <= 0 || p->nnz < 0) return fail(AHIP_EINVAL, "bad plastic projection");
-    if (kind < 0 || kind > 2 || (kind != 1 && !p->spiking) || (kind == 1 && !p->syn->has_update) ||
-        (kind == 2 && !p->syn->has_post))
-        return fail(AHIP_EINVAL, "the projection has no kernel of kind %d", kind);
-    if (kind == 1 && !p->syn_update) return fail(AHIP_ESTATE, "update_synapse is disabled");
+int ahip_time_kernel(ahip_net* net, int32_t what, int32_t index, int32_t iters, double* avg_ms) {
+    if (!net || !avg_ms || iters <= 0) return fail(AHIP_EINVAL, "bad arguments");
     HIPCHK(hipSetDevice(net->device));
     int rc = prepare(net);
     if (rc) return rc;
+    switch (what) {
+        case AHIP_PROBE_PRE_EVENTS:
+        case AHIP_PROBE_SYN_UPDATE:
+        case AHIP_PROBE_POST_EVENTS: {
+            Proj* p = get_proj(net, index);
+            if (!p || !p->syn || p->nnz <= 0) return fail(AHIP_EINVAL, "bad plastic projection");
+            if ((what != AHIP_PROBE_SYN_UPDATE && !p->spiking) ||
+                (what == AHIP_PROBE_SYN_UPDATE && (!p->syn->has_update || !p->syn_update)) ||
+                (what == AHIP_PROBE_POST_EVENTS && !p->syn->has_post))
+                return fail(AHIP_EINVAL, "the projection launches no such kernel");
+            break;
+        }
+        case AHIP_PROBE_UPDATE:
+        case AHIP_PROBE_COMPACT: {
+            Pop* p = get_pop(net, index);
+            if (!p || (what == AHIP_PROBE_COMPACT && !p->model->spiking)) return fail(AHIP_EINVAL, "bad population");
+            break;
+        }
+        case AHIP_PROBE_SCATTER:
+            if (index < 0 || index >= (int32_t)net->scatter_groups.size()) return fail(AHIP_EINVAL, "bad scatter group");
+            break;
+        default:
+            return fail(AHIP_EINVAL, "unknown kernel kind %d", what);
+    }
     std::vector<int32_t> zb;
     assign_accumulate(net, zb);
     HIPCHK(hipEventCreate(&net->probe_ev[0]));
     HIPCHK(hipEventCreate(&net->probe_ev[1]));
-    net->probe_proj = proj;
-    net->probe_kind = kind;
+    net->probe_proj = index;
+    net->probe_kind = what;
     // `iters` ordinary steps, launched eagerly, with the events around the
     // probed kernel: its time in the context of a real step
     double total = 0.0;
@@ -1660,6 +1692,14 @@ int ahip_proj_time_syn(ahip_net* net, int32_t proj, int32_t kind, int32_t iters,
     net->probe_ev[0] = net->probe_ev[1] = nullptr;
     if (rc) return rc;
     *avg_ms = total / iters;
+    return 0;
+}
+
+int ahip_scatter_groups(ahip_net* net, int32_t* n_groups) {
+    if (!net || !n_groups) return fail(AHIP_EINVAL, "bad arguments");
+    int rc = prepare(net);
+    if (rc) return rc;
+    *n_groups = (int32_t)net->scatter_groups.size();
     return 0;
 }
 

@@ -550,9 +550,7 @@ class Projection:
         post-event kernel inside `iters` ordinary steps, which advance the
         simulation."""
         k = {"pre": 0, "update": 1, "post": 2}[kind]
-        ms = C.c_double()
-        _lib.check(_lib.lib().ahip_proj_time_syn(self.net._h, self.id, k, iters, C.byref(ms)))
-        return ms.value
+        return self.net.time_kernel(k, self.id, iters)
 
     def time_psp(self, iters=20) -> float:
         ms = C.c_double()
@@ -761,6 +759,18 @@ class Network:
         if monitors:
             for m in self.monitors:
                 m.clear()
+
+    def time_kernel(self, what: int, index: int, iters: int = 20) -> float:
+        """Mean time (ms) of one kernel inside `iters` ordinary eager steps
+        (ahip_time_kernel; the steps advance the simulation)."""
+        ms = C.c_double()
+        _lib.check(_lib.lib().ahip_time_kernel(self._h, int(what), int(index), int(iters), C.byref(ms)))
+        return ms.value
+
+    def scatter_groups(self) -> int:
+        n = C.c_int32()
+        _lib.check(_lib.lib().ahip_scatter_groups(self._h, C.byref(n)))
+        return n.value
 
     def enable_learning(self, projections=None, period=None, offset=None):
         """Network.enable_learning: every (or the given) plastic projection."""

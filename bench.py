@@ -282,13 +282,20 @@ def bench_c0(args):
 def bench_c2(args):
     """COBA LIF, 400k neurons (320k E / 80k I), fixed_number_pre 1000 (800 E +
     200 I) drawn on the device, 4e8 synapses with per-synapse weights,
-    Poisson 5 Hz into 10%."""
+    Poisson 5 Hz into 10%.  roofline: the longer of the population update of
+    the 400k LIF neurons (60 B per neuron in FP64: v, g_exc, g_inh read and
+    written, refractory, refractory_remaining read and written) and the fused
+    scatter of the E and I projections (per spike 20 B, per synapse 4 + 3*s
+    B: post index, weight, conductance atomic), each timed with HIP events
+    inside 20 real steps.  cpu_baseline: the oracle's spiking step on the
+    same connectivity (Poisson draws from numpy), a bounded number of steps."""
     from annarchy_amd import connectivity as cn
 
     steps = args.steps if args.steps != 200 else 2000
     N, ne = 400_000, 320_000
     scale = 80.0 / 1000.0  # same mean input conductance as the 4000-neuron benchmark
     prec = args.precision
+    es = 8 if prec == "double" else 4
     we, wi = 0.6 * scale, 6.7 * scale
 
     def conn(net, P, ne_):
@@ -296,25 +303,64 @@ def bench_c2(args):
         return [net.connect(P[:ne_], P, "exc").fixed_number_pre(800, cn.Uniform(we, we), generator="device"),
                 net.connect(P[ne_:], P, "inh").fixed_number_pre(200, cn.Uniform(wi, wi), generator="device")]
     t_b = time.perf_counter()
-    net, P, Q, projs, mon = _coba(N, ne, conn, steps + args.warmup, poisson=(N // 10, 5.0, 0.6),
+    net, P, Q, projs, mon = _coba(N, ne, conn, steps + args.warmup + 40, poisson=(N // 10, 5.0, 0.6),
                                   precision=prec)
     log(f"c2 network ({sum(p.nb_synapses for p in projs)} synapses, connectivity drawn on the device) "
         f"set up in {time.perf_counter() - t_b:.1f} s")
+    v0 = P.v.reshape(-1).copy()
     net.set_graph(True, 100)
     net.run(args.warmup)
     t0 = time.perf_counter()
     net.run(steps)
     el = time.perf_counter() - t0
     counts, idx = mon.raw_spikes()
-    c_t = counts[args.warmup:]
-    idx_t = idx[int(counts[:args.warmup].sum()):]
+    c_t = counts[args.warmup:args.warmup + steps]
+    idx_t = idx[int(counts[:args.warmup].sum()):int(counts[:args.warmup + steps].sum())]
     events = _events(projs[:2], P, c_t, idx_t)
+    # kernel times inside 2 x 20 further steps
+    n_done = args.warmup + steps
+    t_upd = net.time_kernel(3, P.id, 20)
+    t_sc = net.time_kernel(5, 0, 20)
+    counts2, idx2 = mon.raw_spikes()
+    starts = np.concatenate([[0], np.cumsum(counts2)])
+    outdeg = np.zeros(N, dtype=np.int64)
+    for pr in projs[:2]:
+        outdeg += np.bincount(pr.csr.col_idx, minlength=N)[:N]
+    # the scatter of step u sends the spikes of step u - 1
+    sc_steps = range(n_done + 19, n_done + 39)
+    sc_syn = sum(int(outdeg[idx2[starts[u]:starts[u + 1]]].sum()) for u in sc_steps) / 20.0
+    sc_spk = sum(int(counts2[u]) for u in sc_steps) / 20.0
+    b_sc = sc_syn * (4 + 3 * es) + sc_spk * 20
+    b_upd = N * (6 * es + 12)
+    kern = {"local_update": {"ms": t_upd, "bytes": b_upd},
+            "scatter_multi": {"ms": t_sc, "bytes": b_sc, "syn_events": sc_syn}}
+    dom = "local_update" if t_upd >= t_sc else "scatter_multi"
+    ach = kern[dom]["bytes"] / (kern[dom]["ms"] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": None, "kernel": dom}
+    cpu = None
+    if not args.no_cpu and prec == "double":
+        from oracle import oracle as orc
+        t1 = time.perf_counter()
+        po = [orc.csrc_projection(projs[0].csr, 0), orc.csrc_projection(projs[1].csr, 1),
+              orc.csrc_projection(projs[2].csr, 0, pre_is_input=True)]
+        log(f"c2 oracle views built in {time.perf_counter() - t1:.1f} s")
+        cs = 4000
+        unif = np.random.default_rng(5).random((cs, N // 10))
+        t1 = time.perf_counter()
+        ref = orc.spiking_run(0, N, 0.1, v0, cs, po, refractory=np.asarray(P.refractory, np.int32),
+                              poisson=dict(M=N // 10, rates=5.0, uniforms=unif), spike_cap=N * 50)
+        cel = time.perf_counter() - t1
+        ev_c = int(outdeg[ref["idx"]].sum())
+        cpu = {"value": ev_c / cel, "unit": "synaptic events/s", "cores": 1, "kind": "port",
+               "sample": f"{cs} steps of the same network from the same initial state (Poisson draws from "
+                         f"numpy), single thread: {cel:.1f} s, {cel / cs * 1e3:.1f} ms/step"}
     return _line("synaptic events/s", events / el, "synaptic events/s", steps, args.warmup,
                  el * 1e3 / steps, "f64" if prec == "double" else "f32",
                  {"workload": "COBA LIF N=400000 fixed_number_pre 1000 (800E/200I) 4e8 synapses, "
                               "Poisson 5 Hz into 10%, dt=0.1ms", "precision": prec,
                   "connectivity": "drawn on the device (Philox stream)"},
-                 None, None, spikes_per_step=float(c_t.mean()),
+                 roof, cpu, kernels=kern, spikes_per_step=float(c_t.mean()),
                  mean_rate_hz=float(c_t.sum() / N / (steps * 1e-4)))
 
 

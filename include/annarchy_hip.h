@@ -210,10 +210,21 @@ int ahip_proj_set_learning(ahip_net* net, int32_t proj, int32_t update, int32_t 
                            int32_t offset);
 /* Measurement helper with no counterpart in the reference: runs `iters`
  * ordinary simulation steps (they advance the simulation, eagerly launched)
- * with HIP events on the network stream around a plastic projection's
- * pre-event (kind 0), update_synapse (1) or post-event (2) kernel inside
- * each step; *avg_ms = mean time of that kernel within real steps. */
-int ahip_proj_time_syn(ahip_net* net, int32_t proj, int32_t kind, int32_t iters, double* avg_ms);
+ * with HIP events on the network stream around one kernel inside each step;
+ * *avg_ms = mean time of that kernel within real steps.  what / index:
+ * plastic projection `index`: its pre events, update_synapse or post
+ * events; population `index`: its update or its spike compaction; fused
+ * spike-scatter launch `index` (0 .. ahip_scatter_groups - 1). */
+#define AHIP_PROBE_PRE_EVENTS 0
+#define AHIP_PROBE_SYN_UPDATE 1
+#define AHIP_PROBE_POST_EVENTS 2
+#define AHIP_PROBE_UPDATE 3
+#define AHIP_PROBE_COMPACT 4
+#define AHIP_PROBE_SCATTER 5
+int ahip_time_kernel(ahip_net* net, int32_t what, int32_t index, int32_t iters, double* avg_ms);
+/* Number of fused spike-scatter launches per step (spiking projections
+ * without per-synapse delays or synapse models, grouped by target). */
+int ahip_scatter_groups(ahip_net* net, int32_t* n_groups);
 /* Bytes the psp kernel must move per step (algorithmic, no padding). */
 int ahip_proj_psp_bytes(ahip_net* net, int32_t proj, double* bytes);
 /* Times `iters` back-to-back launches of the projection's psp kernel with HIP

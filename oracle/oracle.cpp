@@ -75,6 +75,26 @@ int64_t orc_csr_from_lil(int32_t num_rows, int32_t n_lil, const int32_t* post_ra
 // CSRC backward (column) view: per pre neuron, the post ranks in ascending
 // order and the forward index of each synapse.
 // Follows include/CSRCMatrix.hpp:164-210.
+// The same view by a counting sort, O(nnz): rows are visited in ascending
+// order, so each column receives its rows ascending, as the map-based
+// restatement below.  Used to build views of large networks; checked equal to
+// orc_csrc_inverse on random matrices.
+void orc_csrc_inverse_fast(int32_t num_rows, int32_t num_cols, const int64_t* row_begin,
+                           const int32_t* col_idx, int64_t* col_ptr, int32_t* row_idx,
+                           int64_t* inv_idx) {
+    const int64_t nnz = row_begin[num_rows];
+    std::vector<int64_t> cur(num_cols + 1, 0);
+    for (int64_t s = 0; s < nnz; s++) cur[col_idx[s] + 1]++;
+    for (int32_t c = 0; c < num_cols; c++) cur[c + 1] += cur[c];
+    for (int32_t c = 0; c <= num_cols; c++) col_ptr[c] = cur[c];
+    for (int32_t i = 0; i < num_rows; i++)
+        for (int64_t s = row_begin[i]; s < row_begin[i + 1]; s++) {
+            const int64_t q = cur[col_idx[s]]++;
+            row_idx[q] = i;
+            inv_idx[q] = s;
+        }
+}
+
 void orc_csrc_inverse(int32_t num_rows, int32_t num_cols, const int64_t* row_begin,
                       const int32_t* col_idx, int64_t* col_ptr, int32_t* row_idx,
                       int64_t* inv_idx) {

@@ -74,6 +74,8 @@ def lib():
         L.orc_csr_from_lil.argtypes = [_I32, _I32, _PI32, _PI64, _PI32, _PI64, _PI32]
         L.orc_csr_from_lil.restype = _I64
         L.orc_csrc_inverse.argtypes = [_I32, _I32, _PI64, _PI32, _PI64, _PI32, _PI64]
+        L.orc_csrc_inverse_fast.argtypes = [_I32, _I32, _PI64, _PI32, _PI64, _PI32, _PI64]
+        L.orc_csrc_inverse_fast.restype = None
         L.orc_csrc_inverse.restype = None
         L.orc_ell_from_lil.argtypes = [_I32, _PI64, _PI32, _PD, _I32, _PI32, _PD, _PI32]
         L.orc_ell_from_lil.restype = _I32
@@ -141,15 +143,20 @@ def csr_from_lil(num_rows, post_ranks, pre_lists):
     return row_begin, col_idx[:nnz]
 
 
-def csrc_inverse(num_rows, num_cols, row_begin, col_idx):
+def csrc_inverse(num_rows, num_cols, row_begin, col_idx, fast=None):
+    """orc_csrc_inverse (the map-based restatement of CSRCMatrix.hpp:164-210)
+    or, for large views (fast=None and nnz > 1e6, or fast=True), the
+    counting sort orc_csrc_inverse_fast that returns the same arrays."""
     nnz = int(row_begin[-1])
+    if fast is None:
+        fast = nnz > 1_000_000
     col_ptr = np.zeros(num_cols + 1, dtype=np.int64)
     row_idx = np.zeros(max(1, nnz), dtype=np.int32)
     inv_idx = np.zeros(max(1, nnz), dtype=np.int64)
     rb = np.ascontiguousarray(row_begin, dtype=np.int64)
     ci = np.ascontiguousarray(col_idx, dtype=np.int32)
-    lib().orc_csrc_inverse(num_rows, num_cols, _p(rb, _I64), _p(ci, _I32), _p(col_ptr, _I64),
-                           _p(row_idx, _I32), _p(inv_idx, _I64))
+    fn = lib().orc_csrc_inverse_fast if fast else lib().orc_csrc_inverse
+    fn(num_rows, num_cols, _p(rb, _I64), _p(ci, _I32), _p(col_ptr, _I64), _p(row_idx, _I32), _p(inv_idx, _I64))
     return col_ptr, row_idx[:nnz], inv_idx[:nnz]
 
 

@@ -92,6 +92,19 @@ def test_csrc_inverse_brute_force():
                 assert ci[inv[q]] == c and rb[row_idx[q]] <= inv[q] < rb[row_idx[q] + 1]
 
 
+def test_csrc_inverse_fast_equals_restatement():
+    """The counting-sort view used for large networks returns the arrays of
+    the map-based restatement (empty rows and columns included)."""
+    rng = np.random.default_rng(4)
+    for trial in range(6):
+        nr, nc = int(rng.integers(1, 300)), int(rng.integers(1, 300))
+        csr = conn.random_in_degree(nc, nr, rng, rng.integers(0, min(nc, 40) + 1, nr), 1.0)
+        a = orc.csrc_inverse(nr, nc, csr.row_ptr, csr.col_idx, fast=False)
+        b = orc.csrc_inverse(nr, nc, csr.row_ptr, csr.col_idx, fast=True)
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+
+
 def test_spmv_dense_product():
     rng = np.random.default_rng(1)
     csr = conn.random_in_degree(500, 300, rng, rng.integers(0, 40, 300), conn.Normal(0, 1, rng=rng))
