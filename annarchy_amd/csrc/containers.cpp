@@ -74,6 +74,15 @@ bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
 }  // namespace
 
+// Panel width for at most `pmax` columns per panel: the same number of
+// panels, all of (about) equal width, so the last panel is not a sliver with
+// too little work per tile.
+static int32_t equal_panels(int32_t n_cols, int32_t pmax) {
+    if (n_cols <= 0 || pmax <= 0) return pmax;
+    const int32_t np = (n_cols + pmax - 1) / pmax;
+    return (n_cols + np - 1) / np;
+}
+
 int build_layout(const Proj& p, int32_t precision, const ahip_format_opts& o, RateLayout& L,
                  std::string& err) {
     const int32_t R = p.n_rows;
@@ -228,6 +237,7 @@ int build_layout(const Proj& p, int32_t precision, const ahip_format_opts& o, Ra
             const size_t es = precision == AHIP_F64 ? 8 : 4;
             int32_t pc = o.panel_cols > 0 ? o.panel_cols : (int32_t)(32768 / es);
             if ((size_t)pc * es > 64 * 1024) { err = "panel exceeds 64 KiB of LDS"; return AHIP_EINVAL; }
+            pc = equal_panels(p.n_cols, pc);
             const int32_t rb_rows = o.chunk > 0 ? o.chunk : 512;
             if (rb_rows < 16 || rb_rows > 8192) { err = "tile rows must be 16..8192"; return AHIP_EINVAL; }
             const int32_t np = std::max<int32_t>(1, (p.n_cols + pc - 1) / pc);
@@ -296,6 +306,7 @@ int build_layout(const Proj& p, int32_t precision, const ahip_format_opts& o, Ra
             const size_t es = precision == AHIP_F64 ? 8 : 4;
             int32_t pc = o.panel_cols > 0 ? o.panel_cols : (int32_t)(32768 / es);
             if (pc > 65535 || (size_t)(pc + 1) * es > 64 * 1024) { err = "panel exceeds 64 KiB of LDS or 65535 columns"; return AHIP_EINVAL; }
+            pc = equal_panels(p.n_cols, pc);
             const int32_t rb_rows = o.chunk > 0 ? o.chunk : 256;
             if (rb_rows < 64 || rb_rows > 8192 || rb_rows % 64) { err = "tile rows must be a multiple of 64 in 64..8192"; return AHIP_EINVAL; }
             const int32_t steps = o.lanes_per_row > 0 ? o.lanes_per_row : 4;

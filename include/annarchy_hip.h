@@ -56,7 +56,8 @@ typedef struct ahip_format_opts {
     int32_t chunk;            /* SELL: rows per slice (C); BSR: tile size; CSR_PANEL / SELL_PANEL: rows per tile */
     int32_t sigma;            /* SELL: sorting window in rows (1 = no sorting); SELL_PANEL: waves per
                                  workgroup (4 or 8; 0 or 1 = 4) */
-    int32_t panel_cols;       /* CSR_PANEL / SELL_PANEL: columns per panel (0 = automatic) */
+    int32_t panel_cols;       /* CSR_PANEL / SELL_PANEL: at most this many columns per panel (0 =
+                                 automatic); the panels are then made equally wide */
 } ahip_format_opts;
 
 /* ---- errors / device ------------------------------------------------------ */
