@@ -362,7 +362,22 @@ __global__ __launch_bounds__(WAVES * kWave) void sell_panel_psp(int32_t n_rows, 
     const int32_t q = t / n_rb, rb = t % n_rb;
     const int32_t c0 = q * panel_cols;
     const int32_t nc = min(panel_cols, n_cols - c0);
-    for (int32_t k = threadIdx.x; k < nc; k += WAVES * kWave) xs[k] = x[c0 + k];
+    // stage the panel of x: each thread loads 8 values before it stores
+    // them, so a panel costs a few load latencies instead of one per value
+    constexpr int32_t kB = 8, kT = WAVES * kWave;
+    for (int32_t k0 = 0; k0 < nc; k0 += kB * kT) {
+        T v[kB];
+#pragma unroll
+        for (int i = 0; i < kB; i++) {
+            const int32_t k = k0 + i * kT + (int32_t)threadIdx.x;
+            v[i] = k < nc ? x[c0 + k] : T(0);
+        }
+#pragma unroll
+        for (int i = 0; i < kB; i++) {
+            const int32_t k = k0 + i * kT + (int32_t)threadIdx.x;
+            if (k < nc) xs[k] = v[i];
+        }
+    }
     if (threadIdx.x == 0) xs[panel_cols] = T(0);
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
