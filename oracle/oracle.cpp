@@ -13,29 +13,41 @@
 // Pinning of each function (what checks it besides retyping it):
 //   orc_csr_from_lil      tests_cpp golden CSR layouts (6x8, 8x6, empty rows)
 //   orc_csrc_inverse      brute-force transpose on random matrices + 6x8 golden
+//   orc_csrc_inverse_fast equal to orc_csrc_inverse on random matrices
 //   orc_ell_from_lil      tests_cpp golden ELL/ELLR row- and column-major
 //   orc_sell_from_lil     tests_cpp golden SELL (S=3) layout
 //   orc_bsr_from_lil      tests_cpp golden BSR (tile 2) layout
 //   orc_dense_from_lil    tests_cpp golden dense row-major layout
 //   orc_spmv_csr          closed form W.r (numpy dense product) as in
 //                         tests_python/RateSynapse/test_RateDelays.py:99-153
+//   orc_spmv_csr_f32      the FP32 summation error bound n 2^-24 sum|w r|
+//                         against the exact product of the FP32 inputs
 //   orc_rate_run          closed-form leaky integration at constant input and
 //                         the explicit-Euler recurrence evaluated by numpy
+//   orc_delayed_psp_run,  tests_python/RateSynapse/test_RateDelays.py uniform
+//   orc_rate_run_delayed  (10 / 5 ms) and per-synapse ([[3],[5],[2]]) known
+//                         answers; delay 1 equals the undelayed run; a delayed
+//                         queue is the shifted input sequence
 //   orc_spiking_run       tests_python/Neuron/test_SpikingNeuron.py and
 //                         test_Monitor.py:294-333 trajectories (counter model),
-//                         test_SpikingTransmission.py:14-78 (5 spikes x w=1),
+//                         test_SpikingTransmission.py:14-78 (5 spikes x w=1)
+//                         and :86-253 (uniform and per-synapse delays),
 //                         COBA: free-membrane exponential relaxation invariant
+//   orc_sweep_instance_run the geometric approach of each input row to
+//                         g (W_in u)_i / (1 - beta) at constant input with
+//                         self-weights beta; rows beyond the input stay 0
 //   orc_mlp_*             finite differences on the CPU (numpy)
 //   orc_rate_plastic_run  closed forms: geometric Hebbian growth of one
 //                         synapse at a constant rate, Oja's fixed point
-//                         1/(r*sqrt(alpha)), the exponential relaxation of
+//                         1/sqrt(alpha), the exponential relaxation of
 //                         IBCM's threshold at a fixed post rate, the update
 //                         period (fewer, larger steps)
 //   orc_plastic_run       hand-derived closed forms of single-synapse spike
 //                         pairings (trace decays between events, the order
 //                         g += w before the weight change, clipping, t_pre /
-//                         t_post) and the pre/post-spike known answers of
-//                         tests_python/SpikeSynapse/test_SpikingSynapse.py
+//                         t_post, STP's depression and facilitation), the
+//                         independence of synapses (an N x M run equals its
+//                         1 x 1 runs), a uniform delay
 
 #include <algorithm>
 #include <climits>

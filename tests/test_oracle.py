@@ -339,3 +339,38 @@ def test_spiking_nonuniform_delay_kat():
     ref = orc.spiking_run(0, 2, 1.0, np.full(2, -60.0), 5, [pr], params=dict(tau_exc=1.0, I=0.0),
                           poisson=_emit_at_step1(5, 5), trace=True)
     np.testing.assert_array_equal(ref["trace_gexc"], [[0, 0], [0, 0], [2, 1], [2, 2], [1, 2]])
+
+
+def test_spmv_csr_f32_error_bound():
+    """FP32 row sums (sequential, j ascending) stay within the classical bound
+    n * 2^-24 * sum |w r| of the exact product of the FP32 inputs."""
+    rng = np.random.default_rng(11)
+    csr = conn.random_in_degree(400, 300, rng, rng.integers(0, 200, 300), conn.Normal(0, 1, rng=rng))
+    w32 = csr.w.astype(np.float32)
+    r32 = rng.random(400).astype(np.float32)
+    got = orc.spmv_csr_f32(csr.row_ptr, csr.col_idx, w32, r32).astype(np.float64)
+    for i in range(300):
+        seg = slice(csr.row_ptr[i], csr.row_ptr[i + 1])
+        terms = w32[seg].astype(np.float64) * r32[csr.col_idx[seg]].astype(np.float64)
+        n = max(1, terms.size)
+        assert abs(got[i] - terms.sum()) <= 2 * n * 2.0 ** -24 * np.abs(terms).sum() + 1e-30
+
+
+def test_sweep_instance_closed_form():
+    """One sweep instance at omega = 0: the input rates are the constant
+    u_k = 1 + 0.5 sin(phase_k) from step 0's update on, so with self-weights
+    beta the rate of an input row relaxes geometrically to g (W_in u)_i / (1 - beta):
+    r_t = r* (1 - a^(t-1)), a = 1 - (1 - beta) dt / tau; rows beyond n_in stay 0."""
+    n, n_in, beta, g, tau, dt, steps = 40, 25, 0.3, 0.7, 10.0, 1.0, 60
+    rng = np.random.default_rng(2)
+    w_in = rng.random((n_in, n_in)) * 0.1
+    phase = rng.random(n_in) * 2 * np.pi
+    eye = conn.from_arrays(n, n, np.arange(n + 1, dtype=np.int64), np.arange(n, dtype=np.int32), np.full(n, beta))
+    r, tr = orc.sweep_instance_run(eye, w_in, phase, 0.0, g, tau, dt, np.zeros(n), steps, trace=True)
+    u = 1.0 + 0.5 * np.sin(phase)
+    r_star = np.zeros(n)
+    r_star[:n_in] = g * (w_in @ u) / (1.0 - beta)
+    a = 1.0 - (1.0 - beta) * dt / tau
+    for t in (1, 5, 30, steps):
+        np.testing.assert_allclose(tr[t - 1], r_star * (1.0 - a ** (t - 1)), rtol=1e-12, atol=1e-15)
+    assert np.all(r[n_in:] == 0.0)
