@@ -133,7 +133,7 @@ struct Monitor {
     int64_t t_start = 0;
     DevBuf data;                  // values [capacity][size] (T), or spike indices
     DevBuf counts;                // spikes: int32[capacity]
-    DevBuf fill;                  // spikes: int64[1] indices written
+    DevBuf fill;                  // spikes: int64[2], indices written before recorded step s in fill[s & 1]
     int64_t idx_capacity = 0;
 };
 
@@ -198,7 +198,7 @@ int launch_rate_psp(const ahip_net* net, const Proj& p, const void* pre_r, void*
                     hipStream_t s);
 int launch_spike_psp(const ahip_net* net, const Proj& p, const Pop& pre, void* target,
                      hipStream_t s);
-int launch_compact(const Pop& p, hipStream_t s);
+int launch_compact(const Pop& p, const Monitor* m, const int64_t* t_base, int32_t step, hipStream_t s);
 int launch_scatter_group(const ahip_net* net, const ScatterGroup& g, int32_t step, hipStream_t s);
 int launch_push_values(const ahip_net* net, const void* src, void* hist, int32_t n, int32_t D, int32_t step,
                        hipStream_t s);

@@ -44,12 +44,30 @@ __device__ inline int32_t block_exclusive_scan(int32_t v, int32_t* lds_waves, in
 // sum of the spike counts of all update blocks before it (read from
 // block_counts written by the update kernel), then a block-wide scan over the
 // per-thread popcounts of the spike bit mask places every index.
+// With a spike monitor (rec_counts != NULL) the kernel also appends the list
+// to it: the monitor's index buffer holds the lists of the recorded steps
+// back to back; fill[s & 1] is the number of indices before the list of
+// recorded step s (written by step s - 1's launch, so no workgroup of this
+// launch reads a value another one writes), the last workgroup writes
+// counts[s] and fill[(s + 1) & 1].
+struct SpikeRec {
+    int32_t* counts;          // int32[capacity] spikes per recorded step, NULL: no monitor
+    int32_t* idx;             // int32[idx_capacity]
+    int64_t* fill;            // int64[2]
+    int64_t idx_capacity;
+    int64_t t_start;
+    int64_t capacity;
+    const int64_t* t_base;
+    int32_t step;
+};
+
 __global__ __launch_bounds__(kCompactThreads) void compact_spikes(int32_t size,
                                                                   int32_t n_update_blocks,
                                                                   const uint64_t* __restrict__ mask,
                                                                   const int32_t* __restrict__ bcnt,
                                                                   int32_t* __restrict__ spiked,
-                                                                  int32_t* __restrict__ count) {
+                                                                  int32_t* __restrict__ count,
+                                                                  SpikeRec rec) {
     __shared__ int32_t lds_waves[kCompactThreads / kWave + 1];
     __shared__ int32_t lds_base[kCompactThreads / kWave];
     const int32_t chunk0 = blockIdx.x * kCompactChunk;
@@ -75,11 +93,27 @@ __global__ __launch_bounds__(kCompactThreads) void compact_spikes(int32_t size,
     int32_t total;
     const int32_t excl = block_exclusive_scan(__popc(bits), lds_waves, total);
     int32_t pos = base + excl;
+    int64_t slot = -1, f = 0;
+    if (rec.counts) {
+        slot = *rec.t_base + rec.step - rec.t_start;
+        if (slot >= 0 && slot < rec.capacity) f = rec.fill[slot & 1];
+        else slot = -1;
+    }
 #pragma unroll
     for (int k = 0; k < kCompactPerThread; k++)
-        if (bits & (1u << k)) spiked[pos++] = n0 + k;
+        if (bits & (1u << k)) {
+            spiked[pos] = n0 + k;
+            if (slot >= 0 && f + pos < rec.idx_capacity) rec.idx[f + pos] = n0 + k;
+            pos++;
+        }
     // grand total: the last chunk knows base + its own total
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *count = base + total;
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        *count = base + total;
+        if (slot >= 0) {
+            rec.counts[slot] = base + total;
+            rec.fill[(slot + 1) & 1] = f + base + total;
+        }
+    }
     (void)n_update_blocks;
 }
 
@@ -202,8 +236,9 @@ __global__ __launch_bounds__(256) void scatter_multi(const ScatterDesc* __restri
     }
 }
 
-// Spike recorder: appends the step's spike list to the monitor buffer.  One
-// workgroup; fill[0] is the number of indices written so far.
+// Spike recorder of a further spike monitor of a population (the first one is
+// filled by the compaction): appends the step's list to the monitor buffer.
+// One workgroup; fill is double-buffered by recorded step as in compact_spikes.
 __global__ __launch_bounds__(256) void record_spikes(const int32_t* __restrict__ spiked,
                                                      const int32_t* __restrict__ count,
                                                      const int64_t* __restrict__ t_base,
@@ -215,25 +250,34 @@ __global__ __launch_bounds__(256) void record_spikes(const int32_t* __restrict__
     const int64_t slot = *t_base + step - t_start;
     if (slot < 0 || slot >= capacity) return;
     const int32_t n = *count;
-    const int64_t f = *fill;
-    __syncthreads();
+    const int64_t f = fill[slot & 1];   // same double-buffered fill as compact_spikes
     for (int32_t k = threadIdx.x; k < n; k += 256)
         if (f + k < idx_capacity) idx[f + k] = spiked[k];
-    __syncthreads();
     if (threadIdx.x == 0) {
         counts[slot] = n;
-        *fill = f + n;
+        fill[(slot + 1) & 1] = f + n;
     }
 }
 
 }  // namespace
 
-int launch_compact(const Pop& p, hipStream_t s) {
+int launch_compact(const Pop& p, const Monitor* m, const int64_t* t_base, int32_t step, hipStream_t s) {
     const int32_t nb = (p.size + kCompactChunk - 1) / kCompactChunk;
     if (nb == 0) return 0;
+    SpikeRec rec{};
+    if (m) {
+        rec.counts = m->counts.as<int32_t>();
+        rec.idx = m->data.as<int32_t>();
+        rec.fill = m->fill.as<int64_t>();
+        rec.idx_capacity = m->idx_capacity;
+        rec.t_start = m->t_start;
+        rec.capacity = m->capacity;
+        rec.t_base = t_base;
+        rec.step = step;
+    }
     hipLaunchKernelGGL(compact_spikes, dim3(nb), dim3(kCompactThreads), 0, s, p.size, p.nblocks(),
                        p.spike_mask.as<uint64_t>(), p.block_counts.as<int32_t>(),
-                       p.spiked.as<int32_t>(), p.spike_count.as<int32_t>());
+                       p.spiked.as<int32_t>(), p.spike_count.as<int32_t>(), rec);
     return (int)hipGetLastError();
 }
 

@@ -493,6 +493,13 @@ ahip_pop_args make_args(ahip_net* net, Pop& p, int32_t step) {
     return a;
 }
 
+// The first spike monitor of a population (filled by its compaction kernel).
+const Monitor* first_spike_monitor(const ahip_net* net, int32_t pop) {
+    for (auto& mp : net->mons)
+        if (mp->spikes && mp->pop == pop) return mp.get();
+    return nullptr;
+}
+
 // Enqueue one simulation step (index `step` inside the current batch).
 int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_before) {
     hipStream_t s = net->stream;
@@ -553,7 +560,8 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
         if (p.model->spiking) {
             const bool probe_c = net->probe_kind == AHIP_PROBE_COMPACT && net->probe_proj == p.id;
             if (probe_c) HIPCHK(hipEventRecord(net->probe_ev[0], s));
-            LCHK(launch_compact(p, s));
+            // the population's first spike monitor is filled by the compaction
+            LCHK(launch_compact(p, first_spike_monitor(net, p.id), net->t_dev.as<int64_t>(), step, s));
             if (probe_c) HIPCHK(hipEventRecord(net->probe_ev[1], s));
         }
         // delay queues: push this step's values (after the update, as the
@@ -586,9 +594,10 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
         Monitor& m = *mp;
         if (m.after_psp) continue;
         Pop& p = *net->pops[m.pop];
-        if (m.spikes)
-            LCHK(launch_record_spikes(m, p, net->t_dev.as<int64_t>(), step, s));
-        else
+        if (m.spikes) {
+            if (first_spike_monitor(net, m.pop) != &m)
+                LCHK(launch_record_spikes(m, p, net->t_dev.as<int64_t>(), step, s));
+        } else
             LCHK(launch_record_values(m, p.slots[m.slot].p, p.size, (int32_t)es,
                                       net->t_dev.as<int64_t>(), step, s));
     }
@@ -1724,7 +1733,7 @@ int ahip_monitor_create(ahip_net* net, int32_t pop, const char* attr, int64_t ca
         m->idx_capacity = cap;
         if ((rc = dev_alloc(m->data, cap * sizeof(int32_t)))) return rc;
         if ((rc = dev_alloc(m->counts, capacity * sizeof(int32_t)))) return rc;
-        if ((rc = dev_alloc(m->fill, sizeof(int64_t)))) return rc;
+        if ((rc = dev_alloc(m->fill, 2 * sizeof(int64_t)))) return rc;
     } else {
         m->slot = p->slot_index(attr);
         if (m->slot < 0 || !p->model->slot_local[m->slot])
@@ -1772,8 +1781,9 @@ int ahip_monitor_get_spikes(ahip_net* net, int32_t mon, int32_t* counts, int64_t
     ahip_monitor_steps(net, mon, &have);
     if (steps != have) return fail(AHIP_EINVAL, "monitor holds %lld steps", (long long)have);
     HIPCHK(hipStreamSynchronize(net->stream));
-    int64_t fill = 0;
-    HIPCHK(hipMemcpy(&fill, m->fill.p, sizeof(int64_t), hipMemcpyDeviceToHost));
+    int64_t fills[2] = {0, 0};
+    HIPCHK(hipMemcpy(fills, m->fill.p, 2 * sizeof(int64_t), hipMemcpyDeviceToHost));
+    const int64_t fill = have > 0 ? fills[have & 1] : 0;   // indices before recorded step `have`
     *total = fill;
     if (fill > m->idx_capacity) return fail(AHIP_ESTATE, "spike monitor overflowed its buffer");
     if (counts && steps > 0) HIPCHK(hipMemcpy(counts, m->counts.p, steps * sizeof(int32_t), hipMemcpyDeviceToHost));
@@ -1790,7 +1800,7 @@ int ahip_monitor_clear(ahip_net* net, int32_t mon) {
     HIPCHK(hipStreamSynchronize(net->stream));
     m->t_start = net->t;
     if (m->spikes) {
-        LCHK(launch_fill_i64(m->fill.as<int64_t>(), 1, 0, net->stream));
+        LCHK(launch_fill_i64(m->fill.as<int64_t>(), 2, 0, net->stream));
         HIPCHK(hipStreamSynchronize(net->stream));
     }
     invalidate(net);
