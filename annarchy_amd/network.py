@@ -518,6 +518,30 @@ class Projection:
     def nb_synapses(self) -> int:
         return self.csr.nnz if self.csr is not None else 0
 
+    # dendrites: the synapses of one post-synaptic neuron (Projection.py:
+    # dendrite(), post_ranks, nb_dendrites; the wrapper's post_rank(),
+    # pre_rank(i), dendrite_size(i), NanoBind/Generator.py:193-204)
+    @property
+    def post_ranks(self) -> list:
+        """Ranks of the post-synaptic neurons that receive synapses."""
+        return [int(r) for r in np.flatnonzero(np.diff(self.csr.row_ptr))]
+
+    @property
+    def nb_dendrites(self) -> int:
+        return len(self.post_ranks)
+
+    def dendrite_size(self, post_rank: int) -> int:
+        return int(self.csr.row_ptr[post_rank + 1] - self.csr.row_ptr[post_rank])
+
+    def pre_ranks(self, post_rank: int) -> list:
+        rp = self.csr.row_ptr
+        return [int(c) for c in self.csr.col_idx[rp[post_rank]:rp[post_rank + 1]]]
+
+    def dendrite(self, post_rank: int) -> "Dendrite":
+        if self.dendrite_size(post_rank) == 0:
+            raise IndexError(f"post-synaptic neuron {post_rank} receives no synapse")
+        return Dendrite(self, post_rank)
+
     @property
     def w(self) -> np.ndarray:
         out = np.empty(self.nb_synapses)
@@ -556,6 +580,53 @@ class Projection:
         ms = C.c_double()
         _lib.check(_lib.lib().ahip_proj_time_psp(self.net._h, self.id, iters, C.byref(ms)))
         return ms.value
+
+
+class Dendrite:
+    """The synapses of one post-synaptic neuron (core/Dendrite.py): w and the
+    synaptic attributes restricted to its row, in pre-rank order."""
+
+    def __init__(self, proj: Projection, post_rank: int):
+        object.__setattr__(self, "proj", proj)
+        object.__setattr__(self, "post_rank", int(post_rank))
+
+    @property
+    def pre_ranks(self) -> list:
+        return self.proj.pre_ranks(self.post_rank)
+
+    @property
+    def size(self) -> int:
+        return self.proj.dendrite_size(self.post_rank)
+
+    def _slice(self):
+        rp = self.proj.csr.row_ptr
+        return slice(int(rp[self.post_rank]), int(rp[self.post_rank + 1]))
+
+    def __getattr__(self, name):
+        if name.startswith("_"):
+            raise AttributeError(name)
+        info = self.proj._syn_info()
+        if name != "w" and name not in info:
+            raise AttributeError(f"the dendrite has no attribute {name}")
+        loc = "local" if name == "w" else info[name][1]
+        v = self.proj.get(name)
+        if loc == "local":
+            return np.asarray(v)[self._slice()]
+        return float(np.asarray(v).reshape(-1)[self.post_rank]) if loc == "semiglobal" else v
+
+    def __setattr__(self, name, value):
+        info = self.proj._syn_info()
+        if name != "w" and name not in info:
+            raise AttributeError(f"the dendrite has no attribute {name}")
+        loc = "local" if name == "w" else info[name][1]
+        if loc == "global":
+            raise AttributeError(f"{name} is one value per projection: set it on the projection")
+        v = np.array(self.proj.get(name), dtype=np.float64).reshape(-1)
+        if loc == "local":
+            v[self._slice()] = value
+        else:
+            v[self.post_rank] = value
+        self.proj.set(name, v)
 
 
 def _sizes(obj) -> int:

@@ -43,28 +43,34 @@ def _kat_net(synapse):
 def test_prespike_kat(gpu):
     syn = ah.Synapse(parameters="g = 1.0", pre_spike="w -= 1.0 : min=0.0", name="BoundedPre")
     net, proj = _kat_net(syn)
-    np.testing.assert_allclose(proj.w, 5.0)
+    np.testing.assert_allclose(proj.dendrite(0).w, [5.0, 5.0])
     net.simulate(5)
-    np.testing.assert_allclose(proj.w, 1.0)
+    np.testing.assert_allclose(proj.dendrite(0).w, [1.0, 1.0])
     net.simulate(5)
+    np.testing.assert_allclose(proj.dendrite(0).w, [0.0, 0.0])
     np.testing.assert_allclose(proj.w, 0.0)
     np.testing.assert_allclose(proj.g, 1.0)
+    # the dendrite view: pre ranks of post neuron 1 (no self-connection), its size
+    assert proj.dendrite(1).pre_ranks == [0, 2] and proj.dendrite(1).size == 2
+    assert proj.post_ranks == [0, 1, 2] and proj.nb_dendrites == 3
 
 
 def test_postspike_kat(gpu):
     syn = ah.Synapse(parameters="constant = 1.0", post_spike="w += 1.0 * constant : max=10.0",
                      name="BoundedPost")
     net, proj = _kat_net(syn)
-    np.testing.assert_allclose(proj.w, 5.0)
+    np.testing.assert_allclose(proj.dendrite(0).w, [5.0, 5.0])
     net.simulate(5)
-    np.testing.assert_allclose(proj.w, 10.0)
+    np.testing.assert_allclose(proj.dendrite(0).w, [10.0, 10.0])
     net.simulate(5)
-    np.testing.assert_allclose(proj.w, 10.0)
+    np.testing.assert_allclose(proj.dendrite(0).w, [10.0, 10.0])
     # a per-synapse parameter set from the host changes the increment
     proj.w = 1.0
     proj.constant = 0.5
+    proj.dendrite(2).constant = 0.25
     net.simulate(4)
-    np.testing.assert_allclose(proj.w, 3.0)
+    np.testing.assert_allclose(proj.dendrite(0).w, [3.0, 3.0])
+    np.testing.assert_allclose(proj.dendrite(2).w, [2.0, 2.0])
 
 
 # Poisson trains; the post neurons read g_exc (reset after every step) but
