@@ -65,6 +65,11 @@ struct Pop {
     // depth x size int32 with counts int32[depth]
     int32_t depth = 1;
     DevBuf hist, hist_count;
+    // mean firing rate over fr_window steps (0: off): a fr_window-bit history
+    // per neuron (uint32 words, word-major) and the number of spikes in it
+    int32_t fr_window = 0;
+    double fr_rate = 0.0;         // 1000 / window (ms)
+    DevBuf fr_bits, fr_count;
     DevBuf rand_inject[AHIP_MAX_RAND];
     int64_t rand_inject_steps[AHIP_MAX_RAND] = {0};
     int64_t rand_t0 = 0;
@@ -199,6 +204,7 @@ int launch_rate_psp(const ahip_net* net, const Proj& p, const void* pre_r, void*
 int launch_spike_psp(const ahip_net* net, const Proj& p, const Pop& pre, void* target,
                      hipStream_t s);
 int launch_compact(const Pop& p, const Monitor* m, const int64_t* t_base, int32_t step, hipStream_t s);
+int launch_update_rate(const ahip_net* net, const Pop& p, int32_t step, hipStream_t s);
 int launch_scatter_group(const ahip_net* net, const ScatterGroup& g, int32_t step, hipStream_t s);
 int launch_push_values(const ahip_net* net, const void* src, void* hist, int32_t n, int32_t D, int32_t step,
                        hipStream_t s);

@@ -563,6 +563,7 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
             // the population's first spike monitor is filled by the compaction
             LCHK(launch_compact(p, first_spike_monitor(net, p.id), net->t_dev.as<int64_t>(), step, s));
             if (probe_c) HIPCHK(hipEventRecord(net->probe_ev[1], s));
+            if (p.fr_window > 0) LCHK(launch_update_rate(net, p, step, s));
         }
         // delay queues: push this step's values (after the update, as the
         // reference's update_delay)
@@ -1002,7 +1003,34 @@ int ahip_pop_reset_spiking(ahip_net* net, int32_t pop) {
     LCHK(launch_fill_i32(p->refractory_remaining.as<int32_t>(), p->size, 0, net->stream));
     LCHK(launch_fill_i64(p->last_spike.as<int64_t>(), p->size, -10000, net->stream));
     LCHK(launch_fill_i32(p->spike_count.as<int32_t>(), 1, 0, net->stream));
+    if (p->fr_window > 0) {
+        // the spike history of the mean firing rate is emptied (reset_FR)
+        LCHK(launch_fill_i32(p->fr_bits.as<int32_t>(), (int64_t)((p->fr_window + 31) / 32) * p->size, 0,
+                             net->stream));
+        LCHK(launch_fill_i32(p->fr_count.as<int32_t>(), p->size, 0, net->stream));
+    }
     HIPCHK(hipStreamSynchronize(net->stream));
+    return 0;
+}
+
+int ahip_pop_compute_firing_rate(ahip_net* net, int32_t pop, double window_ms) {
+    Pop* p = get_pop(net, pop);
+    if (!p || !p->model->spiking) return fail(AHIP_EINVAL, "compute_firing_rate applies to spiking populations");
+    if (p->slot_index("r") < 0) return fail(AHIP_EMODEL, "the neuron model has no variable r");
+    if (!(window_ms > 0.0)) return 0;   // as the reference: a non-positive window leaves the state alone
+    const int64_t W = (int64_t)(window_ms / net->dt);
+    if (W < 1 || W > (1 << 20)) return fail(AHIP_EINVAL, "the window must span 1 .. 2^20 steps");
+    HIPCHK(hipSetDevice(net->device));
+    HIPCHK(hipStreamSynchronize(net->stream));
+    if (W != p->fr_window) {
+        int rc;
+        const int64_t words = (W + 31) / 32;
+        if ((rc = dev_alloc(p->fr_bits, (size_t)(words * p->size) * sizeof(uint32_t)))) return rc;
+        if ((rc = dev_alloc(p->fr_count, (size_t)p->size * sizeof(int32_t)))) return rc;
+        p->fr_window = (int32_t)W;
+    }
+    p->fr_rate = 1000.0 / window_ms;
+    invalidate(net);
     return 0;
 }
 

@@ -357,6 +357,14 @@ def analyse_neuron(parameters: str = "", equations: str = "", spike: str | None 
                                   ctype="int" if "int" in flags else precision))
     declared = {v.name for v in params + variables}
     spiking = spike is not None
+    if spiking:
+        # spiking neurons hold their mean firing rate in r (0 unless
+        # compute_firing_rate() is enabled), AnalyseNeuron.py:145-160
+        if "r" in declared:
+            raise EquationError('spiking neurons use the variable "r" for the mean firing rate: use another name')
+        variables.append(Variable(name="r", eq="", locality="local", flags=[], bounds={}, init=0.0,
+                                  ctype=precision))
+        declared.add("r")
     # conductances of spiking neurons and targets of rate neurons
     targets = []
     for v in variables:
@@ -403,6 +411,8 @@ def analyse_neuron(parameters: str = "", equations: str = "", spike: str | None 
     tr = _Translator(desc, precision)
     local_names = desc.local_names() + [r.name for r in randoms if r.locality == "local"]
     for v in variables:
+        if v.eq == "":
+            continue            # attribute without an equation (the implicit r of spiking neurons)
         kind = _equation_type(v.name, v.eq)
         method = "explicit"
         for m in ("exponential", "explicit"):

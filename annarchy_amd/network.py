@@ -122,11 +122,22 @@ class Population:
         if not self.net.compiled:
             v = self._init.get(name, info[name][2])
             return v
-        n = self.size if info[name][1] else 1
+        local = info[name][1]
+        n = self.size if local else 1
         out = np.empty(n, dtype=np.float64)
         _lib.check(_lib.lib().ahip_pop_get(self.net._h, self.id, name.encode(),
                                            _lib.ptr(out, C.c_double), n))
-        return out.reshape(self.geometry) if n > 1 else float(out[0])
+        # local attributes keep the population's shape (also for one neuron)
+        return out.reshape(self.geometry) if local else float(out[0])
+
+    def compute_firing_rate(self, window: float):
+        """core/Population.py:700-723: spiking neurons hold in r their mean
+        firing rate over the last `window` ms (Hz), updated every step."""
+        if not self.spiking:
+            raise ValueError("compute_firing_rate(): the neuron is already rate-coded")
+        self._fr_window = float(window)
+        if self.net.compiled:
+            _lib.check(_lib.lib().ahip_pop_compute_firing_rate(self.net._h, self.id, self._fr_window))
 
     def reset(self, attributes=None):
         """Population.reset (core/Population.py:309-343): parameters and
@@ -791,6 +802,8 @@ class Network:
             for name, value in p._init.items():
                 p.set(name, value)
             p._upload_refractory()
+            if getattr(p, "_fr_window", None):
+                p.compute_firing_rate(p._fr_window)
         for m in self.monitors:
             m._instantiate()
         return self

@@ -116,6 +116,11 @@ int ahip_pop_get_last_spike(ahip_net* net, int32_t pop, int64_t* values, int64_t
 int ahip_pop_spikes(ahip_net* net, int32_t pop, int32_t* out, int32_t cap, int32_t* n);
 /* reset(): refractory_remaining = 0, last_spike = -10000, spiked cleared. */
 int ahip_pop_reset_spiking(ahip_net* net, int32_t pop);
+/* compute_firing_rate(window) (core/Population.py:700-723,
+ * Population/OpenMPGenerator.py:562-618): after every step r = 1000/window x
+ * the neuron's spikes in the last window/dt steps (spiking populations; r is
+ * 0 until enabled; window <= 0 leaves the state unchanged). */
+int ahip_pop_compute_firing_rate(ahip_net* net, int32_t pop, double window_ms);
 /* reset() of the delay queues (Population/OpenMPTemplates.py:193-197, the
  * "reset" code of attribute_delayed; Projection reset_ring_buffer,
  * core/Network.py:652-655): every slot of a rate-coded population's queue
