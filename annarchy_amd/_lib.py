@@ -83,6 +83,8 @@ _SIGS = {
     "ahip_pop_reset_spiking": ([_P, _I32], _I32),
     "ahip_pop_reset_delays": ([_P, _I32], _I32),
     "ahip_pop_compute_firing_rate": ([_P, _I32, _D], _I32),
+    "ahip_pop_set_active": ([_P, _I32, _I32], _I32),
+    "ahip_proj_set_transmission": ([_P, _I32, _I32], _I32),
     "ahip_pop_size": ([_P, _I32, _PI32], _I32),
     "ahip_pop_inject_random": ([_P, _I32, _I32, _PD, _I64], _I32),
     "ahip_proj_create": ([_P, _I32, _I32, C.c_char_p, _I32, _PI32], _I32),

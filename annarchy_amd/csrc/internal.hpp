@@ -67,6 +67,8 @@ struct Pop {
     DevBuf hist, hist_count;
     // mean firing rate over fr_window steps (0: off): a fr_window-bit history
     // per neuron (uint32 words, word-major) and the number of spikes in it
+    bool active = true;           // activate(false): no update, no spike gather
+    DevBuf ring_tmp;              // inactive with a delay queue: staging of the ring rotation
     int32_t fr_window = 0;
     double fr_rate = 0.0;         // 1000 / window (ms)
     DevBuf fr_bits, fr_count;
@@ -90,6 +92,7 @@ struct Proj {
     bool accumulate = false;      // rate: add into the target instead of overwriting it
     bool single_w = false;
     double w_const = 0.0;
+    bool transmission = true;     // _transmission: compute_psp (and the learning rules) on / off
     // host copy of the forward view (for re-layout and get/set)
     std::vector<int64_t> h_row_ptr;
     std::vector<int32_t> h_col;
@@ -204,7 +207,7 @@ int launch_rate_psp(const ahip_net* net, const Proj& p, const void* pre_r, void*
 int launch_spike_psp(const ahip_net* net, const Proj& p, const Pop& pre, void* target,
                      hipStream_t s);
 int launch_compact(const Pop& p, const Monitor* m, const int64_t* t_base, int32_t step, hipStream_t s);
-int launch_update_rate(const ahip_net* net, const Pop& p, int32_t step, hipStream_t s);
+int launch_update_rate(const ahip_net* net, const Pop& p, int32_t step, hipStream_t s, bool active = true);
 int launch_scatter_group(const ahip_net* net, const ScatterGroup& g, int32_t step, hipStream_t s);
 int launch_push_values(const ahip_net* net, const void* src, void* hist, int32_t n, int32_t D, int32_t step,
                        hipStream_t s);

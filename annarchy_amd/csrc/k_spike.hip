@@ -269,7 +269,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void update_rate(int32_t size, int32_t W, const int64_t* __restrict__ t_base,
                                                    int32_t step, const uint64_t* __restrict__ mask,
                                                    uint32_t* __restrict__ bits, int32_t* __restrict__ count,
-                                                   T* __restrict__ r, T rate) {
+                                                   T* __restrict__ r, T rate, int32_t active) {
     const int32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= size) return;
     const int64_t t = *t_base + step;
@@ -277,17 +277,19 @@ __global__ __launch_bounds__(256) void update_rate(int32_t size, int32_t W, cons
     uint32_t* wp = bits + (pos >> 5) * (int64_t)size + i;
     const uint32_t b = 1u << (pos & 31);
     const uint32_t word = *wp;
-    const int32_t spiked = (int32_t)((mask[i >> 6] >> (i & 63)) & 1ull);
+    // an inactive population only lets step t - W leave the window (the
+    // reference drops spike times <= t - W at its next active step); r stays
+    const int32_t spiked = active ? (int32_t)((mask[i >> 6] >> (i & 63)) & 1ull) : 0;
     const int32_t old = (word & b) ? 1 : 0;
     *wp = spiked ? (word | b) : (word & ~b);
     const int32_t c = count[i] + spiked - old;
     count[i] = c;
-    r[i] = rate * (T)c;
+    if (active) r[i] = rate * (T)c;
 }
 
 }  // namespace
 
-int launch_update_rate(const ahip_net* net, const Pop& p, int32_t step, hipStream_t s) {
+int launch_update_rate(const ahip_net* net, const Pop& p, int32_t step, hipStream_t s, bool active) {
     if (p.fr_window <= 0) return 0;
     const unsigned nb = (unsigned)((p.size + 255) / 256);
     const int rs = p.slot_index("r");
@@ -295,11 +297,11 @@ int launch_update_rate(const ahip_net* net, const Pop& p, int32_t step, hipStrea
     if (net->precision == AHIP_F64)
         hipLaunchKernelGGL(update_rate<double>, dim3(nb), dim3(256), 0, s, p.size, p.fr_window,
                            net->t_dev.as<int64_t>(), step, p.spike_mask.as<uint64_t>(), p.fr_bits.as<uint32_t>(),
-                           p.fr_count.as<int32_t>(), p.slots[rs].as<double>(), p.fr_rate);
+                           p.fr_count.as<int32_t>(), p.slots[rs].as<double>(), p.fr_rate, active ? 1 : 0);
     else
         hipLaunchKernelGGL(update_rate<float>, dim3(nb), dim3(256), 0, s, p.size, p.fr_window,
                            net->t_dev.as<int64_t>(), step, p.spike_mask.as<uint64_t>(), p.fr_bits.as<uint32_t>(),
-                           p.fr_count.as<int32_t>(), p.slots[rs].as<float>(), (float)p.fr_rate);
+                           p.fr_count.as<int32_t>(), p.slots[rs].as<float>(), (float)p.fr_rate, active ? 1 : 0);
     return (int)hipGetLastError();
 }
 

@@ -356,6 +356,28 @@ int launch_syn_now(ahip_net* net, Proj& p, int kind, int32_t step) {
     return 0;
 }
 
+// An inactive population does not push into its delay queue: the reference's
+// deque keeps its lists while the clock runs on (Population/OpenMPGenerator.py:
+// 440-455 inside `if (_active)`).  The ring here is indexed by step (slot
+// u mod D = end of step u), so it is rotated by one slot per inactive step:
+// slot (t - d) then still holds what deque entry d - 1 holds.
+int rotate_ring(DevBuf& ring, size_t slot_bytes, int32_t D, DevBuf& tmp, hipStream_t s) {
+    char* r = (char*)ring.p;
+    char* t = (char*)tmp.p;
+    if (!r || D <= 1) return 0;
+    if (!t || tmp.bytes < slot_bytes * D) return fail(AHIP_ESTATE, "delay ring staging missing");
+    HIPCHK(hipMemcpyAsync(t + slot_bytes, r, slot_bytes * (D - 1), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(t, r + slot_bytes * (D - 1), slot_bytes, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(r, t, slot_bytes * D, hipMemcpyDeviceToDevice, s));
+    return 0;
+}
+
+// compute_psp of a projection runs when it transmits and its post-synaptic
+// population is active (Projection/OpenMP/CSR.py: _transmission && post _active)
+bool transmits(const ahip_net* net, const Proj& p) {
+    return p.transmission && net->pops[p.post]->active;
+}
+
 int prepare(ahip_net* net) {
     if (!net->initialized) return fail(AHIP_ESTATE, "network not initialized");
     for (auto& pp : net->projs) {
@@ -364,6 +386,11 @@ int prepare(ahip_net* net) {
     }
     int rc0 = ensure_delay_buffers(net);
     if (rc0) return rc0;
+    for (auto& pp : net->pops) {
+        Pop& p = *pp;
+        if (p.active || p.depth <= 1 || !p.hist.p) continue;
+        if (p.ring_tmp.bytes < p.hist.bytes && (rc0 = dev_alloc(p.ring_tmp, p.hist.bytes))) return rc0;
+    }
     // fused scatter groups: spiking projections in creation order, each in the
     // first group after the last one that already writes its target
     if (net->scatter_dirty) {
@@ -373,7 +400,7 @@ int prepare(ahip_net* net) {
         std::vector<std::vector<const void*>> targets;
         for (auto& pp : net->projs) {
             Proj& p = *pp;
-            if (!p.spiking || p.nnz == 0 || p.delay_per_syn || p.syn) continue;
+            if (!p.spiking || p.nnz == 0 || p.delay_per_syn || p.syn || !transmits(net, p)) continue;
             Pop& pre = *net->pops[p.pre];
             void* g = net->pops[p.post]->slots[p.target_slot].p;
             size_t k = 0;
@@ -518,6 +545,7 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
         Pop& post = *net->pops[p.post];
         void* target = post.slots[p.target_slot].p;
         if (p.spiking) {
+            if (!transmits(net, p)) continue;
             if (p.syn) {
                 if (p.nnz > 0) {
                     int rc = launch_syn(net, p, 0, step);
@@ -526,6 +554,14 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
             } else if (p.delay_per_syn && p.nnz > 0) {
                 LCHK(launch_spike_ring(net, p, pre, target, step, s));
             }
+            continue;
+        }
+        if (!transmits(net, p)) {
+            // the sums of an active population are still cleared at every
+            // step (OpenMPTemplates.py:266-276): a first writer that does not
+            // transmit leaves 0 in its target; an inactive one keeps its sums
+            if (post.active && (zero_before[p.id] || !p.accumulate))
+                LCHK(launch_fill(target, post.size, (int32_t)es, 0.0, s));
             continue;
         }
         if (zero_before[p.id]) LCHK(launch_fill(target, post.size, (int32_t)es, 0.0, s));
@@ -549,6 +585,16 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
     // 3. neuron updates (+ spike flags), 4. compaction
     for (auto& pp : net->pops) {
         Pop& p = *pp;
+        if (!p.active) {
+            if (p.depth > 1 && p.hist.p) {
+                const size_t slot = (size_t)p.size * (p.model->spiking ? sizeof(int32_t) : es);
+                int rc = rotate_ring(p.hist, slot, p.depth, p.ring_tmp, s);
+                if (!rc && p.model->spiking) rc = rotate_ring(p.hist_count, sizeof(int32_t), p.depth, p.ring_tmp, s);
+                if (rc) return rc;
+            }
+            if (p.model->spiking && p.fr_window > 0) LCHK(launch_update_rate(net, p, step, s, false));
+            continue;
+        }
         ahip_pop_args a = make_args(net, p, step);
         ahip_launch_fn fn = net->precision == AHIP_F64 ? p.model->launch_f64 : p.model->launch_f32;
         const bool probe_u = net->probe_kind == AHIP_PROBE_UPDATE && net->probe_proj == p.id;
@@ -580,7 +626,7 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
     // then the post events (OpenMPBaseTemplate.py:217-325)
     for (auto& pp : net->projs) {
         Proj& p = *pp;
-        if (!p.syn || p.nnz == 0) continue;
+        if (!p.syn || p.nnz == 0 || !transmits(net, p)) continue;
         if (p.syn->has_update && p.syn_update) {
             int rc = launch_syn(net, p, 1, step);
             if (rc) return rc;
@@ -596,7 +642,9 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
         if (m.after_psp) continue;
         Pop& p = *net->pops[m.pop];
         if (m.spikes) {
-            if (first_spike_monitor(net, m.pop) != &m)
+            // an inactive population keeps its last spike list, which its
+            // monitors record again (its compaction does not run)
+            if (first_spike_monitor(net, m.pop) != &m || !p.active)
                 LCHK(launch_record_spikes(m, p, net->t_dev.as<int64_t>(), step, s));
         } else
             LCHK(launch_record_values(m, p.slots[m.slot].p, p.size, (int32_t)es,
@@ -1010,6 +1058,15 @@ int ahip_pop_reset_spiking(ahip_net* net, int32_t pop) {
         LCHK(launch_fill_i32(p->fr_count.as<int32_t>(), p->size, 0, net->stream));
     }
     HIPCHK(hipStreamSynchronize(net->stream));
+    return 0;
+}
+
+int ahip_pop_set_active(ahip_net* net, int32_t pop, int32_t active) {
+    Pop* p = get_pop(net, pop);
+    if (!p) return fail(AHIP_EINVAL, "bad population");
+    HIPCHK(hipStreamSynchronize(net->stream));
+    p->active = active != 0;
+    invalidate(net);
     return 0;
 }
 
@@ -1617,6 +1674,15 @@ int ahip_proj_get_last_event(ahip_net* net, int32_t proj, int64_t* values, int64
     Proj* p = get_proj(net, proj);
     if (!p || !p->syn || !values || n != p->nnz) return fail(AHIP_EINVAL, "bad plastic projection or size");
     return syn_field_io(net, *p, -2, nullptr, values, false);
+}
+
+int ahip_proj_set_transmission(ahip_net* net, int32_t proj, int32_t transmission) {
+    Proj* p = get_proj(net, proj);
+    if (!p) return fail(AHIP_EINVAL, "bad projection");
+    HIPCHK(hipStreamSynchronize(net->stream));
+    p->transmission = transmission != 0;
+    invalidate(net);
+    return 0;
 }
 
 int ahip_proj_set_learning(ahip_net* net, int32_t proj, int32_t update, int32_t plasticity, int32_t period,

@@ -130,6 +130,27 @@ class Population:
         # local attributes keep the population's shape (also for one neuron)
         return out.reshape(self.geometry) if local else float(out[0])
 
+    # -- enable / disable (core/Population.py:355-375) -------------------------
+    @property
+    def enabled(self) -> bool:
+        return getattr(self, "_enabled", True)
+
+    def enable(self) -> None:
+        """(Re-)enables the population's computations after disable()."""
+        self._enabled = True
+        self._upload_active()
+
+    def disable(self) -> None:
+        """Freezes the population: no neuron update, no spike emission (the
+        last spike list stays as it is) and the projections into it do not
+        transmit, until enable()."""
+        self._enabled = False
+        self._upload_active()
+
+    def _upload_active(self):
+        if self.net.compiled:
+            _lib.check(_lib.lib().ahip_pop_set_active(self.net._h, self.id, 1 if self.enabled else 0))
+
     def compute_firing_rate(self, window: float):
         """core/Population.py:700-723: spiking neurons hold in r their mean
         firing rate over the last `window` ms (Hz), updated every step."""
@@ -334,6 +355,41 @@ class Projection:
             u, p, n, off = self._learning
             _lib.check(_lib.lib().ahip_proj_set_learning(self.net._h, self.id, u, p, n, off))
 
+    # -- projection flags (core/Projection.py:1066-1100) -------------------------
+    @property
+    def transmission(self) -> bool:
+        """False: the projection delivers nothing (no psp, no pre/post events,
+        no update_synapse); its target sum reads 0."""
+        return getattr(self, "_transmission", True)
+
+    @transmission.setter
+    def transmission(self, value: bool):
+        self._transmission = bool(value)
+        if self.net.compiled:
+            _lib.check(_lib.lib().ahip_proj_set_transmission(self.net._h, self.id, 1 if self._transmission else 0))
+
+    @property
+    def update(self) -> bool:
+        """update_synapse on (the _update flag)."""
+        return bool(getattr(self, "_learning", (1, 1, 1, 0))[0])
+
+    @update.setter
+    def update(self, value: bool):
+        u, p, n, off = getattr(self, "_learning", (1, 1, 1, 0))
+        self._learning = (1 if value else 0, p, n, off)
+        self._upload_learning()
+
+    @property
+    def plasticity(self) -> bool:
+        """Weights change (the _plasticity flag)."""
+        return bool(getattr(self, "_learning", (1, 1, 1, 0))[1])
+
+    @plasticity.setter
+    def plasticity(self, value: bool):
+        u, p, n, off = getattr(self, "_learning", (1, 1, 1, 0))
+        self._learning = (u, 1 if value else 0, n, off)
+        self._upload_learning()
+
     @property
     def last_event(self) -> np.ndarray:
         """Step of the last pre or post event of every synapse (forward order)."""
@@ -523,6 +579,8 @@ class Projection:
             self.set(name, value)
         if hasattr(self, "_learning"):
             self._upload_learning()
+        if not self.transmission:
+            self.transmission = False
 
     # -- accessors ------------------------------------------------------------
     @property
@@ -804,6 +862,8 @@ class Network:
             p._upload_refractory()
             if getattr(p, "_fr_window", None):
                 p.compute_firing_rate(p._fr_window)
+            if not p.enabled:
+                p._upload_active()
         for m in self.monitors:
             m._instantiate()
         return self

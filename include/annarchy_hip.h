@@ -116,6 +116,11 @@ int ahip_pop_get_last_spike(ahip_net* net, int32_t pop, int64_t* values, int64_t
 int ahip_pop_spikes(ahip_net* net, int32_t pop, int32_t* out, int32_t cap, int32_t* n);
 /* reset(): refractory_remaining = 0, last_spike = -10000, spiked cleared. */
 int ahip_pop_reset_spiking(ahip_net* net, int32_t pop);
+/* activate(bool) (OpenMPTemplates.py:55, core/Population.py:355-372
+ * enable/disable): an inactive population is neither updated nor gathered
+ * (its last spike list stays as it is), and the projections into it do not
+ * transmit. */
+int ahip_pop_set_active(ahip_net* net, int32_t pop, int32_t active);
 /* compute_firing_rate(window) (core/Population.py:700-723,
  * Population/OpenMPGenerator.py:562-618): after every step r = 1000/window x
  * the neuron's spikes in the last window/dt steps (spiking populations; r is
@@ -208,6 +213,10 @@ int ahip_proj_syn_get(ahip_net* net, int32_t proj, const char* attr, double* val
 int ahip_proj_syn_set(ahip_net* net, int32_t proj, const char* attr, const double* values, int64_t n);
 /* _last_event of every synapse (forward order), in steps. */
 int ahip_proj_get_last_event(ahip_net* net, int32_t proj, int64_t* values, int64_t n);
+/* _transmission (core/Projection.py:1066-1100): 0 skips the projection's
+ * compute_psp, its pre/post events and update_synapse; its target sum still
+ * reads 0 at the step, as the reference clears sums every step. */
+int ahip_proj_set_transmission(ahip_net* net, int32_t proj, int32_t transmission);
 /* enable_learning / disable_learning (core/Projection.py:1440-1510): update
  * = _update (0 skips update_synapse), plasticity = _plasticity (0 leaves w
  * unchanged in events and updates), update_synapse every `period` steps when
