@@ -145,7 +145,8 @@ def generate(desc: NeuronDescription) -> ModelSource:
     """Build the HIP source of the update kernel of one neuron model."""
     slots = []  # (name, local, init, kind)
     for p in desc.parameters:
-        slots.append((p.name, p.locality == "local", p.init, 0))
+        # kind 4: a network constant (written by ahip_net_set_constant)
+        slots.append((p.name, p.locality == "local", p.init, 4 if "constant" in p.flags else 0))
     for v in desc.variables:
         if any(s[0] == v.name for s in slots):
             continue
@@ -277,8 +278,8 @@ def generate(desc: NeuronDescription) -> ModelSource:
 
 
 def generate_from_text(name: str, parameters: str = "", equations: str = "", spike=None,
-                       reset=None, refractory=None) -> ModelSource:
-    return generate(analyse_neuron(parameters, equations, spike, reset, refractory, name))
+                       reset=None, refractory=None, constants=None) -> ModelSource:
+    return generate(analyse_neuron(parameters, equations, spike, reset, refractory, name, constants=constants))
 
 
 # ---------------------------------------------------------------------------
@@ -286,6 +287,7 @@ def generate_from_text(name: str, parameters: str = "", equations: str = "", spi
 # ---------------------------------------------------------------------------
 
 SYNAPSE_ABI = 2  # AHIP_SYNAPSE_ABI in include/annarchy_hip_synapse.h
+SYN_CONSTANT = 3  # AHIP_SYN_CONSTANT: a network constant, one value per projection
 
 _SYN_KERNELS = r"""
 #include "annarchy_hip_synapse.h"
@@ -593,7 +595,8 @@ def generate_synapse(desc) -> SynapseSource:
         "launch_global": launch_global,
         "launch_local": launch_local,
         "attr_names": ", ".join(f'"{n}"' for n, _, _ in attrs) or "nullptr",
-        "attr_loc": ", ".join(str(SYN_LOCALITY[loc]) for _, loc, _ in attrs) or "0",
+        "attr_loc": ", ".join(str(SYN_CONSTANT if n in desc.constant_names() else SYN_LOCALITY[loc])
+                              for n, loc, _ in attrs) or "0",
         "attr_init": ", ".join(repr(float(v)) for _, _, v in attrs) or "0.0",
         "pre_deps": ", ".join(f'"{n}"' for n in desc.pre_deps) or "nullptr",
         "post_deps": ", ".join(f'"{n}"' for n in desc.post_deps) or "nullptr",
@@ -613,6 +616,7 @@ def generate_synapse(desc) -> SynapseSource:
 
 
 def generate_synapse_from_text(name: str, parameters: str = "", equations: str = "", pre_spike=None,
-                               post_spike=None) -> SynapseSource:
+                               post_spike=None, constants=None) -> SynapseSource:
     from .equations import analyse_synapse
-    return generate_synapse(analyse_synapse(parameters, equations, pre_spike, post_spike, name))
+    return generate_synapse(analyse_synapse(parameters, equations, pre_spike, post_spike, name,
+                                            constants=constants))

@@ -1070,6 +1070,34 @@ int ahip_pop_set_active(ahip_net* net, int32_t pop, int32_t active) {
     return 0;
 }
 
+int ahip_pop_size_in_bytes(ahip_net* net, int32_t pop, int64_t* bytes) {
+    Pop* p = get_pop(net, pop);
+    if (!p || !bytes) return fail(AHIP_EINVAL, "bad population");
+    size_t b = 0;
+    for (const DevBuf& d : p->slots) b += d.bytes;
+    for (const DevBuf* d : std::initializer_list<const DevBuf*>{&p->refractory, &p->refractory_remaining, &p->last_spike, &p->spike_mask,
+                            &p->block_counts, &p->spiked, &p->spike_count, &p->hist, &p->hist_count,
+                            &p->ring_tmp, &p->fr_bits, &p->fr_count})
+        b += d->bytes;
+    for (const DevBuf& d : p->rand_inject) b += d.bytes;
+    *bytes = (int64_t)b;
+    return 0;
+}
+
+int ahip_proj_size_in_bytes(ahip_net* net, int32_t proj, int64_t* bytes) {
+    Proj* p = get_proj(net, proj);
+    if (!p || !bytes) return fail(AHIP_EINVAL, "bad projection");
+    const RateLayout& L = p->rate;
+    size_t b = 0;
+    for (const DevBuf* d : std::initializer_list<const DevBuf*>{&L.ptr, &L.idx, &L.val, &L.rl, &L.perm, &L.aux, &L.tiles, &p->col_ptr, &p->row_idx,
+                            &p->w_col, &p->delay_dev, &p->ring, &p->xd, &p->syn_rec, &p->last_event,
+                            &p->fwd2col, &p->row_ptr_dev, &p->col_dev})
+        b += d->bytes;
+    for (const DevBuf& d : p->syn_attr) b += d.bytes;
+    *bytes = (int64_t)b;
+    return 0;
+}
+
 int ahip_pop_compute_firing_rate(ahip_net* net, int32_t pop, double window_ms) {
     Pop* p = get_pop(net, pop);
     if (!p || !p->model->spiking) return fail(AHIP_EINVAL, "compute_firing_rate applies to spiking populations");
@@ -1668,6 +1696,32 @@ int ahip_proj_syn_set(ahip_net* net, int32_t proj, const char* attr, const doubl
     if (n != size) return fail(AHIP_EINVAL, "%s holds %lld values", attr, (long long)size);
     if (local) return syn_field_io(net, *p, k, const_cast<double*>(values), nullptr, true);
     return syn_plain_io(net, p->syn_attr[k].p, const_cast<double*>(values), n, true);
+}
+
+// set_<constant>(value) of the reference module (core/Constant.py:98-120):
+// every population slot and synaptic attribute that mirrors the constant
+int ahip_net_set_constant(ahip_net* net, const char* name, double value, int32_t* n_set) {
+    if (!net || !name) return fail(AHIP_EINVAL, "bad network or constant name");
+    int32_t count = 0;
+    for (auto& pp : net->pops) {
+        Pop& p = *pp;
+        const int k = p.slot_index(name);
+        if (k < 0 || p.model->slot_kind[k] != 4) continue;   // absent, or an attribute hiding the constant
+        int rc = ahip_pop_set(net, p.id, name, &value, 1);
+        if (rc) return rc;
+        count++;
+    }
+    for (auto& pp : net->projs) {
+        Proj& p = *pp;
+        if (!p.syn || p.nnz < 0) continue;
+        const int k = syn_attr(&p, name);
+        if (k < 0 || p.syn->attr_locality[k] != AHIP_SYN_CONSTANT) continue;
+        int rc = ahip_proj_syn_set(net, p.id, name, &value, 1);
+        if (rc) return rc;
+        count++;
+    }
+    if (n_set) *n_set = count;
+    return 0;
 }
 
 int ahip_proj_get_last_event(ahip_net* net, int32_t proj, int64_t* values, int64_t n) {

@@ -330,10 +330,29 @@ def _equation_type(name: str, expression: str) -> str:
 # Public entry point
 # ---------------------------------------------------------------------------
 
+def _constant_params(constants, declared, texts, precision: str = "double") -> list:
+    """Network constants read by a model (core/Constant.py:16-60): each one
+    becomes a model-wide parameter flagged "constant" (initial value 0; the
+    network writes the constant's value into it), unless the model declares
+    an attribute of the same name, which hides the constant."""
+    if not constants:
+        return []
+    blob = "\n".join(t for t in texts if t)
+    out = []
+    for cname in sorted(constants):
+        if cname in declared:
+            continue
+        if re.search(r"(?<![\w.])" + re.escape(cname) + r"(?!\w)", blob):
+            out.append(Variable(name=cname, eq=f"{cname} = 0.0", locality="global", flags=["constant"],
+                                bounds={}, init=0.0, is_parameter=True, ctype=precision))
+    return out
+
+
 def analyse_neuron(parameters: str = "", equations: str = "", spike: str | None = None,
                    reset: str | None = None, refractory=None, name: str = "neuron",
-                   precision: str = "double") -> NeuronDescription:
-    """Analyse a neuron definition into C snippets (AnalyseNeuron.py:27-380)."""
+                   precision: str = "double", constants=None) -> NeuronDescription:
+    """Analyse a neuron definition into C snippets (AnalyseNeuron.py:27-380).
+    `constants`: names of the network constants an equation may read."""
     params = []
     for line in _lines(parameters):
         eq, constraint = _split_constraint(line)
@@ -356,6 +375,10 @@ def analyse_neuron(parameters: str = "", equations: str = "", spike: str | None 
                                   flags=flags, bounds=bounds, init=init,
                                   ctype="int" if "int" in flags else precision))
     declared = {v.name for v in params + variables}
+    for c in _constant_params(constants, declared | {"r"} if spike is not None else declared,
+                              [equations, spike, reset], precision):
+        params.append(c)
+        declared.add(c.name)
     spiking = spike is not None
     if spiking:
         # spiking neurons hold their mean firing rate in r (0 unless
@@ -551,6 +574,9 @@ class SynapseDescription:
         """(name, locality, init) of every attribute besides w."""
         return [(a.name, a.locality, a.init) for a in self.parameters + self.variables if a.name != "w"]
 
+    def constant_names(self):
+        return [a.name for a in self.parameters if "constant" in a.flags]
+
     def local_names(self):
         return ["w"] + [n for n, loc, _ in self.attrs() if loc == "local"]
 
@@ -616,9 +642,10 @@ def _statement(tr, line, desc) -> str:
 
 def analyse_synapse(parameters: str = "", equations: str = "", pre_spike: str | None = None,
                     post_spike: str | None = None, name: str = "synapse",
-                    precision: str = "double") -> SynapseDescription:
+                    precision: str = "double", constants=None) -> SynapseDescription:
     """Analyse a synapse definition (AnalyseSynapse.py) into the C text of its
-    pre- and post-synaptic events and of its continuous updates."""
+    pre- and post-synaptic events and of its continuous updates.
+    `constants`: names of the network constants an equation may read."""
     params = []
     for line in _lines(parameters):
         eq, constraint = _split_constraint(line)
@@ -655,7 +682,10 @@ def analyse_synapse(parameters: str = "", equations: str = "", pre_spike: str | 
     for a in params:
         if a.name == "w":
             raise EquationError("w is the synaptic weight: define it with an equation, not as a parameter")
-    declared = {a.name for a in params + desc.variables}
+    declared = {a.name for a in params + desc.variables} | {"w"}
+    for c in _constant_params(constants, declared, [equations, pre_spike, post_spike], precision):
+        params.append(c)
+        declared.add(c.name)
     tr = _Translator(desc, precision)
     # attributes become _s_<name> in the kernels so that no attribute name
     # can collide with a kernel variable

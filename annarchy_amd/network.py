@@ -20,6 +20,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import os
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -31,11 +32,59 @@ from . import connectivity as conn
 from .models import Neuron, Synapse
 
 
-def model_library(neuron: Neuron, allow_build: bool = True) -> Path:
+# -- constants (core/Constant.py:16-120, core/Network.py:482-512) ------------
+_GLOBAL_CONSTANTS: dict = {}
+_NETWORKS = weakref.WeakSet()
+
+
+class Constant(float):
+    """A named value that every neuron and synapse equation of a network may
+    read; ``set()`` changes it, also after compile.  Declared at the module
+    level (``Constant(name, value)``), it is copied into every network created
+    afterwards and ``set()`` reaches all of them; ``net.get_constant(name)``
+    (or ``net.constant(name, value)``) is the copy of one network.  A model
+    attribute of the same name hides the constant."""
+
+    def __new__(cls, name: str, value: float, net: "Network | None" = None):
+        return float.__new__(cls, float(value))
+
+    def __init__(self, name: str, value: float, net: "Network | None" = None):
+        self.name = name
+        self.value = float(value)
+        self._net = net
+        if net is None:
+            _GLOBAL_CONSTANTS[name] = self
+        else:
+            if net.compiled and name not in net._constants:
+                raise RuntimeError("constants are added before compile()")
+            net._constants[name] = self
+
+    def __str__(self):
+        return str(self.value)
+
+    __repr__ = __str__
+
+    def set(self, value: float) -> None:
+        self.value = float(value)
+        if self._net is None:
+            for net in list(_NETWORKS):
+                local = net._constants.get(self.name)
+                if local is not None:
+                    local.set(value)
+        elif self._net.compiled:
+            _lib.check(_lib.lib().ahip_net_set_constant(self._net._h, self.name.encode(), self.value, None))
+
+
+def _constant_names(net) -> list:
+    return sorted(net._constants) if net is not None else sorted(_GLOBAL_CONSTANTS)
+
+
+def model_library(neuron: Neuron, allow_build: bool = True, constants=None) -> Path:
     """Path of the compiled model library of `neuron` (built on demand when
-    hipcc is available and `allow_build`)."""
+    hipcc is available and `allow_build`); `constants`: the names of the
+    network constants its equations may read."""
     src = codegen.generate_from_text(neuron.name, neuron.parameters, neuron.equations,
-                                     neuron.spike, neuron.reset, neuron.refractory)
+                                     neuron.spike, neuron.reset, neuron.refractory, constants=constants)
     path = _build.model_lib_path(neuron.name, src.source)
     if path.exists():
         return path
@@ -44,10 +93,10 @@ def model_library(neuron: Neuron, allow_build: bool = True) -> Path:
     return _build.build_model(neuron.name, src.source)
 
 
-def synapse_library(synapse: Synapse, allow_build: bool = True) -> Path:
+def synapse_library(synapse: Synapse, allow_build: bool = True, constants=None) -> Path:
     """Path of the compiled event-kernel library of a spiking synapse type."""
     src = codegen.generate_synapse_from_text(synapse.name, synapse.parameters, synapse.equations,
-                                             synapse.pre_spike, synapse.post_spike)
+                                             synapse.pre_spike, synapse.post_spike, constants=constants)
     path = _build.model_lib_path(synapse.name, src.source, "syn")
     if path.exists():
         return path
@@ -87,8 +136,10 @@ class Population:
         if self._attrs is None:
             src = codegen.generate_from_text(self.neuron.name, self.neuron.parameters,
                                              self.neuron.equations, self.neuron.spike,
-                                             self.neuron.reset, self.neuron.refractory)
-            self._attrs = {s[0]: s for s in src.slots}
+                                             self.neuron.reset, self.neuron.refractory,
+                                             constants=_constant_names(self.net))
+            # the slots mirroring network constants are not population attributes
+            self._attrs = {s[0]: s for s in src.slots if s[3] != 4}
         return self._attrs
 
     def set(self, name, value):
@@ -129,6 +180,14 @@ class Population:
                                            _lib.ptr(out, C.c_double), n))
         # local attributes keep the population's shape (also for one neuron)
         return out.reshape(self.geometry) if local else float(out[0])
+
+    def size_in_bytes(self) -> int:
+        """Device memory held by the population (0 before compile)."""
+        if not self.net.compiled:
+            return 0
+        b = C.c_int64()
+        _lib.check(_lib.lib().ahip_pop_size_in_bytes(self.net._h, self.id, C.byref(b)))
+        return b.value
 
     # -- enable / disable (core/Population.py:355-375) -------------------------
     @property
@@ -264,8 +323,10 @@ class Projection:
             else:
                 src = codegen.generate_synapse_from_text(self.synapse.name, self.synapse.parameters,
                                                          self.synapse.equations, self.synapse.pre_spike,
-                                                         self.synapse.post_spike)
-                self._syn_attrs = {a[0]: a for a in src.attrs}
+                                                         self.synapse.post_spike,
+                                                         constants=_constant_names(self.net))
+                consts = set(src.desc.constant_names())
+                self._syn_attrs = {a[0]: a for a in src.attrs if a[0] not in consts}
         return self._syn_attrs
 
     def __getattr__(self, name):
@@ -561,7 +622,7 @@ class Projection:
                                       int(spiking), C.byref(pid)))
         self.id = pid.value
         if self.synapse is not None:
-            lib_path = synapse_library(self.synapse, self.net._allow_build)
+            lib_path = synapse_library(self.synapse, self.net._allow_build, _constant_names(self.net))
             _lib.check(L.ahip_proj_set_synapse(self.net._h, self.id, str(lib_path).encode()))
         if self._generator == "device":
             self._generate_on_device()
@@ -601,6 +662,25 @@ class Projection:
 
     def dendrite_size(self, post_rank: int) -> int:
         return int(self.csr.row_ptr[post_rank + 1] - self.csr.row_ptr[post_rank])
+
+    def size_in_bytes(self) -> int:
+        """Device memory held by the projection (0 before compile)."""
+        if not self.net.compiled:
+            return 0
+        b = C.c_int64()
+        _lib.check(_lib.lib().ahip_proj_size_in_bytes(self.net._h, self.id, C.byref(b)))
+        return b.value
+
+    def nb_efferent_synapses(self) -> dict:
+        """{pre rank: number of its synapses} for every pre-synaptic neuron
+        with at least one (CSRCMatrix.hpp nb_efferent_synapses; spiking
+        projections, core/Projection.py:916-930)."""
+        if not _base(self.pre).spiking:
+            raise ValueError("nb_efferent_synapses() is defined for spiking projections")
+        if self.csr is None:
+            raise RuntimeError("nb_efferent_synapses() needs the connectivity (after compile())")
+        counts = np.bincount(self.csr.col_idx, minlength=self.csr.n_cols)
+        return {int(r): int(counts[r]) for r in np.flatnonzero(counts)}
 
     def pre_ranks(self, post_rank: int) -> list:
         rp = self.csr.row_ptr
@@ -807,6 +887,21 @@ class Network:
         self.monitors: list[Monitor] = []
         self.compiled = False
         self._h = None
+        # the module-level constants declared so far, copied (core/Network.py:502-512)
+        self._constants = {}
+        for c in list(_GLOBAL_CONSTANTS.values()):
+            Constant(c.name, c.value, net=self)
+        _NETWORKS.add(self)
+
+    def constant(self, name: str, value: float) -> Constant:
+        """A constant of this network only (core/Network.py:482-500)."""
+        return Constant(name, value, net=self)
+
+    def get_constant(self, name: str) -> Constant:
+        """This network's copy of the constant `name`."""
+        if name not in self._constants:
+            raise KeyError(f"no constant named {name}")
+        return self._constants[name]
 
     def create(self, geometry, neuron: Neuron, name: str | None = None) -> Population:
         if self.compiled:
@@ -847,7 +942,7 @@ class Network:
         if graph_batch:
             _lib.check(L.ahip_set_graph_batch(h, graph_batch))
         for p in self.populations:
-            lib_path = model_library(p.neuron, allow_build)
+            lib_path = model_library(p.neuron, allow_build, _constant_names(self))
             pid = C.c_int32()
             _lib.check(L.ahip_pop_create(h, p.size, str(lib_path).encode(), C.byref(pid)))
             p.id = pid.value
@@ -864,6 +959,9 @@ class Network:
                 p.compute_firing_rate(p._fr_window)
             if not p.enabled:
                 p._upload_active()
+        # constants: the libraries hold 0, the network writes the values
+        for c in self._constants.values():
+            _lib.check(L.ahip_net_set_constant(h, c.name.encode(), c.value, None))
         for m in self.monitors:
             m._instantiate()
         return self

@@ -81,6 +81,12 @@ int ahip_net_destroy(ahip_net* net);
 int ahip_net_initialize(ahip_net* net, double dt, int32_t precision);
 /* set_seed(seed, num_sources, use_seed_seq): key of the on-device generator. */
 int ahip_set_seed(ahip_net* net, uint64_t seed);
+/* set_<constant>(value) of the generated module (core/Constant.py:98-120,
+ * Network.py:482-512): writes `value` into every population slot of kind 4
+ * and every synaptic attribute of locality AHIP_SYN_CONSTANT named `name`
+ * (the models that read the constant and do not hide it with an attribute
+ * of their own); *n_set (may be NULL) = the number of objects written. */
+int ahip_net_set_constant(ahip_net* net, const char* name, double value, int32_t* n_set);
 /* run(nbSteps): whole steps are captured in a HIP graph and replayed. */
 int ahip_run(ahip_net* net, int64_t nsteps);
 /* step() */
@@ -121,6 +127,13 @@ int ahip_pop_reset_spiking(ahip_net* net, int32_t pop);
  * (its last spike list stays as it is), and the projections into it do not
  * transmit. */
 int ahip_pop_set_active(ahip_net* net, int32_t pop, int32_t active);
+/* size_in_bytes() (NanoBind/Population.py, OpenMPTemplates.py): here the
+ * device memory the population holds (attributes, spike structures, delay
+ * ring, firing-rate history, injected draws). */
+int ahip_pop_size_in_bytes(ahip_net* net, int32_t pop, int64_t* bytes);
+/* size_in_bytes() of a projection (CSRCMatrix.hpp size_in_bytes): the device
+ * memory of its psp layout, backward view, delays and synaptic state. */
+int ahip_proj_size_in_bytes(ahip_net* net, int32_t proj, int64_t* bytes);
 /* compute_firing_rate(window) (core/Population.py:700-723,
  * Population/OpenMPGenerator.py:562-618): after every step r = 1000/window x
  * the neuron's spikes in the last window/dt steps (spiking populations; r is

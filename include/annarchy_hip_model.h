@@ -61,7 +61,9 @@ typedef struct ahip_model_desc {
     const char* const* slot_names;      /* attributes, _sum_<target>, pre-loop step sizes */
     const int32_t* slot_local;          /* 1: one value per neuron, 0: population-wide */
     const double* slot_init;            /* initial value of each slot */
-    const int32_t* slot_kind;           /* 0 parameter, 1 variable, 2 psp sum, 3 pre-loop */
+    const int32_t* slot_kind;           /* 0 parameter, 1 variable, 2 psp sum, 3 pre-loop,
+                                           4 network constant (population-wide, written by
+                                           ahip_net_set_constant) */
     int32_t n_rand;                     /* random variables drawn per step */
     const int32_t* rand_dist;           /* 0 Uniform(a,b), 1 Normal(mu,sigma) */
     const double* rand_args;            /* two per random variable */

@@ -37,6 +37,8 @@ extern "C" {
 #define AHIP_SYN_LOCAL 0       /* one value per synapse (state order) */
 #define AHIP_SYN_SEMIGLOBAL 1  /* one value per post-synaptic neuron (": postsynaptic") */
 #define AHIP_SYN_GLOBAL 2      /* one value per projection (": projection") */
+#define AHIP_SYN_CONSTANT 3    /* a network constant read by the model: one value per
+                                  projection, written by ahip_net_set_constant */
 
 typedef struct ahip_syn_args {
     /* pre events: spike list of step t-d of the pre population.  pre_depth
