@@ -392,19 +392,30 @@ int prepare(ahip_net* net) {
         if (p.ring_tmp.bytes < p.hist.bytes && (rc0 = dev_alloc(p.ring_tmp, p.hist.bytes))) return rc0;
     }
     // fused scatter groups: spiking projections in creation order, each in the
-    // first group after the last one that already writes its target
+    // first group after the last one that already writes its target, so a
+    // target receives the projections' sums in the reference's order.  A
+    // target that some projection with per-synapse weights writes has no
+    // such order (its atomic sum is order-dependent in any case), so its
+    // projections share groups freely.
     if (net->scatter_dirty) {
         net->scatter_groups.clear();
         std::vector<std::vector<ScatterDesc>> groups;
         std::vector<int64_t> blocks;
         std::vector<std::vector<const void*>> targets;
+        auto fused = [&](const Proj& p) {
+            return p.spiking && p.nnz != 0 && !p.delay_per_syn && !p.syn && transmits(net, p);
+        };
+        std::vector<const void*> unordered;
+        for (auto& pp : net->projs)
+            if (fused(*pp) && !pp->single_w) unordered.push_back(net->pops[pp->post]->slots[pp->target_slot].p);
         for (auto& pp : net->projs) {
             Proj& p = *pp;
-            if (!p.spiking || p.nnz == 0 || p.delay_per_syn || p.syn || !transmits(net, p)) continue;
+            if (!fused(p)) continue;
             Pop& pre = *net->pops[p.pre];
             void* g = net->pops[p.post]->slots[p.target_slot].p;
+            const bool ordered = std::find(unordered.begin(), unordered.end(), g) == unordered.end();
             size_t k = 0;
-            for (size_t q = 0; q < groups.size(); q++)
+            for (size_t q = 0; q < groups.size() && ordered; q++)
                 for (const void* t : targets[q])
                     if (t == g) k = q + 1;
             while (k < groups.size() && (int)groups[k].size() >= kMaxScatter) k++;
