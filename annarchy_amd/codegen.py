@@ -42,8 +42,19 @@ template <typename T> __host__ __device__ inline T positive(T x) { return x > T(
 template <typename T> __host__ __device__ inline T negative(T x) { return x < T(0) ? x : T(0); }
 template <typename T, typename A, typename B>
 __host__ __device__ inline T clip(T x, A lo, B hi) { return x < T(lo) ? T(lo) : (x > T(hi) ? T(hi) : x); }
-template <typename A, typename B> __host__ __device__ inline double power(A x, B y) { return pow((double)x, (double)y); }
-template <typename A, typename B> __host__ __device__ inline A modulo(A x, B y) { return x %% y; }
+// power(x, a): x multiplied a times by itself, the exponent taken as an
+// unsigned integer (OpenMPBaseTemplate.py:544-552; a = 0 gives 1 here)
+template <typename A> struct ahip_real { typedef double type; };
+template <> struct ahip_real<float> { typedef float type; };
+template <typename A>
+__host__ __device__ inline typename ahip_real<A>::type power(A x, unsigned int a) {
+    typedef typename ahip_real<A>::type R;
+    R res = (R)x;
+    for (unsigned int i = 1; i < a; i++) res *= (R)x;
+    return a == 0 ? R(1) : res;
+}
+// modulo(a, b) of the integer parts (OpenMPBaseTemplate.py:539)
+template <typename A, typename B> __host__ __device__ inline long modulo(A x, B y) { return (long)x %% (long)y; }
 template <typename C, typename A, typename B> __host__ __device__ inline A ite(C c, A a, B b) { return c ? a : (A)b; }
 
 """

@@ -127,6 +127,98 @@ def _lines(text: str | None) -> list[str]:
     return out
 
 
+_CONSTRAINT_KEYWORDS = ("init", "min", "max", "population", "postsynaptic", "projection", "explicit",
+                        "implicit", "semiimplicit", "exponential", "midpoint", "rk4", "runge-kutta4", "exact",
+                        "event-driven", "unless_refractory", "int", "bool", "float")
+
+
+def _is_constraint(text: str) -> bool:
+    """The text after the last ':' of a line is a constraint when it opens
+    with a reserved keyword (StringManipulation.py:7-35, :162-168)."""
+    e = "," + text.replace(" ", "") + ","
+    return any(re.match(r"([,]+)" + re.escape(k) + r"([=,]+)", e) for k in _CONSTRAINT_KEYWORDS)
+
+
+def _equations(text: str | None) -> list[tuple[str, str]]:
+    """(equation, constraint) of every variable (StringManipulation.py:
+    153-245): the ':' of an if-then-else is not a constraint separator, and a
+    line without an assignment continues the previous equation (multi-line
+    if-then-else statements)."""
+    out = []
+    for line in _lines(text):
+        eq, cons = line, ""
+        if ":" in line:
+            head, tail = line.rsplit(":", 1)
+            if _is_constraint(tail):
+                eq, cons = head.strip(), tail.strip()
+        if not re.findall(r"([\s\w\+\-\*\/\)]+)=([^=])", eq) and out:
+            prev_eq, prev_cons = out[-1]
+            out[-1] = (prev_eq + " " + eq.strip(), (prev_cons + ", " + cons) if prev_cons and cons else prev_cons + cons)
+        else:
+            out.append((eq.strip(), cons))
+    return out
+
+
+_ITE_PH = "__conditional__0"
+
+
+def _logical(expr: str) -> str:
+    """Natural logical operators in sympy form (Equation.py:16-27, applied to
+    every expression before its spaces go)."""
+    return (expr.replace(" and ", " & ").replace(" or ", " | ").replace(" is not ", " != ")
+            .replace(" not ", " Not ").replace(" not(", " Not(").replace(" is ", " == "))
+
+
+def _extract_ite(eq: str):
+    """One if-then-else right of the '=' (ITE.py:88-180): returns the equation
+    with the placeholder __conditional__0 and the nested [cond, then, else]
+    tree, or (eq, None) without a conditional."""
+    if "if " not in eq:
+        return eq, None
+    left, right = eq.split("=", 1)
+    if right.count(":") == 0:
+        raise EquationError(f"conditional statements define then and else values: {eq}")
+    if not right.strip().startswith("if"):
+        raise EquationError(f"the right term must start with the if statement: {eq}")
+    if right.count(":") != 2 * len(re.findall("else", right)):
+        raise EquationError(f"conditional statements must use both : and else: {eq}")
+    tokens = []
+    for item in (a.strip() for a in right.split(":")):
+        if item.startswith("if "):
+            tokens.append(item)
+        elif item.endswith("else"):
+            tokens.append(item[: -len("else")].strip())
+            tokens.append("else")
+        else:
+            tokens.append(item)
+
+    def tree(toks):
+        out = []
+        while toks:
+            if toks[0].startswith("if"):
+                node = [toks.pop(0)[2:].strip(), tree(toks)]
+                if toks and toks[0] == "else":
+                    toks.pop(0)
+                    node.append(tree(toks))
+                else:
+                    raise EquationError(f"an if without else: {eq}")
+                out.append(node)
+            elif toks[0] != "else":
+                out.append(toks.pop(0))
+            else:
+                break
+        return out[0]
+
+    return left + "= " + _ITE_PH, tree(tokens)
+
+
+def _ite_code(tr, node) -> str:
+    if isinstance(node, list):
+        return ("(" + tr.condition(node[0]) + " ? " + _ite_code(tr, node[1]) + " : "
+                + _ite_code(tr, node[2]) + ")")
+    return tr.expression(node)
+
+
 def _split_constraint(line: str) -> tuple[str, str]:
     # "eq : flags" -- the last colon that is not inside an ite(...) / brackets
     depth = 0
@@ -196,8 +288,10 @@ class _Translator:
 
     def parse(self, expr: str, extra: dict | None = None):
         d = dict(self.local_dict)
+        d.setdefault("Not", sp.Not)
         if extra:
             d.update(extra)
+        expr = _logical(expr)
         try:
             return parse_expr(expr, local_dict=d,
                               transformations=(standard_transformations + (convert_xor,)),
@@ -364,8 +458,7 @@ def analyse_neuron(parameters: str = "", equations: str = "", spike: str | None 
                                flags=flags, bounds=bounds, init=_parse_value(val), is_parameter=True,
                                ctype="int" if "int" in flags else precision))
     variables = []
-    for line in _lines(equations):
-        eq, constraint = _split_constraint(line)
+    for eq, constraint in _equations(equations):
         bounds, flags = _flags(constraint)
         vname = _variable_name(eq)
         init = 0.0
@@ -436,7 +529,9 @@ def analyse_neuron(parameters: str = "", equations: str = "", spike: str | None 
     for v in variables:
         if v.eq == "":
             continue            # attribute without an equation (the implicit r of spiking neurons)
-        kind = _equation_type(v.name, v.eq)
+        eq, ite = _extract_ite(v.eq)
+        eq = _logical(eq)
+        kind = _equation_type(v.name, eq)
         method = "explicit"
         for m in ("exponential", "explicit"):
             if m in v.flags:
@@ -446,19 +541,21 @@ def analyse_neuron(parameters: str = "", equations: str = "", spike: str | None 
                 raise EquationError(f"numerical method '{m}' is not supported by this backend")
         v.method = method
         if kind == "ODE":
-            expr = v.eq
+            expr = eq
             expr = expr.replace("=", "- (", 1) + ")" if "=" in expr else expr
             expr = expr.replace(" ", "")
-            if method == "exponential":
+            if method == "exponential" and ite is None:
                 v.pre_loop, v.cpp, v.switch = tr.exponential(v.name, expr, local_names)
             else:
                 v.pre_loop, v.cpp, v.switch = tr.explicit(v.name, expr)
         elif kind == "inc":
-            v.cpp = tr.increment(v.eq)
+            v.cpp = tr.increment(eq)
         elif kind == "simple":
-            v.cpp = tr.assignment(v.eq)
+            v.cpp = tr.assignment(eq)
         else:
             raise EquationError(f"not an equation: {v.eq}")
+        if ite is not None:
+            v.cpp = v.cpp.replace(_ITE_PH, _ite_code(tr, ite))
         # bounds (generator/Utils.py:48-66)
         code = ""
         idx = "%(local_index)s" if v.locality == "local" else "%(global_index)s"
@@ -659,8 +756,7 @@ def analyse_synapse(parameters: str = "", equations: str = "", pre_spike: str | 
                               spiking=bool(pre_spike) or bool(post_spike))
     if not desc.spiking and re.search(r"\bt_(pre|post)\b", (equations or "") + (parameters or "")):
         raise EquationError("t_pre / t_post are spike times: only spiking synapses use them")
-    for line in _lines(equations):
-        eq, constraint = _split_constraint(line)
+    for eq, constraint in _equations(equations):
         bounds, flags = _flags(constraint)
         eq = _deps_to_symbols(eq, desc)
         vname = _variable_name(eq)
@@ -720,22 +816,26 @@ def analyse_synapse(parameters: str = "", equations: str = "", pre_spike: str | 
             desc.event_driven.append((v.name, code))
             v.cpp = ""
             continue
-        kind = _equation_type(v.name, v.eq)
+        eq, ite = _extract_ite(v.eq)
+        eq = _logical(eq)
+        kind = _equation_type(v.name, eq)
         if kind == "ODE":
-            expr = v.eq.replace("=", "- (", 1) + ")"
+            expr = eq.replace("=", "- (", 1) + ")"
             expr = expr.replace(" ", "")
-            if v.method == "exponential":
+            if v.method == "exponential" and ite is None:
                 v.pre_loop, v.cpp, v.switch = tr.exponential(v.name, expr, local_names)
             else:
                 v.pre_loop, v.cpp, v.switch = tr.explicit(v.name, expr)
             if v.pre_loop:
                 desc.preloop.append((v.pre_loop["name"], v.pre_loop["value"]))
         elif kind == "inc":
-            v.cpp = tr.increment(v.eq)
+            v.cpp = tr.increment(eq)
         elif kind == "simple":
-            v.cpp = tr.assignment(v.eq)
+            v.cpp = tr.assignment(eq)
         else:
             raise EquationError(f"not an equation: {v.eq}")
+        if ite is not None:
+            v.cpp = v.cpp.replace(_ITE_PH, _ite_code(tr, ite))
         v.bound_code = _bound_code(tr, v.name, v.bounds)
     for v in desc.variables:
         if v.name not in declared:
