@@ -152,8 +152,53 @@ def _c_name(name: str) -> str:
     return re.sub(r"\W", "_", name)
 
 
-def generate(desc: NeuronDescription) -> ModelSource:
-    """Build the HIP source of the update kernel of one neuron model."""
+_STOP_KERNEL = r"""
+namespace {
+// stop_condition of the population (Population/OpenMPGenerator.py:488-555):
+// the number of neurons for which the condition holds
+template <typename T>
+__global__ __launch_bounds__(AHIP_UPDATE_BLOCK) void stop_count(ahip_pop_args a, int32_t* count) {
+    const int32_t i = blockIdx.x * AHIP_UPDATE_BLOCK + threadIdx.x;
+    const int64_t t = ahip::sim_time(a);
+    const T dt = (T)a.dt;
+    (void)t; (void)dt;
+%(global_loads)s
+%(local_ptrs)s
+    const bool c = i < a.size && (%(cond)s);
+    const unsigned long long m = __ballot(c);
+    if ((threadIdx.x & 63) == 0 && m != 0ull) atomicAdd(count, (int32_t)__popcll(m));
+}
+
+template <typename T>
+int stop_launch(const ahip_pop_args* a, int32_t* count, void* stream) {
+    const int nb = (a->size + AHIP_UPDATE_BLOCK - 1) / AHIP_UPDATE_BLOCK;
+    if (nb > 0) hipLaunchKernelGGL(stop_count<T>, dim3(nb), dim3(AHIP_UPDATE_BLOCK), 0, (hipStream_t)stream, *a, count);
+    return (int)hipGetLastError();
+}
+}  // namespace
+
+extern "C" int ahip_model_stop_f64(const ahip_pop_args* a, int32_t* count, void* s) { return stop_launch<double>(a, count, s); }
+extern "C" int ahip_model_stop_f32(const ahip_pop_args* a, int32_t* count, void* s) { return stop_launch<float>(a, count, s); }
+"""
+
+
+def parse_stop_condition(text: str | None) -> tuple[str | None, str]:
+    """stop_condition "expr : any|all" (Extraction.py extract_stop_condition):
+    (expression or None for "spiked", mode); mode is "any" unless ": all"."""
+    if not text:
+        return None, ""
+    cond, mode = text, "any"
+    if ":" in text:
+        head, tail = text.rsplit(":", 1)
+        if tail.strip() in ("any", "all"):
+            cond, mode = head, tail.strip()
+    cond = cond.strip()
+    return (None if cond == "spiked" else cond), mode
+
+
+def generate(desc: NeuronDescription, stop_condition: str | None = None) -> ModelSource:
+    """Build the HIP source of the update kernel of one neuron model (and of
+    the population's stop condition, when it has one)."""
     slots = []  # (name, local, init, kind)
     for p in desc.parameters:
         # kind 4: a network constant (written by ahip_net_set_constant)
@@ -283,14 +328,20 @@ def generate(desc: NeuronDescription) -> ModelSource:
         "preloop_ptr": preloop_ptr,
         "hash": "%(hash)s",
     }
+    cond, _ = parse_stop_condition(stop_condition)
+    if cond is not None:
+        from .equations import _Translator
+        src_wo_hash += _STOP_KERNEL % {"global_loads": global_loads, "local_ptrs": local_ptrs,
+                                       "cond": subst(_Translator(desc, "double").condition(cond))}
     h = hashlib.sha1((body + src_wo_hash).encode()).hexdigest()[:12]
     source = body + src_wo_hash.replace("%(hash)s", h)
     return ModelSource(desc, source, slots, refractory)
 
 
 def generate_from_text(name: str, parameters: str = "", equations: str = "", spike=None,
-                       reset=None, refractory=None, constants=None) -> ModelSource:
-    return generate(analyse_neuron(parameters, equations, spike, reset, refractory, name, constants=constants))
+                       reset=None, refractory=None, constants=None, stop_condition=None) -> ModelSource:
+    return generate(analyse_neuron(parameters, equations, spike, reset, refractory, name, constants=constants),
+                    stop_condition)
 
 
 # ---------------------------------------------------------------------------

@@ -67,6 +67,12 @@ struct Pop {
     DevBuf hist, hist_count;
     // mean firing rate over fr_window steps (0: off): a fr_window-bit history
     // per neuron (uint32 words, word-major) and the number of spikes in it
+    // stop_condition: 0 none, 1 the model's condition for any neuron, 2 for
+    // all neurons, 3 "spiked : any", 4 "spiked : all"
+    int32_t stop_mode = 0;
+    int (*stop_f64)(const ahip_pop_args*, int32_t*, void*) = nullptr;
+    int (*stop_f32)(const ahip_pop_args*, int32_t*, void*) = nullptr;
+    DevBuf stop_count;
     bool active = true;           // activate(false): no update, no spike gather
     DevBuf ring_tmp;              // inactive with a delay queue: staging of the ring rotation
     int32_t fr_window = 0;

@@ -886,25 +886,71 @@ int ahip_run(ahip_net* net, int64_t nsteps) {
 
 int ahip_step(ahip_net* net) { return ahip_run(net, 1); }
 
+// stop_condition() of one population after a step
+static int stop_reached(ahip_net* net, Pop& p, bool* out) {
+    int32_t c = 0;
+    if (p.stop_mode >= 3) {
+        HIPCHK(hipMemcpy(&c, p.spike_count.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+        *out = p.stop_mode == 3 ? c > 0 : c == p.size;
+        return 0;
+    }
+    auto fn = net->precision == AHIP_F64 ? p.stop_f64 : p.stop_f32;
+    if (!fn) return fail(AHIP_EMODEL, "population %d has no compiled stop condition", p.id);
+    if (!p.stop_count.p) {
+        int rc = dev_alloc(p.stop_count, sizeof(int32_t));
+        if (rc) return rc;
+    }
+    HIPCHK(hipMemsetAsync(p.stop_count.p, 0, sizeof(int32_t), net->stream));
+    ahip_pop_args a = make_args(net, p, 0);
+    int rc = fn(&a, p.stop_count.as<int32_t>(), (void*)net->stream);
+    if (rc) return fail(AHIP_EHIP, "stop condition of population %d failed: %s", p.id,
+                        hipGetErrorString((hipError_t)rc));
+    HIPCHK(hipMemcpyAsync(&c, p.stop_count.p, sizeof(int32_t), hipMemcpyDeviceToHost, net->stream));
+    HIPCHK(hipStreamSynchronize(net->stream));
+    *out = p.stop_mode == 1 ? c > 0 : c == p.size;
+    return 0;
+}
+
+int ahip_pop_set_stop_condition(ahip_net* net, int32_t pop, int32_t mode) {
+    Pop* p = get_pop(net, pop);
+    if (!p || mode < 0 || mode > 4) return fail(AHIP_EINVAL, "bad population or stop mode");
+    if ((mode == 1 || mode == 2) && !(net->precision == AHIP_F64 ? p->stop_f64 : p->stop_f32))
+        return fail(AHIP_EMODEL, "the model library of population %d has no stop condition", pop);
+    if (mode >= 3 && !p->model->spiking) return fail(AHIP_EINVAL, "\"spiked\" needs a spiking population");
+    p->stop_mode = mode;
+    return 0;
+}
+
 int ahip_run_until(ahip_net* net, int64_t max_steps, const int32_t* pops, int32_t npops,
                    int32_t and_or, int64_t* done) {
     if (!net || !done || (npops > 0 && !pops)) return fail(AHIP_EINVAL, "null argument");
-    for (int32_t k = 0; k < npops; k++) {
-        Pop* p = get_pop(net, pops[k]);
-        if (!p || !p->model->spiking) return fail(AHIP_EINVAL, "run_until needs spiking populations");
+    for (int32_t k = 0; k < npops; k++)
+        if (!get_pop(net, pops[k])) return fail(AHIP_EINVAL, "bad population %d", pops[k]);
+    // without any stop condition in the network: run(steps) (OpenMPBaseTemplate.py:485-489)
+    bool any_cond = false;
+    for (auto& pp : net->pops) any_cond = any_cond || pp->stop_mode != 0;
+    if (!any_cond) {
+        int rc = ahip_run(net, max_steps);
+        if (rc) return rc;
+        *done = max_steps;
+        return 0;
     }
+    // step by step; stop starts as and_or and folds the listed populations'
+    // conditions with && (and_or = 1) or || (OpenMPBaseTemplate.py:490-517)
     int64_t n = 0;
     while (n < max_steps) {
         int rc = ahip_run(net, 1);
         if (rc) return rc;
         n++;
-        int hits = 0;
+        bool stop = and_or != 0;
         for (int32_t k = 0; k < npops; k++) {
-            int32_t c = 0;
-            HIPCHK(hipMemcpy(&c, net->pops[pops[k]]->spike_count.p, sizeof(int32_t), hipMemcpyDeviceToHost));
-            if (c > 0) hits++;
+            Pop& p = *net->pops[pops[k]];
+            if (p.stop_mode == 0) continue;
+            bool c = false;
+            if ((rc = stop_reached(net, p, &c))) return rc;
+            stop = and_or != 0 ? (stop && c) : (stop || c);
         }
-        if (npops > 0 && ((and_or == 0 && hits > 0) || (and_or != 0 && hits == npops))) break;
+        if (stop && npops > 0) break;
     }
     *done = n;
     return 0;
@@ -932,6 +978,9 @@ int ahip_pop_create(ahip_net* net, int32_t size, const char* model_lib, int32_t*
     p->lib_path = model_lib;
     p->dl = dl;
     p->model = d;
+    // optional: the population's stop condition compiled into the library
+    p->stop_f64 = (int (*)(const ahip_pop_args*, int32_t*, void*))dlsym(dl, "ahip_model_stop_f64");
+    p->stop_f32 = (int (*)(const ahip_pop_args*, int32_t*, void*))dlsym(dl, "ahip_model_stop_f32");
     const size_t es = net->elem();
     p->slots.resize(d->n_slots);
     p->slot_host_global.assign(d->n_slots, 0.0);

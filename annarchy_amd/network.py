@@ -79,12 +79,14 @@ def _constant_names(net) -> list:
     return sorted(net._constants) if net is not None else sorted(_GLOBAL_CONSTANTS)
 
 
-def model_library(neuron: Neuron, allow_build: bool = True, constants=None) -> Path:
+def model_library(neuron: Neuron, allow_build: bool = True, constants=None, stop_condition=None) -> Path:
     """Path of the compiled model library of `neuron` (built on demand when
     hipcc is available and `allow_build`); `constants`: the names of the
-    network constants its equations may read."""
+    network constants its equations may read; `stop_condition`: the
+    population's stop condition, compiled into the same library."""
     src = codegen.generate_from_text(neuron.name, neuron.parameters, neuron.equations,
-                                     neuron.spike, neuron.reset, neuron.refractory, constants=constants)
+                                     neuron.spike, neuron.reset, neuron.refractory, constants=constants,
+                                     stop_condition=stop_condition)
     path = _build.model_lib_path(neuron.name, src.source)
     if path.exists():
         return path
@@ -106,8 +108,10 @@ def synapse_library(synapse: Synapse, allow_build: bool = True, constants=None) 
 
 
 class Population:
-    def __init__(self, net: "Network", geometry, neuron: Neuron, name: str | None = None):
+    def __init__(self, net: "Network", geometry, neuron: Neuron, name: str | None = None,
+                 stop_condition: str | None = None):
         self.net = net
+        self.stop_condition = stop_condition
         self.geometry = (geometry,) if isinstance(geometry, int) else tuple(geometry)
         self.size = int(np.prod(self.geometry))
         self.neuron = neuron
@@ -122,12 +126,13 @@ class Population:
     # -- attribute access (Population.py:375-538) --------------------------
     def __getattr__(self, name):
         if name.startswith("_") or name in ("net", "geometry", "size", "neuron", "id", "ranks",
-                                             "spiking", "name"):
+                                             "spiking", "name", "stop_condition"):
             raise AttributeError(name)
         return self.get(name)
 
     def __setattr__(self, name, value):
-        if name in ("net", "geometry", "size", "neuron", "name", "id", "ranks", "spiking") or name.startswith("_"):
+        if name in ("net", "geometry", "size", "neuron", "name", "id", "ranks", "spiking",
+                    "stop_condition") or name.startswith("_"):
             object.__setattr__(self, name, value)
         else:
             self.set(name, value)
@@ -903,10 +908,13 @@ class Network:
             raise KeyError(f"no constant named {name}")
         return self._constants[name]
 
-    def create(self, geometry, neuron: Neuron, name: str | None = None) -> Population:
+    def create(self, geometry, neuron: Neuron, name: str | None = None,
+               stop_condition: str | None = None) -> Population:
+        """Population(geometry, neuron, name, stop_condition): `stop_condition`
+        ("expr : any|all", or "spiked : any|all") ends simulate_until()."""
         if self.compiled:
             raise RuntimeError("the network is already compiled")
-        p = Population(self, geometry, neuron, name)
+        p = Population(self, geometry, neuron, name, stop_condition)
         self.populations.append(p)
         return p
 
@@ -942,7 +950,7 @@ class Network:
         if graph_batch:
             _lib.check(L.ahip_set_graph_batch(h, graph_batch))
         for p in self.populations:
-            lib_path = model_library(p.neuron, allow_build, _constant_names(self))
+            lib_path = model_library(p.neuron, allow_build, _constant_names(self), p.stop_condition)
             pid = C.c_int32()
             _lib.check(L.ahip_pop_create(h, p.size, str(lib_path).encode(), C.byref(pid)))
             p.id = pid.value
@@ -959,6 +967,10 @@ class Network:
                 p.compute_firing_rate(p._fr_window)
             if not p.enabled:
                 p._upload_active()
+            if p.stop_condition:
+                cond, mode = codegen.parse_stop_condition(p.stop_condition)
+                code = (1 if mode == "any" else 2) if cond is not None else (3 if mode == "any" else 4)
+                _lib.check(L.ahip_pop_set_stop_condition(h, p.id, code))
         # constants: the libraries hold 0, the network writes the values
         for c in self._constants.values():
             _lib.check(L.ahip_net_set_constant(h, c.name.encode(), c.value, None))
@@ -973,6 +985,19 @@ class Network:
 
     def run(self, nsteps: int):
         _lib.check(_lib.lib().ahip_run(self._h, int(nsteps)))
+
+    def simulate_until(self, max_duration: float, population, operator: str = "and") -> float:
+        """core/Simulate.py:173-229: runs at most ceil(max_duration/dt) steps
+        and stops after the first step at which the stop conditions of the
+        given population(s), combined with `operator` ('and' / 'or'), hold.
+        Returns the number of steps divided by dt, as the reference does."""
+        nb_steps = math.ceil(float(max_duration) / self.dt)
+        pops = population if isinstance(population, list) else [population]
+        ids = np.array([_base(p).id for p in pops], dtype=np.int32)
+        done = C.c_int64()
+        _lib.check(_lib.lib().ahip_run_until(self._h, nb_steps, _lib.ptr(ids, C.c_int32), ids.size,
+                                             1 if operator == "and" else 0, C.byref(done)))
+        return float(done.value) / self.dt
 
     def step(self):
         _lib.check(_lib.lib().ahip_step(self._h))

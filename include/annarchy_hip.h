@@ -91,11 +91,18 @@ int ahip_net_set_constant(ahip_net* net, const char* name, double value, int32_t
 int ahip_run(ahip_net* net, int64_t nsteps);
 /* step() */
 int ahip_step(ahip_net* net);
-/* run_until(steps, populations, and_or): stops at the end of the first step
- * where any (and_or=0) or all (and_or=1) listed spiking populations emitted a
- * spike; *done = steps simulated.  (Simulate.py:173-266) */
+/* run_until(steps, populations, or_and) (Simulate.py:173-266,
+ * OpenMPBaseTemplate.py:485-517): without a stop condition anywhere in the
+ * network, run(steps); otherwise step by step until the stop conditions of
+ * the listed populations, folded with && (and_or = 1) or || (and_or = 0),
+ * hold after a step.  *done = steps simulated. */
 int ahip_run_until(ahip_net* net, int64_t max_steps, const int32_t* pops, int32_t npops,
                    int32_t and_or, int64_t* done);
+/* stop_condition of a population (Population/OpenMPGenerator.py:488-555):
+ * mode 0 none, 1 / 2 the condition compiled into its model library
+ * (ahip_model_stop_f64/_f32) holds for any / all neurons, 3 / 4
+ * "spiked : any" / "spiked : all". */
+int ahip_pop_set_stop_condition(ahip_net* net, int32_t pop, int32_t mode);
 /* get_time() / set_time(t) */
 int ahip_get_time(ahip_net* net, int64_t* t);
 int ahip_set_time(ahip_net* net, int64_t t);
