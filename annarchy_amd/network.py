@@ -691,6 +691,12 @@ class Projection:
         rp = self.csr.row_ptr
         return [int(c) for c in self.csr.col_idx[rp[post_rank]:rp[post_rank + 1]]]
 
+    @property
+    def dendrites(self) -> list:
+        """The dendrites of every post-synaptic neuron that receives synapses
+        (core/Projection.py dendrites)."""
+        return [Dendrite(self, r) for r in self.post_ranks]
+
     def dendrite(self, post_rank: int) -> "Dendrite":
         if self.dendrite_size(post_rank) == 0:
             raise IndexError(f"post-synaptic neuron {post_rank} receives no synapse")
