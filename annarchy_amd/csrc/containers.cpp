@@ -305,7 +305,7 @@ int build_layout(const Proj& p, int32_t precision, const ahip_format_opts& o, Ra
             // of a row, -1 for the others.
             const size_t es = precision == AHIP_F64 ? 8 : 4;
             int32_t pc = o.panel_cols > 0 ? o.panel_cols : (int32_t)(32768 / es);
-            if (pc > 65535 || (size_t)(pc + 1) * es > 64 * 1024) { err = "panel exceeds 64 KiB of LDS or 65535 columns"; return AHIP_EINVAL; }
+            if (pc > 65535 || (size_t)(pc + 1) * es > 160 * 1024) { err = "panel exceeds 160 KiB of LDS or 65535 columns"; return AHIP_EINVAL; }
             pc = equal_panels(p.n_cols, pc);
             const int32_t rb_rows = o.chunk > 0 ? o.chunk : 256;
             if (rb_rows < 64 || rb_rows > 8192 || rb_rows % 64) { err = "tile rows must be a multiple of 64 in 64..8192"; return AHIP_EINVAL; }

@@ -90,16 +90,19 @@ def main():
     ap.add_argument("--degree", type=int, default=1000)
     ap.add_argument("--lognormal", action="store_true")
     ap.add_argument("--precision", default="double", choices=["double", "float"])
-    # default: sliced tiles (1024-row blocks x 8191-column panels staged in
-    # LDS, sorted 64-row slices, one lane per row, 2 steps of 4 synapses in
-    # flight, 8 waves per workgroup): the fastest FP64 layout of the sweep
+    # default: sliced tiles (2048-row blocks, 1024 in FP32, x 20479-column
+    # panels staged in 160 KiB of LDS, sorted 64-row slices, one lane per row,
+    # 2 steps of 4 synapses in flight, 8 waves per workgroup): the fastest
+    # layout of the sweep
     ap.add_argument("--format", default="sell_panel")
     ap.add_argument("--lanes", type=int, default=2,
                     help="csr/csr_vec/csr_panel: lanes per row; sell: threads per row; sell_panel: steps in flight")
     ap.add_argument("--sigma", type=int, default=1, help="sell: sorting window")
     ap.add_argument("--waves", type=int, default=8, help="sell_panel: waves per workgroup")
-    ap.add_argument("--panel-cols", type=int, default=8191)
-    ap.add_argument("--chunk", type=int, default=1024)
+    ap.add_argument("--panel-cols", type=int, default=20479,
+                    help="sell_panel: columns per panel at most (20479: 160 KiB of LDS in FP64)")
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="sell_panel: rows per tile (0: 2048 in FP64, 1024 in FP32)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2", "c3", "stdp"],
@@ -125,9 +128,10 @@ def main():
     net = ah.Network(dt=1.0, seed=42, precision=args.precision)
     pop = net.create(args.neurons, ah.RateNoisy)
     proj = net.connect(pop, pop, "exc")
+    chunk = args.chunk or (2048 if args.precision == "double" else 1024)
     proj.from_csr(csr, storage_format=args.format, lanes=args.lanes,
                   sigma=args.waves if args.format == "sell_panel" else args.sigma,
-                  panel_cols=args.panel_cols, chunk=args.chunk)
+                  panel_cols=args.panel_cols, chunk=chunk)
     net.compile(allow_build=False)
     pop.r = np.random.default_rng(7).random(args.neurons)
     log("network uploaded")

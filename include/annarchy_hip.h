@@ -57,7 +57,9 @@ typedef struct ahip_format_opts {
     int32_t sigma;            /* SELL: sorting window in rows (1 = no sorting); SELL_PANEL: waves per
                                  workgroup (4 or 8; 0 or 1 = 4) */
     int32_t panel_cols;       /* CSR_PANEL / SELL_PANEL: at most this many columns per panel (0 =
-                                 automatic); the panels are then made equally wide */
+                                 automatic); the panels are then made equally wide.  A panel
+                                 is staged in LDS: at most 64 KiB (CSR_PANEL) or 160 KiB
+                                 (SELL_PANEL, one workgroup per CU at that size) */
 } ahip_format_opts;
 
 /* ---- errors / device ------------------------------------------------------ */

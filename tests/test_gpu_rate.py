@@ -162,6 +162,26 @@ def test_sell_panel_heavy_rows(gpu):
         np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-13 * np.max(np.abs(ref)))
 
 
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_sell_panel_large_lds_panels(gpu, precision):
+    # panels of up to 160 KiB of LDS (20000 FP64 / 40000 FP32 columns, one
+    # workgroup per CU), the layout of the 1e8-synapse benchmark
+    n = 45000
+    csr = _random_csr(n, 40, 31, ragged=True)
+    r0 = np.random.default_rng(8).random(n)
+    ref = orc.spmv_csr(csr.row_ptr, csr.col_idx, csr.weights(), r0)
+    pc = 20479 if precision == "double" else 40959
+    fmt = dict(storage_format="sell_panel", lanes=2, sigma=8, panel_cols=pc, chunk=2048)
+    net, pop, proj = _net_one_step(csr, r0, fmt, precision)
+    assert proj.get_format()["panel_cols"] > 8192   # more than 64 KiB of LDS in FP64
+    got = pop.get("_sum_exc").reshape(-1)
+    if precision == "double":
+        np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-13 * np.max(np.abs(ref)))
+    else:
+        # FP32 with compensated sums: within 1e-5 of the FP64 oracle
+        assert _rel_err(got, ref) < 1e-5
+
+
 def test_sell_panel_graph_run_matches_oracle(gpu):
     n, steps = 3000, 48
     csr = _random_csr(n, 120, 6, ragged=True)
