@@ -266,9 +266,35 @@ class Population:
         _lib.check(_lib.lib().ahip_pop_set_int(self.net._h, self.id, b"refractory",
                                                _lib.ptr(refs, C.c_int32), self.size))
 
-    def __getitem__(self, sl):
-        ranks = self.ranks[sl]
-        return PopulationView(self, np.atleast_1d(ranks))
+    def __getitem__(self, key):
+        """pop[rank], pop[list or 1-D array of ranks], pop[start:stop:step] or,
+        with the population's geometry, pop[i, j, ...] with ints or slices
+        per dimension (core/Population.py:782-880; ranks in row-major order)."""
+        if isinstance(key, tuple):
+            if len(key) != len(self.geometry):
+                raise IndexError(f"{len(key)} coordinates for a {len(self.geometry)}-D population")
+            axes = [np.atleast_1d(np.arange(d)[k]) for k, d in zip(key, self.geometry)]
+            grid = np.meshgrid(*axes, indexing="ij")
+            ranks = np.ravel_multi_index(tuple(g.reshape(-1) for g in grid), self.geometry)
+            return PopulationView(self, ranks)
+        if isinstance(key, (int, np.integer)):
+            if not -self.size <= int(key) < self.size:
+                raise IndexError(f"rank {key} out of range")
+            return PopulationView(self, np.array([int(key) % self.size]))
+        return PopulationView(self, np.atleast_1d(self.ranks[key]))
+
+    def rank_from_coordinates(self, coord) -> int:
+        """Rank of the neuron at these coordinates (row-major)."""
+        return int(np.ravel_multi_index(tuple(int(c) for c in np.atleast_1d(coord)), self.geometry))
+
+    def coordinates_from_rank(self, rank: int) -> tuple:
+        """Coordinates of the neuron of this rank (row-major)."""
+        return tuple(int(c) for c in np.unravel_index(int(rank), self.geometry))
+
+    def neuron(self, *coord) -> "PopulationView":
+        """The one-neuron view of a rank or of coordinates."""
+        rank = coord[0] if len(coord) == 1 else self.rank_from_coordinates(coord)
+        return self[int(rank)]
 
     # -- spikes ---------------------------------------------------------------
     def spikes(self) -> np.ndarray:
@@ -290,14 +316,79 @@ class Population:
 
 
 class PopulationView:
+    """A subset of a population's neurons (core/PopulationView.py): usable as
+    the pre or post side of a projection and for attribute access."""
+
+    _OWN = ("population", "ranks", "size", "geometry")
+
     def __init__(self, pop: Population, ranks: np.ndarray):
-        self.population = pop
-        self.ranks = np.asarray(ranks)
-        self.size = len(self.ranks)
+        object.__setattr__(self, "population", pop)
+        object.__setattr__(self, "ranks", np.asarray(ranks))
+        object.__setattr__(self, "size", len(self.ranks))
+        object.__setattr__(self, "geometry", (self.size,))
 
     @property
     def spiking(self):
         return self.population.spiking
+
+    @property
+    def rank(self):
+        """The rank of a one-neuron view; the ranks otherwise."""
+        return int(self.ranks[0]) if self.size == 1 else self.ranks
+
+    def __len__(self):
+        return self.size
+
+    def __add__(self, other):
+        """The union of two views of the same population, in ascending rank
+        order (core/PopulationView.py:320-333)."""
+        if not isinstance(other, PopulationView) or other.population is not self.population:
+            raise ValueError("only views of the same population can be added")
+        return PopulationView(self.population, np.array(sorted(set(self.ranks.tolist()) | set(other.ranks.tolist()))))
+
+    def get(self, name):
+        """The values of a local attribute for the view's neurons; a
+        population-wide attribute is returned as it is."""
+        val = self.population.get(name)
+        if np.ndim(val) == 0:
+            return val
+        return np.asarray(val).reshape(-1)[self.ranks]
+
+    def set(self, values: dict):
+        """Sets local attributes of the view's neurons (one value or one per
+        neuron); population-wide attributes cannot be set from a view."""
+        pop = self.population
+        for name, value in values.items():
+            info = pop._slot_info()
+            if name not in info:
+                raise AttributeError(f"population {pop.name} has no attribute {name}")
+            if not info[name][1]:
+                raise ValueError(f"{name} is population-wide: set it on the population")
+            if isinstance(value, conn.RandomDistribution):
+                value = value.get_values(self.size)
+            v = np.asarray(value, dtype=np.float64).reshape(-1)
+            if v.size not in (1, self.size):
+                raise ValueError(f"{name}: one value or {self.size} values expected")
+            cur = np.array(pop.get(name), dtype=np.float64).reshape(-1)
+            if cur.size != pop.size:
+                cur = np.full(pop.size, float(cur.reshape(-1)[0]))
+            cur[self.ranks] = v
+            pop.set(name, cur)
+
+    def __getattr__(self, name):
+        if name.startswith("_"):
+            raise AttributeError(name)
+        if name in self.population._slot_info():
+            return self.get(name)
+        raise AttributeError(name)
+
+    def __setattr__(self, name, value):
+        if name in PopulationView._OWN or name.startswith("_"):
+            object.__setattr__(self, name, value)
+        elif name in self.population._slot_info():
+            self.set({name: value})
+        else:
+            object.__setattr__(self, name, value)
 
 
 def _base(obj):
