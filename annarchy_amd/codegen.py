@@ -339,9 +339,10 @@ def generate(desc: NeuronDescription, stop_condition: str | None = None) -> Mode
 
 
 def generate_from_text(name: str, parameters: str = "", equations: str = "", spike=None,
-                       reset=None, refractory=None, constants=None, stop_condition=None) -> ModelSource:
-    return generate(analyse_neuron(parameters, equations, spike, reset, refractory, name, constants=constants),
-                    stop_condition)
+                       reset=None, refractory=None, constants=None, stop_condition=None,
+                       incoming_targets=None) -> ModelSource:
+    return generate(analyse_neuron(parameters, equations, spike, reset, refractory, name, constants=constants,
+                                   incoming_targets=incoming_targets), stop_condition)
 
 
 # ---------------------------------------------------------------------------

@@ -444,9 +444,11 @@ def _constant_params(constants, declared, texts, precision: str = "double") -> l
 
 def analyse_neuron(parameters: str = "", equations: str = "", spike: str | None = None,
                    reset: str | None = None, refractory=None, name: str = "neuron",
-                   precision: str = "double", constants=None) -> NeuronDescription:
+                   precision: str = "double", constants=None, incoming_targets=None) -> NeuronDescription:
     """Analyse a neuron definition into C snippets (AnalyseNeuron.py:27-380).
-    `constants`: names of the network constants an equation may read."""
+    `constants`: names of the network constants an equation may read;
+    `incoming_targets`: targets of the rate-coded projections into the population,
+    each of which gets its sum even when no equation reads it."""
     params = []
     for line in _lines(parameters):
         eq, constraint = _split_constraint(line)
@@ -487,6 +489,10 @@ def analyse_neuron(parameters: str = "", equations: str = "", spike: str | None 
         for m in re.finditer(r"sum\(\s*(\w+)\s*\)", v.eq):
             if m.group(1) not in targets:
                 targets.append(m.group(1))
+    if not spiking:
+        for tgt in incoming_targets or []:
+            if tgt not in targets:
+                targets.append(tgt)
     if spiking:
         for v in variables:
             if v.name.startswith("g_") and v.name[2:] not in targets:

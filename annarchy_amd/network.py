@@ -79,14 +79,15 @@ def _constant_names(net) -> list:
     return sorted(net._constants) if net is not None else sorted(_GLOBAL_CONSTANTS)
 
 
-def model_library(neuron: Neuron, allow_build: bool = True, constants=None, stop_condition=None) -> Path:
+def model_library(neuron: Neuron, allow_build: bool = True, constants=None, stop_condition=None,
+                  incoming_targets=None) -> Path:
     """Path of the compiled model library of `neuron` (built on demand when
     hipcc is available and `allow_build`); `constants`: the names of the
     network constants its equations may read; `stop_condition`: the
     population's stop condition, compiled into the same library."""
     src = codegen.generate_from_text(neuron.name, neuron.parameters, neuron.equations,
                                      neuron.spike, neuron.reset, neuron.refractory, constants=constants,
-                                     stop_condition=stop_condition)
+                                     stop_condition=stop_condition, incoming_targets=incoming_targets)
     path = _build.model_lib_path(neuron.name, src.source)
     if path.exists():
         return path
@@ -142,7 +143,8 @@ class Population:
             src = codegen.generate_from_text(self.neuron.name, self.neuron.parameters,
                                              self.neuron.equations, self.neuron.spike,
                                              self.neuron.reset, self.neuron.refractory,
-                                             constants=_constant_names(self.net))
+                                             constants=_constant_names(self.net),
+                                             incoming_targets=self.net._incoming_targets(self))
             # the slots mirroring network constants are not population attributes
             self._attrs = {s[0]: s for s in src.slots if s[3] != 4}
         return self._attrs
@@ -557,6 +559,9 @@ class Projection:
     # -- connectors (core/ConnectorMethods.py) ---------------------------------
     def _store(self, kind, *args, storage_format=None, **fmt):
         self._connector = (kind, args)
+        # force_multiple_weights: one weight per synapse even when all are equal
+        self._force_multi = bool(fmt.pop("force_multiple_weights", False))
+        fmt.pop("storage_order", None)   # post_to_pre is the only order here
         if storage_format is not None or fmt:
             self._format = _lib.opts(storage_format or "csr", fmt.get("lanes", 0),
                                      fmt.get("chunk", 0), fmt.get("sigma", 1), fmt.get("panel_cols", 0))
@@ -724,6 +729,8 @@ class Projection:
             self._generate_on_device()
         else:
             self.csr = self._build_csr()
+            if getattr(self, "_force_multi", False) and self.csr.w is None:
+                self.csr.w = np.full(self.csr.nnz, float(self.csr.w_const))
             c = self.csr
             w = c.w
             _lib.check(L.ahip_proj_init_csr(self.net._h, self.id, c.n_rows, c.n_cols,
@@ -873,11 +880,57 @@ class Dendrite:
         if loc == "global":
             raise AttributeError(f"{name} is one value per projection: set it on the projection")
         v = np.array(self.proj.get(name), dtype=np.float64).reshape(-1)
+        if v.size == 1 and loc == "local":
+            v = np.full(self.proj.nb_synapses, float(v[0]))
         if loc == "local":
             v[self._slice()] = value
         else:
             v[self.post_rank] = value
         self.proj.set(name, v)
+
+    @property
+    def rank(self) -> int:
+        return self.post_rank
+
+    def get(self, name):
+        return getattr(self, name)
+
+    def set(self, values: dict):
+        for name, value in values.items():
+            setattr(self, name, value)
+
+    def __len__(self):
+        return self.size
+
+    def __getitem__(self, index: int) -> "Synapse_":
+        """The synapse at position `index` of the dendrite (pre-rank order)."""
+        if not -self.size <= int(index) < self.size:
+            raise IndexError(f"the dendrite has {self.size} synapses")
+        return Synapse_(self, int(index) % self.size)
+
+
+class Synapse_:
+    """One synapse of a dendrite (core/Dendrite.py IndividualSynapse): its
+    per-synapse attributes by name."""
+
+    def __init__(self, dendrite: Dendrite, index: int):
+        object.__setattr__(self, "dendrite", dendrite)
+        object.__setattr__(self, "index", index)
+
+    @property
+    def pre_rank(self) -> int:
+        return self.dendrite.pre_ranks[self.index]
+
+    def __getattr__(self, name):
+        if name.startswith("_"):
+            raise AttributeError(name)
+        v = getattr(self.dendrite, name)
+        return float(v[self.index]) if np.ndim(v) else v
+
+    def __setattr__(self, name, value):
+        v = np.array(getattr(self.dendrite, name), dtype=np.float64).reshape(-1)
+        v[self.index] = value
+        setattr(self.dendrite, name, v)
 
 
 def _sizes(obj) -> int:
@@ -995,6 +1048,12 @@ class Network:
             Constant(c.name, c.value, net=self)
         _NETWORKS.add(self)
 
+    def _incoming_targets(self, pop) -> list:
+        """Targets of the rate-coded projections into `pop`: the population
+        holds a sum for each, read by its equations or not."""
+        return sorted({pr.target for pr in self.projections
+                       if _base(pr.post) is pop and not _base(pr.pre).spiking})
+
     def constant(self, name: str, value: float) -> Constant:
         """A constant of this network only (core/Network.py:482-500)."""
         return Constant(name, value, net=self)
@@ -1047,7 +1106,8 @@ class Network:
         if graph_batch:
             _lib.check(L.ahip_set_graph_batch(h, graph_batch))
         for p in self.populations:
-            lib_path = model_library(p.neuron, allow_build, _constant_names(self), p.stop_condition)
+            lib_path = model_library(p.neuron, allow_build, _constant_names(self), p.stop_condition,
+                                     self._incoming_targets(p))
             pid = C.c_int32()
             _lib.check(L.ahip_pop_create(h, p.size, str(lib_path).encode(), C.byref(pid)))
             p.id = pid.value
