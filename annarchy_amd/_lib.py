@@ -84,6 +84,9 @@ _SIGS = {
     "ahip_pop_reset_delays": ([_P, _I32], _I32),
     "ahip_pop_compute_firing_rate": ([_P, _I32, _D], _I32),
     "ahip_pop_set_active": ([_P, _I32, _I32], _I32),
+    "ahip_monitor_set_period": ([_P, _I32, _I32, _I64], _I32),
+    "ahip_monitor_pause": ([_P, _I32], _I32),
+    "ahip_monitor_resume": ([_P, _I32], _I32),
     "ahip_pop_set_stop_condition": ([_P, _I32, _I32], _I32),
     "ahip_net_set_constant": ([_P, C.c_char_p, _D, _PI32], _I32),
     "ahip_pop_size_in_bytes": ([_P, _I32, _PI64], _I32),

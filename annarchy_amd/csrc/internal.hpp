@@ -144,7 +144,15 @@ struct Monitor {
     bool spikes = false;
     bool after_psp = false;       // targets are recorded right after compute_psp
     int64_t capacity = 0;
+    // spikes: the step of the first recorded list.  Values: the first step of
+    // the current recording segment at which (t - phase) mod period == 0;
+    // sample k of the segment (step t_start + k * period) goes to row
+    // seg_base + k.  A paused monitor records nothing.
     int64_t t_start = 0;
+    int32_t period = 1;
+    int64_t phase = 0;
+    int64_t seg_base = 0;
+    bool paused = false;
     DevBuf data;                  // values [capacity][size] (T), or spike indices
     DevBuf counts;                // spikes: int32[capacity]
     DevBuf fill;                  // spikes: int64[2], indices written before recorded step s in fill[s & 1]

@@ -21,9 +21,12 @@ __global__ void advance_time(int64_t* t, int64_t n) {
 template <typename T>
 __global__ void record_values(const T* __restrict__ src, int32_t size, T* __restrict__ dst,
                               const int64_t* __restrict__ t_base, int32_t step, int64_t t_start,
-                              int64_t capacity) {
-    const int64_t slot = *t_base + step - t_start;
-    if (slot < 0 || slot >= capacity) return;
+                              int32_t period, int64_t base, int64_t capacity) {
+    // every period steps from t_start on, into rows base, base + 1, ...
+    const int64_t d = *t_base + step - t_start;
+    if (d < 0 || d % period != 0) return;
+    const int64_t slot = base + d / period;
+    if (slot >= capacity) return;
     T* out = dst + slot * (int64_t)size;
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < size; i += gridDim.x * blockDim.x)
         out[i] = src[i];
@@ -78,10 +81,12 @@ int launch_record_values(const Monitor& m, const void* src, int32_t size, int32_
     const int nb = grid_for(size);
     if (elem == 8)
         hipLaunchKernelGGL(record_values<double>, dim3(nb), dim3(256), 0, s, (const double*)src,
-                           size, m.data.as<double>(), t_base, step, m.t_start, m.capacity);
+                           size, m.data.as<double>(), t_base, step, m.t_start, m.period, m.seg_base,
+                           m.capacity);
     else
         hipLaunchKernelGGL(record_values<float>, dim3(nb), dim3(256), 0, s, (const float*)src,
-                           size, m.data.as<float>(), t_base, step, m.t_start, m.capacity);
+                           size, m.data.as<float>(), t_base, step, m.t_start, m.period, m.seg_base,
+                           m.capacity);
     return (int)hipGetLastError();
 }
 

@@ -588,7 +588,7 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
     // 2. record targets
     for (auto& mp : net->mons) {
         Monitor& m = *mp;
-        if (!m.after_psp) continue;
+        if (!m.after_psp || m.paused) continue;
         Pop& p = *net->pops[m.pop];
         LCHK(launch_record_values(m, p.slots[m.slot].p, p.size, (int32_t)es,
                                   net->t_dev.as<int64_t>(), step, s));
@@ -650,7 +650,7 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
     // 5. record variables and spikes
     for (auto& mp : net->mons) {
         Monitor& m = *mp;
-        if (m.after_psp) continue;
+        if (m.after_psp || m.paused) continue;
         Pop& p = *net->pops[m.pop];
         if (m.spikes) {
             // an inactive population keeps its last spike list, which its
@@ -1956,11 +1956,67 @@ int ahip_monitor_create(ahip_net* net, int32_t pop, const char* attr, int64_t ca
     return 0;
 }
 
+// first step >= t at which (step - phase) mod period == 0
+static int64_t first_record_step(int64_t t, int64_t phase, int32_t period) {
+    const int64_t r = ((t - phase) % period + period) % period;
+    return r == 0 ? t : t + (period - r);
+}
+
+// rows a value monitor holds after the steps before `now`
+static int64_t recorded_rows(const Monitor& m, int64_t now) {
+    int64_t n = m.seg_base;
+    if (!m.paused && now > m.t_start) n += (now - 1 - m.t_start) / m.period + 1;
+    return std::max<int64_t>(0, std::min<int64_t>(n, m.capacity));
+}
+
 int ahip_monitor_steps(ahip_net* net, int32_t mon, int64_t* steps) {
     Monitor* m = get_mon(net, mon);
     if (!m || !steps) return fail(AHIP_EINVAL, "bad monitor");
-    int64_t s = net->t - m->t_start;
-    *steps = std::max<int64_t>(0, std::min<int64_t>(s, m->capacity));
+    if (m->spikes) {
+        int64_t s = net->t - m->t_start;
+        *steps = std::max<int64_t>(0, std::min<int64_t>(s, m->capacity));
+    } else {
+        *steps = recorded_rows(*m, net->t);
+    }
+    return 0;
+}
+
+int ahip_monitor_set_period(ahip_net* net, int32_t mon, int32_t period, int64_t phase) {
+    Monitor* m = get_mon(net, mon);
+    if (!m || period < 1) return fail(AHIP_EINVAL, "bad monitor or period");
+    if (m->spikes && period != 1) return fail(AHIP_EINVAL, "spike monitors record every step");
+    HIPCHK(hipStreamSynchronize(net->stream));
+    if (!m->spikes) {
+        m->seg_base = recorded_rows(*m, net->t);
+        m->period = period;
+        m->phase = phase;
+        m->t_start = first_record_step(net->t, phase, period);
+    }
+    invalidate(net);
+    return 0;
+}
+
+int ahip_monitor_pause(ahip_net* net, int32_t mon) {
+    Monitor* m = get_mon(net, mon);
+    if (!m) return fail(AHIP_EINVAL, "bad monitor");
+    if (m->spikes) return fail(AHIP_EINVAL, "spike monitors cannot be paused");
+    if (m->paused) return 0;
+    HIPCHK(hipStreamSynchronize(net->stream));
+    m->seg_base = recorded_rows(*m, net->t);
+    m->paused = true;
+    invalidate(net);
+    return 0;
+}
+
+int ahip_monitor_resume(ahip_net* net, int32_t mon) {
+    Monitor* m = get_mon(net, mon);
+    if (!m) return fail(AHIP_EINVAL, "bad monitor");
+    if (m->spikes) return fail(AHIP_EINVAL, "spike monitors cannot be paused");
+    if (!m->paused) return 0;
+    HIPCHK(hipStreamSynchronize(net->stream));
+    m->paused = false;
+    m->t_start = first_record_step(net->t, m->phase, m->period);
+    invalidate(net);
     return 0;
 }
 
@@ -2006,10 +2062,13 @@ int ahip_monitor_clear(ahip_net* net, int32_t mon) {
     Monitor* m = get_mon(net, mon);
     if (!m) return fail(AHIP_EINVAL, "bad monitor");
     HIPCHK(hipStreamSynchronize(net->stream));
-    m->t_start = net->t;
     if (m->spikes) {
+        m->t_start = net->t;
         LCHK(launch_fill_i64(m->fill.as<int64_t>(), 2, 0, net->stream));
         HIPCHK(hipStreamSynchronize(net->stream));
+    } else {
+        m->seg_base = 0;
+        m->t_start = first_record_step(net->t, m->phase, m->period);
     }
     invalidate(net);
     return 0;

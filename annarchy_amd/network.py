@@ -942,20 +942,57 @@ class Monitor:
     generator/Monitor/BaseTemplates.py:5-149; here each recorded attribute has
     a device buffer of `capacity` steps filled inside the captured step)."""
 
-    def __init__(self, net: "Network", obj, variables, capacity: int = 10000):
+    def __init__(self, net: "Network", obj, variables, capacity: int = 10000, period: float | None = None,
+                 period_offset: float | None = None, start: bool = True):
         self.net = net
         self.obj = obj
         self.variables = [variables] if isinstance(variables, str) else list(variables)
         self.capacity = capacity
         self._ids = {}
+        # period / period_offset in ms (core/Monitor.py:120-145); spikes are
+        # recorded every step
+        self.period = float(period) if period is not None else net.dt
+        self.period_offset = float(period_offset) if period_offset else 0.0
+        if self.period_offset >= self.period:
+            raise ValueError("Monitor(): period_offset must be smaller than period")
+        self._recording = bool(start)
 
     def _instantiate(self):
         pop = _base(self.obj)
+        L = _lib.lib()
+        period = max(1, int(round(self.period / self.net.dt)))
+        # recorded steps: (t - offset) mod period == period_offset with
+        # offset = the current step mod period (core/Monitor.py:318-332)
+        phase = self.net.get_current_step() % period + int(self.period_offset / self.net.dt)
         for v in self.variables:
             mid = C.c_int32()
-            _lib.check(_lib.lib().ahip_monitor_create(self.net._h, pop.id, v.encode(), self.capacity,
-                                                      C.byref(mid)))
+            _lib.check(L.ahip_monitor_create(self.net._h, pop.id, v.encode(), self.capacity, C.byref(mid)))
             self._ids[v] = mid.value
+            if v != "spike":
+                if period != 1 or phase != 0:
+                    _lib.check(L.ahip_monitor_set_period(self.net._h, mid.value, period, phase))
+                if not self._recording:
+                    _lib.check(L.ahip_monitor_pause(self.net._h, mid.value))
+
+    def pause(self):
+        """Stops recording the variables until resume() (core/Monitor.py:436-471)."""
+        self._recording = False
+        if self.net.compiled:
+            for v, mid in self._ids.items():
+                if v != "spike":
+                    _lib.check(_lib.lib().ahip_monitor_pause(self.net._h, mid))
+
+    def resume(self):
+        """Records again, appending to what was recorded before."""
+        self._recording = True
+        if self.net.compiled:
+            for v, mid in self._ids.items():
+                if v != "spike":
+                    _lib.check(_lib.lib().ahip_monitor_resume(self.net._h, mid))
+
+    def start(self):
+        """Starts a monitor created with start=False."""
+        self.resume()
 
     def clear(self):
         """Empty every recorded buffer; recording restarts at the current step."""
@@ -1083,8 +1120,9 @@ class Network:
         self.projections.append(p)
         return p
 
-    def monitor(self, obj, variables, capacity: int = 10000) -> Monitor:
-        m = Monitor(self, obj, variables, capacity)
+    def monitor(self, obj, variables, capacity: int = 10000, period: float | None = None,
+                period_offset: float | None = None, start: bool = True) -> Monitor:
+        m = Monitor(self, obj, variables, capacity, period, period_offset, start)
         self.monitors.append(m)
         if self.compiled:
             m._instantiate()

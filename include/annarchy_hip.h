@@ -304,6 +304,16 @@ int ahip_monitor_get_spikes(ahip_net* net, int32_t mon, int32_t* counts, int64_t
                             int32_t* idx, int64_t idx_cap, int64_t* total);
 /* Clear the recorded data and restart recording at the current step. */
 int ahip_monitor_clear(ahip_net* net, int32_t mon);
+/* period / period_offset of a value monitor (generator/Monitor/
+ * OpenMPTemplates.py:77, core/Monitor.py:131-166, :323-332): it records at
+ * the steps t with (t - phase) mod period == 0 (phase = offset +
+ * period_offset in steps), one row per recorded step.  Spike monitors
+ * record every step (period 1 only). */
+int ahip_monitor_set_period(ahip_net* net, int32_t mon, int32_t period, int64_t phase);
+/* pause() / resume() of a value monitor (core/Monitor.py:436-500): no row
+ * is written while paused; rows go on from where they stopped. */
+int ahip_monitor_pause(ahip_net* net, int32_t mon);
+int ahip_monitor_resume(ahip_net* net, int32_t mon);
 
 /* ---- batched instances (parallel_run, core/Network.py:751-926) ------------
  * Many rate-network instances of uneven size advanced together: instance i
