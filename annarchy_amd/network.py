@@ -956,6 +956,8 @@ class Monitor:
         if self.period_offset >= self.period:
             raise ValueError("Monitor(): period_offset must be smaller than period")
         self._recording = bool(start)
+        if not start and "spike" in self.variables:
+            raise ValueError("spike recording runs every step: start=False applies to variables only")
 
     def _instantiate(self):
         pop = _base(self.obj)
@@ -976,6 +978,8 @@ class Monitor:
 
     def pause(self):
         """Stops recording the variables until resume() (core/Monitor.py:436-471)."""
+        if "spike" in self.variables:
+            raise ValueError("spike recording runs every step and cannot be paused")
         self._recording = False
         if self.net.compiled:
             for v, mid in self._ids.items():
