@@ -304,10 +304,18 @@ int build_layout(const Proj& p, int32_t precision, const ahip_format_opts& o, Ra
             // weight 0.  rowof[s][lane] = (g << 16) | row for the first lane
             // of a row, -1 for the others.
             const size_t es = precision == AHIP_F64 ? 8 : 4;
-            int32_t pc = o.panel_cols > 0 ? o.panel_cols : (int32_t)(32768 / es);
+            // automatic: panels of up to 20479 columns (160 KiB of LDS in
+            // FP64, one workgroup per CU), and the tallest tiles (2048 rows
+            // down to 256) that still give every CU a workgroup
+            int32_t pc = o.panel_cols > 0 ? o.panel_cols : 20479;
             if (pc > 65535 || (size_t)(pc + 1) * es > 160 * 1024) { err = "panel exceeds 160 KiB of LDS or 65535 columns"; return AHIP_EINVAL; }
             pc = equal_panels(p.n_cols, pc);
-            const int32_t rb_rows = o.chunk > 0 ? o.chunk : 256;
+            int32_t rb_rows = o.chunk;
+            if (rb_rows <= 0) {
+                const int64_t panels = std::max<int32_t>(1, (p.n_cols + pc - 1) / pc);
+                rb_rows = 2048;
+                while (rb_rows > 256 && panels * ((R + rb_rows - 1) / rb_rows) < 256) rb_rows /= 2;
+            }
             if (rb_rows < 64 || rb_rows > 8192 || rb_rows % 64) { err = "tile rows must be a multiple of 64 in 64..8192"; return AHIP_EINVAL; }
             const int32_t steps = o.lanes_per_row > 0 ? o.lanes_per_row : 4;
             const int32_t waves = o.sigma > 1 ? o.sigma : 4;

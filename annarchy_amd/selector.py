@@ -26,7 +26,7 @@ import numpy as np
 from . import connectivity as conn
 
 CLASSES = [
-    dict(fmt="sell_panel", lanes=4, sigma=4, chunk=256),
+    dict(fmt="sell_panel", lanes=4, sigma=4, chunk=256, panel_cols=4096),
     dict(fmt="csr", lanes=4),
     dict(fmt="csr", lanes=8),
     dict(fmt="csr", lanes=16),

@@ -53,11 +53,13 @@ typedef struct ahip_format_opts {
     int32_t format;           /* AHIP_FMT_* */
     int32_t lanes_per_row;    /* CSR: 1..64 (power of two); SELL: threads per row; CSR_PANEL: 4..32;
                                  SELL_PANEL: 4-synapse steps in flight per lane (1, 2, 4, 8) */
-    int32_t chunk;            /* SELL: rows per slice (C); BSR: tile size; CSR_PANEL / SELL_PANEL: rows per tile */
+    int32_t chunk;            /* SELL: rows per slice (C); BSR: tile size; CSR_PANEL / SELL_PANEL: rows per
+                                 tile (SELL_PANEL 0 = the tallest of 2048 ... 256 rows that still
+                                 makes at least 256 tiles) */
     int32_t sigma;            /* SELL: sorting window in rows (1 = no sorting); SELL_PANEL: waves per
                                  workgroup (4 or 8; 0 or 1 = 4) */
     int32_t panel_cols;       /* CSR_PANEL / SELL_PANEL: at most this many columns per panel (0 =
-                                 automatic); the panels are then made equally wide.  A panel
+                                 automatic: 20479 for SELL_PANEL); the panels are then made equally wide.  A panel
                                  is staged in LDS: at most 64 KiB (CSR_PANEL) or 160 KiB
                                  (SELL_PANEL, one workgroup per CU at that size) */
 } ahip_format_opts;

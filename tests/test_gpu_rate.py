@@ -41,6 +41,7 @@ FORMATS = [
     dict(storage_format="sell_panel", lanes=4, panel_cols=4096, chunk=256),
     dict(storage_format="sell_panel", lanes=8, sigma=8, panel_cols=1000, chunk=192),
     dict(storage_format="sell_panel", lanes=4, sigma=8, panel_cols=2048, chunk=1024),
+    dict(storage_format="sell_panel"),
 ]
 
 
