@@ -1048,7 +1048,28 @@ class Monitor:
                                                       C.byref(total)))
         return counts, idx[: total.value]
 
-    # -- analysis (core/Monitor.py:1172-1330) -----------------------------------
+    # -- analysis (core/Monitor.py:712-1330) ------------------------------------
+    def _spike_data(self, spikes):
+        if spikes is None:
+            return self.get("spike", keep=True)
+        return spikes["spike"] if "spike" in spikes else spikes
+
+    def raster_plot(self, spikes: dict | None = None) -> tuple:
+        from . import utils
+        return utils.raster_plot(self._spike_data(spikes), self.net.dt)
+
+    def histogram(self, spikes: dict | None = None, bins=None, per_neuron=False, recording_window=None):
+        from . import utils
+        return utils.histogram(self._spike_data(spikes), bins, per_neuron, recording_window, self.net.dt)
+
+    def inter_spike_interval(self, spikes: dict | None = None, ranks=None, per_neuron: bool = False):
+        from . import utils
+        return utils.inter_spike_interval(self._spike_data(spikes), ranks, per_neuron, self.net.dt)
+
+    def coefficient_of_variation(self, spikes: dict | None = None, ranks=None):
+        from . import utils
+        return utils.coefficient_of_variation(self._spike_data(spikes), ranks, False, self.net.dt)
+
     @staticmethod
     def mean_fr(spikes: dict, duration_ms: float) -> float:
         n = sum(len(v) for v in spikes.values())

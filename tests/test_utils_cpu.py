@@ -45,3 +45,33 @@ def test_rate_projection_has_no_efferent_count():
     proj = net.connect(a, a, "exc")
     with pytest.raises(ValueError):
         proj.nb_efferent_synapses()
+
+
+SPIKES = {0: [1, 4, 10], 1: [2, 3], 2: [], 3: [5, 6, 7, 9]}
+
+
+def test_raster_plot():
+    t, r = utils.raster_plot(SPIKES, dt=0.5)
+    np.testing.assert_array_equal(t, 0.5 * np.array([1, 4, 10, 2, 3, 5, 6, 7, 9]))
+    np.testing.assert_array_equal(r, [0, 0, 0, 1, 1, 3, 3, 3, 3])
+
+
+def test_histogram_counts_every_spike_once():
+    h = utils.histogram(SPIKES)
+    assert h.sum() == 9 and h.size == 10            # steps 1 .. 10
+    np.testing.assert_array_equal(h, [1, 1, 1, 1, 1, 1, 1, 0, 1, 1])
+    h2 = utils.histogram(SPIKES, bins=2.0)          # bins of 2 steps from step 1
+    np.testing.assert_array_equal(h2, [2, 2, 2, 1, 2])
+    hp = utils.histogram(SPIKES, per_neuron=True)
+    assert hp.shape == (4, 10) and hp.sum() == 9 and hp[3].sum() == 4
+
+
+def test_isi_and_cv_by_hand():
+    isi = utils.inter_spike_interval(SPIKES, per_neuron=True, dt=0.5)
+    assert isi == {0: [1.5, 3.0], 1: [0.5], 3: [0.5, 0.5, 1.0]}
+    assert utils.inter_spike_interval(SPIKES, ranks=[1]) == [1]
+    cv = utils.coefficient_of_variation(SPIKES, per_neuron=True)
+    # neuron 0: ISIs 3, 6 -> std 1.5 / mean 4.5; neuron 3: 1, 1, 2 -> sqrt(2)/3 / (4/3)
+    assert set(cv) == {0, 3}
+    assert abs(cv[0] - 1.5 / 4.5) < 1e-15
+    assert abs(cv[3] - (np.sqrt(2.0) / 3.0) / (4.0 / 3.0)) < 1e-15
