@@ -1075,22 +1075,6 @@ class Monitor:
         n = sum(len(v) for v in spikes.values())
         return n / len(spikes) / (duration_ms / 1000.0) if spikes else 0.0
 
-    @staticmethod
-    def inter_spike_interval(spikes: dict, dt: float, per_neuron: bool = False):
-        isis = [list(np.diff(np.asarray(ts) * dt)) for ts in spikes.values() if len(ts) > 1]
-        if per_neuron:
-            return isis
-        return [x for l in isis for x in l]
-
-    @staticmethod
-    def coefficient_of_variation(spikes: dict, dt: float) -> list:
-        """std(ISI)/mean(ISI) per neuron with at least two ISIs (Monitor.py:1210-1240)."""
-        out = []
-        for ts in spikes.values():
-            if len(ts) > 2:
-                isi = np.diff(np.asarray(ts) * dt)
-                out.append(float(np.std(isi) / np.mean(isi)))
-        return out
 
 
 class Network:
