@@ -285,6 +285,11 @@ class Population:
             return PopulationView(self, np.array([int(key) % self.size]))
         return PopulationView(self, np.atleast_1d(self.ranks[key]))
 
+    def sum(self, target: str):
+        """The weighted sum of the rate-coded projections into the
+        population through `target` (Population.sum)."""
+        return self.get("_sum_" + target)
+
     def rank_from_coordinates(self, coord) -> int:
         """Rank of the neuron at these coordinates (row-major)."""
         return int(np.ravel_multi_index(tuple(int(c) for c in np.atleast_1d(coord)), self.geometry))
@@ -371,7 +376,10 @@ class PopulationView:
             v = np.asarray(value, dtype=np.float64).reshape(-1)
             if v.size not in (1, self.size):
                 raise ValueError(f"{name}: one value or {self.size} values expected")
-            cur = np.array(pop.get(name), dtype=np.float64).reshape(-1)
+            cur = pop.get(name)
+            if isinstance(cur, conn.RandomDistribution):   # an initial distribution not drawn yet
+                cur = cur.get_values(pop.size)
+            cur = np.array(cur, dtype=np.float64).reshape(-1)
             if cur.size != pop.size:
                 cur = np.full(pop.size, float(cur.reshape(-1)[0]))
             cur[self.ranks] = v

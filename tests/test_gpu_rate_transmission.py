@@ -33,3 +33,4 @@ def test_rate_transmission_kat(gpu, fmt):
     assert [d.size for d in fnp.dendrites] == [3] * 9
     net.simulate(2)   # r of pop1 is 1 from the end of step 0 on
     np.testing.assert_allclose(pop2.r.reshape(-1), 0.1 + 0.9 + 0.3, rtol=1e-15)
+    np.testing.assert_allclose(np.asarray(pop2.sum("exc")).reshape(-1), 0.1 + 0.9 + 0.3, rtol=1e-15)
