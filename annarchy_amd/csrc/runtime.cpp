@@ -534,7 +534,7 @@ ahip_pop_args make_args(ahip_net* net, Pop& p, int32_t step) {
 // The first spike monitor of a population (filled by its compaction kernel).
 const Monitor* first_spike_monitor(const ahip_net* net, int32_t pop) {
     for (auto& mp : net->mons)
-        if (mp->spikes && mp->pop == pop) return mp.get();
+        if (mp->spikes && mp->pop == pop && !mp->paused) return mp.get();
     return nullptr;
 }
 
@@ -1973,7 +1973,9 @@ int ahip_monitor_steps(ahip_net* net, int32_t mon, int64_t* steps) {
     Monitor* m = get_mon(net, mon);
     if (!m || !steps) return fail(AHIP_EINVAL, "bad monitor");
     if (m->spikes) {
-        int64_t s = net->t - m->t_start;
+        // t_start is the effective start: rows recorded before the current
+        // segment are counted in it
+        int64_t s = m->paused ? m->seg_base : net->t - m->t_start;
         *steps = std::max<int64_t>(0, std::min<int64_t>(s, m->capacity));
     } else {
         *steps = recorded_rows(*m, net->t);
@@ -1999,10 +2001,15 @@ int ahip_monitor_set_period(ahip_net* net, int32_t mon, int32_t period, int64_t 
 int ahip_monitor_pause(ahip_net* net, int32_t mon) {
     Monitor* m = get_mon(net, mon);
     if (!m) return fail(AHIP_EINVAL, "bad monitor");
-    if (m->spikes) return fail(AHIP_EINVAL, "spike monitors cannot be paused");
     if (m->paused) return 0;
     HIPCHK(hipStreamSynchronize(net->stream));
-    m->seg_base = recorded_rows(*m, net->t);
+    if (m->spikes) {
+        int64_t rows = 0;
+        ahip_monitor_steps(net, mon, &rows);
+        m->seg_base = rows;
+    } else {
+        m->seg_base = recorded_rows(*m, net->t);
+    }
     m->paused = true;
     invalidate(net);
     return 0;
@@ -2011,11 +2018,13 @@ int ahip_monitor_pause(ahip_net* net, int32_t mon) {
 int ahip_monitor_resume(ahip_net* net, int32_t mon) {
     Monitor* m = get_mon(net, mon);
     if (!m) return fail(AHIP_EINVAL, "bad monitor");
-    if (m->spikes) return fail(AHIP_EINVAL, "spike monitors cannot be paused");
     if (!m->paused) return 0;
     HIPCHK(hipStreamSynchronize(net->stream));
     m->paused = false;
-    m->t_start = first_record_step(net->t, m->phase, m->period);
+    // spikes: the row of step t is t - t_start, so the rows go on from
+    // seg_base (the fill counts of row seg_base were left by the last
+    // recorded step); values: the next step on the period grid
+    m->t_start = m->spikes ? net->t - m->seg_base : first_record_step(net->t, m->phase, m->period);
     invalidate(net);
     return 0;
 }
@@ -2064,6 +2073,7 @@ int ahip_monitor_clear(ahip_net* net, int32_t mon) {
     HIPCHK(hipStreamSynchronize(net->stream));
     if (m->spikes) {
         m->t_start = net->t;
+        m->seg_base = 0;
         LCHK(launch_fill_i64(m->fill.as<int64_t>(), 2, 0, net->stream));
         HIPCHK(hipStreamSynchronize(net->stream));
     } else {

@@ -964,8 +964,8 @@ class Monitor:
         if self.period_offset >= self.period:
             raise ValueError("Monitor(): period_offset must be smaller than period")
         self._recording = bool(start)
-        if not start and "spike" in self.variables:
-            raise ValueError("spike recording runs every step: start=False applies to variables only")
+        # spikes: (first row, its step) of every recorded segment
+        self._segments = []
 
     def _instantiate(self):
         pop = _base(self.obj)
@@ -978,29 +978,27 @@ class Monitor:
             mid = C.c_int32()
             _lib.check(L.ahip_monitor_create(self.net._h, pop.id, v.encode(), self.capacity, C.byref(mid)))
             self._ids[v] = mid.value
-            if v != "spike":
-                if period != 1 or phase != 0:
-                    _lib.check(L.ahip_monitor_set_period(self.net._h, mid.value, period, phase))
-                if not self._recording:
-                    _lib.check(L.ahip_monitor_pause(self.net._h, mid.value))
+            if v != "spike" and (period != 1 or phase != 0):
+                _lib.check(L.ahip_monitor_set_period(self.net._h, mid.value, period, phase))
+            if not self._recording:
+                _lib.check(L.ahip_monitor_pause(self.net._h, mid.value))
+        self._segments = [(0, self.net.get_current_step())] if self._recording else []
 
     def pause(self):
-        """Stops recording the variables until resume() (core/Monitor.py:436-471)."""
-        if "spike" in self.variables:
-            raise ValueError("spike recording runs every step and cannot be paused")
+        """Stops recording until resume() (core/Monitor.py:436-471)."""
         self._recording = False
         if self.net.compiled:
-            for v, mid in self._ids.items():
-                if v != "spike":
-                    _lib.check(_lib.lib().ahip_monitor_pause(self.net._h, mid))
+            for mid in self._ids.values():
+                _lib.check(_lib.lib().ahip_monitor_pause(self.net._h, mid))
 
     def resume(self):
         """Records again, appending to what was recorded before."""
-        self._recording = True
-        if self.net.compiled:
+        if self.net.compiled and not self._recording:
             for v, mid in self._ids.items():
-                if v != "spike":
-                    _lib.check(_lib.lib().ahip_monitor_resume(self.net._h, mid))
+                if v == "spike":
+                    self._segments.append((self._steps(mid), self.net.get_current_step()))
+                _lib.check(_lib.lib().ahip_monitor_resume(self.net._h, mid))
+        self._recording = True
 
     def start(self):
         """Starts a monitor created with start=False."""
@@ -1010,6 +1008,15 @@ class Monitor:
         """Empty every recorded buffer; recording restarts at the current step."""
         for mid in self._ids.values():
             _lib.check(_lib.lib().ahip_monitor_clear(self.net._h, mid))
+        self._segments = [(0, self.net.get_current_step())] if self._recording else []
+
+    def _row_steps(self, rows: int) -> np.ndarray:
+        """The simulation step of every recorded spike row."""
+        out = np.empty(rows, dtype=np.int64)
+        for k, (r0, t0) in enumerate(self._segments):
+            r1 = self._segments[k + 1][0] if k + 1 < len(self._segments) else rows
+            out[r0:r1] = t0 + np.arange(max(0, min(r1, rows) - r0))
+        return out
 
     def _steps(self, mid) -> int:
         s = C.c_int64()
@@ -1023,16 +1030,17 @@ class Monitor:
         pop = _base(self.obj)
         if name == "spike":
             counts, idx = self.raw_spikes()
-            t0 = self.net.time - steps
+            row_step = self._row_steps(steps)
             out = {int(r): [] for r in np.asarray(self.obj.ranks)}
             pos = 0
             for s in range(steps):
                 for r in idx[pos: pos + counts[s]]:
                     if int(r) in out:
-                        out[int(r)].append(t0 + s)
+                        out[int(r)].append(int(row_step[s]))
                 pos += counts[s]
             if not keep:
                 _lib.check(_lib.lib().ahip_monitor_clear(self.net._h, mid))
+                self._segments = [(0, self.net.get_current_step())] if self._recording else []
             return out
         out = np.empty(steps * pop.size)
         _lib.check(_lib.lib().ahip_monitor_get(self.net._h, mid, _lib.ptr(out, C.c_double), out.size))

@@ -312,8 +312,9 @@ int ahip_monitor_clear(ahip_net* net, int32_t mon);
  * period_offset in steps), one row per recorded step.  Spike monitors
  * record every step (period 1 only). */
 int ahip_monitor_set_period(ahip_net* net, int32_t mon, int32_t period, int64_t phase);
-/* pause() / resume() of a value monitor (core/Monitor.py:436-500): no row
- * is written while paused; rows go on from where they stopped. */
+/* pause() / resume() of a monitor (core/Monitor.py:436-500): no row is
+ * written while paused; rows go on from where they stopped (for spikes one
+ * row per step of the recorded segments). */
 int ahip_monitor_pause(ahip_net* net, int32_t mon);
 int ahip_monitor_resume(ahip_net* net, int32_t mon);
 

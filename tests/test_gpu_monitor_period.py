@@ -76,3 +76,29 @@ def test_period_with_pause(gpu):
     m.resume()
     net.simulate(6)     # 15 .. 20: rows at 16, 20
     np.testing.assert_array_equal(m.get("r")[:, 0], [0.0, 4.0, 8.0, 16.0, 20.0])
+
+
+Every3 = ah.Neuron(equations="v = v + 1.0\nmp = t", spike="v == 3.0", reset="v = 0.0", name="Every3")
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_spike_monitor_pause_and_start(gpu, graph):
+    # spikes at 2, 5, 8, 11, 14, 17; the first monitor is paused over 6..11,
+    # the second starts at 6; the paused one hands the fused recording over
+    net = ah.Network(dt=1.0)
+    pop = net.create(2, Every3)
+    m1 = net.monitor(pop, ["spike"], capacity=30)
+    m2 = net.monitor(pop, ["spike"], capacity=30, start=False)
+    net.compile()
+    net.set_graph(graph, 4)
+    net.simulate(6)
+    m1.pause()
+    m2.start()
+    net.simulate(6)
+    m1.resume()
+    net.simulate(6)
+    assert m1.get("spike") == {0: [2, 5, 14, 17], 1: [2, 5, 14, 17]}
+    assert m2.get("spike") == {0: [8, 11, 14, 17], 1: [8, 11, 14, 17]}
+    # after a clear, recording goes on from the current step
+    net.simulate(3)
+    assert m1.get("spike") == {0: [20], 1: [20]}
