@@ -183,6 +183,58 @@ def test_lif_free_membrane_and_decay():
     np.testing.assert_allclose(out["g_inh"], 1.0 * (1 - dt / 10.0) ** steps, rtol=1e-12)
 
 
+def _coba_constant_g_closed_form(v0, ge, gi, dt, k, El=-60.0, Ee=0.0, Ei=-80.0, I=20.0, tau=20.0):
+    """Explicit Euler of tau dv/dt = El - v + gE (Ee - v) + gI (Ei - v) + I at
+    constant conductances: v_k = v* + (v0 - v*) (1 - dt (1 + gE + gI) / tau)^k
+    with v* = (El + gE Ee + gI Ei + I) / (1 + gE + gI)."""
+    vs = (El + ge * Ee + gi * Ei + I) / (1.0 + ge + gi)
+    return vs + (v0 - vs) * (1.0 - dt * (1.0 + ge + gi) / tau) ** k
+
+
+def test_coba_conductance_coupled_membrane_closed_form():
+    # Population/OpenMPGenerator.py:870-997: the COBA membrane is driven by
+    # g_exc (Erev_exc = 0) and g_inh (Erev_inh = -80).  Infinite conductance
+    # time constants hold g exactly (-g/inf = -0), so v follows the geometric
+    # closed form.  The gI-only and gE-only rows fail if the reversal
+    # potentials are swapped or a coupling term has the wrong sign.
+    N, dt, steps = 4, 0.1, 80
+    v0 = np.array([-70.0, -62.0, -58.0, -75.0])
+    ge = np.array([0.0, 0.5, 0.0, 0.3])
+    gi = np.array([0.0, 0.0, 0.7, 1.2])
+    out = orc.spiking_run(0, N, dt, v0, steps, params=dict(tau_exc=np.inf, tau_inh=np.inf, Vt=1e9),
+                          g_exc=ge, g_inh=gi, trace=True)
+    np.testing.assert_array_equal(out["g_exc"], ge)
+    np.testing.assert_array_equal(out["g_inh"], gi)
+    k = np.arange(1, steps + 1)[:, None]
+    np.testing.assert_allclose(out["trace_v"], _coba_constant_g_closed_form(v0, ge, gi, dt, k), rtol=1e-12)
+    # the fixed points themselves: gE-only pulls towards Erev_exc, gI-only
+    # below El + I towards Erev_inh
+    vinf = orc.spiking_run(0, N, dt, v0, 20000, params=dict(tau_exc=np.inf, tau_inh=np.inf, Vt=1e9),
+                           g_exc=ge, g_inh=gi)["v"]
+    np.testing.assert_allclose(vinf, [-40.0, -40.0 / 1.5, -96.0 / 1.7, (-40.0 - 96.0) / 2.5], rtol=1e-10)
+
+
+def test_coba_inhibitory_projection_reaches_membrane():
+    # one input spike at step 0 through an inhibitory (target 1) projection of
+    # weight 0.4: g_inh = 0.4 from step 1 on (held by tau_inh = inf), g_exc
+    # stays 0, and v follows the gI-only closed form from v_1
+    dt, steps = 0.1, 40
+    unif = np.ones((steps, 1))
+    unif[0, 0] = 0.0
+    csr = conn.all_to_all(np.arange(1), np.arange(1), 1, 1, 0.4, True)
+    proj = orc.csrc_projection(csr, 1, pre_is_input=True)
+    v0 = np.array([-55.0])
+    out = orc.spiking_run(0, 1, dt, v0, steps, [proj], params=dict(tau_exc=np.inf, tau_inh=np.inf, Vt=1e9),
+                          poisson=dict(M=1, rates=1.0, uniforms=unif), trace=True)
+    assert out["g_exc"][0] == 0.0 and out["g_inh"][0] == 0.4
+    # step 0 runs free (v_1 = trace_v[0]); the spike of step 0 is delivered by
+    # step 1's compute_psp, so trace_v[k] is k steps of gI = 0.4 from v_1
+    v1 = _coba_constant_g_closed_form(v0, 0.0, 0.0, dt, 1)
+    np.testing.assert_allclose(out["trace_v"][0], v1, rtol=1e-14)
+    k = np.arange(1, steps)[:, None]
+    np.testing.assert_allclose(out["trace_v"][1:], _coba_constant_g_closed_form(v1, 0.0, 0.4, dt, k), rtol=1e-12)
+
+
 def test_lif_refractory_period():
     # strong drive: every spike is followed by exactly 50 refractory steps at v = Vr
     out = orc.spiking_run(0, 1, 0.1, np.array([-50.5]), 400, params=dict(I=100.0),
