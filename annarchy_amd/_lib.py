@@ -17,6 +17,8 @@ from .build import CORE_LIB
 
 AHIP_F64 = 0
 AHIP_F32 = 1
+AHIP_DELIVERY_ORDERED = 0
+AHIP_DELIVERY_ATOMIC = 1
 
 FORMATS = {"csr": 0, "ell": 1, "ellr": 2, "sell": 3, "bsr": 4, "dense": 5, "csr_panel": 6, "csr_vec": 7,
            "sell_panel": 8}
@@ -73,6 +75,7 @@ _SIGS = {
     "ahip_set_time": ([_P, _I64], _I32),
     "ahip_set_graph_batch": ([_P, _I32], _I32),
     "ahip_set_graph_enabled": ([_P, _I32], _I32),
+    "ahip_graph_prepare": ([_P], _I32),
     "ahip_pop_create": ([_P, _I32, C.c_char_p, _PI32], _I32),
     "ahip_pop_set": ([_P, _I32, C.c_char_p, _PD, _I64], _I32),
     "ahip_pop_get": ([_P, _I32, C.c_char_p, _PD, _I64], _I32),
@@ -108,6 +111,8 @@ _SIGS = {
     "ahip_proj_set_learning": ([_P, _I32, _I32, _I32, _I32, _I32], _I32),
     "ahip_time_kernel": ([_P, _I32, _I32, _I32, _PD], _I32),
     "ahip_scatter_groups": ([_P, _PI32], _I32),
+    "ahip_net_set_spike_delivery": ([_P, _I32], _I32),
+    "ahip_ordered_targets": ([_P, _PI32], _I32),
     "ahip_proj_psp_bytes": ([_P, _I32, _PD], _I32),
     "ahip_proj_time_psp": ([_P, _I32, _I32, _PD], _I32),
     "ahip_proj_set_launch": ([_P, _I32, _I32], _I32),

@@ -176,8 +176,40 @@ struct ScatterDesc {
     int32_t delay;            // list of step t - delay = ring slot (t - delay) mod depth
     int32_t n_cols;
     int32_t chunks;           // waves per spike (256 synapses each)
+    // ordered delivery (cnt != NULL): the event of column-order synapse j is
+    // appended as key gid_off + j to the bucket of its post neuron instead of
+    // being added to g; deliver_ordered adds the bucket in ascending keys
+    int32_t* cnt;             // events per post neuron this step
+    const int64_t* boff;      // bucket of post neuron i: [boff[i], boff[i+1])
+    uint32_t* bucket;
+    uint32_t gid_off;
 };
 constexpr int kMaxScatter = 8;
+// One projection feeding an ordered target: its keys are [gid_off, gid_end),
+// key k names column-order synapse k - gid_off of that projection.
+struct DeliverSrc {
+    const void* w_col;        // weights in column order, or null for w_const
+    double w_const;
+    uint32_t gid_off, gid_end;
+};
+constexpr int kMaxDeliverSrc = 32;
+// A conductance g_<target> fed by at least one spiking projection with
+// per-synapse weights.  The reference adds the events projection by
+// projection, spike by spike in ascending pre-synaptic rank
+// (Projection/OpenMP/CSR.py:870-885); keys gid_off + j of the projections in
+// creation order sort exactly in that order, so adding each post neuron's
+// bucket in ascending keys reproduces every rounding of the reference.
+struct OrderedTarget {
+    void* g = nullptr;
+    int32_t pop = 0, size = 0;
+    int64_t cap = 0;              // bucket entries (the summed in-degree)
+    int32_t n_src = 0;
+    std::vector<int32_t> projs;   // projection ids, creation order
+    DevBuf boff;                  // int64[size + 1]
+    DevBuf cnt;                   // int32[size], zero between steps
+    DevBuf bucket;                // uint32[cap]
+    DevBuf src;                   // DeliverSrc[n_src]
+};
 // Spiking projections launched together: their targets are distinct, so the
 // atomics of one projection never interleave with another's into the same
 // conductance and every sum keeps the reference's projection order.
@@ -210,6 +242,8 @@ struct ahip_net {
     int32_t probe_proj = -1, probe_kind = -1;
     hipEvent_t probe_ev[2] = {nullptr, nullptr};
     std::vector<ahip::ScatterGroup> scatter_groups;
+    std::vector<ahip::OrderedTarget> ordered;   // targets delivered in the reference's order
+    int32_t delivery = AHIP_DELIVERY_ORDERED;
     size_t elem() const { return precision == AHIP_F64 ? sizeof(double) : sizeof(float); }
 };
 
@@ -223,6 +257,7 @@ int launch_spike_psp(const ahip_net* net, const Proj& p, const Pop& pre, void* t
 int launch_compact(const Pop& p, const Monitor* m, const int64_t* t_base, int32_t step, hipStream_t s);
 int launch_update_rate(const ahip_net* net, const Pop& p, int32_t step, hipStream_t s, bool active = true);
 int launch_scatter_group(const ahip_net* net, const ScatterGroup& g, int32_t step, hipStream_t s);
+int launch_deliver(const ahip_net* net, const OrderedTarget& o, hipStream_t s);
 int launch_push_values(const ahip_net* net, const void* src, void* hist, int32_t n, int32_t D, int32_t step,
                        hipStream_t s);
 int launch_gather_delayed(const ahip_net* net, const void* hist, void* dst, int32_t n, int32_t D, int32_t d,
@@ -284,6 +319,9 @@ int batch_launch_input(const BatchDev& B, int32_t step, hipStream_t s);
 int batch_launch_gemm(const BatchDev& B, hipStream_t s);
 int batch_launch_spmv(const BatchDev& B, int32_t parity, hipStream_t s);
 int batch_launch_advance(const BatchDev& B, int64_t n, hipStream_t s);
+
+// Sets the message returned by ahip_last_error(); returns `code`.
+int report_error(int code, const char* msg);
 
 // Format builders (host, from the forward CSR view).
 int build_layout(const Proj& p, int32_t precision, const ahip_format_opts& o, RateLayout& out,

@@ -217,6 +217,32 @@ __global__ __launch_bounds__(256) void scatter_multi(const ScatterDesc* __restri
         const int32_t pre = d.list[list_slot(d, t) * d.list_stride + s];
         if (pre < 0 || pre >= d.n_cols) continue;
         const int64_t b = d.col_ptr[pre], e = d.col_ptr[pre + 1];
+        if (d.cnt) {
+            // ordered delivery: append key gid_off + j to the post neuron's
+            // bucket; each synapse fires at most once per step, so a bucket
+            // never holds more entries than the neuron's summed in-degree
+            for (int64_t base = b + (int64_t)c * 256; base < e; base += (int64_t)d.chunks * 256) {
+                int32_t post[4], slot[4];
+                int64_t lo[4], hi[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int64_t j = base + lane + u * kWave;
+                    post[u] = j < e ? d.row_idx[j] : -1;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (post[u] >= 0) {
+                        slot[u] = atomicAdd(d.cnt + post[u], 1);
+                        lo[u] = d.boff[post[u]];
+                        hi[u] = d.boff[post[u] + 1];
+                    }
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (post[u] >= 0 && lo[u] + slot[u] < hi[u])
+                        d.bucket[lo[u] + slot[u]] = d.gid_off + (uint32_t)(base + lane + u * kWave);
+            }
+            continue;
+        }
         const T* wc = static_cast<const T*>(d.w_col);
         const T wk = (T)d.w_const;
         T* g = static_cast<T*>(d.g);
@@ -234,6 +260,87 @@ __global__ __launch_bounds__(256) void scatter_multi(const ScatterDesc* __restri
                 if (post[u] >= 0) unsafeAtomicAdd(g + post[u], v[u]);
         }
     }
+}
+
+// Ordered delivery of one conductance (after every scatter of the step): post
+// neuron i adds the weights of its bucket's events in ascending key, i.e.
+// projection by projection in creation order and in ascending pre-synaptic
+// rank, each addition rounded as the reference's g[post] += w
+// (Projection/OpenMP/CSR.py:870-885), then empties the bucket.  Up to 16
+// events are sorted in registers by a bitonic network; a longer bucket
+// (more than 16 of the neuron's inputs fired in one step) is walked by
+// repeated minimum, O(L^2) reads of the bucket.
+template <typename T>
+__device__ inline T deliver_weight(const DeliverSrc* __restrict__ src, int32_t n_src, uint32_t key) {
+    int32_t k = 0;
+    while (k < n_src - 1 && key >= src[k].gid_end) k++;
+    const T* wc = static_cast<const T*>(src[k].w_col);
+    return wc ? wc[key - src[k].gid_off] : (T)src[k].w_const;
+}
+
+template <int N>
+__device__ inline void bitonic_sort_keys(uint32_t (&a)[N]) {
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint32_t x = a[i], y = a[l];
+                    const bool up = (i & k) == 0;
+                    const uint32_t lo = x < y ? x : y, hi = x < y ? y : x;
+                    a[i] = up ? lo : hi;
+                    a[l] = up ? hi : lo;
+                }
+            }
+}
+
+constexpr int kDeliverSort = 16;
+
+template <typename T>
+__global__ __launch_bounds__(256) void deliver_ordered(int32_t n, const int64_t* __restrict__ boff,
+                                                       int32_t* __restrict__ cnt,
+                                                       const uint32_t* __restrict__ bucket,
+                                                       const DeliverSrc* __restrict__ src, int32_t n_src,
+                                                       T* __restrict__ g) {
+    __shared__ DeliverSrc ls[kMaxDeliverSrc];
+    if (threadIdx.x < n_src) ls[threadIdx.x] = src[threadIdx.x];
+    __syncthreads();
+    const int32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int32_t L = cnt[i];
+    if (L == 0) return;
+    const int64_t b = boff[i];
+    T acc = g[i];
+    if (L == 1) {
+        acc += deliver_weight<T>(ls, n_src, bucket[b]);
+    } else if (L <= kDeliverSort) {
+        uint32_t k[kDeliverSort];
+#pragma unroll
+        for (int m = 0; m < kDeliverSort; m++) k[m] = m < L ? bucket[b + m] : 0xffffffffu;
+        bitonic_sort_keys(k);
+        T w[kDeliverSort];
+#pragma unroll
+        for (int m = 0; m < kDeliverSort; m++) w[m] = m < L ? deliver_weight<T>(ls, n_src, k[m]) : T(0);
+#pragma unroll
+        for (int m = 0; m < kDeliverSort; m++)
+            if (m < L) acc += w[m];
+    } else {
+        uint32_t last = 0;
+        for (int32_t m = 0; m < L; m++) {
+            uint32_t best = 0xffffffffu;
+            for (int32_t q = 0; q < L; q++) {
+                const uint32_t v = bucket[b + q];
+                if ((m == 0 || v > last) && v < best) best = v;
+            }
+            acc += deliver_weight<T>(ls, n_src, best);
+            last = best;
+        }
+    }
+    g[i] = acc;
+    cnt[i] = 0;
 }
 
 // Spike recorder of a further spike monitor of a population (the first one is
@@ -333,6 +440,20 @@ int launch_scatter_group(const ahip_net* net, const ScatterGroup& g, int32_t ste
     else
         hipLaunchKernelGGL(scatter_multi<float>, dim3(g.blocks), dim3(256), 0, s,
                            g.table.as<ScatterDesc>(), g.n, net->t_dev.as<int64_t>(), step);
+    return (int)hipGetLastError();
+}
+
+int launch_deliver(const ahip_net* net, const OrderedTarget& o, hipStream_t s) {
+    if (o.size == 0 || o.n_src == 0) return 0;
+    const unsigned nb = (unsigned)((o.size + 255) / 256);
+    if (net->precision == AHIP_F64)
+        hipLaunchKernelGGL(deliver_ordered<double>, dim3(nb), dim3(256), 0, s, o.size, o.boff.as<int64_t>(),
+                           o.cnt.as<int32_t>(), o.bucket.as<uint32_t>(), o.src.as<DeliverSrc>(), o.n_src,
+                           (double*)o.g);
+    else
+        hipLaunchKernelGGL(deliver_ordered<float>, dim3(nb), dim3(256), 0, s, o.size, o.boff.as<int64_t>(),
+                           o.cnt.as<int32_t>(), o.bucket.as<uint32_t>(), o.src.as<DeliverSrc>(), o.n_src,
+                           (float*)o.g);
     return (int)hipGetLastError();
 }
 

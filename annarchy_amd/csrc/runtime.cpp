@@ -378,6 +378,67 @@ bool transmits(const ahip_net* net, const Proj& p) {
     return p.transmission && net->pops[p.post]->active;
 }
 
+// Ordered targets of the fused spike scatter (`targets`: the conductances
+// fed by a projection with per-synapse weights).  Buffers of a previous
+// build of the same conductance are kept when they are large enough.
+int build_ordered_targets(ahip_net* net, const std::vector<const void*>& targets) {
+    std::vector<OrderedTarget> old;
+    old.swap(net->ordered);
+    for (const void* g : targets) {
+        OrderedTarget o;
+        o.g = const_cast<void*>(g);
+        std::vector<DeliverSrc> src;
+        int64_t total = 0;
+        for (auto& pp : net->projs) {
+            Proj& p = *pp;
+            if (!(p.spiking && p.nnz != 0 && !p.delay_per_syn && !p.syn && transmits(net, p))) continue;
+            if (net->pops[p.post]->slots[p.target_slot].p != g) continue;
+            o.pop = p.post;
+            o.projs.push_back(p.id);
+            DeliverSrc d;
+            d.w_col = p.single_w ? nullptr : p.w_col.p;
+            d.w_const = p.w_const;
+            d.gid_off = (uint32_t)total;
+            total += p.nnz;
+            d.gid_end = (uint32_t)std::min<int64_t>(total, UINT32_MAX);
+            src.push_back(d);
+        }
+        if (total >= (int64_t)UINT32_MAX)
+            return fail(AHIP_EINVAL, "ordered spike delivery keys 32-bit synapse indices: %lld synapses into one "
+                        "conductance (use AHIP_DELIVERY_ATOMIC)", (long long)total);
+        if ((int32_t)src.size() > kMaxDeliverSrc)
+            return fail(AHIP_EINVAL, "ordered spike delivery: more than %d projections into one conductance",
+                        kMaxDeliverSrc);
+        const Pop& post = *net->pops[o.pop];
+        o.size = post.size;
+        o.cap = total;
+        o.n_src = (int32_t)src.size();
+        std::vector<int64_t> boff(o.size + 1, 0);
+        for (int32_t id : o.projs) {
+            const Proj& p = *net->projs[id];
+            for (int32_t i = 0; i < o.size; i++) boff[i + 1] += p.h_row_ptr[i + 1] - p.h_row_ptr[i];
+        }
+        for (int32_t i = 0; i < o.size; i++) boff[i + 1] += boff[i];
+        for (OrderedTarget& q : old)
+            if (q.g == g) {
+                if (q.boff.bytes >= boff.size() * sizeof(int64_t)) o.boff = std::move(q.boff);
+                if (q.cnt.bytes >= (size_t)o.size * sizeof(int32_t)) o.cnt = std::move(q.cnt);
+                if (q.bucket.bytes >= (size_t)o.cap * sizeof(uint32_t)) o.bucket = std::move(q.bucket);
+            }
+        int rc;
+        if (!o.boff.p && (rc = dev_alloc(o.boff, boff.size() * sizeof(int64_t)))) return rc;
+        if (!o.cnt.p && (rc = dev_alloc(o.cnt, (size_t)o.size * sizeof(int32_t)))) return rc;
+        if (!o.bucket.p && (rc = dev_alloc(o.bucket, (size_t)o.cap * sizeof(uint32_t)))) return rc;
+        if ((rc = dev_alloc(o.src, src.size() * sizeof(DeliverSrc)))) return rc;
+        HIPCHK(hipMemcpy(o.boff.p, boff.data(), boff.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(o.src.p, src.data(), src.size() * sizeof(DeliverSrc), hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(o.cnt.p, 0, (size_t)o.size * sizeof(int32_t)));
+        HIPCHK(hipDeviceSynchronize());
+        net->ordered.push_back(std::move(o));
+    }
+    return 0;
+}
+
 int prepare(ahip_net* net) {
     if (!net->initialized) return fail(AHIP_ESTATE, "network not initialized");
     for (auto& pp : net->projs) {
@@ -408,6 +469,12 @@ int prepare(ahip_net* net) {
         std::vector<const void*> unordered;
         for (auto& pp : net->projs)
             if (fused(*pp) && !pp->single_w) unordered.push_back(net->pops[pp->post]->slots[pp->target_slot].p);
+        // ordered delivery: every fused projection into such a target appends
+        // its events to the target's buckets (single-weight ones included, so
+        // the projections' sums keep their creation order)
+        int rc_o = build_ordered_targets(net, net->delivery == AHIP_DELIVERY_ORDERED ? unordered
+                                                                                     : std::vector<const void*>());
+        if (rc_o) return rc_o;
         for (auto& pp : net->projs) {
             Proj& p = *pp;
             if (!fused(p)) continue;
@@ -446,6 +513,22 @@ int prepare(ahip_net* net) {
             d.w_const = p.w_const;
             d.n_cols = p.n_cols;
             d.chunks = p.scatter_chunks;
+            d.cnt = nullptr;
+            d.boff = nullptr;
+            d.bucket = nullptr;
+            d.gid_off = 0;
+            for (OrderedTarget& o : net->ordered) {
+                if (o.g != g) continue;
+                uint32_t off = 0;
+                for (int32_t id : o.projs) {
+                    if (id == p.id) break;
+                    off += (uint32_t)net->projs[id]->nnz;
+                }
+                d.cnt = o.cnt.as<int32_t>();
+                d.boff = o.boff.as<int64_t>();
+                d.bucket = o.bucket.as<uint32_t>();
+                d.gid_off = off;
+            }
             groups[k].push_back(d);
             targets[k].push_back(g);
             blocks[k] += ((int64_t)pre.size * p.scatter_chunks + 3) / 4;
@@ -548,6 +631,12 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
         const bool probe = net->probe_kind == AHIP_PROBE_SCATTER && net->probe_proj == (int32_t)gi;
         if (probe) HIPCHK(hipEventRecord(net->probe_ev[0], s));
         LCHK(launch_scatter_group(net, net->scatter_groups[gi], step, s));
+        if (probe) HIPCHK(hipEventRecord(net->probe_ev[1], s));
+    }
+    for (size_t oi = 0; oi < net->ordered.size(); oi++) {
+        const bool probe = net->probe_kind == AHIP_PROBE_DELIVER && net->probe_proj == (int32_t)oi;
+        if (probe) HIPCHK(hipEventRecord(net->probe_ev[0], s));
+        LCHK(launch_deliver(net, net->ordered[oi], s));
         if (probe) HIPCHK(hipEventRecord(net->probe_ev[1], s));
     }
     for (auto& pp : net->projs) {
@@ -714,6 +803,13 @@ void to_host_t(const void* dev, double* out, int64_t n) {
 
 }  // namespace
 
+// The message of ahip_last_error() for failures detected in other
+// translation units; returns `code`.
+int ahip::report_error(int code, const char* msg) {
+    g_err = msg;
+    return code;
+}
+
 void ahip::DevBuf::release() {
     if (p) hipFree(p);
     p = nullptr;
@@ -836,6 +932,23 @@ int ahip_set_graph_enabled(ahip_net* net, int32_t enabled) {
     if (!net) return fail(AHIP_EINVAL, "null network");
     net->graph_enabled = enabled != 0;
     invalidate(net);
+    return 0;
+}
+
+int ahip_graph_prepare(ahip_net* net) {
+    if (!net) return fail(AHIP_EINVAL, "null network");
+    if (!net->graph_enabled) return fail(AHIP_ESTATE, "graph capture is disabled");
+    HIPCHK(hipSetDevice(net->device));
+    int rc = prepare(net);
+    if (rc) return rc;
+    std::vector<int32_t> zb;
+    assign_accumulate(net, zb);
+    const int32_t G = auto_batch(net);
+    if (net->dirty || !net->graph_exec || net->graph_steps != G) {
+        if ((rc = build_graph(net, G, zb))) return rc;
+    }
+    HIPCHK(hipGraphUpload(net->graph_exec, net->stream));
+    HIPCHK(hipStreamSynchronize(net->stream));
     return 0;
 }
 
@@ -1879,6 +1992,9 @@ int ahip_time_kernel(ahip_net* net, int32_t what, int32_t index, int32_t iters, 
         case AHIP_PROBE_SCATTER:
             if (index < 0 || index >= (int32_t)net->scatter_groups.size()) return fail(AHIP_EINVAL, "bad scatter group");
             break;
+        case AHIP_PROBE_DELIVER:
+            if (index < 0 || index >= (int32_t)net->ordered.size()) return fail(AHIP_EINVAL, "bad ordered target");
+            break;
         default:
             return fail(AHIP_EINVAL, "unknown kernel kind %d", what);
     }
@@ -1909,6 +2025,25 @@ int ahip_time_kernel(ahip_net* net, int32_t what, int32_t index, int32_t iters, 
     net->probe_ev[0] = net->probe_ev[1] = nullptr;
     if (rc) return rc;
     *avg_ms = total / iters;
+    return 0;
+}
+
+int ahip_net_set_spike_delivery(ahip_net* net, int32_t mode) {
+    if (!net || (mode != AHIP_DELIVERY_ORDERED && mode != AHIP_DELIVERY_ATOMIC))
+        return fail(AHIP_EINVAL, "bad network or delivery mode");
+    if (net->delivery != mode) {
+        HIPCHK(hipStreamSynchronize(net->stream));
+        net->delivery = mode;
+        invalidate(net);
+    }
+    return 0;
+}
+
+int ahip_ordered_targets(ahip_net* net, int32_t* n) {
+    if (!net || !n) return fail(AHIP_EINVAL, "bad arguments");
+    int rc = prepare(net);
+    if (rc) return rc;
+    *n = (int32_t)net->ordered.size();
     return 0;
 }
 

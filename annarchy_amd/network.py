@@ -286,9 +286,16 @@ class Population:
         return PopulationView(self, np.atleast_1d(self.ranks[key]))
 
     def sum(self, target: str):
-        """The weighted sum of the rate-coded projections into the
-        population through `target` (Population.sum)."""
-        return self.get("_sum_" + target)
+        """Population.sum(target) (core/Population.py:617-635): the weighted
+        sum of the rate-coded projections into the population through
+        `target`, the conductance g_<target> of a spiking population, and
+        zeros (with a warning) for a target no projection uses."""
+        name = ("g_" if self.spiking else "_sum_") + target
+        if name not in self._slot_info():
+            import warnings
+            warnings.warn(f"population {self.name} receives no projection with target {target}")
+            return np.zeros(self.geometry)
+        return self.get(name)
 
     def rank_from_coordinates(self, coord) -> int:
         """Rank of the neuron at these coordinates (row-major)."""
@@ -1094,11 +1101,21 @@ class Monitor:
 
 
 class Network:
-    def __init__(self, dt: float = 1.0, seed: int | None = None, precision: str = "double"):
+    def __init__(self, dt: float = 1.0, seed: int | None = None, precision: str = "double",
+                 spike_delivery: str = "ordered"):
+        """spike_delivery: "ordered" (default) adds the events of spiking
+        projections with per-synapse weights in the reference's order
+        (projection by projection, ascending pre-synaptic rank,
+        generator/Projection/OpenMP/CSR.py:870-885), so conductances are
+        bit-identical to it; "atomic" adds them with device atomics in
+        arbitrary order (faster; one rounding of difference per event)."""
+        if spike_delivery not in ("ordered", "atomic"):
+            raise ValueError("spike_delivery must be 'ordered' or 'atomic'")
         self.dt = float(dt)
         self.seed = seed
         self.rng = np.random.default_rng(seed)
         self.precision = precision
+        self.spike_delivery = spike_delivery
         self.populations: list[Population] = []
         self.projections: list[Projection] = []
         self.monitors: list[Monitor] = []
@@ -1143,6 +1160,8 @@ class Network:
             raise RuntimeError("the network is already compiled")
         p = Projection(self, pre, post, target, synapse)
         self.projections.append(p)
+        # a new target adds a sum slot to the post population
+        _base(post)._attrs = None
         return p
 
     def monitor(self, obj, variables, capacity: int = 10000, period: float | None = None,
@@ -1168,7 +1187,10 @@ class Network:
         _lib.check(L.ahip_set_seed(h, C.c_uint64(seed)))
         if graph_batch:
             _lib.check(L.ahip_set_graph_batch(h, graph_batch))
+        _lib.check(L.ahip_net_set_spike_delivery(
+            h, _lib.AHIP_DELIVERY_ORDERED if self.spike_delivery == "ordered" else _lib.AHIP_DELIVERY_ATOMIC))
         for p in self.populations:
+            p._attrs = None
             lib_path = model_library(p.neuron, allow_build, _constant_names(self), p.stop_condition,
                                      self._incoming_targets(p))
             pid = C.c_int32()
@@ -1259,6 +1281,13 @@ class Network:
         _lib.check(_lib.lib().ahip_scatter_groups(self._h, C.byref(n)))
         return n.value
 
+    def ordered_targets(self) -> int:
+        """Number of conductances delivered in the reference's order (one
+        deliver kernel each per step)."""
+        n = C.c_int32()
+        _lib.check(_lib.lib().ahip_ordered_targets(self._h, C.byref(n)))
+        return n.value
+
     def enable_learning(self, projections=None, period=None, offset=None):
         """Network.enable_learning: every (or the given) plastic projection."""
         for p in projections or self.projections:
@@ -1273,6 +1302,11 @@ class Network:
     def set_graph(self, enabled: bool = True, batch: int = 0):
         _lib.check(_lib.lib().ahip_set_graph_enabled(self._h, int(enabled)))
         _lib.check(_lib.lib().ahip_set_graph_batch(self._h, int(batch)))
+
+    def prepare_graph(self):
+        """Capture, instantiate and upload the step graph now (simulates
+        nothing), so that the next run() only replays it."""
+        _lib.check(_lib.lib().ahip_graph_prepare(self._h))
 
     def destroy(self):
         """Release the device network now (instead of at garbage collection)."""

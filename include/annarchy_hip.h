@@ -114,6 +114,10 @@ int ahip_set_time(ahip_net* net, int64_t t);
 int ahip_set_graph_batch(ahip_net* net, int32_t steps);
 /* Disable (0) or enable (1) graph capture; eager launches otherwise. */
 int ahip_set_graph_enabled(ahip_net* net, int32_t enabled);
+/* Captures, instantiates and uploads the graph of one batch of steps now
+ * (run() would do so at its first full batch), so that a timed run only
+ * replays it.  Simulates nothing.  No counterpart in the reference. */
+int ahip_graph_prepare(ahip_net* net);
 
 /* ---- populations (pop<id>_wrapper, NanoBind/Population.py:6-72) --------- */
 /* pop<id>_wrapper(size, max_delay): model_lib is the path of a generated
@@ -261,9 +265,26 @@ int ahip_proj_set_learning(ahip_net* net, int32_t proj, int32_t update, int32_t 
 #define AHIP_PROBE_COMPACT 4
 #define AHIP_PROBE_SCATTER 5
 int ahip_time_kernel(ahip_net* net, int32_t what, int32_t index, int32_t iters, double* avg_ms);
+#define AHIP_PROBE_DELIVER 6   /* ordered delivery of conductance `index` (0 .. ahip_ordered_targets - 1) */
 /* Number of fused spike-scatter launches per step (spiking projections
  * without per-synapse delays or synapse models, grouped by target). */
 int ahip_scatter_groups(ahip_net* net, int32_t* n_groups);
+/* Spike delivery of projections with per-synapse weights.  ORDERED (the
+ * default) adds every post neuron's events projection by projection and in
+ * ascending pre-synaptic rank, the order of the reference's CSRC loop
+ * (generator/Projection/OpenMP/CSR.py:870-885), so conductances are
+ * bit-identical to the reference's; the scatter appends each event to the
+ * post neuron's bucket and one more kernel per conductance adds the buckets.
+ * ATOMIC adds the weights with device atomics in arbitrary order (one
+ * rounding of difference per event; runs can differ from each other).
+ * Projections with a single weight are order-free either way.  No
+ * counterpart in the reference, which has only the ordered form. */
+#define AHIP_DELIVERY_ORDERED 0
+#define AHIP_DELIVERY_ATOMIC 1
+int ahip_net_set_spike_delivery(ahip_net* net, int32_t mode);
+/* Number of conductances delivered in order (ahip_time_kernel index range of
+ * AHIP_PROBE_DELIVER). */
+int ahip_ordered_targets(ahip_net* net, int32_t* n);
 /* Bytes the psp kernel must move per step (algorithmic, no padding). */
 int ahip_proj_psp_bytes(ahip_net* net, int32_t proj, double* bytes);
 /* Times `iters` back-to-back launches of the projection's psp kernel with HIP
