@@ -1,16 +1,36 @@
 """Benchmark of the simulation step on one MI355X.
 
-Default workload: the rate network of 100,000 neurons with one recurrent
-projection of fixed in-degree 1000 (1e8 synapses, weights N(0, 1/sqrt(1000)),
-int32 indices), tau dr/dt = -r + sum(exc) + N(0, 0.1), r >= 0, dt = 1 ms, FP64,
-CSR.  A step = compute_psp (SpMV) + the fused neuron update with on-device
-noise, captured in a HIP graph.  metric = synaptic events/s = nnz / step time.
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-roofline: the psp kernel's algorithmic bytes per launch
-  nnz*(8+4) + (N+1)*4 + N*8 + 2*N*8
-divided by its mean launch time from HIP events on the network stream.
-cpu_baseline: the oracle's OpenMP CSR step (the reference's default scalar
-loop) on the host cores, a bounded number of steps of the same network.
+prints ONE JSON line.  Its top level is the headline workload: the rate
+network of 100,000 neurons with one recurrent projection of fixed in-degree
+1000 (1e8 synapses, weights N(0, 1/sqrt(1000)), int32 indices),
+tau dr/dt = -r + sum(exc) + N(0, 0.1), r >= 0, dt = 1 ms, FP64, sliced-tile
+layout.  A step = compute_psp (SpMV) + the neuron update with on-device noise.
+W warm-up steps, then K steps timed; the timed steps replay a HIP graph
+captured, instantiated and uploaded before the timer starts (graph batch =
+the largest divisor of K up to 200).  metric = synaptic events/s = nnz per
+step / step time.
+
+Under "configs" the line holds one record of the same shape per further
+workload, each run in a child process of its own (this process never touches
+the GPU):
+  c0  COBA LIF, 4000 neurons, p = 0.02, 10,000 steps (parity run, bit-exact
+      spike trains against the oracle);
+  c2  COBA LIF, 400,000 neurons, fixed_number_pre 800 E + 200 I drawn on the
+      device, 4e8 synapses with per-synapse weights, Poisson 5 Hz into 10%;
+  c3  256 rate-network instances of uneven size in one batched launch, plain
+      per-instance FP64 loop beside it;
+  c4  the learned storage-format selector: held-out regret against
+      exhaustive tuning, timed live on a sample of held-out matrices.
+
+roofline: the dominant kernel's algorithmic bytes (or flop) per launch over
+its mean launch time from HIP events on the network stream, against the
+MI355X peaks (HBM 8 TB/s; dense BF16 MFMA 2.5 PFLOP/s).
+cpu_baseline: the oracle (a C++/OpenMP restatement of the reference's
+generated step) timed on this host's cores, a bounded sample of the same
+workload; the record states the threads used, the host CPU model, nproc and
+the CPUs this process may use.
 """
 
 from __future__ import annotations
@@ -18,6 +38,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -28,10 +49,88 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+SUB_CONFIGS = ("c0", "c2", "c3", "c4")
+CHILD_TIMEOUT_S = {"c1": 240, "c0": 120, "c2": 240, "c3": 300, "c4": 180}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# helpers
+# ---------------------------------------------------------------------------
+def host_cpu() -> dict:
+    """nproc, the CPUs this process may run on, the thread share it uses
+    (OMP_NUM_THREADS when set, as on a one-GPU box's CPU share) and the model."""
+    info = {"nproc": os.cpu_count() or 1, "affinity": len(os.sched_getaffinity(0))}
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    info["share"] = int(share) if share.isdigit() and int(share) > 0 else info["affinity"]
+    info["threads"] = max(1, min(info["affinity"], info["share"]))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    info["model"] = model
+    return info
+
+
+def graph_batch_for(steps: int, cap: int = 200) -> int:
+    """Largest divisor of `steps` up to `cap` (every timed step is a replay)."""
+    for b in range(min(cap, max(steps, 1)), 0, -1):
+        if steps % b == 0:
+            return b
+    return 1
+
+
+def traffic_from_profiles(name):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary."""
+    f = ROOT / "profiles" / name
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("hbm_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def roofline(bound, achieved, peak, unit, traffic, kernel, **extra):
+    out = {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak,
+           "traffic": traffic, "kernel": kernel}
+    out.update(extra)
+    return out
+
+
+def record(metric, value, unit, steps, warmup, ms_step, dtype, config, roof, cpu, higher=True, **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": ms_step, "higher_is_better": higher, "scaling": "weak", "vs_baseline": None,
+           "dtype": dtype, "data": "synthetic (seeded)", "config": config, "roofline": roof,
+           "cpu_baseline": cpu}
+    out.update(extra)
+    return out
+
+
+def copy_probe():
+    """Streaming copy of 2 GiB: the achievable HBM rate on this box."""
+    import ctypes as C
+    from annarchy_amd import _lib
+    ms = C.c_double()
+    _lib.check(_lib.lib().ahip_probe_copy(1 << 31, 10, C.byref(ms)))
+    return 2.0 * (1 << 31) / (ms.value * 1e-3) / 1e9
+
+
+# ---------------------------------------------------------------------------
+# C1: the rate network (headline)
+# ---------------------------------------------------------------------------
+KERNEL_OF_FORMAT = {"csr": "csr_rows", "sell": "sell_slices", "csr_panel": "tile_psp", "csr_vec": "csr_vec",
+                    "ell": "ell_rows", "ellr": "ell_rows", "bsr": "bsr_rows", "dense": "dense_rows",
+                    "tiles": "tile_psp", "sell_panel": "sell_panel_psp"}
 
 
 def build_c1(n, k, seed, lognormal=False):
@@ -47,29 +146,14 @@ def build_c1(n, k, seed, lognormal=False):
     return conn.random_in_degree(n, n, rng, deg, w)
 
 
-KERNEL_OF_FORMAT = {"csr": "csr_rows", "sell": "sell_slices", "csr_panel": "tile_psp", "csr_vec": "csr_vec",
-                    "ell": "ell_rows", "ellr": "ell_rows", "bsr": "bsr_rows", "dense": "dense_rows",
-                    "tiles": "tile_psp", "sell_panel": "sell_panel_psp"}
-
-
-def traffic_from_profiles(name):
-    """HBM bytes per launch from a committed rocprofv3 --pmc summary."""
-    f = ROOT / "profiles" / name
-    if not f.exists():
-        return None
-    for line in f.read_text().splitlines():
-        if line.startswith("hbm_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
-
-
-def cpu_baseline(csr, seconds, threads):
+def c1_cpu_baseline(csr, seconds, threads):
+    """Steps of the oracle's CSR loop (the reference's default scalar row
+    loop) + rate update on `threads` OpenMP threads, for about `seconds`."""
     from oracle import oracle as orc
 
     n = csr.n_rows
     r = np.random.default_rng(1).random(n)
     noise = np.random.default_rng(2).normal(0.0, 0.1, (1, n))
-    # warm-up step
     orc.rate_run(csr.row_ptr, csr.col_idx, csr.w, r, 10.0, 1.0, noise, 1, nthreads=threads)
     steps, t0 = 0, time.perf_counter()
     while True:
@@ -81,49 +165,25 @@ def cpu_baseline(csr, seconds, threads):
     return csr.nnz * steps / el, steps, el
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--neurons", type=int, default=100_000)
-    ap.add_argument("--degree", type=int, default=1000)
-    ap.add_argument("--lognormal", action="store_true")
-    ap.add_argument("--precision", default="double", choices=["double", "float"])
-    # default: sliced tiles (2048-row blocks, 1024 in FP32, x 20479-column
-    # panels staged in 160 KiB of LDS, sorted 64-row slices, one lane per row,
-    # 2 steps of 4 synapses in flight, 8 waves per workgroup): the fastest
-    # layout of the sweep
-    ap.add_argument("--format", default="sell_panel")
-    ap.add_argument("--lanes", type=int, default=2,
-                    help="csr/csr_vec/csr_panel: lanes per row; sell: threads per row; sell_panel: steps in flight")
-    ap.add_argument("--sigma", type=int, default=1, help="sell: sorting window")
-    ap.add_argument("--waves", type=int, default=8, help="sell_panel: waves per workgroup")
-    ap.add_argument("--panel-cols", type=int, default=20479,
-                    help="sell_panel: columns per panel at most (20479: 160 KiB of LDS in FP64)")
-    ap.add_argument("--chunk", type=int, default=0,
-                    help="sell_panel: rows per tile (0: 2048 in FP64, 1024 in FP32)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2", "c3", "stdp"],
-                    help="c1 (default): rate network 1e8 synapses; c0: COBA 4000 neurons; "
-                         "c2: COBA 400k neurons / 4e8 synapses; c3: batched sweep of 256 instances")
-    args = ap.parse_args()
-
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: the simulation step of this backend is a "
-              "single-device path (multi-GPU sharding is a design note, not built).", file=sys.stderr)
-        sys.exit(2)
-
-    if args.config != "c1":
-        print(json.dumps(globals()["bench_" + args.config](args)))
-        return
-
+def bench_c1(args):
     import annarchy_amd as ah
 
     t_build = time.perf_counter()
     csr = build_c1(args.neurons, args.degree, 1234, args.lognormal)
-    log(f"connectivity: {csr.nnz} synapses in {time.perf_counter() - t_build:.1f} s")
+    log(f"c1: connectivity {csr.nnz} synapses in {time.perf_counter() - t_build:.1f} s")
+    cpu = None
+    if not args.no_cpu:
+        h = host_cpu()
+        v1, s1, e1 = c1_cpu_baseline(csr, args.cpu_seconds, 1)
+        log(f"c1: cpu baseline 1 thread {v1:.3e} events/s")
+        vn, sn, en = c1_cpu_baseline(csr, args.cpu_seconds, h["threads"])
+        log(f"c1: cpu baseline {h['threads']} threads {vn:.3e} events/s")
+        cpu = {"value": vn, "unit": "synaptic events/s", "cores": h["threads"], "kind": "port",
+               "sample": f"{sn} full steps (SpMV + update) of the same {csr.nnz}-synapse network in "
+                         f"{en:.1f} s, oracle CSR loop FP64, OpenMP on {h['threads']} threads (the CPU share of "
+                         f"this process; nproc {h['nproc']}, affinity {h['affinity']}, {h['model']})",
+               "one_thread": {"value": v1, "steps": s1, "seconds": e1},
+               "host": h}
 
     net = ah.Network(dt=1.0, seed=42, precision=args.precision)
     pop = net.create(args.neurons, ah.RateNoisy)
@@ -134,7 +194,10 @@ def main():
                   panel_cols=args.panel_cols, chunk=chunk)
     net.compile(allow_build=False)
     pop.r = np.random.default_rng(7).random(args.neurons)
-    log("network uploaded")
+    G = graph_batch_for(args.steps)
+    net.set_graph(True, G)
+    net.prepare_graph()
+    log(f"c1: network uploaded, graph of {G} steps ready")
 
     net.run(args.warmup)
     t0 = time.perf_counter()
@@ -143,88 +206,30 @@ def main():
     ms_step = el * 1000.0 / args.steps
     events = csr.nnz * args.steps / el
 
-    import ctypes as C
-    from annarchy_amd import _lib
-
-    copy_ms = C.c_double()
-    copy_bytes = 1 << 31
-    _lib.check(_lib.lib().ahip_probe_copy(copy_bytes, 10, C.byref(copy_ms)))
-    copy_gbs = 2.0 * copy_bytes / (copy_ms.value * 1e-3) / 1e9
-    log(f"streaming copy of {copy_bytes >> 20} MiB: {copy_ms.value:.3f} ms, {copy_gbs:.0f} GB/s (read + write)")
-
+    copy_gbs = copy_probe()
     psp_ms = proj.time_psp(50)
     psp_bytes = proj.psp_bytes()
     achieved = psp_bytes / (psp_ms * 1e-3) / 1e9
     dtype = "f64" if args.precision == "double" else "f32"
     kname = KERNEL_OF_FORMAT.get(args.format, args.format)
     tag = f"{kname}_{dtype}_c1_{'lognormal' if args.lognormal else 'fixed'}{args.degree}"
-    traffic = traffic_from_profiles(f"{tag}_traffic.txt")
-    log(f"step {ms_step:.3f} ms, psp {psp_ms * 1e3:.1f} us, {achieved:.0f} GB/s")
-
-    cpu = None
-    if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        v, nsteps, secs = cpu_baseline(csr, args.cpu_seconds, threads)
-        cpu = {"value": v, "unit": "synaptic events/s", "cores": threads, "kind": "port",
-               "sample": f"{nsteps} full steps (SpMV + update) of the same {csr.nnz}-synapse network, "
-                         f"{secs:.1f} s, oracle CSR loop FP64 with OpenMP"}
-        log(f"cpu baseline {v:.3e} events/s on {threads} threads")
-
-    out = {
-        "metric": "synaptic events/s",
-        "value": events,
-        "unit": "synaptic events/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": dtype,
-        "data": "synthetic (seeded random connectivity, random-init rates)",
-        "config": {
-            "workload": f"rate network N={args.neurons}, in-degree {'lognormal mean ' if args.lognormal else ''}"
-                        f"{args.degree}, nnz={csr.nnz}, dt=1ms, {args.format} {dtype}",
-            "neurons": args.neurons,
-            "synapses": csr.nnz,
-            "format": proj.get_format(),
-            "parallelism": "single GPU",
-        },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "kernel": f"{kname}{' + tile_reduce' if args.format in ('csr_panel', 'sell_panel') else ''}"
-                      f" (compute_psp, {args.format})",
-            "kernel_ms": psp_ms,
-            "algorithmic_bytes": psp_bytes,
-        },
-        "cpu_baseline": cpu,
-        "hbm_gbs_step": psp_bytes / (ms_step * 1e-3) / 1e9,
-        "hbm_copy_gbs": copy_gbs,
-    }
-    print(json.dumps(out))
+    log(f"c1: step {ms_step:.3f} ms, psp {psp_ms * 1e3:.1f} us, {achieved:.0f} GB/s; copy {copy_gbs:.0f} GB/s")
+    roof = roofline("hbm", achieved, HBM_PEAK_GBS, "GB/s", traffic_from_profiles(f"{tag}_traffic.txt"),
+                    f"{kname}{' + tile_reduce' if args.format in ('csr_panel', 'sell_panel') else ''}"
+                    f" (compute_psp, {args.format})", kernel_ms=psp_ms, algorithmic_bytes=psp_bytes)
+    return record("synaptic events/s", events, "synaptic events/s", args.steps, args.warmup, ms_step, dtype,
+                  {"workload": f"rate network N={args.neurons}, in-degree "
+                               f"{'lognormal mean ' if args.lognormal else ''}{args.degree}, nnz={csr.nnz}, "
+                               f"dt=1ms, {args.format} {dtype}",
+                   "neurons": args.neurons, "synapses": csr.nnz, "format": proj.get_format(),
+                   "graph_steps": G, "parallelism": "single GPU"},
+                  roof, cpu, hbm_gbs_step=psp_bytes / (ms_step * 1e-3) / 1e9, hbm_copy_gbs=copy_gbs)
 
 
 # ---------------------------------------------------------------------------
-# Other workloads (parity configurations and chosen forms); each returns one
-# JSON object of the same shape as the default line.
+# spiking workloads
 # ---------------------------------------------------------------------------
-def _line(metric, value, unit, steps, warmup, ms_step, dtype, config, roofline, cpu, **extra):
-    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
-           "warmup": warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded)", "config": config,
-           "roofline": roofline, "cpu_baseline": cpu}
-    out.update(extra)
-    return out
-
-
-def _coba(N, ne, conn_fn, steps_cap, seed=7, poisson=None, weights=(0.6, 6.7), I=20.0,
-          precision="double"):
+def _coba(N, ne, conn_fn, steps_cap, seed=7, poisson=None, I=20.0, precision="double"):
     import annarchy_amd as ah
     from annarchy_amd import models
 
@@ -243,113 +248,161 @@ def _coba(N, ne, conn_fn, steps_cap, seed=7, poisson=None, weights=(0.6, 6.7), I
     return net, P, Q, projs, mon
 
 
-def _events(projs, P, counts, idx):
-    """Delivered events = sum over spikes of the out-degree (core/Utils.py:223-239)."""
+def _outdeg(projs, P):
     out = np.zeros(P.size, dtype=np.int64)
     for pr in projs:
         if pr.pre is P or getattr(pr.pre, "population", None) is P:
             out += np.bincount(pr.csr.col_idx, minlength=P.size)[: P.size]
-    return int(out[idx].sum())
+    return out
+
+
+def _events(projs, P, idx):
+    """Delivered events = sum over spikes of the out-degree (core/Utils.py:223-239)."""
+    return int(_outdeg(projs, P)[idx].sum())
+
+
+def _spiking_kernels(net, P, projs, outdeg, mon, es, n_steps_done):
+    """Mean time of each kernel of a spiking step (HIP events inside 20 real
+    eager steps per kernel) and its algorithmic bytes over exactly those
+    steps.  Kernel bytes:
+      update      N * (6 s + 12): v, g_exc, g_inh read and written, refractory
+                  read, refractory_remaining read and written;
+      scatter     per spike 20 B (list entry, column range), per synapse event
+                  4 + 3 s (post rank, weight, conductance read and written) with
+                  atomic delivery or one weight, and 4 + 8 + 16 + 4 (post rank,
+                  bucket counter read and written, bucket range, key) ordered;
+      deliver     per conductance N * 4 (counters) + per event 4 + s (key,
+                  weight) + per neuron with events 16 + 2 s (range, g).
+    """
+    from annarchy_amd import _lib  # noqa: F401
+
+    t = {}
+    t["update"] = net.time_kernel(3, P.id, 20)
+    t["compact"] = net.time_kernel(4, P.id, 20)
+    ng = net.scatter_groups()
+    t["scatter"] = sum(net.time_kernel(5, g, 20) for g in range(ng))
+    no = net.ordered_targets()
+    t["deliver"] = sum(net.time_kernel(6, k, 20) for k in range(no)) if no else 0.0
+    counts, idx = mon.raw_spikes()
+    starts = np.concatenate([[0], np.cumsum(counts)])
+    # the kernels above ran 20 + 20 + 20 ng + 20 no steps after n_steps_done;
+    # the scatter of step u sends the spikes of step u - 1: average over the
+    # steps of the scatter and deliver probes
+    first = n_steps_done + 40
+    last = n_steps_done + 40 + 20 * (ng + no)
+    us = range(first - 1, min(last - 1, len(counts)))
+    syn = np.mean([outdeg[idx[starts[u]:starts[u + 1]]].sum() for u in us]) if len(us) else 0.0
+    spk = np.mean([counts[u] for u in us]) if len(us) else 0.0
+    N = P.size
+    per_syn = (4 + 8 + 16 + 4) if no else (4 + 3 * es)
+    b = {"update": N * (6 * es + 12), "compact": N / 8 + spk * 4 + 8,
+         "scatter": syn * per_syn + spk * 20,
+         "deliver": (no * N * 4 + syn * (4 + es) + min(syn, N) * (16 + 2 * es)) if no else 0.0}
+    kern = {k: {"ms": t[k], "bytes": float(b[k])} for k in t if t[k] > 0}
+    return kern, float(syn), float(spk)
 
 
 def bench_c0(args):
-    """COBA LIF, 4000 neurons (3200 E / 800 I), p = 0.02, 10,000 steps."""
+    """COBA LIF, 4000 neurons (3200 E / 800 I), p = 0.02, 10,000 steps: one
+    graph-replayed parity run from the seeded initial state."""
     from oracle import oracle as orc
 
     steps = 10000
+    h = host_cpu()
+
     def conn(net, P, ne):
         return [net.connect(P[:ne], P, "exc").fixed_probability(0.02, 0.6),
                 net.connect(P[ne:], P, "inh").fixed_probability(0.02, 6.7)]
-    net, P, _, projs, mon = _coba(4000, 3200, conn, steps)
+    net, P, _, projs, mon = _coba(4000, 3200, conn, steps + 200)
     net.set_graph(True, 200)
+    net.prepare_graph()
     v0 = P.v.reshape(-1).copy()
     t0 = time.perf_counter()
     net.run(steps)
     el = time.perf_counter() - t0
     counts, idx = mon.raw_spikes()
-    events = _events(projs, P, counts, idx)
+    events = _events(projs, P, idx)
+    log(f"c0: {steps} steps in {el:.3f} s, {int(counts.sum())} spikes")
+    outdeg = _outdeg(projs, P)
+    kern, syn, spk = _spiking_kernels(net, P, projs, outdeg, mon, 8, steps)
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    ach = kern[dom]["bytes"] / (kern[dom]["ms"] * 1e-3) / 1e9
+    roof = roofline("hbm", ach, HBM_PEAK_GBS, "GB/s", None, dom, kernel_ms=kern[dom]["ms"],
+                    algorithmic_bytes=kern[dom]["bytes"],
+                    note="latency-bound: three launches of a few microseconds per step")
     cpu = None
     if not args.no_cpu:
         po = [orc.csrc_projection(projs[0].csr, 0), orc.csrc_projection(projs[1].csr, 1)]
-        t1 = time.perf_counter()
-        ref = orc.spiking_run(0, 4000, 0.1, v0, steps, po, refractory=np.asarray(P.refractory, np.int32))
-        cel = time.perf_counter() - t1
-        exact = bool(np.array_equal(ref["counts"], counts) and np.array_equal(ref["idx"], idx))
+        times = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            ref = orc.spiking_run(0, 4000, 0.1, v0, steps, po, refractory=np.asarray(P.refractory, np.int32))
+            times.append(time.perf_counter() - t1)
+        cel = float(np.median(times))
+        exact = bool(np.array_equal(ref["counts"], counts[:steps]) and
+                     np.array_equal(ref["idx"], idx[:int(counts[:steps].sum())]))
         cpu = {"value": steps / cel, "unit": "steps/s", "cores": 1, "kind": "port",
-               "sample": f"the full 10000-step run, single thread; spike trains bit-exact: {exact}"}
-    return _line("steps/s", steps / el, "steps/s", steps, 0, el * 1e3 / steps, "f64",
-                 {"workload": "COBA LIF N=4000 (3200E/800I) p=0.02, dt=0.1ms, 10000 steps"},
-                 None, cpu, synaptic_events_per_s=events / el, spikes=int(counts.sum()),
-                 mean_rate_hz=float(counts.sum() / 4000 / (steps * 1e-4)))
+               "sample": f"the full {steps}-step run, single thread (the oracle's spiking step is the "
+                         f"reference's single-thread loop), median of 3; spike trains bit-exact: {exact}",
+               "host": h}
+        log(f"c0: oracle {steps / cel:.0f} steps/s, bit-exact {exact}")
+    return record("steps/s", steps / el, "steps/s", steps, 0, el * 1e3 / steps, "f64",
+                  {"workload": "COBA LIF N=4000 (3200E/800I) p=0.02, dt=0.1ms, 10000 steps", "graph_steps": 200},
+                  roof, cpu, synaptic_events_per_s=events / el, spikes=int(counts[:steps].sum()),
+                  mean_rate_hz=float(counts[:steps].sum() / 4000 / (steps * 1e-4)), kernels=kern)
 
 
 def bench_c2(args):
     """COBA LIF, 400k neurons (320k E / 80k I), fixed_number_pre 1000 (800 E +
-    200 I) drawn on the device, 4e8 synapses with per-synapse weights,
-    Poisson 5 Hz into 10%.  roofline: the longer of the population update of
-    the 400k LIF neurons (60 B per neuron in FP64: v, g_exc, g_inh read and
-    written, refractory, refractory_remaining read and written) and the fused
-    scatter of the E and I projections (per spike 20 B, per synapse 4 + 3*s
-    B: post index, weight, conductance atomic), each timed with HIP events
-    inside 20 real steps.  cpu_baseline: the oracle's spiking step on the
-    same connectivity (Poisson draws from numpy), a bounded number of steps."""
+    200 I) drawn on the device, 4e8 synapses with per-synapse weights drawn
+    from Uniform(0.8 w, 1.2 w), Poisson 5 Hz into 10%, ordered delivery."""
     from annarchy_amd import connectivity as cn
 
-    steps = args.steps if args.steps != 200 else 2000
+    steps = args.c2_steps
+    warmup = 20
     N, ne = 400_000, 320_000
-    scale = 80.0 / 1000.0  # same mean input conductance as the 4000-neuron benchmark
+    scale = 80.0 / 1000.0  # same mean input conductance as the 4000-neuron network
     prec = args.precision
     es = 8 if prec == "double" else 4
     we, wi = 0.6 * scale, 6.7 * scale
 
     def conn(net, P, ne_):
-        # one weight array entry per synapse (Uniform(w, w) draws w exactly)
-        return [net.connect(P[:ne_], P, "exc").fixed_number_pre(800, cn.Uniform(we, we), generator="device"),
-                net.connect(P[ne_:], P, "inh").fixed_number_pre(200, cn.Uniform(wi, wi), generator="device")]
+        return [net.connect(P[:ne_], P, "exc").fixed_number_pre(800, cn.Uniform(0.8 * we, 1.2 * we),
+                                                                generator="device"),
+                net.connect(P[ne_:], P, "inh").fixed_number_pre(200, cn.Uniform(0.8 * wi, 1.2 * wi),
+                                                                generator="device")]
     t_b = time.perf_counter()
-    net, P, Q, projs, mon = _coba(N, ne, conn, steps + args.warmup + 40, poisson=(N // 10, 5.0, 0.6),
-                                  precision=prec)
-    log(f"c2 network ({sum(p.nb_synapses for p in projs)} synapses, connectivity drawn on the device) "
-        f"set up in {time.perf_counter() - t_b:.1f} s")
+    net, P, Q, projs, mon = _coba(N, ne, conn, steps + warmup + 200, poisson=(N // 10, 5.0, 0.6), precision=prec)
+    log(f"c2: network ({sum(p.nb_synapses for p in projs)} synapses drawn on the device) set up in "
+        f"{time.perf_counter() - t_b:.1f} s")
     v0 = P.v.reshape(-1).copy()
-    net.set_graph(True, 100)
-    net.run(args.warmup)
+    G = graph_batch_for(steps, 100)
+    net.set_graph(True, G)
+    net.prepare_graph()
+    net.run(warmup)
     t0 = time.perf_counter()
     net.run(steps)
     el = time.perf_counter() - t0
     counts, idx = mon.raw_spikes()
-    c_t = counts[args.warmup:args.warmup + steps]
-    idx_t = idx[int(counts[:args.warmup].sum()):int(counts[:args.warmup + steps].sum())]
-    events = _events(projs[:2], P, c_t, idx_t)
-    # kernel times inside 2 x 20 further steps
-    n_done = args.warmup + steps
-    t_upd = net.time_kernel(3, P.id, 20)
-    t_sc = net.time_kernel(5, 0, 20)
-    counts2, idx2 = mon.raw_spikes()
-    starts = np.concatenate([[0], np.cumsum(counts2)])
-    outdeg = np.zeros(N, dtype=np.int64)
-    for pr in projs[:2]:
-        outdeg += np.bincount(pr.csr.col_idx, minlength=N)[:N]
-    # the scatter of step u sends the spikes of step u - 1
-    sc_steps = range(n_done + 19, n_done + 39)
-    sc_syn = sum(int(outdeg[idx2[starts[u]:starts[u + 1]]].sum()) for u in sc_steps) / 20.0
-    sc_spk = sum(int(counts2[u]) for u in sc_steps) / 20.0
-    b_sc = sc_syn * (4 + 3 * es) + sc_spk * 20
-    b_upd = N * (6 * es + 12)
-    kern = {"local_update": {"ms": t_upd, "bytes": b_upd},
-            "scatter_multi": {"ms": t_sc, "bytes": b_sc, "syn_events": sc_syn}}
-    dom = "local_update" if t_upd >= t_sc else "scatter_multi"
+    c_t = counts[warmup:warmup + steps]
+    idx_t = idx[int(counts[:warmup].sum()):int(counts[:warmup + steps].sum())]
+    outdeg = _outdeg(projs[:2], P)
+    events = int(outdeg[idx_t].sum())
+    log(f"c2: {steps} steps in {el:.3f} s ({el / steps * 1e6:.1f} us/step), {events / el:.3e} events/s")
+    kern, syn, spk = _spiking_kernels(net, P, projs[:2], outdeg, mon, es, warmup + steps)
+    dom = max(kern, key=lambda k: kern[k]["ms"])
     ach = kern[dom]["bytes"] / (kern[dom]["ms"] * 1e-3) / 1e9
-    roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "traffic": None, "kernel": dom}
+    traffic = traffic_from_profiles(f"c2_{dom}_{'f64' if prec == 'double' else 'f32'}_traffic.txt")
+    roof = roofline("hbm", ach, HBM_PEAK_GBS, "GB/s", traffic, dom, kernel_ms=kern[dom]["ms"],
+                    algorithmic_bytes=kern[dom]["bytes"])
     cpu = None
     if not args.no_cpu and prec == "double":
         from oracle import oracle as orc
         t1 = time.perf_counter()
         po = [orc.csrc_projection(projs[0].csr, 0), orc.csrc_projection(projs[1].csr, 1),
               orc.csrc_projection(projs[2].csr, 0, pre_is_input=True)]
-        log(f"c2 oracle views built in {time.perf_counter() - t1:.1f} s")
-        cs = 4000
+        log(f"c2: oracle views built in {time.perf_counter() - t1:.1f} s")
+        cs = 2000
         unif = np.random.default_rng(5).random((cs, N // 10))
         t1 = time.perf_counter()
         ref = orc.spiking_run(0, N, 0.1, v0, cs, po, refractory=np.asarray(P.refractory, np.int32),
@@ -357,156 +410,89 @@ def bench_c2(args):
         cel = time.perf_counter() - t1
         ev_c = int(outdeg[ref["idx"]].sum())
         cpu = {"value": ev_c / cel, "unit": "synaptic events/s", "cores": 1, "kind": "port",
-               "sample": f"{cs} steps of the same network from the same initial state (Poisson draws from "
-                         f"numpy), single thread: {cel:.1f} s, {cel / cs * 1e3:.1f} ms/step"}
-    return _line("synaptic events/s", events / el, "synaptic events/s", steps, args.warmup,
-                 el * 1e3 / steps, "f64" if prec == "double" else "f32",
-                 {"workload": "COBA LIF N=400000 fixed_number_pre 1000 (800E/200I) 4e8 synapses, "
-                              "Poisson 5 Hz into 10%, dt=0.1ms", "precision": prec,
-                  "connectivity": "drawn on the device (Philox stream)"},
-                 roof, cpu, kernels=kern, spikes_per_step=float(c_t.mean()),
-                 mean_rate_hz=float(c_t.sum() / N / (steps * 1e-4)))
+               "sample": f"{cs} steps of the same network from the same initial state (Poisson draws from numpy), "
+                         f"single thread: {cel:.1f} s, {cel / cs * 1e3:.1f} ms/step", "host": host_cpu()}
+        log(f"c2: oracle {ev_c / cel:.3e} events/s")
+    return record("synaptic events/s", events / el, "synaptic events/s", steps, warmup, el * 1e3 / steps,
+                  "f64" if prec == "double" else "f32",
+                  {"workload": "COBA LIF N=400000 fixed_number_pre 1000 (800E/200I) 4e8 synapses with per-synapse "
+                               "weights, Poisson 5 Hz into 10%, dt=0.1ms", "precision": prec,
+                   "delivery": "ordered (the reference's summation order)", "graph_steps": G,
+                   "connectivity": "drawn on the device (Philox stream)"},
+                  roof, cpu, kernels=kern, spikes_per_step=float(c_t.mean()),
+                  mean_rate_hz=float(c_t.sum() / N / (steps * 1e-4)))
 
 
-def bench_stdp(args):
-    """STDP (models/Synapses.py:232-310) on 5e7 plastic synapses: 100,000
-    Poisson pre-synaptic neurons at 20 Hz -> 50,000 post-synaptic Poisson
-    neurons at 10 Hz (their spikes do not depend on g), fixed_number_pre
-    1000 drawn on the device, weights U(0.2, 0.8), dt = 1 ms.  metric =
-    synaptic events/s: pre spikes x out-degree + post spikes x in-degree
-    (core/Utils.py:223-239).  roofline: the pre-event and post-event kernels
-    timed with HIP events inside 20 further steps each; algorithmic bytes per synapse
-    event: pre 20 + 8*s (row index, w, x, y, _last_event read and written,
-    the conductance atomic), post 24 + 6*s (state position, w, x, y,
-    _last_event), s = bytes per value.  cpu_baseline: the oracle's plastic
-    projection (orc_plastic_run) over the warm-up steps' recorded spike
-    trains, single thread; its weights are compared with the GPU's."""
-    import annarchy_amd as ah
-    from annarchy_amd import connectivity as cn
+# ---------------------------------------------------------------------------
+# C3: the batched sweep
+# ---------------------------------------------------------------------------
+_SWEEP = None
 
-    steps = args.steps if args.steps != 200 else 1000
-    warmup = args.warmup if args.warmup != 20 else 200
-    n_pre, n_post, k = 100_000, 50_000, 1000
-    prec = args.precision
-    es = 8 if prec == "double" else 4
-    pre_n = ah.Neuron(parameters="rates = 40.0", equations="p = Uniform(0.0, 1.0) * 1000.0 / dt",
-                      spike="p < rates", name="PoissonPre")
-    post_n = ah.Neuron(parameters="rates = 30.0", equations="p = Uniform(0.0, 1.0) * 1000.0 / dt\nmp = g_exc",
-                       spike="p < rates", name="PoissonPostG")
-    stdp = ah.STDP(tau_plus=20.0, tau_minus=20.0, A_plus=0.01, A_minus=0.0105, w_min=0.0, w_max=1.0)
-    t_b = time.perf_counter()
-    net = ah.Network(dt=1.0, seed=17, precision=prec)
-    pre = net.create(n_pre, pre_n)
-    post = net.create(n_post, post_n)
-    proj = net.connect(pre, post, "exc", synapse=stdp).fixed_number_pre(k, cn.Uniform(0.2, 0.8),
-                                                                         generator="device")
-    m_pre = net.monitor(pre, ["spike"], capacity=warmup + steps + 40)
-    m_post = net.monitor(post, ["spike"], capacity=warmup + steps + 40)
-    net.compile(allow_build=False)
-    pre.rates = 20.0
-    post.rates = 10.0
-    net.set_graph(True, 50)
-    log(f"stdp network ({proj.nb_synapses} plastic synapses) set up in {time.perf_counter() - t_b:.1f} s")
-    w0 = proj.w.copy()
-    net.run(warmup)
-    w_warm = proj.w.copy()
+
+def _sweep_worker(args):
+    """One instance of the sweep on one CPU core (parallel_run semantics:
+    every instance is its own single-threaded simulation)."""
+    from oracle import oracle as orc
+
+    i, steps = args
+    sw = _SWEEP
     t0 = time.perf_counter()
-    net.run(steps)
-    el = time.perf_counter() - t0
-    pc, pi = m_pre.raw_spikes()
-    qc, qi = m_post.raw_spikes()
-    pc, qc = pc[:warmup + steps], qc[:warmup + steps]
-    pi, qi = pi[:int(pc.sum())], qi[:int(qc.sum())]
-    csr = proj.csr
-    outdeg = np.bincount(csr.col_idx, minlength=n_pre)
-    n_pw = int(pc[:warmup].sum())
-    ev_pre = int(outdeg[pi[n_pw:]].sum())
-    ev_post = int(qc[warmup:].sum()) * k
-    events = ev_pre + ev_post
-    log(f"stdp: {steps} steps in {el:.3f} s, {events / steps:.3e} synapse events/step")
-    # kernel times inside 2 x 20 further steps (HIP events around the kernel
-    # in each step); algorithmic bytes of exactly those steps' events
-    n_done = warmup + steps
-    t_pre = proj.time_events("pre", 20)
-    t_post = proj.time_events("post", 20)
-    pc2, pi2 = m_pre.raw_spikes()
-    qc2, qi2 = m_post.raw_spikes()
-    starts = np.concatenate([[0], np.cumsum(pc2)])
-    # pre events of step u handle the pre spikes of step u - 1
-    pre_steps = range(n_done - 1, n_done + 19)
-    syn_pre = sum(int(outdeg[pi2[starts[u]:starts[u + 1]]].sum()) for u in pre_steps) / 20.0
-    spk_pre = sum(int(pc2[u]) for u in pre_steps) / 20.0
-    spk_post = float(qc2[n_done + 20:n_done + 40].mean())
-    syn_post = spk_post * k
-    b_pre = syn_pre * (20 + 8 * es) + spk_pre * 20
-    b_post = syn_post * (24 + 6 * es) + spk_post * 20
-    # streaming-copy probe: the achievable HBM rate and the counter calibration
-    import ctypes as C
-    from annarchy_amd import _lib
-    copy_ms = C.c_double()
-    _lib.check(_lib.lib().ahip_probe_copy(1 << 31, 10, C.byref(copy_ms)))
-    copy_gbs = 2.0 * (1 << 31) / (copy_ms.value * 1e-3) / 1e9
-    kern = {"pre_event": {"ms": t_pre, "bytes": b_pre, "syn_events": syn_pre},
-            "post_event": {"ms": t_post, "bytes": b_post, "syn_events": syn_post}}
-    dom = "pre_event" if t_pre >= t_post else "post_event"
-    ach = kern[dom]["bytes"] / (kern[dom]["ms"] * 1e-3) / 1e9
-    traffic = traffic_from_profiles("stdp_events_f64_5e7syn_traffic.txt") if (prec == "double" and
-                                                                                dom == "pre_event") else None
-    roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "traffic": traffic, "kernel": dom}
-    cpu, parity = None, None
-    if not args.no_cpu and prec == "double":
-        from oracle import oracle as orc
-        n_qw = int(qc[:warmup].sum())
-        t1 = time.perf_counter()
-        ref = orc.plastic_run(0, csr.row_ptr, csr.col_idx, n_pre, w0, [20.0, 20.0, 0.01, 0.0105, 0.0, 1.0], 1.0,
-                              pc[:warmup], pi[:n_pw], qc[:warmup], qi[:n_qw])
-        cel = time.perf_counter() - t1
-        ev_w = int(outdeg[pi[:n_pw]].sum()) + n_qw * k
-        parity = float(np.max(np.abs(ref["w"] - w_warm)))
-        cpu = {"value": ev_w / cel, "unit": "synaptic events/s", "cores": 1, "kind": "port",
-               "sample": f"the {warmup} warm-up steps ({ev_w:.3e} synapse events) of the same network, "
-                         f"single thread; max |w_gpu - w_oracle| = {parity:.2e}"}
-    return _line("synaptic events/s", events / el, "synaptic events/s", steps, warmup, el * 1e3 / steps,
-                 "f64" if prec == "double" else "f32",
-                 {"workload": "STDP 100000 Poisson(20 Hz) -> 50000 Poisson(10 Hz), fixed_number_pre 1000 "
-                              "(5e7 plastic synapses), dt=1ms", "precision": prec},
-                 roof, cpu, kernels=kern, w_max_abs_diff_vs_oracle=parity, hbm_copy_gbs=copy_gbs,
-                 pre_spikes_per_step=float(pc[warmup:].mean()), post_spikes_per_step=float(qc[warmup:].mean()))
+    orc.sweep_instance_run(sw.recurrent[i], sw.w_in, sw.phase[:, i], sw.omega, float(sw.gains[i]), sw.tau,
+                           sw.dt, sw.r0[i], steps)
+    return time.perf_counter() - t0
+
+
+def c3_cpu_baseline(sw, steps, workers):
+    """core/Network.py:751-926 (parallel_run): one process per instance,
+    `workers` at a time, each advancing its instance `steps` steps through
+    the oracle.  Forked before this process touches the GPU."""
+    import multiprocessing as mp
+
+    global _SWEEP
+    _SWEEP = sw
+    ctx = mp.get_context("fork")
+    order = sorted(range(sw.n_inst), key=lambda i: -int(sw.recurrent[i].nnz))
+    t0 = time.perf_counter()
+    with ctx.Pool(workers) as pool:
+        per = pool.map(_sweep_worker, [(i, steps) for i in order], chunksize=1)
+    wall = time.perf_counter() - t0
+    _SWEEP = None
+    return steps / wall, wall, float(sum(per))
 
 
 def bench_c3(args):
-    """Batched sweep: 256 instances, N_i ~ U{500..8000}, recurrent p=0.05,
+    """Batched sweep: 256 instances, N_i ~ U{500..8000}, recurrent p = 0.05,
     shared dense 2000 x 2000 input, gains linspace(0.1, 2.0)."""
-    from annarchy_amd.batched import RateSweep, time_plain
-
-    steps = args.steps if args.steps != 200 else 2000
+    steps = args.c3_steps
     rng = np.random.default_rng(2026)
     sizes = rng.integers(500, 8001, 256)
     t_b = time.perf_counter()
+    from annarchy_amd.batched import RateSweep, time_plain
     sw = RateSweep(sizes, n_in=2000, p=0.05, seed=3)
-    log(f"c3 sweep: {sw.total} neurons, {sw.nnz} synapses, built in {time.perf_counter() - t_b:.0f} s")
+    log(f"c3: sweep of {sw.total} neurons, {sw.nnz} synapses built in {time.perf_counter() - t_b:.0f} s")
+    cpu = None
+    if not args.no_cpu:
+        h = host_cpu()
+        cs = 20
+        sps, wall, busy = c3_cpu_baseline(sw, cs, h["threads"])
+        cpu = {"value": sps, "unit": "steps/s", "cores": h["threads"], "kind": "port",
+               "sample": f"{cs} steps of all 256 instances, one single-threaded oracle process per instance, "
+                         f"{h['threads']} at a time (parallel_run, core/Network.py:751-926): {wall:.1f} s wall, "
+                         f"{busy:.1f} s of CPU work", "host": h}
+        log(f"c3: cpu baseline {sps:.2f} steps/s on {h['threads']} processes")
     b = sw.batched(lanes=16)
-    b.run(20)
+    G = graph_batch_for(steps, 50)
+    b.run(20, G)
     t0 = time.perf_counter()
-    b.run(steps)
+    b.run(steps, G)
     el_b = time.perf_counter() - t0
-    # streaming-copy probe: the achievable HBM rate, and the calibration of the
-    # HBM byte counters of a profiled run
-    import ctypes as C
-    from annarchy_amd import _lib
-    copy_ms = C.c_double()
-    _lib.check(_lib.lib().ahip_probe_copy(1 << 31, 10, C.byref(copy_ms)))
-    copy_gbs = 2.0 * (1 << 31) / (copy_ms.value * 1e-3) / 1e9
     info = b.info()
     ms_spmv = b.time_kernel(2, 20)
     ms_gemm = b.time_kernel(1, 50)
-    log(f"batched {el_b * 1e3 / steps:.3f} ms/step, spmv {ms_spmv * 1e3:.0f} us, gemm {ms_gemm * 1e3:.1f} us")
-    # plain per-instance FP64 loop, same initial state, same number of steps
+    log(f"c3: batched {el_b * 1e3 / steps:.3f} ms/step, spmv {ms_spmv * 1e3:.0f} us, gemm {ms_gemm * 1e3:.1f} us")
     nets = sw.plain_networks()
     plain_steps = min(steps, 200)
     el_p = time_plain(nets, plain_steps)
-    # error of the batched form against the FP64 plain form after `steps`
     for net, exc in nets:
         net.run(20 + steps - plain_steps)
     got = b.rates()
@@ -518,19 +504,171 @@ def bench_c3(args):
     sps_b = steps / el_b
     sps_p = plain_steps / el_p
     achieved = info["sparse_bytes"] / (ms_spmv * 1e-3) / 1e9
-    return _line("steps/s (batched sweep)", sps_b, "steps/s", steps, 20, el_b * 1e3 / steps, "f32/bf16",
-                 {"workload": "256 rate instances N~U{500..8000} p=0.05 + shared dense 2000x2000 input",
-                  "neurons": sw.total, "synapses": sw.nnz},
-                 {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                  "frac": achieved / HBM_PEAK_GBS,
-                  "traffic": traffic_from_profiles("batch_sell_f32_c3_sweep256_traffic.txt") if info["mode"] == 2
-                  else None,
-                  "kernel": {0: "batch_spmv_update", 1: "batch_spmv_lds", 2: "batch_sell"}[info["mode"]],
-                  "kernel_ms": ms_spmv},
-                 None, plain_steps_per_s=sps_p, speedup_vs_plain=sps_b / sps_p,
-                 max_rel_rate_error_vs_fp64=worst, max_rel_population_rate_error=pop_err,
-                 gemm={"ms": ms_gemm, "tflops": 3 * info["dense_flops"] / (ms_gemm * 1e-3) / 1e12,
-                       "peak_bf16_tflops": 2500.0}, hbm_copy_gbs=copy_gbs)
+    tflops = info["dense_flops"] / (ms_gemm * 1e-3) / 1e12
+    roof = roofline("hbm", achieved, HBM_PEAK_GBS, "GB/s",
+                    traffic_from_profiles("batch_sell_f32_c3_sweep256_traffic.txt") if info["mode"] == 2 else None,
+                    {0: "batch_spmv_update", 1: "batch_spmv_lds", 2: "batch_sell"}[info["mode"]],
+                    kernel_ms=ms_spmv, algorithmic_bytes=info["sparse_bytes"])
+    return record("steps/s (batched sweep)", sps_b, "steps/s", steps, 20, el_b * 1e3 / steps, "f32/bf16",
+                  {"workload": "256 rate instances N~U{500..8000} p=0.05 + shared dense 2000x2000 input, 2000 steps",
+                   "neurons": sw.total, "synapses": sw.nnz, "graph_steps": G},
+                  roof, cpu, plain_steps_per_s=sps_p, speedup_vs_plain=sps_b / sps_p,
+                  max_rel_rate_error_vs_fp64=worst, max_rel_population_rate_error=pop_err,
+                  gemm={"kernel": "batch_gemm", "ms": ms_gemm, "useful_flop": info["dense_flops"],
+                        "tflops": tflops, "peak_bf16_tflops": BF16_PEAK_TFLOPS,
+                        "frac": tflops / BF16_PEAK_TFLOPS})
+
+
+# ---------------------------------------------------------------------------
+# C4: the learned selector
+# ---------------------------------------------------------------------------
+def bench_c4(args):
+    """Held-out regret of the trained selector against exhaustive tuning,
+    timed live: a sample of held-out corpus matrices is regenerated from its
+    seeds, every feasible configuration is timed on the device (the
+    exhaustive autotuner), and the MLP's and the reference heuristic's picks
+    are compared with the best."""
+    import json as js
+
+    from annarchy_amd import selector as S
+    from oracle import oracle as orc
+
+    d = np.load(S.DATA_DIR / "selector_corpus.npz")
+    specs = [js.loads(s) for s in d["specs"]]
+    heur = np.array([s["heuristic"] for s in specs])
+    _, te = S.holdout_split(d["times"], heur)
+    pick = te[:: max(1, len(te) // args.c4_sample)][: args.c4_sample]
+    sel = S.MLPSelector.load()
+    T, choice, hc, nnz = [], [], [], []
+    t0 = time.perf_counter()
+    for i in pick:
+        csr = S.generate(specs[i])
+        f = S.features(csr, specs[i]["seed"] & 0xFFFF)
+        c = int(sel.predict(f)[0])
+        t = S.autotune(csr)
+        T.append(t)
+        choice.append(c)
+        hc.append(int(heur[i]))
+        nnz.append(csr.nnz)
+    el = time.perf_counter() - t0
+    T = np.array(T)
+    choice = S.with_fallback(T, np.array(choice))
+    reg = S.regret(T, choice)
+    reg_h = S.regret(T, S.with_fallback(T, np.array(hc)))
+    # roofline: the selected configuration's psp on the largest matrix of the sample
+    k = int(np.argmax(nnz))
+    csr = S.generate(specs[pick[k]])
+    from annarchy_amd.models import RateNoisy
+    from annarchy_amd.network import Network
+    net = Network(dt=1.0, seed=0)
+    pop = net.create(csr.n_rows, RateNoisy)
+    proj = net.connect(pop, pop, "exc")
+    proj.from_csr(csr, storage_format="csr", lanes=64)
+    net.compile(allow_build=False)
+    proj.set_format(**S.CLASSES[int(choice[k])])
+    ms = proj.time_psp(50)
+    pb = proj.psp_bytes()
+    ach = pb / (ms * 1e-3) / 1e9
+    roof = roofline("hbm", ach, HBM_PEAK_GBS, "GB/s", None,
+                    f"psp of the selected configuration {S.CLASSES[int(choice[k])]}",
+                    kernel_ms=ms, algorithmic_bytes=pb, matrix_nnz=int(csr.nnz))
+    cpu = None
+    if not args.no_cpu:
+        n = csr.n_rows
+        r = np.random.default_rng(1).random(n)
+        noise = np.zeros((1, n))
+        t1, steps = time.perf_counter(), 0
+        while time.perf_counter() - t1 < 2.0:
+            orc.rate_run(csr.row_ptr, csr.col_idx, csr.w, r, 10.0, 1.0, noise, 1, nthreads=1)
+            steps += 1
+        cel = time.perf_counter() - t1
+        cpu = {"value": csr.nnz * steps / cel, "unit": "synaptic events/s", "cores": 1, "kind": "port",
+               "sample": f"{steps} oracle CSR steps of the largest sampled matrix ({csr.nnz} synapses), single thread",
+               "host": host_cpu()}
+    net.destroy()
+    log(f"c4: {len(pick)} held-out matrices tuned in {el:.1f} s: MLP regret {reg.mean():.4f}, "
+        f"heuristic {reg_h.mean():.3f}")
+    return record("held-out regret vs exhaustive tuning", float(reg.mean()), "fraction", len(pick), 0,
+                  el * 1e3 / len(pick), "f32",
+                  {"workload": f"learned selector, {len(pick)} held-out matrices of the 2000-matrix corpus "
+                               "(N 1e3..1e6), MLP 16-64-64-12, 12 configurations timed live"},
+                  roof, cpu, higher=False, heuristic_regret=float(reg_h.mean()),
+                  regret_median=float(np.median(reg)), regret_p90=float(np.quantile(reg, 0.9)),
+                  total_time_ratio=float(T[np.arange(len(T)), choice].sum() / T.min(axis=1).sum()))
+
+
+# ---------------------------------------------------------------------------
+# driver
+# ---------------------------------------------------------------------------
+def run_child(cfg, args):
+    cmd = [sys.executable, "-u", str(Path(__file__).resolve()), "--config", cfg, "--single",
+           "--steps", str(args.steps), "--warmup", str(args.warmup)]
+    if args.no_cpu:
+        cmd.append("--no-cpu")
+    log(f"bench: {cfg} ...")
+    t0 = time.perf_counter()
+    try:
+        res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, text=True,
+                             timeout=CHILD_TIMEOUT_S.get(cfg, 300))
+    except subprocess.TimeoutExpired:
+        return {"error": f"{cfg} exceeded {CHILD_TIMEOUT_S.get(cfg, 300)} s"}
+    lines = [ln for ln in res.stdout.splitlines() if ln.strip().startswith("{")]
+    if res.returncode != 0 or not lines:
+        return {"error": f"{cfg} exited with {res.returncode}"}
+    out = json.loads(lines[-1])
+    out["child_wall_s"] = time.perf_counter() - t0
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--neurons", type=int, default=100_000)
+    ap.add_argument("--degree", type=int, default=1000)
+    ap.add_argument("--lognormal", action="store_true")
+    ap.add_argument("--precision", default="double", choices=["double", "float"])
+    # default: sliced tiles (2048-row blocks, 1024 in FP32, x 20479-column
+    # panels staged in 160 KiB of LDS, sorted 64-row slices, one lane per row,
+    # 2 steps of 4 synapses in flight, 8 waves per workgroup)
+    ap.add_argument("--format", default="sell_panel")
+    ap.add_argument("--lanes", type=int, default=2,
+                    help="csr/csr_vec/csr_panel: lanes per row; sell: threads per row; sell_panel: steps in flight")
+    ap.add_argument("--sigma", type=int, default=1, help="sell: sorting window")
+    ap.add_argument("--waves", type=int, default=8, help="sell_panel: waves per workgroup")
+    ap.add_argument("--panel-cols", type=int, default=20479,
+                    help="sell_panel: columns per panel at most (20479: 160 KiB of LDS in FP64)")
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="sell_panel: rows per tile (0: 2048 in FP64, 1024 in FP32)")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", default="all", choices=["all", "c0", "c1", "c2", "c3", "c4"],
+                    help="all (default): c1 headline + c0, c2, c3, c4 records, each in a child process")
+    ap.add_argument("--single", action="store_true", help="run one config in this process")
+    ap.add_argument("--c2-steps", type=int, default=2000)
+    ap.add_argument("--c3-steps", type=int, default=2000)
+    ap.add_argument("--c4-sample", type=int, default=24)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: every GPU box of this setup has one GPU, and the simulation "
+              "step of this backend is a single-device path (sharding is a design note, not built).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    if args.single or args.config not in ("all",):
+        if args.config == "all":
+            args.config = "c1"
+        print(json.dumps(globals()["bench_" + args.config](args)), flush=True)
+        return
+
+    head = run_child("c1", args)
+    if "error" in head:
+        print(json.dumps({"metric": "synaptic events/s", "value": None, "error": head["error"]}))
+        sys.exit(1)
+    head["configs"] = {cfg: run_child(cfg, args) for cfg in SUB_CONFIGS}
+    print(json.dumps(head), flush=True)
 
 
 if __name__ == "__main__":

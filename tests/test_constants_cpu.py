@@ -60,3 +60,24 @@ def test_constant_copies():
             ah.Network().get_constant("own_c")
     finally:
         ahnet._GLOBAL_CONSTANTS.pop("tmp_c", None)
+
+
+def test_slot_list_follows_projections_added_after_an_attribute_write():
+    # setting an attribute before connect() must not freeze the slot list:
+    # the projection adds the population's _sum_exc (core/Population.py:617-635)
+    import warnings
+
+    import annarchy_amd as ah
+
+    net = ah.Network(dt=1.0)
+    pre = net.create(4, ah.Neuron(equations="r = 1", name="SlotPre"))
+    post = net.create(3, ah.Neuron(parameters="b = 0.0", equations="r = b", name="SlotPost"))
+    post.b = 1.0
+    assert "_sum_exc" not in post._slot_info()
+    net.connect(pre, post, "exc")
+    assert "_sum_exc" in post._slot_info()
+    # a target no projection uses: zeros of the population's shape, with a warning
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        z = post.sum("inh")
+    assert z.shape == (3,) and not z.any() and len(w) == 1
