@@ -238,6 +238,8 @@ struct ahip_net {
     int32_t graph_steps = 0;
     bool dirty = true;             // structure changed since the last capture
     bool scatter_dirty = true;     // the fused scatter tables need rebuilding
+    uint64_t scatter_gen = 0;       // bumped whenever the scatter tables are rebuilt
+    uint64_t graph_scatter_gen = ~0ull;  // the tables generation the step graph was captured with
     // kernel-time probe: HIP events around one plastic kernel inside eager steps
     int32_t probe_proj = -1, probe_kind = -1;
     hipEvent_t probe_ev[2] = {nullptr, nullptr};

@@ -310,9 +310,10 @@ __global__ __launch_bounds__(256) void deliver_ordered(int32_t n, const int64_t*
     __syncthreads();
     const int32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const int32_t L = cnt[i];
-    if (L == 0) return;
     const int64_t b = boff[i];
+    // a bucket never holds more events than the neuron's summed in-degree
+    const int32_t L = (int32_t)min((int64_t)cnt[i], boff[i + 1] - b);
+    if (L <= 0) return;
     T acc = g[i];
     if (L == 1) {
         acc += deliver_weight<T>(ls, n_src, bucket[b]);

@@ -87,14 +87,22 @@ Monitor* get_mon(ahip_net* net, int32_t id) {
     return net->mons[id].get();
 }
 
-void invalidate(ahip_net* net) {
+// Drops the instantiated step graph: its kernel nodes hold the device
+// addresses of the buffers and tables they were captured with, so any
+// reallocation of one of them must come with a recapture.
+void drop_graph(ahip_net* net) {
     net->dirty = true;
-    net->scatter_dirty = true;
     if (net->graph_exec) {
         hipGraphExecDestroy(net->graph_exec);
         net->graph_exec = nullptr;
         net->graph_steps = 0;
     }
+}
+
+// Topology or buffers changed: rebuild the scatter tables and the graph.
+void invalidate(ahip_net* net) {
+    drop_graph(net);
+    net->scatter_dirty = true;
 }
 
 bool layout_covers_all_rows(const Proj& p) {
@@ -384,7 +392,10 @@ bool transmits(const ahip_net* net, const Proj& p) {
 int build_ordered_targets(ahip_net* net, const std::vector<const void*>& targets) {
     std::vector<OrderedTarget> old;
     old.swap(net->ordered);
+    std::vector<const void*> seen;
     for (const void* g : targets) {
+        if (std::find(seen.begin(), seen.end(), g) != seen.end()) continue;  // one target per conductance
+        seen.push_back(g);
         OrderedTarget o;
         o.g = const_cast<void*>(g);
         std::vector<DeliverSrc> src;
@@ -450,7 +461,10 @@ int prepare(ahip_net* net) {
     for (auto& pp : net->pops) {
         Pop& p = *pp;
         if (p.active || p.depth <= 1 || !p.hist.p) continue;
-        if (p.ring_tmp.bytes < p.hist.bytes && (rc0 = dev_alloc(p.ring_tmp, p.hist.bytes))) return rc0;
+        if (p.ring_tmp.bytes < p.hist.bytes) {
+            drop_graph(net);  // the rotation's copy nodes address ring_tmp
+            if ((rc0 = dev_alloc(p.ring_tmp, p.hist.bytes))) return rc0;
+        }
     }
     // fused scatter groups: spiking projections in creation order, each in the
     // first group after the last one that already writes its target, so a
@@ -459,6 +473,9 @@ int prepare(ahip_net* net) {
     // such order (its atomic sum is order-dependent in any case), so its
     // projections share groups freely.
     if (net->scatter_dirty) {
+        // the tables freed here are the ones a captured graph reads
+        drop_graph(net);
+        net->scatter_gen++;
         net->scatter_groups.clear();
         std::vector<std::vector<ScatterDesc>> groups;
         std::vector<int64_t> blocks;
@@ -762,8 +779,11 @@ int enqueue_batch(ahip_net* net, int32_t nsteps, const std::vector<int32_t>& zb)
     return 0;
 }
 
+// Captures `steps` steps with the scatter tables and buffers that prepare()
+// left in place (it must run first); only the previous executable graph is
+// dropped, so the tables stay valid for as long as the new graph lives.
 int build_graph(ahip_net* net, int32_t steps, const std::vector<int32_t>& zb) {
-    invalidate(net);
+    drop_graph(net);
     HIPCHK(hipStreamBeginCapture(net->stream, hipStreamCaptureModeThreadLocal));
     int rc = enqueue_batch(net, steps, zb);
     hipGraph_t g = nullptr;
@@ -780,7 +800,15 @@ int build_graph(ahip_net* net, int32_t steps, const std::vector<int32_t>& zb) {
         return fail(AHIP_EHIP, "graph instantiation failed: %s", hipGetErrorString(e));
     }
     net->graph_steps = steps;
+    net->graph_scatter_gen = net->scatter_gen;
     net->dirty = false;
+    return 0;
+}
+
+// A captured graph launches only with the scatter tables it was captured with.
+int check_graph_tables(const ahip_net* net) {
+    if (net->graph_scatter_gen != net->scatter_gen || net->scatter_dirty)
+        return fail(AHIP_ESTATE, "internal: the step graph was captured with scatter tables that were rebuilt since");
     return 0;
 }
 
@@ -947,6 +975,7 @@ int ahip_graph_prepare(ahip_net* net) {
     if (net->dirty || !net->graph_exec || net->graph_steps != G) {
         if ((rc = build_graph(net, G, zb))) return rc;
     }
+    if ((rc = check_graph_tables(net))) return rc;
     HIPCHK(hipGraphUpload(net->graph_exec, net->stream));
     HIPCHK(hipStreamSynchronize(net->stream));
     return 0;
@@ -981,6 +1010,7 @@ int ahip_run(ahip_net* net, int64_t nsteps) {
             rc = build_graph(net, G, zb);
             if (rc) return rc;
         }
+        if ((rc = check_graph_tables(net))) return rc;
         while (left >= G) {
             HIPCHK(hipGraphLaunch(net->graph_exec, net->stream));
             left -= G;

@@ -175,3 +175,36 @@ def test_c2_shape_fp32_rate_and_isi_cv(gpu):
     assert r64 > 0.5
     assert abs(r32 / r64 - 1.0) < 0.02
     assert abs(cv32 / cv64 - 1.0) < 0.02
+
+
+def test_prepared_graph_survives_eager_steps(gpu):
+    # Regression: a graph captured by prepare_graph() must still address live
+    # scatter tables after an eager run of fewer steps than the graph batch
+    # (the sequence of the benchmark: prepare, warm up eagerly, replay).
+    # 4000 COBA neurons with per-synapse Normal weights (ordered delivery),
+    # graph of 20 steps: 7 eager steps, then 60 replayed, then 5 eager;
+    # spike lists, v and g bit-exact against the oracle.
+    steps_a, steps_b, steps_c, G = 7, 60, 5, 20
+    steps = steps_a + steps_b + steps_c
+    net = ah.Network(dt=0.1, seed=5)
+    P = net.create(4000, ah.COBA)
+    ce = net.connect(P[:3200], P, "exc").fixed_probability(0.02, cn.Normal(0.6, 0.15))
+    ci = net.connect(P[3200:], P, "inh").fixed_probability(0.02, cn.Uniform(3.0, 9.0))
+    P.v = ah.Uniform(-60.0, -50.0, rng=net.rng)
+    mon = net.monitor(P, ["spike"], capacity=steps)
+    net.compile(graph_batch=G)
+    P.I = 20.0
+    assert net.ordered_targets() == 2
+    v0 = P.v.reshape(-1).copy()
+    net.prepare_graph()
+    net.run(steps_a)
+    net.run(steps_b)
+    net.run(steps_c)
+    po = [orc.csrc_projection(ce.csr, 0), orc.csrc_projection(ci.csr, 1)]
+    ref = orc.spiking_run(0, 4000, 0.1, v0, steps, po, refractory=np.asarray(P.refractory, np.int32))
+    counts, idx = mon.raw_spikes()
+    assert counts.sum() > 100, "network is silent"
+    np.testing.assert_array_equal(counts, ref["counts"])
+    np.testing.assert_array_equal(idx, ref["idx"])
+    assert np.array_equal(P.v.reshape(-1), ref["v"])
+    assert np.array_equal(P.g_exc.reshape(-1), ref["g_exc"])
