@@ -138,6 +138,7 @@ _SIGS = {
     "ahip_batch_run": ([_P, _I64, _I32], _I32),
     "ahip_batch_time": ([_P, _I32, _I32, _PD], _I32),
     "ahip_batch_info": ([_P, _PI64, _PI32, _PD, _PD], _I32),
+    "ahip_batch_chunk_steps": ([_P, _PI32], _I32),
     "ahip_batch_set_mode": ([_P, _I32], _I32),
     "ahip_batch_get_mode": ([_P, _PI32], _I32),
     "ahip_mlp_train": ([_PF, _PF, _I32, _PF, _I32, C.c_float, C.c_float, _PF, _PF], _I32),

@@ -9,8 +9,10 @@ The reference runs a sweep as independent networks, one process each
 * :class:`BatchedSweep` -- every instance in one device state: the recurrent
   projections as one block-diagonal CSR (FP32, compensated sums) fused with
   the neuron update, and the input projection whose weights all instances
-  share as one GEMM on MFMA (split-BF16 operands, FP32 accumulation); three
-  kernels per step captured in a HIP graph.
+  share as one GEMM on MFMA per graph batch (its input rates depend on t
+  alone, so the rows of the product are (step, instance) pairs; split-BF16
+  operands, FP32 accumulation); one SpMV + update launch per step, all
+  captured in a HIP graph.
 
 Instance i:  input population (n_in neurons)  r = 1 + 0.5*sin(omega*t + phase)
              exc population (N_i neurons)  tau dr/dt = -r + sum(exc) + gain_i*sum(inp), r >= 0
@@ -179,9 +181,11 @@ class BatchedSweep:
     def info(self):
         nnz, total, sb, df = C.c_int64(), C.c_int32(), C.c_double(), C.c_double()
         self._check(_lib.lib().ahip_batch_info(self._h, C.byref(nnz), C.byref(total), C.byref(sb), C.byref(df)))
-        mode = C.c_int32()
+        mode, chunk = C.c_int32(), C.c_int32()
         self._check(_lib.lib().ahip_batch_get_mode(self._h, C.byref(mode)))
-        return dict(nnz=nnz.value, total=total.value, sparse_bytes=sb.value, dense_flops=df.value, mode=mode.value)
+        self._check(_lib.lib().ahip_batch_chunk_steps(self._h, C.byref(chunk)))
+        return dict(nnz=nnz.value, total=total.value, sparse_bytes=sb.value, dense_flops=df.value, mode=mode.value,
+                    chunk_steps=chunk.value)
 
     def __del__(self):
         try:

@@ -7,6 +7,7 @@ separate spike_gather that appends with atomics).  Here one kernel per model
 does the whole per-neuron work of a step, one thread per neuron, 256 threads
 per workgroup (4 waves):
 
+    ordered spike delivery into g_<target>               (when the runtime asks)
     in_ref = refractory_remaining[i] > 0 ? 0 : 1         (if refractory)
     random draws (Philox, or the injected stream)
     ODE increments, switches (masked), bounds            (spliced text)
@@ -27,7 +28,7 @@ import re
 
 from .equations import DISTRIBUTIONS, NeuronDescription, analyse_neuron, equation_code
 
-ABI = 3  # AHIP_MODEL_ABI in include/annarchy_hip_model.h
+ABI = 4  # AHIP_MODEL_ABI in include/annarchy_hip_model.h
 
 _PRELUDE = r"""// Generated neuron-model library for the MI355X simulation backend.
 // Model: %(name)s
@@ -79,6 +80,9 @@ __global__ __launch_bounds__(AHIP_UPDATE_BLOCK) void local_update(ahip_pop_args 
     const int64_t t = ahip::sim_time(a);
     const T dt = (T)a.dt;
     (void)t; (void)dt;
+    // this step's spike events of the ordered conductances (end of compute_psp)
+    if (a.n_deliver > 0 && i < a.size) ahip::deliver_buckets<T>(a, i);
+    asm volatile("" ::: "memory");  // the equations below read the delivered values
 %(global_loads)s
 %(local_ptrs)s
     bool _spike = false;

@@ -24,6 +24,7 @@ struct ahip_batch {
     hipGraphExec_t graph = nullptr;
     int32_t graph_steps = 0;
     int32_t graph_parity = 0;
+    int32_t last_chunk = 0;        // steps of the last chunk enqueued (its Y^T is in B.y)
     std::vector<int32_t> sizes;
     std::vector<int64_t> offsets;
 };
@@ -82,14 +83,34 @@ void drop_graph(ahip_batch* b) {
     b->graph_steps = 0;
 }
 
+// X and Y buffers for chunks of up to `steps` steps (reallocated only to grow).
+int ensure_chunk(ahip_batch* b, int32_t steps) {
+    BatchDev& B = b->B;
+    if (steps <= B.cap_steps) return 0;
+    BCHK(hipStreamSynchronize(b->stream));
+    drop_graph(b);
+    const size_t frag = (size_t)B.k_steps * 64 * 8 * sizeof(uint16_t);
+    const size_t c_tiles = (size_t)steps * B.n_tiles;
+    int rc;
+    if ((rc = balloc(B.x_hi, frag * c_tiles))) return rc;
+    if ((rc = balloc(B.x_lo, frag * c_tiles))) return rc;
+    if ((rc = balloc(B.y, c_tiles * 32 * (size_t)B.m_tiles * 32 * sizeof(float)))) return rc;
+    B.cap_steps = steps;
+    return 0;
+}
+
 int enqueue_steps(ahip_batch* b, int32_t n, int32_t parity0) {
-    // reference step order: both psp first (from the rates of step t-1: the
-    // input rates X were produced by the previous step's input update), then
-    // the updates of the exc population (fused into the SpMV) and of the input
-    for (int32_t j = 0; j < n; j++) {
-        BLCHK(batch_launch_gemm(b->B, b->stream));
-        BLCHK(batch_launch_spmv(b->B, (parity0 + j) & 1, b->stream));
-        BLCHK(batch_launch_input(b->B, j, b->stream));
+    // reference step order: both psp first, from the rates of step t-1 (the
+    // input rates of step t-1 are a function of t alone, so the input psp of
+    // a whole chunk of steps is computed up front), then the update of the
+    // exc population, fused into the SpMV
+    const int32_t cap = b->B.cap_steps;
+    for (int32_t j0 = 0; j0 < n; j0 += cap) {
+        const int32_t m = std::min(cap, n - j0);
+        BLCHK(batch_launch_input(b->B, j0, m, b->stream));
+        BLCHK(batch_launch_gemm(b->B, m, b->stream));
+        for (int32_t j = 0; j < m; j++) BLCHK(batch_launch_spmv(b->B, (parity0 + j0 + j) & 1, j, b->stream));
+        b->last_chunk = m;
     }
     BLCHK(batch_launch_advance(b->B, n, b->stream));
     return 0;
@@ -119,7 +140,7 @@ int ahip_batch_create(int32_t n_inst, const int32_t* sizes, int32_t n_in, double
     b->B.total = (int32_t)b->offsets[n_inst];
     b->B.m_tiles = (n_in + 31) / 32;
     b->B.n_tiles = (n_inst + 31) / 32;
-    b->B.k_steps = (n_in + 7) / 8;
+    b->B.k_steps = (n_in + 15) / 16;
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete b; return bfail(AHIP_EHIP, "stream: %s", hipGetErrorString(e)); }
     int rc;
@@ -137,14 +158,12 @@ int ahip_batch_create(int32_t n_inst, const int32_t* sizes, int32_t n_in, double
     if ((rc = balloc(B.sum_exc, (size_t)B.total * 4))) return rc;
     std::vector<float> g(n_inst, 1.0f);
     if ((rc = balloc(B.gain, g.size() * 4, g.data()))) return rc;
-    const size_t frag = (size_t)B.k_steps * 64 * 4 * sizeof(uint16_t);
+    const size_t frag = (size_t)B.k_steps * 64 * 8 * sizeof(uint16_t);
     if ((rc = balloc(B.w_hi, frag * B.m_tiles))) return rc;
     if ((rc = balloc(B.w_lo, frag * B.m_tiles))) return rc;
-    if ((rc = balloc(B.x_hi, frag * B.n_tiles))) return rc;
-    if ((rc = balloc(B.x_lo, frag * B.n_tiles))) return rc;
     if ((rc = balloc(B.phase, (size_t)n_in * n_inst * 8))) return rc;
-    if ((rc = balloc(B.y, (size_t)B.m_tiles * 32 * B.n_tiles * 32 * 4))) return rc;
     if ((rc = balloc(B.t_dev, 8))) return rc;
+    if ((rc = ensure_chunk(b, 1))) return rc;
     *out = b;
     return 0;
 }
@@ -291,15 +310,16 @@ int ahip_batch_set_input(ahip_batch* b, const double* w_in, const double* phase,
     if (!b || !w_in || !phase) return bfail(AHIP_EINVAL, "null argument");
     BatchDev& B = b->B;
     const int32_t n = B.n_in;
-    std::vector<uint16_t> hi((size_t)B.m_tiles * B.k_steps * 64 * 4, 0), lo(hi.size(), 0);
+    // B operand W^T: element (k, m) of tile (mt, ks) at lane m%32 + 32*((k%16)/8), item k%8
+    std::vector<uint16_t> hi((size_t)B.m_tiles * B.k_steps * 64 * 8, 0), lo(hi.size(), 0);
     for (int32_t mt = 0; mt < B.m_tiles; mt++)
         for (int32_t ks = 0; ks < B.k_steps; ks++)
             for (int lane = 0; lane < 64; lane++)
-                for (int j = 0; j < 4; j++) {
-                    const int32_t row = mt * 32 + lane % 32;
-                    const int32_t k = ks * 8 + 4 * (lane / 32) + j;
-                    float v = (row < n && k < n) ? (float)w_in[(size_t)row * n + k] : 0.0f;
-                    const size_t pos = (((size_t)mt * B.k_steps + ks) * 64 + lane) * 4 + j;
+                for (int j = 0; j < 8; j++) {
+                    const int32_t m = mt * 32 + lane % 32;
+                    const int32_t k = ks * 16 + 8 * (lane / 32) + j;
+                    float v = (m < n && k < n) ? (float)w_in[(size_t)m * n + k] : 0.0f;
+                    const size_t pos = (((size_t)mt * B.k_steps + ks) * 64 + lane) * 8 + j;
                     hi[pos] = bf16_rne(v);
                     lo[pos] = bf16_rne(v - bf16_f(hi[pos]));
                 }
@@ -343,12 +363,16 @@ int ahip_batch_get_rates(ahip_batch* b, double* r) {
 int ahip_batch_get_input_sums(ahip_batch* b, double* y) {
     if (!b || !y) return bfail(AHIP_EINVAL, "null argument");
     const BatchDev& B = b->B;
-    std::vector<float> f((size_t)B.m_tiles * 32 * B.n_tiles * 32);
+    if (b->last_chunk < 1) return bfail(AHIP_ESTATE, "no step has run");
+    // the last step of the last chunk: rows [instance] x columns [input neuron]
+    const int32_t ld = B.m_tiles * 32;
+    const size_t block = (size_t)B.n_tiles * 32 * ld;
+    std::vector<float> f(block);
     BCHK(hipStreamSynchronize(b->stream));
-    BCHK(hipMemcpy(f.data(), B.y.p, f.size() * 4, hipMemcpyDeviceToHost));
-    const int32_t ld = B.n_tiles * 32;
+    BCHK(hipMemcpy(f.data(), B.y.as<float>() + (size_t)(b->last_chunk - 1) * block, block * 4,
+                   hipMemcpyDeviceToHost));
     for (int32_t m = 0; m < B.n_in; m++)
-        for (int32_t i = 0; i < B.n_inst; i++) y[(size_t)m * B.n_inst + i] = f[(size_t)m * ld + i];
+        for (int32_t i = 0; i < B.n_inst; i++) y[(size_t)m * B.n_inst + i] = f[(size_t)i * ld + m];
     return 0;
 }
 
@@ -357,6 +381,9 @@ int ahip_batch_run(ahip_batch* b, int64_t steps, int32_t graph_batch) {
     if (!b->ready_sparse || !b->ready_input) return bfail(AHIP_ESTATE, "set the recurrent and input projections first");
     int32_t G = graph_batch > 0 ? graph_batch : 50;
     if (G % 2) G++;  // even batches keep the ping-pong parity of the captured graph fixed
+    // one input chunk per graph batch: X and Y for G steps
+    int rc0 = ensure_chunk(b, G);
+    if (rc0) return rc0;
     int64_t left = steps;
     if (left >= G) {
         if (!b->graph || b->graph_steps != G || b->graph_parity != b->parity) {
@@ -396,14 +423,15 @@ int ahip_batch_time(ahip_batch* b, int32_t which, int32_t iters, double* avg_ms)
     hipEvent_t e0, e1;
     BCHK(hipEventCreate(&e0));
     BCHK(hipEventCreate(&e1));
+    // input and GEMM: one launch over a full chunk (cap_steps steps; every run
+    // recomputes its chunks, so the buffers they overwrite carry no state)
+    const int32_t n = b->B.cap_steps;
     auto one = [&]() -> int {
-        // step -1 recomputes the inputs of the last completed step: the staged
-        // values the next step reads stay the same
-        if (which == 0) return batch_launch_input(b->B, -1, b->stream);
-        if (which == 1) return batch_launch_gemm(b->B, b->stream);
+        if (which == 0) return batch_launch_input(b->B, 0, n, b->stream);
+        if (which == 1) return batch_launch_gemm(b->B, n, b->stream);
         // reads the current rates, writes only the other buffer and the psp
         // record, both rewritten by the next step
-        return batch_launch_spmv(b->B, b->parity, b->stream);
+        return batch_launch_spmv(b->B, b->parity, 0, b->stream);
     };
     BLCHK(one());
     BCHK(hipEventRecord(e0, b->stream));
@@ -415,6 +443,12 @@ int ahip_batch_time(ahip_batch* b, int32_t which, int32_t iters, double* avg_ms)
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     *avg_ms = (double)ms / iters;
+    return 0;
+}
+
+int ahip_batch_chunk_steps(ahip_batch* b, int32_t* steps) {
+    if (!b || !steps) return bfail(AHIP_EINVAL, "null argument");
+    *steps = b->B.cap_steps;
     return 0;
 }
 

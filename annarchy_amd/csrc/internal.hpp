@@ -60,6 +60,8 @@ struct Pop {
     std::vector<double> slot_host_global;   // host mirror of population-wide slots
     DevBuf refractory, refractory_remaining, last_spike, spike_mask, block_counts;
     DevBuf spiked, spike_count;             // int32[size], int32[1]
+    DevBuf deliver;                         // ahip_deliver_desc[n_deliver]: ordered targets of the update
+    int32_t n_deliver = 0;
     // delay queue (ring of `depth` steps, slot t mod depth = end of step t):
     // rate-coded: the psp variable, depth x size; spiking: spike lists,
     // depth x size int32 with counts int32[depth]
@@ -182,17 +184,36 @@ struct ScatterDesc {
     int32_t* cnt;             // events per post neuron this step
     const int64_t* boff;      // bucket of post neuron i: [boff[i], boff[i+1])
     uint32_t* bucket;
+    // post neurons with events this step, sharded by post & (shards - 1):
+    // shard s lists its neurons at touched[s * shard_cap ..] and counts them
+    // in scnt[s * kShardStride] (the delivery clears it)
+    int32_t* touched;
+    int32_t* scnt;
+    int32_t shards, shard_cap;
     uint32_t gid_off;
 };
 constexpr int kMaxScatter = 8;
 // One projection feeding an ordered target: its keys are [gid_off, gid_end),
 // key k names column-order synapse k - gid_off of that projection.
-struct DeliverSrc {
-    const void* w_col;        // weights in column order, or null for w_const
-    double w_const;
-    uint32_t gid_off, gid_end;
+using DeliverSrc = ahip_deliver_src;
+constexpr int kMaxDeliverSrc = 32;   // projections per ordered conductance
+constexpr int kMaxDeliverLds = 64;   // projections over all ordered conductances
+// One ordered conductance in the merged delivery launch: its touched post
+// neurons of the step and its sources src[src0 .. src0 + n_src) of the
+// network's DeliverSrc table.
+struct DeliverTarget {
+    void* g;
+    const int64_t* boff;
+    int32_t* cnt;
+    const uint32_t* bucket;
+    const int32_t* touched;
+    int32_t* scnt;
+    int32_t shards, shard_cap;
+    int32_t wave0;            // first wave of this target in the launch (one wave per shard)
+    int32_t src0, n_src;
 };
-constexpr int kMaxDeliverSrc = 32;
+constexpr int kShardStride = 16;     // one 64-byte line per shard counter
+constexpr int kMaxShards = 1024;
 // A conductance g_<target> fed by at least one spiking projection with
 // per-synapse weights.  The reference adds the events projection by
 // projection, spike by spike in ascending pre-synaptic rank
@@ -208,7 +229,15 @@ struct OrderedTarget {
     DevBuf boff;                  // int64[size + 1]
     DevBuf cnt;                   // int32[size], zero between steps
     DevBuf bucket;                // uint32[cap]
-    DevBuf src;                   // DeliverSrc[n_src]
+    int32_t shards = 1, shard_cap = 0;
+    DevBuf touched;               // int32[shards * shard_cap]
+    DevBuf scnt;                  // int32[shards * kShardStride]
+    std::vector<DeliverSrc> src;  // in creation order of the projections
+    // delivered by the update kernel of the post population (no other writer
+    // of the conductance, no monitor reading it right after compute_psp);
+    // otherwise by the merged delivery launch after the scatters
+    bool in_update = false;
+    DevBuf src_dev;               // DeliverSrc[n_src] (in_update)
 };
 // Spiking projections launched together: their targets are distinct, so the
 // atomics of one projection never interleave with another's into the same
@@ -245,6 +274,9 @@ struct ahip_net {
     hipEvent_t probe_ev[2] = {nullptr, nullptr};
     std::vector<ahip::ScatterGroup> scatter_groups;
     std::vector<ahip::OrderedTarget> ordered;   // targets delivered in the reference's order
+    ahip::DevBuf deliver_targets;               // DeliverTarget[ordered.size()]
+    ahip::DevBuf deliver_src;                   // DeliverSrc of every ordered target
+    int32_t deliver_n_tg = 0, deliver_n_src = 0, deliver_waves = 0;  // the merged delivery launch
     int32_t delivery = AHIP_DELIVERY_ORDERED;
     size_t elem() const { return precision == AHIP_F64 ? sizeof(double) : sizeof(float); }
 };
@@ -259,7 +291,7 @@ int launch_spike_psp(const ahip_net* net, const Proj& p, const Pop& pre, void* t
 int launch_compact(const Pop& p, const Monitor* m, const int64_t* t_base, int32_t step, hipStream_t s);
 int launch_update_rate(const ahip_net* net, const Pop& p, int32_t step, hipStream_t s, bool active = true);
 int launch_scatter_group(const ahip_net* net, const ScatterGroup& g, int32_t step, hipStream_t s);
-int launch_deliver(const ahip_net* net, const OrderedTarget& o, hipStream_t s);
+int launch_deliver(const ahip_net* net, hipStream_t s);
 int launch_push_values(const ahip_net* net, const void* src, void* hist, int32_t n, int32_t D, int32_t step,
                        hipStream_t s);
 int launch_gather_delayed(const ahip_net* net, const void* hist, void* dst, int32_t n, int32_t D, int32_t d,
@@ -292,7 +324,9 @@ int launch_copy16(const void* src, void* dst, int64_t n16, hipStream_t s);
 // Device state of the batched multi-instance rate sweep (k_batched.hip).
 struct BatchDev {
     int32_t n_inst = 0, n_in = 0, total = 0;
-    int32_t m_tiles = 0, n_tiles = 0, k_steps = 0;  // GEMM tiling (32 x 32 x 8)
+    int32_t m_tiles = 0, n_tiles = 0, k_steps = 0;  // GEMM tiling (32 x 32 x 16): input neurons,
+                                                    // instances per step, depth
+    int32_t cap_steps = 0;           // steps per input chunk the X and Y buffers hold
     int32_t lanes = 16;
     double dt = 1.0, tau = 10.0, omega = 0.0;
     DevBuf row_ptr, col, w;          // block-diagonal CSR, FP32 weights
@@ -300,10 +334,10 @@ struct BatchDev {
     DevBuf gain;                     // float [n_inst]
     DevBuf r[2];                     // float [total] ping-pong
     DevBuf sum_exc;                  // float [total], psp of the last step
-    DevBuf w_hi, w_lo;               // bf16 fragments [m_tiles][k_steps][64][4]
-    DevBuf x_hi, x_lo;               // bf16 fragments [n_tiles][k_steps][64][4]
+    DevBuf w_hi, w_lo;               // W^T as bf16 B fragments [m_tiles][k_steps][64][8]
+    DevBuf x_hi, x_lo;               // X^T of a chunk as bf16 A fragments [cap_steps*n_tiles][k_steps][64][8]
     DevBuf phase;                    // double [n_in][n_inst]
-    DevBuf y;                        // float [m_tiles*32][n_tiles*32]
+    DevBuf y;                        // Y^T of a chunk: float [cap_steps][n_tiles*32][m_tiles*32]
     DevBuf t_dev;                    // int64
     int32_t mode = 1;                // 0: gathers from global memory, 1: instance rates staged in LDS,
                                      // 2: sliced chunks (one lane per row, uint16 columns)
@@ -317,9 +351,9 @@ struct BatchDev {
     DevBuf sl_off, sl_row;           // int64 [slices + 1], int32 [slices][64] (-1: no row)
     DevBuf sl_col, sl_w;             // uint16 / float [sl_stored]
 };
-int batch_launch_input(const BatchDev& B, int32_t step, hipStream_t s);
-int batch_launch_gemm(const BatchDev& B, hipStream_t s);
-int batch_launch_spmv(const BatchDev& B, int32_t parity, hipStream_t s);
+int batch_launch_input(const BatchDev& B, int32_t step0, int32_t n, hipStream_t s);
+int batch_launch_gemm(const BatchDev& B, int32_t n, hipStream_t s);
+int batch_launch_spmv(const BatchDev& B, int32_t parity, int32_t j, hipStream_t s);
 int batch_launch_advance(const BatchDev& B, int64_t n, hipStream_t s);
 
 // Sets the message returned by ahip_last_error(); returns `code`.

@@ -8,14 +8,16 @@
 //     exc -> exc  sparse (block-diagonal CSR over all instances, FP32 + Kahan)
 //     inp -> exc  dense n_in x n_in weights shared by every instance, applied to
 //                 the first min(N_i, n_in) neurons of instance i
-// One step = three kernels captured in a graph:
-//   1. input update: X[k][i] for all instances, split in BF16 hi + lo
-//   2. Y = W_in . X on MFMA (v_mfma_f32_32x32x8_bf16), FP32 accumulation;
+// The input rates depend on t only, so the input projection of a whole
+// chunk of steps (a captured graph batch) is one GEMM:
+//   1. input: X of every step of the chunk, split in BF16 hi + lo;
+//   2. Y^T = X^T W^T on MFMA (v_mfma_f32_32x32x16_bf16), FP32 accumulation;
 //      W and X are each split in two BF16 terms and the three significant
 //      products (hi.hi + hi.lo + lo.hi) are accumulated, which gives ~16
-//      mantissa bits per operand instead of 8
-//   3. block-diagonal SpMV (FP32, compensated) fused with the exc update
-//      (rates ping-pong between two buffers, so every row reads step-t rates)
+//      mantissa bits per operand instead of 8;
+//   3. per step: block-diagonal SpMV (FP32, compensated) fused with the exc
+//      update (rates ping-pong between two buffers, so every row reads
+//      step-t rates), reading the step's row block of Y^T.
 #include <hip/hip_bfloat16.h>
 
 #include "ahip_device.hpp"
@@ -23,7 +25,8 @@
 
 namespace ahip {
 
-typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 
@@ -38,60 +41,120 @@ __device__ inline short bf16_bits(float f) {
 }
 __device__ inline float bf16_to_f(short b) { return __uint_as_float(((uint32_t)(uint16_t)b) << 16); }
 
-// X[k][i] = 1 + 0.5*sin(omega*(t*dt) + phase[k][i]), stored as B fragments:
-// element (k, n) of tile (ntile, kstep) sits at lane = n%32 + 32*((k%8)/4),
-// register (k%4).
-__global__ __launch_bounds__(256) void batch_input(int32_t n_in, int32_t n_inst, int32_t n_tiles,
+// Input rates of `n` consecutive steps as split-BF16 A fragments of X^T.
+// Step j of a chunk that starts at step `step0` of the current batch reads
+// the input population's rates of the step before it,
+//   X_j[k][i] = 1 + 0.5*sin(omega*((t - 1)*dt) + phase[k][i]),  t = t_base + step0 + j,
+// and zero for t = 0 (the input rates start at 0 before the first update).
+// Row c of X^T is (step j, instance i) = (c / NP, c % NP); fragment element
+// (c, k) of tile (ct, ks) sits at lane c%32 + 32*((k%16)/8), item k%8 (the
+// 32x32x16 BF16 operand layout).  One thread writes one lane's 8 items.
+__global__ __launch_bounds__(256) void batch_input(int32_t n_in, int32_t n_inst, int32_t np, int32_t c_tiles,
                                                    int32_t k_steps, const double* __restrict__ phase,
                                                    double omega, double dt, const int64_t* __restrict__ t_base,
-                                                   int32_t step, short* __restrict__ xh,
-                                                   short* __restrict__ xl) {
-    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t total = (int64_t)k_steps * 8 * n_tiles * 32;
-    if (idx >= total) return;
-    const int32_t n = (int32_t)(idx % (n_tiles * 32));
-    const int32_t k = (int32_t)(idx / (n_tiles * 32));
-    float v = 0.0f;
-    if (k < n_in && n < n_inst) {
-        const double t = (double)(*t_base + step);
-        v = (float)(1.0 + 0.5 * sin(omega * (t * dt) + phase[(int64_t)k * n_inst + n]));
+                                                   int32_t step0, v8s* __restrict__ xh, v8s* __restrict__ xl) {
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= (int64_t)c_tiles * k_steps * 64) return;
+    const int32_t lane = (int32_t)(g & 63);
+    const int64_t tile = g >> 6;
+    const int32_t ks = (int32_t)(tile % k_steps), ct = (int32_t)(tile / k_steps);
+    const int32_t c = ct * 32 + (lane & 31);
+    const int32_t j = c / np, i = c % np;
+    const int32_t k0 = ks * 16 + 8 * (lane >> 5);
+    const int64_t tprev = *t_base + step0 + j - 1;
+    v8s h, l;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int32_t k = k0 + e;
+        float v = 0.0f;
+        if (k < n_in && i < n_inst && tprev >= 0)
+            v = (float)(1.0 + 0.5 * sin(omega * ((double)tprev * dt) + phase[(int64_t)k * n_inst + i]));
+        h[e] = bf16_bits(v);
+        l[e] = bf16_bits(v - bf16_to_f(h[e]));
     }
-    const short h = bf16_bits(v);
-    const short l = bf16_bits(v - bf16_to_f(h));
-    const int32_t nt = n / 32, ks = k / 8, kk = k % 8;
-    const int32_t lane = (n % 32) + 32 * (kk / 4);
-    const int64_t pos = (((int64_t)nt * k_steps + ks) * 64 + lane) * 4 + (kk % 4);
-    xh[pos] = h;
-    xl[pos] = l;
+    xh[g] = h;
+    xl[g] = l;
 }
 
-// One wave per 32 x 32 tile of Y; 4 waves per block take 4 column tiles.
-__global__ __launch_bounds__(256) void batch_gemm(int32_t m_tiles, int32_t n_tiles, int32_t k_steps,
-                                                  const v4s* __restrict__ wh, const v4s* __restrict__ wl,
-                                                  const v4s* __restrict__ xh, const v4s* __restrict__ xl,
-                                                  float* __restrict__ y) {
+// C = X^T W^T for a chunk of steps (C[c][m] = sum_k X[k][c] W[m][k]), on
+// v_mfma_f32_32x32x16_bf16 with FP32 accumulation.  Each operand is split in
+// two BF16 terms and the three significant products lo.hi + hi.lo + hi.hi
+// are accumulated (about 16 mantissa bits per operand).  A wave owns
+// GM x GN tiles of 32 x 32 (rows: steps x instances, columns: input
+// neurons); the fragments of the next k step are loaded while the current
+// one is multiplied.  Tiles past the edge load a valid tile and store nothing.
+template <int GM, int GN>
+__global__ __launch_bounds__(256) void batch_gemm(int32_t c_tiles, int32_t m_tiles, int32_t k_steps,
+                                                  const v8bf* __restrict__ xh, const v8bf* __restrict__ xl,
+                                                  const v8bf* __restrict__ wh, const v8bf* __restrict__ wl,
+                                                  float* __restrict__ y, int32_t ldy) {
     const int lane = threadIdx.x & 63;
+    const int32_t mblocks = (m_tiles + GN - 1) / GN;
     const int32_t wave = blockIdx.x * 4 + threadIdx.x / 64;
-    if (wave >= m_tiles * n_tiles) return;
-    const int32_t mt = wave / n_tiles, nt = wave % n_tiles;
-    v16f acc = {0};
-    const v4s* a_h = wh + (int64_t)mt * k_steps * 64 + lane;
-    const v4s* a_l = wl + (int64_t)mt * k_steps * 64 + lane;
-    const v4s* b_h = xh + (int64_t)nt * k_steps * 64 + lane;
-    const v4s* b_l = xl + (int64_t)nt * k_steps * 64 + lane;
-    for (int32_t ks = 0; ks < k_steps; ks++) {
-        const v4s ah = a_h[ks * 64], al = a_l[ks * 64];
-        const v4s bh = b_h[ks * 64], bl = b_l[ks * 64];
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(al, bh, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, bl, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, bh, acc, 0, 0, 0);
-    }
-    const int32_t ld = n_tiles * 32;
-    const int32_t col = nt * 32 + (lane % 32);
+    const int32_t cb = wave / mblocks, mb = wave % mblocks;
+    if (cb * GM >= c_tiles) return;
+    const v8bf* ah[GM];
+    const v8bf* al[GM];
+    const v8bf* bh[GN];
+    const v8bf* bl[GN];
 #pragma unroll
-    for (int i = 0; i < 16; i++) {
-        const int32_t row = mt * 32 + 8 * (i / 4) + 4 * (lane / 32) + (i % 4);
-        y[(int64_t)row * ld + col] = acc[i];
+    for (int a = 0; a < GM; a++) {
+        const int32_t ct = min(cb * GM + a, c_tiles - 1);
+        ah[a] = xh + (int64_t)ct * k_steps * 64 + lane;
+        al[a] = xl + (int64_t)ct * k_steps * 64 + lane;
+    }
+#pragma unroll
+    for (int b = 0; b < GN; b++) {
+        const int32_t mt = min(mb * GN + b, m_tiles - 1);
+        bh[b] = wh + (int64_t)mt * k_steps * 64 + lane;
+        bl[b] = wl + (int64_t)mt * k_steps * 64 + lane;
+    }
+    v16f acc[GM][GN];
+#pragma unroll
+    for (int a = 0; a < GM; a++)
+#pragma unroll
+        for (int b = 0; b < GN; b++) acc[a][b] = v16f{0};
+    v8bf fah[GM], fal[GM], fbh[GN], fbl[GN];
+#pragma unroll
+    for (int a = 0; a < GM; a++) { fah[a] = ah[a][0]; fal[a] = al[a][0]; }
+#pragma unroll
+    for (int b = 0; b < GN; b++) { fbh[b] = bh[b][0]; fbl[b] = bl[b][0]; }
+    for (int32_t ks = 0; ks < k_steps; ks++) {
+        const int64_t nx = (int64_t)min(ks + 1, k_steps - 1) * 64;
+        v8bf nah[GM], nal[GM], nbh[GN], nbl[GN];
+#pragma unroll
+        for (int a = 0; a < GM; a++) { nah[a] = ah[a][nx]; nal[a] = al[a][nx]; }
+#pragma unroll
+        for (int b = 0; b < GN; b++) { nbh[b] = bh[b][nx]; nbl[b] = bl[b][nx]; }
+#pragma unroll
+        for (int a = 0; a < GM; a++)
+#pragma unroll
+            for (int b = 0; b < GN; b++) {
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fal[a], fbh[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fah[a], fbl[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fah[a], fbh[b], acc[a][b], 0, 0, 0);
+            }
+#pragma unroll
+        for (int a = 0; a < GM; a++) { fah[a] = nah[a]; fal[a] = nal[a]; }
+#pragma unroll
+        for (int b = 0; b < GN; b++) { fbh[b] = nbh[b]; fbl[b] = nbl[b]; }
+    }
+    // D[i][j] of a 32 x 32 tile: lane j + 32*((i/4)%2), item (i%4) + 4*(i/8)
+#pragma unroll
+    for (int a = 0; a < GM; a++) {
+        const int32_t ct = cb * GM + a;
+        if (ct >= c_tiles) continue;
+#pragma unroll
+        for (int b = 0; b < GN; b++) {
+            const int32_t mt = mb * GN + b;
+            if (mt >= m_tiles) continue;
+            const int32_t col = mt * 32 + (lane & 31);
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int32_t row = ct * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+                y[(int64_t)row * ldy + col] = acc[a][b][r];
+            }
+        }
     }
 }
 
@@ -125,7 +188,7 @@ __global__ __launch_bounds__(256) void batch_spmv_update(
     if (lane == 0) {
         const int32_t inst = inst_of_row[row];
         const int32_t n = local_of_row[row];
-        const float s_in = n < n_in ? y[(int64_t)n * ld_y + inst] : 0.0f;
+        const float s_in = n < n_in ? y[(int64_t)inst * ld_y + n] : 0.0f;
         const float r = r_in[row];
         const float _r = (s + gain[inst] * s_in - r) / tau;
         float rn = r + dt * _r;
@@ -178,7 +241,7 @@ __global__ __launch_bounds__(256) void batch_spmv_lds(
         const float s = group_sum_kahan<L>(k0.s, k0.c);
         if (lane == 0) {
             const int32_t nl = (int32_t)(row - base);
-            const float s_in = nl < n_in ? y[(int64_t)nl * ld_y + inst] : 0.0f;
+            const float s_in = nl < n_in ? y[(int64_t)inst * ld_y + nl] : 0.0f;
             const float r = xs[nl];
             const float _r = (s + gain[inst] * s_in - r) / tau;
             float rn = r + dt * _r;
@@ -250,7 +313,7 @@ __global__ __launch_bounds__(512) void batch_sell(
         if (row >= 0) {
             const float sum = k0.value();
             const int32_t nl = (int32_t)(row - base);
-            const float s_in = nl < n_in ? y[(int64_t)nl * ld_y + inst] : 0.0f;
+            const float s_in = nl < n_in ? y[(int64_t)inst * ld_y + nl] : 0.0f;
             const float r = xs[nl];
             const float _r = (sum + gain[inst] * s_in - r) / tau;
             float rn = r + dt * _r;
@@ -267,25 +330,34 @@ __global__ void batch_advance(int64_t* t, int64_t n) {
 
 }  // namespace
 
-int batch_launch_input(const BatchDev& B, int32_t step, hipStream_t s) {
-    const int64_t total = (int64_t)B.k_steps * 8 * B.n_tiles * 32;
-    const int64_t nb = (total + 255) / 256;
-    hipLaunchKernelGGL(batch_input, dim3(nb), dim3(256), 0, s, B.n_in, B.n_inst, B.n_tiles, B.k_steps,
-                       B.phase.as<double>(), B.omega, B.dt, B.t_dev.as<int64_t>(), step,
-                       B.x_hi.as<short>(), B.x_lo.as<short>());
+int batch_launch_input(const BatchDev& B, int32_t step0, int32_t n, hipStream_t s) {
+    if (n < 1 || n > B.cap_steps) return (int)hipErrorInvalidValue;
+    const int32_t c_tiles = n * B.n_tiles;
+    const int64_t nb = ((int64_t)c_tiles * B.k_steps * 64 + 255) / 256;
+    hipLaunchKernelGGL(batch_input, dim3(nb), dim3(256), 0, s, B.n_in, B.n_inst, B.n_tiles * 32, c_tiles,
+                       B.k_steps, B.phase.as<double>(), B.omega, B.dt, B.t_dev.as<int64_t>(), step0,
+                       B.x_hi.as<v8s>(), B.x_lo.as<v8s>());
     return (int)hipGetLastError();
 }
 
-int batch_launch_gemm(const BatchDev& B, hipStream_t s) {
-    const int64_t waves = (int64_t)B.m_tiles * B.n_tiles;
+constexpr int kGemmGM = 2, kGemmGN = 4;
+
+int batch_launch_gemm(const BatchDev& B, int32_t n, hipStream_t s) {
+    if (n < 1 || n > B.cap_steps) return (int)hipErrorInvalidValue;
+    const int32_t c_tiles = n * B.n_tiles;
+    const int64_t waves = (int64_t)((c_tiles + kGemmGM - 1) / kGemmGM) * ((B.m_tiles + kGemmGN - 1) / kGemmGN);
     const int64_t nb = (waves + 3) / 4;
-    hipLaunchKernelGGL(batch_gemm, dim3(nb), dim3(256), 0, s, B.m_tiles, B.n_tiles, B.k_steps,
-                       B.w_hi.as<v4s>(), B.w_lo.as<v4s>(), B.x_hi.as<v4s>(), B.x_lo.as<v4s>(),
-                       B.y.as<float>());
+    hipLaunchKernelGGL((batch_gemm<kGemmGM, kGemmGN>), dim3(nb), dim3(256), 0, s, c_tiles, B.m_tiles, B.k_steps,
+                       B.x_hi.as<v8bf>(), B.x_lo.as<v8bf>(), B.w_hi.as<v8bf>(), B.w_lo.as<v8bf>(),
+                       B.y.as<float>(), B.m_tiles * 32);
     return (int)hipGetLastError();
 }
 
-int batch_launch_spmv(const BatchDev& B, int32_t parity, hipStream_t s) {
+int batch_launch_spmv(const BatchDev& B, int32_t parity, int32_t j, hipStream_t s) {
+    if (j < 0 || j >= B.cap_steps) return (int)hipErrorInvalidValue;
+    // row block of step j in Y^T: [instance][input neuron], leading dimension m_tiles*32
+    const int32_t ld_y = B.m_tiles * 32;
+    const float* yj = B.y.as<float>() + (int64_t)j * B.n_tiles * 32 * ld_y;
     const int64_t threads = (int64_t)B.total * B.lanes;
     const int64_t nb = (threads + 255) / 256;
     const float* rin = B.r[parity].as<float>();
@@ -295,7 +367,7 @@ int batch_launch_spmv(const BatchDev& B, int32_t parity, hipStream_t s) {
         hipLaunchKernelGGL(batch_sell<2>, dim3(B.n_schunks), dim3(512), lds, s, B.schunks.as<int32_t>(),
                            B.schunk_slices.as<int32_t>(), B.inst_base.as<int64_t>(), B.sl_off.as<int64_t>(),
                            B.sl_row.as<int32_t>(), B.sl_col.as<uint16_t>(), B.sl_w.as<float>(), B.gain.as<float>(),
-                           B.y.as<float>(), B.n_tiles * 32, B.n_in, rin, rout, B.sum_exc.as<float>(), (float)B.dt,
+                           yj, ld_y, B.n_in, rin, rout, B.sum_exc.as<float>(), (float)B.dt,
                            (float)B.tau);
         return (int)hipGetLastError();
     }
@@ -305,8 +377,8 @@ int batch_launch_spmv(const BatchDev& B, int32_t parity, hipStream_t s) {
     case N:                                                                                       \
         hipLaunchKernelGGL(batch_spmv_lds<N>, dim3(B.n_chunks), dim3(256), lds, s,                \
                            B.chunks.as<int32_t>(), B.inst_base.as<int64_t>(), B.row_ptr.as<int64_t>(), \
-                           B.col.as<int32_t>(), B.w.as<float>(), B.gain.as<float>(), B.y.as<float>(), \
-                           B.n_tiles * 32, B.n_in, rin, rout, B.sum_exc.as<float>(), (float)B.dt,     \
+                           B.col.as<int32_t>(), B.w.as<float>(), B.gain.as<float>(), yj, \
+                           ld_y, B.n_in, rin, rout, B.sum_exc.as<float>(), (float)B.dt,     \
                            (float)B.tau);                                                          \
         break;
         switch (B.lanes) {
@@ -326,7 +398,7 @@ int batch_launch_spmv(const BatchDev& B, int32_t parity, hipStream_t s) {
         hipLaunchKernelGGL(batch_spmv_update<N>, dim3(nb), dim3(256), 0, s, B.total,               \
                            B.row_ptr.as<int64_t>(), B.col.as<int32_t>(), B.w.as<float>(),          \
                            B.inst_of_row.as<int32_t>(), B.local_of_row.as<int32_t>(),             \
-                           B.gain.as<float>(), B.y.as<float>(), B.n_tiles * 32, B.n_in, rin, rout, \
+                           B.gain.as<float>(), yj, ld_y, B.n_in, rin, rout, \
                            B.sum_exc.as<float>(), (float)B.dt, (float)B.tau);                      \
         break;
     switch (B.lanes) {

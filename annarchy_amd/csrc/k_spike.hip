@@ -238,8 +238,16 @@ __global__ __launch_bounds__(256) void scatter_multi(const ScatterDesc* __restri
                     }
 #pragma unroll
                 for (int u = 0; u < 4; u++)
-                    if (post[u] >= 0 && lo[u] + slot[u] < hi[u])
+                    if (post[u] >= 0 && lo[u] + slot[u] < hi[u]) {
                         d.bucket[lo[u] + slot[u]] = d.gid_off + (uint32_t)(base + lane + u * kWave);
+                        // the first event of a post neuron in this step lists it in its
+                        // shard (targets delivered by their own launch)
+                        if (slot[u] == 0 && d.touched) {
+                            const int32_t sh = post[u] & (d.shards - 1);
+                            const int32_t q = atomicAdd(d.scnt + sh * kShardStride, 1);
+                            d.touched[(int64_t)sh * d.shard_cap + q] = post[u];
+                        }
+                    }
             }
             continue;
         }
@@ -262,86 +270,40 @@ __global__ __launch_bounds__(256) void scatter_multi(const ScatterDesc* __restri
     }
 }
 
-// Ordered delivery of one conductance (after every scatter of the step): post
-// neuron i adds the weights of its bucket's events in ascending key, i.e.
-// projection by projection in creation order and in ascending pre-synaptic
-// rank, each addition rounded as the reference's g[post] += w
-// (Projection/OpenMP/CSR.py:870-885), then empties the bucket.  Up to 16
-// events are sorted in registers by a bitonic network; a longer bucket
-// (more than 16 of the neuron's inputs fired in one step) is walked by
-// repeated minimum, O(L^2) reads of the bucket.
+// One launch delivers every ordered conductance: one wave per shard of the
+// touched post neurons of each target (the shards of a target follow one
+// another); the wave delivers the shard's neurons and clears its counter.
 template <typename T>
-__device__ inline T deliver_weight(const DeliverSrc* __restrict__ src, int32_t n_src, uint32_t key) {
-    int32_t k = 0;
-    while (k < n_src - 1 && key >= src[k].gid_end) k++;
-    const T* wc = static_cast<const T*>(src[k].w_col);
-    return wc ? wc[key - src[k].gid_off] : (T)src[k].w_const;
+__device__ inline void deliver_one(const DeliverTarget& d, const DeliverSrc* __restrict__ sk, int32_t i) {
+    const int64_t b = d.boff[i];
+    // a bucket never holds more events than the neuron's summed in-degree
+    const int32_t L = (int32_t)min((int64_t)d.cnt[i], d.boff[i + 1] - b);
+    d.cnt[i] = 0;
+    if (L <= 0) return;
+    T* g = static_cast<T*>(d.g);
+    g[i] = ordered_sum<T>(g[i], d.bucket, b, L, sk, d.n_src);
 }
 
-template <int N>
-__device__ inline void bitonic_sort_keys(uint32_t (&a)[N]) {
-#pragma unroll
-    for (int k = 2; k <= N; k <<= 1)
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1)
-#pragma unroll
-            for (int i = 0; i < N; i++) {
-                const int l = i ^ j;
-                if (l > i) {
-                    const uint32_t x = a[i], y = a[l];
-                    const bool up = (i & k) == 0;
-                    const uint32_t lo = x < y ? x : y, hi = x < y ? y : x;
-                    a[i] = up ? lo : hi;
-                    a[l] = up ? hi : lo;
-                }
-            }
-}
-
-constexpr int kDeliverSort = 16;
-
 template <typename T>
-__global__ __launch_bounds__(256) void deliver_ordered(int32_t n, const int64_t* __restrict__ boff,
-                                                       int32_t* __restrict__ cnt,
-                                                       const uint32_t* __restrict__ bucket,
+__global__ __launch_bounds__(256) void deliver_ordered(const DeliverTarget* __restrict__ tg, int32_t n_tg,
                                                        const DeliverSrc* __restrict__ src, int32_t n_src,
-                                                       T* __restrict__ g) {
-    __shared__ DeliverSrc ls[kMaxDeliverSrc];
+                                                       int32_t n_waves) {
+    __shared__ DeliverSrc ls[kMaxDeliverLds];
     if (threadIdx.x < n_src) ls[threadIdx.x] = src[threadIdx.x];
     __syncthreads();
-    const int32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const int64_t b = boff[i];
-    // a bucket never holds more events than the neuron's summed in-degree
-    const int32_t L = (int32_t)min((int64_t)cnt[i], boff[i + 1] - b);
-    if (L <= 0) return;
-    T acc = g[i];
-    if (L == 1) {
-        acc += deliver_weight<T>(ls, n_src, bucket[b]);
-    } else if (L <= kDeliverSort) {
-        uint32_t k[kDeliverSort];
-#pragma unroll
-        for (int m = 0; m < kDeliverSort; m++) k[m] = m < L ? bucket[b + m] : 0xffffffffu;
-        bitonic_sort_keys(k);
-        T w[kDeliverSort];
-#pragma unroll
-        for (int m = 0; m < kDeliverSort; m++) w[m] = m < L ? deliver_weight<T>(ls, n_src, k[m]) : T(0);
-#pragma unroll
-        for (int m = 0; m < kDeliverSort; m++)
-            if (m < L) acc += w[m];
-    } else {
-        uint32_t last = 0;
-        for (int32_t m = 0; m < L; m++) {
-            uint32_t best = 0xffffffffu;
-            for (int32_t q = 0; q < L; q++) {
-                const uint32_t v = bucket[b + q];
-                if ((m == 0 || v > last) && v < best) best = v;
-            }
-            acc += deliver_weight<T>(ls, n_src, best);
-            last = best;
-        }
-    }
-    g[i] = acc;
-    cnt[i] = 0;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int32_t w = blockIdx.x * 4 + threadIdx.x / kWave;
+    if (w >= n_waves) return;
+    int32_t k = 0;
+    while (k < n_tg - 1 && w >= tg[k + 1].wave0) k++;
+    const DeliverTarget& d = tg[k];
+    const int32_t sh = w - d.wave0;
+    int32_t* sc = d.scnt + sh * kShardStride;
+    const int32_t n = *sc;
+    const int32_t* list = d.touched + (int64_t)sh * d.shard_cap;
+    for (int32_t q = lane; q < n; q += kWave) deliver_one<T>(d, ls + d.src0, list[q]);
+    // every lane of the wave has read the count above
+    if (lane == 0 && n) *sc = 0;
 }
 
 // Spike recorder of a further spike monitor of a population (the first one is
@@ -444,17 +406,16 @@ int launch_scatter_group(const ahip_net* net, const ScatterGroup& g, int32_t ste
     return (int)hipGetLastError();
 }
 
-int launch_deliver(const ahip_net* net, const OrderedTarget& o, hipStream_t s) {
-    if (o.size == 0 || o.n_src == 0) return 0;
-    const unsigned nb = (unsigned)((o.size + 255) / 256);
+int launch_deliver(const ahip_net* net, hipStream_t s) {
+    if (net->deliver_waves == 0) return 0;
+    const int32_t n_tg = net->deliver_n_tg;
+    const unsigned nb = (unsigned)((net->deliver_waves + 3) / 4);
     if (net->precision == AHIP_F64)
-        hipLaunchKernelGGL(deliver_ordered<double>, dim3(nb), dim3(256), 0, s, o.size, o.boff.as<int64_t>(),
-                           o.cnt.as<int32_t>(), o.bucket.as<uint32_t>(), o.src.as<DeliverSrc>(), o.n_src,
-                           (double*)o.g);
+        hipLaunchKernelGGL(deliver_ordered<double>, dim3(nb), dim3(256), 0, s, net->deliver_targets.as<DeliverTarget>(),
+                           n_tg, net->deliver_src.as<DeliverSrc>(), net->deliver_n_src, net->deliver_waves);
     else
-        hipLaunchKernelGGL(deliver_ordered<float>, dim3(nb), dim3(256), 0, s, o.size, o.boff.as<int64_t>(),
-                           o.cnt.as<int32_t>(), o.bucket.as<uint32_t>(), o.src.as<DeliverSrc>(), o.n_src,
-                           (float*)o.g);
+        hipLaunchKernelGGL(deliver_ordered<float>, dim3(nb), dim3(256), 0, s, net->deliver_targets.as<DeliverTarget>(),
+                           n_tg, net->deliver_src.as<DeliverSrc>(), net->deliver_n_src, net->deliver_waves);
     return (int)hipGetLastError();
 }
 

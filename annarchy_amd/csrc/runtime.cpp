@@ -398,7 +398,6 @@ int build_ordered_targets(ahip_net* net, const std::vector<const void*>& targets
         seen.push_back(g);
         OrderedTarget o;
         o.g = const_cast<void*>(g);
-        std::vector<DeliverSrc> src;
         int64_t total = 0;
         for (auto& pp : net->projs) {
             Proj& p = *pp;
@@ -412,18 +411,18 @@ int build_ordered_targets(ahip_net* net, const std::vector<const void*>& targets
             d.gid_off = (uint32_t)total;
             total += p.nnz;
             d.gid_end = (uint32_t)std::min<int64_t>(total, UINT32_MAX);
-            src.push_back(d);
+            o.src.push_back(d);
         }
         if (total >= (int64_t)UINT32_MAX)
             return fail(AHIP_EINVAL, "ordered spike delivery keys 32-bit synapse indices: %lld synapses into one "
                         "conductance (use AHIP_DELIVERY_ATOMIC)", (long long)total);
-        if ((int32_t)src.size() > kMaxDeliverSrc)
+        if ((int32_t)o.src.size() > kMaxDeliverSrc)
             return fail(AHIP_EINVAL, "ordered spike delivery: more than %d projections into one conductance",
                         kMaxDeliverSrc);
         const Pop& post = *net->pops[o.pop];
         o.size = post.size;
         o.cap = total;
-        o.n_src = (int32_t)src.size();
+        o.n_src = (int32_t)o.src.size();
         std::vector<int64_t> boff(o.size + 1, 0);
         for (int32_t id : o.projs) {
             const Proj& p = *net->projs[id];
@@ -435,17 +434,103 @@ int build_ordered_targets(ahip_net* net, const std::vector<const void*>& targets
                 if (q.boff.bytes >= boff.size() * sizeof(int64_t)) o.boff = std::move(q.boff);
                 if (q.cnt.bytes >= (size_t)o.size * sizeof(int32_t)) o.cnt = std::move(q.cnt);
                 if (q.bucket.bytes >= (size_t)o.cap * sizeof(uint32_t)) o.bucket = std::move(q.bucket);
+                if (q.touched.bytes >= (size_t)o.size * sizeof(int32_t)) o.touched = std::move(q.touched);
             }
+        // shards of at most 64 neurons each (one wave), a power of two up to kMaxShards
+        o.shards = 1;
+        while (o.shards < kMaxShards && (int64_t)o.shards * 64 < o.size) o.shards *= 2;
+        o.shard_cap = (o.size + o.shards - 1) / o.shards;
         int rc;
         if (!o.boff.p && (rc = dev_alloc(o.boff, boff.size() * sizeof(int64_t)))) return rc;
         if (!o.cnt.p && (rc = dev_alloc(o.cnt, (size_t)o.size * sizeof(int32_t)))) return rc;
         if (!o.bucket.p && (rc = dev_alloc(o.bucket, (size_t)o.cap * sizeof(uint32_t)))) return rc;
-        if ((rc = dev_alloc(o.src, src.size() * sizeof(DeliverSrc)))) return rc;
+        const size_t touched_n = (size_t)o.shards * o.shard_cap;
+        if (o.touched.bytes < touched_n * sizeof(int32_t) && (rc = dev_alloc(o.touched, touched_n * sizeof(int32_t))))
+            return rc;
+        if ((rc = dev_alloc(o.scnt, (size_t)o.shards * kShardStride * sizeof(int32_t)))) return rc;
         HIPCHK(hipMemcpy(o.boff.p, boff.data(), boff.size() * sizeof(int64_t), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(o.src.p, src.data(), src.size() * sizeof(DeliverSrc), hipMemcpyHostToDevice));
         HIPCHK(hipMemset(o.cnt.p, 0, (size_t)o.size * sizeof(int32_t)));
+        // the post population's update adds the buckets when no other kernel
+        // writes the conductance and no monitor records it after compute_psp
+        o.in_update = true;
+        for (auto& pp : net->projs) {
+            const Proj& p = *pp;
+            if (net->pops[p.post]->slots[p.target_slot].p == g &&
+                std::find(o.projs.begin(), o.projs.end(), p.id) == o.projs.end() && transmits(net, p))
+                o.in_update = false;
+        }
+        for (auto& mp : net->mons) {
+            const Monitor& m = *mp;
+            if (m.after_psp && m.pop == o.pop && net->pops[o.pop]->slots[m.slot].p == g) o.in_update = false;
+        }
+        if (o.in_update) {
+            if ((rc = dev_alloc(o.src_dev, o.src.size() * sizeof(DeliverSrc)))) return rc;
+            HIPCHK(hipMemcpy(o.src_dev.p, o.src.data(), o.src.size() * sizeof(DeliverSrc), hipMemcpyHostToDevice));
+        }
         HIPCHK(hipDeviceSynchronize());
         net->ordered.push_back(std::move(o));
+    }
+    // the descriptors of every population's update
+    for (auto& pp : net->pops) {
+        Pop& p = *pp;
+        std::vector<ahip_deliver_desc> dd;
+        for (OrderedTarget& o : net->ordered) {
+            if (!o.in_update || o.pop != p.id) continue;
+            ahip_deliver_desc d;
+            std::memset(&d, 0, sizeof(d));
+            d.g = o.g;
+            d.boff = o.boff.as<int64_t>();
+            d.cnt = o.cnt.as<int32_t>();
+            d.bucket = o.bucket.as<uint32_t>();
+            d.src = o.src_dev.as<DeliverSrc>();
+            d.n_src = o.n_src;
+            dd.push_back(d);
+        }
+        p.n_deliver = (int32_t)dd.size();
+        p.deliver.release();
+        if (!dd.empty()) {
+            int rc;
+            if ((rc = dev_alloc(p.deliver, dd.size() * sizeof(ahip_deliver_desc)))) return rc;
+            HIPCHK(hipMemcpy(p.deliver.p, dd.data(), dd.size() * sizeof(ahip_deliver_desc), hipMemcpyHostToDevice));
+        }
+    }
+    // the merged delivery launch: one DeliverTarget per conductance, their
+    // sources in one table (staged in LDS by the kernel)
+    std::vector<DeliverTarget> tg;
+    std::vector<DeliverSrc> src;
+    int32_t waves = 0;
+    for (OrderedTarget& o : net->ordered) {
+        if (o.in_update) continue;
+        DeliverTarget d;
+        d.g = o.g;
+        d.boff = o.boff.as<int64_t>();
+        d.cnt = o.cnt.as<int32_t>();
+        d.bucket = o.bucket.as<uint32_t>();
+        d.touched = o.touched.as<int32_t>();
+        d.scnt = o.scnt.as<int32_t>();
+        d.shards = o.shards;
+        d.shard_cap = o.shard_cap;
+        d.wave0 = waves;
+        d.src0 = (int32_t)src.size();
+        d.n_src = o.n_src;
+        src.insert(src.end(), o.src.begin(), o.src.end());
+        tg.push_back(d);
+        waves += o.shards;
+    }
+    if ((int32_t)src.size() > kMaxDeliverLds)
+        return fail(AHIP_EINVAL, "ordered spike delivery: more than %d projections into ordered conductances",
+                    kMaxDeliverLds);
+    net->deliver_targets.release();
+    net->deliver_src.release();
+    net->deliver_n_tg = (int32_t)tg.size();
+    net->deliver_n_src = (int32_t)src.size();
+    net->deliver_waves = waves;
+    if (!tg.empty()) {
+        int rc;
+        if ((rc = dev_alloc(net->deliver_targets, tg.size() * sizeof(DeliverTarget)))) return rc;
+        if ((rc = dev_alloc(net->deliver_src, src.size() * sizeof(DeliverSrc)))) return rc;
+        HIPCHK(hipMemcpy(net->deliver_targets.p, tg.data(), tg.size() * sizeof(DeliverTarget), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(net->deliver_src.p, src.data(), src.size() * sizeof(DeliverSrc), hipMemcpyHostToDevice));
     }
     return 0;
 }
@@ -533,6 +618,10 @@ int prepare(ahip_net* net) {
             d.cnt = nullptr;
             d.boff = nullptr;
             d.bucket = nullptr;
+            d.touched = nullptr;
+            d.scnt = nullptr;
+            d.shards = 1;
+            d.shard_cap = 0;
             d.gid_off = 0;
             for (OrderedTarget& o : net->ordered) {
                 if (o.g != g) continue;
@@ -544,6 +633,11 @@ int prepare(ahip_net* net) {
                 d.cnt = o.cnt.as<int32_t>();
                 d.boff = o.boff.as<int64_t>();
                 d.bucket = o.bucket.as<uint32_t>();
+                // targets delivered by the post update need no touched list
+                d.touched = o.in_update ? nullptr : o.touched.as<int32_t>();
+                d.scnt = o.scnt.as<int32_t>();
+                d.shards = o.shards;
+                d.shard_cap = o.shard_cap;
                 d.gid_off = off;
             }
             groups[k].push_back(d);
@@ -628,6 +722,8 @@ ahip_pop_args make_args(ahip_net* net, Pop& p, int32_t step) {
     a.last_spike = p.last_spike.as<int64_t>();
     a.spike_mask = p.spike_mask.as<uint64_t>();
     a.block_counts = p.block_counts.as<int32_t>();
+    a.deliver = p.deliver.as<ahip_deliver_desc>();
+    a.n_deliver = p.n_deliver;
     return a;
 }
 
@@ -650,10 +746,10 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
         LCHK(launch_scatter_group(net, net->scatter_groups[gi], step, s));
         if (probe) HIPCHK(hipEventRecord(net->probe_ev[1], s));
     }
-    for (size_t oi = 0; oi < net->ordered.size(); oi++) {
-        const bool probe = net->probe_kind == AHIP_PROBE_DELIVER && net->probe_proj == (int32_t)oi;
+    if (net->deliver_waves > 0) {
+        const bool probe = net->probe_kind == AHIP_PROBE_DELIVER;
         if (probe) HIPCHK(hipEventRecord(net->probe_ev[0], s));
-        LCHK(launch_deliver(net, net->ordered[oi], s));
+        LCHK(launch_deliver(net, s));
         if (probe) HIPCHK(hipEventRecord(net->probe_ev[1], s));
     }
     for (auto& pp : net->projs) {
@@ -2023,7 +2119,8 @@ int ahip_time_kernel(ahip_net* net, int32_t what, int32_t index, int32_t iters, 
             if (index < 0 || index >= (int32_t)net->scatter_groups.size()) return fail(AHIP_EINVAL, "bad scatter group");
             break;
         case AHIP_PROBE_DELIVER:
-            if (index < 0 || index >= (int32_t)net->ordered.size()) return fail(AHIP_EINVAL, "bad ordered target");
+            if (index != 0 || net->deliver_waves == 0)
+                return fail(AHIP_EINVAL, "no separate ordered delivery launch (index 0): the updates deliver");
             break;
         default:
             return fail(AHIP_EINVAL, "unknown kernel kind %d", what);

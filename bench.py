@@ -266,15 +266,19 @@ def _spiking_kernels(net, P, projs, outdeg, mon, es, n_steps_done):
     eager steps per kernel) and its algorithmic bytes over exactly those
     steps.  Kernel bytes:
       update      N * (6 s + 12): v, g_exc, g_inh read and written, refractory
-                  read, refractory_remaining read and written;
+                  read, refractory_remaining read and written; when it
+                  delivers the ordered conductances also N * 4 per
+                  conductance (counters), per event 4 + s (key, weight) and
+                  per touched neuron 20 (bucket range, counter cleared);
       scatter     per spike 20 B (list entry, column range), per synapse event
                   4 + 3 s (post rank, weight, conductance read and written) with
                   atomic delivery or one weight, and 4 + 8 + 16 + 4 (post rank,
                   bucket counter read and written, bucket range, key) ordered;
-      deliver     per conductance N * 4 (counters) + per event 4 + s (key,
-                  weight) + per neuron with events 16 + 2 s (range, g).
+      deliver     per touched post neuron 4 + 4 + 16 + 2 s (list entry, counter,
+                  bucket range, g read and written) + per event 4 + s (key,
+                  weight).
     """
-    from annarchy_amd import _lib  # noqa: F401
+    from annarchy_amd import _lib
 
     t = {}
     t["update"] = net.time_kernel(3, P.id, 20)
@@ -282,22 +286,38 @@ def _spiking_kernels(net, P, projs, outdeg, mon, es, n_steps_done):
     ng = net.scatter_groups()
     t["scatter"] = sum(net.time_kernel(5, g, 20) for g in range(ng))
     no = net.ordered_targets()
-    t["deliver"] = sum(net.time_kernel(6, k, 20) for k in range(no)) if no else 0.0
+    # ordered conductances are delivered by the post population's update
+    # unless another kernel writes them: a separate launch only then
+    sep = 0
+    if no:
+        try:
+            t["deliver"] = net.time_kernel(6, 0, 20)
+            sep = 1
+        except _lib.AhipError:
+            t["deliver"] = 0.0
+    else:
+        t["deliver"] = 0.0
     counts, idx = mon.raw_spikes()
     starts = np.concatenate([[0], np.cumsum(counts)])
-    # the kernels above ran 20 + 20 + 20 ng + 20 no steps after n_steps_done;
-    # the scatter of step u sends the spikes of step u - 1: average over the
-    # steps of the scatter and deliver probes
+    # the kernels above ran 20 + 20 + 20 ng + 20 (one delivery launch) steps
+    # after n_steps_done; the scatter of step u sends the spikes of step
+    # u - 1: average over the steps of the scatter and deliver probes
     first = n_steps_done + 40
-    last = n_steps_done + 40 + 20 * (ng + no)
+    last = n_steps_done + 40 + 20 * (ng + sep)
     us = range(first - 1, min(last - 1, len(counts)))
     syn = np.mean([outdeg[idx[starts[u]:starts[u + 1]]].sum() for u in us]) if len(us) else 0.0
     spk = np.mean([counts[u] for u in us]) if len(us) else 0.0
     N = P.size
     per_syn = (4 + 8 + 16 + 4) if no else (4 + 3 * es)
-    b = {"update": N * (6 * es + 12), "compact": N / 8 + spk * 4 + 8,
+    # touched post neurons: the expected number of distinct targets of `syn`
+    # events spread over N neurons, N (1 - exp(-syn / N))
+    touched = N * -np.expm1(-syn / N)
+    # update: + per ordered conductance N * 4 (counters) when it delivers,
+    # and per event 4 + s (key, weight), per touched neuron 16 + 4 (range, clear)
+    deliver_in_update = (no * N * 4 + syn * (4 + es) + touched * 20) if (no and not sep) else 0.0
+    b = {"update": N * (6 * es + 12) + deliver_in_update, "compact": N / 8 + spk * 4 + 8,
          "scatter": syn * per_syn + spk * 20,
-         "deliver": (no * N * 4 + syn * (4 + es) + min(syn, N) * (16 + 2 * es)) if no else 0.0}
+         "deliver": (syn * (4 + es) + touched * (24 + 2 * es)) if sep else 0.0}
     kern = {k: {"ms": t[k], "bytes": float(b[k])} for k in t if t[k] > 0}
     return kern, float(syn), float(spk)
 
@@ -489,22 +509,27 @@ def bench_c3(args):
     info = b.info()
     ms_spmv = b.time_kernel(2, 20)
     ms_gemm = b.time_kernel(1, 50)
-    log(f"c3: batched {el_b * 1e3 / steps:.3f} ms/step, spmv {ms_spmv * 1e3:.0f} us, gemm {ms_gemm * 1e3:.1f} us")
-    nets = sw.plain_networks()
-    plain_steps = min(steps, 200)
-    el_p = time_plain(nets, plain_steps)
-    for net, exc in nets:
-        net.run(20 + steps - plain_steps)
-    got = b.rates()
-    worst, pop_err = 0.0, 0.0
-    for i, (net, exc) in enumerate(nets):
-        ref = exc.r.reshape(-1)
-        worst = max(worst, float(np.max(np.abs(got[i] - ref)) / max(np.max(np.abs(ref)), 1e-3)))
-        pop_err = max(pop_err, abs(float(got[i].mean() - ref.mean())) / max(abs(ref.mean()), 1e-9))
+    log(f"c3: batched {el_b * 1e3 / steps:.3f} ms/step, spmv {ms_spmv * 1e3:.0f} us, gemm {ms_gemm * 1e3:.1f} us "
+        f"per launch of {info['chunk_steps']} steps")
     sps_b = steps / el_b
-    sps_p = plain_steps / el_p
+    sps_p = worst = pop_err = None
+    if not args.no_plain:
+        nets = sw.plain_networks()
+        plain_steps = min(steps, 200)
+        el_p = time_plain(nets, plain_steps)
+        for net, exc in nets:
+            net.run(20 + steps - plain_steps)
+        got = b.rates()
+        worst, pop_err = 0.0, 0.0
+        for i, (net, exc) in enumerate(nets):
+            ref = exc.r.reshape(-1)
+            worst = max(worst, float(np.max(np.abs(got[i] - ref)) / max(np.max(np.abs(ref)), 1e-3)))
+            pop_err = max(pop_err, abs(float(got[i].mean() - ref.mean())) / max(abs(ref.mean()), 1e-9))
+        sps_p = plain_steps / el_p
     achieved = info["sparse_bytes"] / (ms_spmv * 1e-3) / 1e9
-    tflops = info["dense_flops"] / (ms_gemm * 1e-3) / 1e12
+    # one GEMM launch covers the input psp of a whole chunk (graph batch) of steps
+    gemm_flop = info["dense_flops"] * info["chunk_steps"]
+    tflops = gemm_flop / (ms_gemm * 1e-3) / 1e12
     roof = roofline("hbm", achieved, HBM_PEAK_GBS, "GB/s",
                     traffic_from_profiles("batch_sell_f32_c3_sweep256_traffic.txt") if info["mode"] == 2 else None,
                     {0: "batch_spmv_update", 1: "batch_spmv_lds", 2: "batch_sell"}[info["mode"]],
@@ -512,11 +537,13 @@ def bench_c3(args):
     return record("steps/s (batched sweep)", sps_b, "steps/s", steps, 20, el_b * 1e3 / steps, "f32/bf16",
                   {"workload": "256 rate instances N~U{500..8000} p=0.05 + shared dense 2000x2000 input, 2000 steps",
                    "neurons": sw.total, "synapses": sw.nnz, "graph_steps": G},
-                  roof, cpu, plain_steps_per_s=sps_p, speedup_vs_plain=sps_b / sps_p,
+                  roof, cpu, plain_steps_per_s=sps_p, speedup_vs_plain=sps_b / sps_p if sps_p else None,
                   max_rel_rate_error_vs_fp64=worst, max_rel_population_rate_error=pop_err,
-                  gemm={"kernel": "batch_gemm", "ms": ms_gemm, "useful_flop": info["dense_flops"],
-                        "tflops": tflops, "peak_bf16_tflops": BF16_PEAK_TFLOPS,
-                        "frac": tflops / BF16_PEAK_TFLOPS})
+                  gemm={"kernel": "batch_gemm", "ms_per_launch": ms_gemm, "steps_per_launch": info["chunk_steps"],
+                        "ms_per_step": ms_gemm / info["chunk_steps"], "useful_flop_per_launch": gemm_flop,
+                        "mfma_flop_per_launch": 3 * gemm_flop, "tflops": tflops,
+                        "peak_bf16_tflops": BF16_PEAK_TFLOPS, "frac": tflops / BF16_PEAK_TFLOPS,
+                        "mfma_frac": 3 * tflops / BF16_PEAK_TFLOPS})
 
 
 # ---------------------------------------------------------------------------
@@ -648,6 +675,8 @@ def main():
     ap.add_argument("--single", action="store_true", help="run one config in this process")
     ap.add_argument("--c2-steps", type=int, default=2000)
     ap.add_argument("--c3-steps", type=int, default=2000)
+    ap.add_argument("--no-plain", action="store_true",
+                    help="c3: skip the plain per-instance FP64 networks (and the error and speedup they give)")
     ap.add_argument("--c4-sample", type=int, default=24)
     args = ap.parse_args()
 

@@ -265,7 +265,7 @@ int ahip_proj_set_learning(ahip_net* net, int32_t proj, int32_t update, int32_t 
 #define AHIP_PROBE_COMPACT 4
 #define AHIP_PROBE_SCATTER 5
 int ahip_time_kernel(ahip_net* net, int32_t what, int32_t index, int32_t iters, double* avg_ms);
-#define AHIP_PROBE_DELIVER 6   /* ordered delivery of conductance `index` (0 .. ahip_ordered_targets - 1) */
+#define AHIP_PROBE_DELIVER 6   /* the ordered delivery launch of every conductance (index 0) */
 /* Number of fused spike-scatter launches per step (spiking projections
  * without per-synapse delays or synapse models, grouped by target). */
 int ahip_scatter_groups(ahip_net* net, int32_t* n_groups);
@@ -274,7 +274,9 @@ int ahip_scatter_groups(ahip_net* net, int32_t* n_groups);
  * ascending pre-synaptic rank, the order of the reference's CSRC loop
  * (generator/Projection/OpenMP/CSR.py:870-885), so conductances are
  * bit-identical to the reference's; the scatter appends each event to the
- * post neuron's bucket and one more kernel per conductance adds the buckets.
+ * post neuron's bucket (and the neuron, at its first event of the step, to
+ * the conductance's touched list), and one more kernel adds the buckets of
+ * the touched neurons of every such conductance.
  * ATOMIC adds the weights with device atomics in arbitrary order (one
  * rounding of difference per event; runs can differ from each other).
  * Projections with a single weight are order-free either way.  No
@@ -282,8 +284,8 @@ int ahip_scatter_groups(ahip_net* net, int32_t* n_groups);
 #define AHIP_DELIVERY_ORDERED 0
 #define AHIP_DELIVERY_ATOMIC 1
 int ahip_net_set_spike_delivery(ahip_net* net, int32_t mode);
-/* Number of conductances delivered in order (ahip_time_kernel index range of
- * AHIP_PROBE_DELIVER). */
+/* Number of conductances delivered in order (all in one launch, probed as
+ * AHIP_PROBE_DELIVER index 0). */
 int ahip_ordered_targets(ahip_net* net, int32_t* n);
 /* Bytes the psp kernel must move per step (algorithmic, no padding). */
 int ahip_proj_psp_bytes(ahip_net* net, int32_t proj, double* bytes);
@@ -364,8 +366,13 @@ int ahip_batch_get_rates(ahip_batch* b, double* r);
 /* Y = W_in . X of the last step, n_in x n_inst */
 int ahip_batch_get_input_sums(ahip_batch* b, double* y);
 int ahip_batch_run(ahip_batch* b, int64_t steps, int32_t graph_batch);
-/* which: 0 input update, 1 GEMM, 2 SpMV + update; mean launch time (ms) */
+/* which: 0 input rates of one chunk, 1 GEMM of one chunk, 2 SpMV + update
+ * of one step; mean launch time (ms).  The input rates depend on t alone, so
+ * the input psp of a chunk of steps (one graph batch) is one GEMM
+ * Y^T = X^T W^T whose rows are (step, instance). */
 int ahip_batch_time(ahip_batch* b, int32_t which, int32_t iters, double* avg_ms);
+/* Steps per input chunk (the GEMM of one launch covers that many steps). */
+int ahip_batch_chunk_steps(ahip_batch* b, int32_t* steps);
 /* 2 (default when every instance has fewer than 16384 neurons): chunks of up
  * to 512 rows of one instance in sorted 64-row slices, one lane per row,
  * uint16 instance-local columns, instance rates staged in LDS; 1: the

@@ -22,10 +22,33 @@
 extern "C" {
 #endif
 
-#define AHIP_MODEL_ABI 3
+#define AHIP_MODEL_ABI 4
 #define AHIP_MAX_SLOTS 48
 #define AHIP_MAX_RAND 8
 #define AHIP_UPDATE_BLOCK 256 /* threads per block of every update kernel */
+
+/* One projection feeding an ordered conductance: its event keys are
+ * [gid_off, gid_end); key k names column-order synapse k - gid_off. */
+typedef struct ahip_deliver_src {
+    const void* w_col;                  /* weights in column order, or NULL: w_const */
+    double w_const;
+    uint32_t gid_off, gid_end;
+} ahip_deliver_src;
+
+/* Ordered spike delivery folded into the update of the post population: the
+ * events of neuron i this step are keys bucket[boff[i] .. boff[i] + cnt[i]),
+ * added to g[i] in ascending key order (projection by projection, ascending
+ * pre-synaptic rank: the reference's summation order) before the equations
+ * read g; cnt[i] is cleared. */
+typedef struct ahip_deliver_desc {
+    void* g;
+    const int64_t* boff;
+    int32_t* cnt;
+    const uint32_t* bucket;
+    const ahip_deliver_src* src;
+    int32_t n_src;
+    int32_t reserved;
+} ahip_deliver_desc;
 
 /* Arguments of one population update launch.  Device pointers unless noted. */
 typedef struct ahip_pop_args {
@@ -44,6 +67,9 @@ typedef struct ahip_pop_args {
     int64_t* last_spike;
     uint64_t* spike_mask;               /* ceil(size/64) words, bit i = neuron spiked */
     int32_t* block_counts;              /* ceil(size/256) spike counts per update block */
+    const ahip_deliver_desc* deliver;   /* ordered conductances delivered by this update, or NULL */
+    int32_t n_deliver;
+    int32_t reserved;
 } ahip_pop_args;
 
 typedef int (*ahip_launch_fn)(const ahip_pop_args* args, void* stream);

@@ -90,3 +90,22 @@ def test_batched_graph_remainder_parity(gpu):
     b.run(50, graph_batch=50)
     for x, y in zip(a.rates(), b.rates()):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("sizes,n_in,steps", [([150, 900, 40, 500, 1200, 75, 333], 100, 5),
+                                               ([300] * 40, 200, 9),
+                                               ([90, 700], 48, 3)])
+def test_chunked_gemm_last_step(gpu, sizes, n_in, steps):
+    # one eager chunk of `steps` steps: the input psp of its last step is
+    # W_in . X(t = steps - 2); shapes with partial tiles along every GEMM
+    # dimension (instances not a multiple of 32, input neurons not a
+    # multiple of 16 or 128, an odd number of row tiles)
+    sw = RateSweep(sizes, n_in=n_in, p=0.05, seed=6)
+    b = sw.batched(lanes=16)
+    b.run(steps, graph_batch=steps + 1)
+    assert b.info()["chunk_steps"] >= steps
+    y = b.input_sums()
+    x = 1.0 + 0.5 * np.sin(sw.omega * (steps - 2) * sw.dt + sw.phase)
+    ref = sw.w_in @ x
+    err = np.max(np.abs(y - ref)) / np.max(np.abs(ref))
+    assert err < 2e-5, err
