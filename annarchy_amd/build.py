@@ -96,9 +96,40 @@ def build_core(jobs: int = 8) -> Path:
     return CORE_LIB
 
 
+# The library name hashes the generated source together with the headers it
+# is compiled against (the plug-in ABI and the device helpers), so a library
+# built against other headers is never picked up under the same name.
+_ABI_HEADERS = (INCLUDE / "annarchy_hip_model.h", INCLUDE / "annarchy_hip_synapse.h", CSRC / "ahip_device.hpp")
+
+
+def _headers_digest() -> bytes:
+    h = hashlib.sha1()
+    for f in _ABI_HEADERS:
+        h.update(f.read_bytes())
+    return h.digest()
+
+
 def model_lib_path(name: str, source: str, kind: str = "model") -> Path:
-    h = hashlib.sha1(source.encode()).hexdigest()[:12]
+    h = hashlib.sha1(source.encode() + _headers_digest()).hexdigest()[:12]
     return LIB_DIR / f"libahip_{kind}_{name}_{h}.so"
+
+
+def prune_model_libs(keep) -> int:
+    """Delete model and synapse libraries (and their generated sources) not
+    in `keep`; returns how many libraries were removed."""
+    keep = {Path(k).name for k in keep}
+    n = 0
+    for f in LIB_DIR.glob("libahip_*.so"):
+        if f.name not in keep:
+            f.unlink()
+            n += 1
+            g = GEN_DIR / (f.stem + ".hip")
+            if g.exists():
+                g.unlink()
+    for g in GEN_DIR.glob("libahip_*.hip"):
+        if g.stem + ".so" not in keep:
+            g.unlink()
+    return n
 
 
 def build_model(name: str, source: str, force: bool = False, kind: str = "model") -> Path:

@@ -174,85 +174,89 @@ __device__ inline void publish_spikes(const ahip_pop_args& a, bool spk) {
 // weights in ascending key (projection by projection in creation order, in
 // ascending pre-synaptic rank) reproduces every rounding of the reference's
 // CSRC loop g[post] += w (generator/Projection/OpenMP/CSR.py:870-885).
-template <int N>
-__device__ inline void bitonic_sort_keys(uint32_t (&a)[N]) {
-#pragma unroll
-    for (int k = 2; k <= N; k <<= 1)
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1)
-#pragma unroll
-            for (int i = 0; i < N; i++) {
-                const int l = i ^ j;
-                if (l > i) {
-                    const uint32_t x = a[i], y = a[l];
-                    const bool up = (i & k) == 0;
-                    const uint32_t lo = x < y ? x : y, hi = x < y ? y : x;
-                    a[i] = up ? lo : hi;
-                    a[l] = up ? hi : lo;
-                }
-            }
-}
+constexpr int kDeliverSort = 4;
 
-constexpr int kDeliverSort = 16;
-
+// acc + the weights w[b .. b + L) in ascending key[b .. b + L).  Up to 4
+// (key, weight) pairs are sorted in registers by a bitonic network on the
+// keys (a neuron rarely receives more events in one step, and a small
+// network keeps the update kernel's registers, hence its occupancy, low);
+// a longer bucket is walked by repeated minimum, O(L^2) reads of the keys.
 template <typename T>
-__device__ inline T deliver_weight(const ahip_deliver_src* __restrict__ src, int32_t n_src, uint32_t key) {
-    int32_t k = 0;
-    while (k < n_src - 1 && key >= src[k].gid_end) k++;
-    const T* wc = static_cast<const T*>(src[k].w_col);
-    return wc ? wc[key - src[k].gid_off] : (T)src[k].w_const;
-}
-
-// acc + the weights of bucket[b .. b + L) in ascending key.  Up to 16 keys
-// are sorted in registers by a bitonic network; a longer bucket (more than
-// 16 of the neuron's inputs fired in one step) is walked by repeated
-// minimum, O(L^2) reads of the bucket.
-template <typename T>
-__device__ inline T ordered_sum(T acc, const uint32_t* __restrict__ bucket, int64_t b, int32_t L,
-                                const ahip_deliver_src* __restrict__ src, int32_t n_src) {
-    if (L == 1) return acc + deliver_weight<T>(src, n_src, bucket[b]);
+__device__ inline T ordered_sum(T acc, const uint32_t* __restrict__ key, const T* __restrict__ w, int64_t b,
+                                int32_t L) {
+    if (L == 1) return acc + w[b];
     if (L <= kDeliverSort) {
-        uint32_t key[kDeliverSort];
+        uint32_t k[kDeliverSort];
+        T v[kDeliverSort];
 #pragma unroll
-        for (int m = 0; m < kDeliverSort; m++) key[m] = m < L ? bucket[b + m] : 0xffffffffu;
-        bitonic_sort_keys(key);
-        T w[kDeliverSort];
+        for (int m = 0; m < kDeliverSort; m++) {
+            k[m] = m < L ? key[b + m] : 0xffffffffu;
+            v[m] = m < L ? w[b + m] : T(0);
+        }
 #pragma unroll
-        for (int m = 0; m < kDeliverSort; m++) w[m] = m < L ? deliver_weight<T>(src, n_src, key[m]) : T(0);
+        for (int s = 2; s <= kDeliverSort; s <<= 1)
+#pragma unroll
+            for (int j = s >> 1; j > 0; j >>= 1)
+#pragma unroll
+                for (int i = 0; i < kDeliverSort; i++) {
+                    const int l = i ^ j;
+                    if (l > i) {
+                        const bool up = (i & s) == 0;
+                        const bool swap = up ? k[i] > k[l] : k[i] < k[l];
+                        const uint32_t ka = k[i], kb = k[l];
+                        const T va = v[i], vb = v[l];
+                        k[i] = swap ? kb : ka;
+                        k[l] = swap ? ka : kb;
+                        v[i] = swap ? vb : va;
+                        v[l] = swap ? va : vb;
+                    }
+                }
 #pragma unroll
         for (int m = 0; m < kDeliverSort; m++)
-            if (m < L) acc += w[m];
+            if (m < L) acc += v[m];
         return acc;
     }
     uint32_t last = 0;
     for (int32_t m = 0; m < L; m++) {
         uint32_t best = 0xffffffffu;
+        int64_t at = b;
         for (int32_t q = 0; q < L; q++) {
-            const uint32_t v = bucket[b + q];
-            if ((m == 0 || v > last) && v < best) best = v;
+            const uint32_t x = key[b + q];
+            if ((m == 0 || x > last) && x < best) {
+                best = x;
+                at = b + q;
+            }
         }
-        acc += deliver_weight<T>(src, n_src, best);
+        acc += w[at];
         last = best;
     }
     return acc;
 }
 
 // The ordered conductances of this population, neuron i: add its buckets
-// and clear its counters (called by the update kernel before the equations
-// read the conductances).
+// and clear their counts (called by the update kernel before the equations
+// read the conductances).  The heads of every conductance are loaded before
+// any bucket is summed.
 template <typename T>
 __device__ inline void deliver_buckets(const ahip_pop_args& a, int32_t i) {
-    for (int32_t k = 0; k < a.n_deliver; k++) {
-        const ahip_deliver_desc& d = a.deliver[k];
-        const int32_t c = d.cnt[i];
+    uint64_t h[AHIP_MAX_DELIVER], hn[AHIP_MAX_DELIVER];
+#pragma unroll
+    for (int k = 0; k < AHIP_MAX_DELIVER; k++) {
+        h[k] = k < a.n_deliver ? a.deliver[k].head[i] : 0;
+        hn[k] = k < a.n_deliver ? a.deliver[k].head[i + 1] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < AHIP_MAX_DELIVER; k++) {
+        const int64_t c = (int64_t)(h[k] & AHIP_HEAD_COUNT);
         if (c == 0) continue;
-        d.cnt[i] = 0;
-        const int64_t b = d.boff[i];
+        const ahip_deliver_desc& d = a.deliver[k];
+        const int64_t b = (int64_t)(h[k] >> AHIP_HEAD_SHIFT);
+        d.head[i] = h[k] & ~AHIP_HEAD_COUNT;
         // a bucket never holds more events than the neuron's summed in-degree
-        const int32_t L = (int32_t)min((int64_t)c, d.boff[i + 1] - b);
+        const int32_t L = (int32_t)min(c, (int64_t)(hn[k] >> AHIP_HEAD_SHIFT) - b);
         if (L <= 0) continue;
         T* g = static_cast<T*>(d.g);
-        g[i] = ordered_sum<T>(g[i], d.bucket, b, L, d.src, d.n_src);
+        g[i] = ordered_sum<T>(g[i], d.bucket, static_cast<const T*>(d.bucket_w), b, L);
     }
 }
 

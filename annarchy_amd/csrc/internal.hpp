@@ -181,9 +181,10 @@ struct ScatterDesc {
     // ordered delivery (cnt != NULL): the event of column-order synapse j is
     // appended as key gid_off + j to the bucket of its post neuron instead of
     // being added to g; deliver_ordered adds the bucket in ascending keys
-    int32_t* cnt;             // events per post neuron this step
-    const int64_t* boff;      // bucket of post neuron i: [boff[i], boff[i+1])
+    uint64_t* head;           // post neuron i: (bucket base << AHIP_HEAD_SHIFT) | events this step
+    int64_t cap;              // bucket entries
     uint32_t* bucket;
+    void* bucket_w;           // the events' weights beside their keys
     // post neurons with events this step, sharded by post & (shards - 1):
     // shard s lists its neurons at touched[s * shard_cap ..] and counts them
     // in scnt[s * kShardStride] (the delivery clears it)
@@ -195,22 +196,18 @@ struct ScatterDesc {
 constexpr int kMaxScatter = 8;
 // One projection feeding an ordered target: its keys are [gid_off, gid_end),
 // key k names column-order synapse k - gid_off of that projection.
-using DeliverSrc = ahip_deliver_src;
-constexpr int kMaxDeliverSrc = 32;   // projections per ordered conductance
-constexpr int kMaxDeliverLds = 64;   // projections over all ordered conductances
-// One ordered conductance in the merged delivery launch: its touched post
-// neurons of the step and its sources src[src0 .. src0 + n_src) of the
-// network's DeliverSrc table.
+// One ordered conductance in the merged delivery launch (conductances that
+// no update can deliver): its touched post neurons of the step, sharded.
 struct DeliverTarget {
     void* g;
-    const int64_t* boff;
-    int32_t* cnt;
+    uint64_t* head;
     const uint32_t* bucket;
+    const void* bucket_w;
     const int32_t* touched;
     int32_t* scnt;
     int32_t shards, shard_cap;
     int32_t wave0;            // first wave of this target in the launch (one wave per shard)
-    int32_t src0, n_src;
+    int32_t reserved;
 };
 constexpr int kShardStride = 16;     // one 64-byte line per shard counter
 constexpr int kMaxShards = 1024;
@@ -224,20 +221,19 @@ struct OrderedTarget {
     void* g = nullptr;
     int32_t pop = 0, size = 0;
     int64_t cap = 0;              // bucket entries (the summed in-degree)
-    int32_t n_src = 0;
     std::vector<int32_t> projs;   // projection ids, creation order
-    DevBuf boff;                  // int64[size + 1]
-    DevBuf cnt;                   // int32[size], zero between steps
-    DevBuf bucket;                // uint32[cap]
+    std::vector<int64_t> key_off; // key offset of each projection (its keys: key_off + column-order index)
+    DevBuf head;                  // uint64[size + 1]: (bucket base << AHIP_HEAD_SHIFT) | count, count 0 between steps
+    DevBuf bucket;                // uint32[cap] keys
+    DevBuf bucket_w;              // [cap] weights, network precision
     int32_t shards = 1, shard_cap = 0;
     DevBuf touched;               // int32[shards * shard_cap]
     DevBuf scnt;                  // int32[shards * kShardStride]
-    std::vector<DeliverSrc> src;  // in creation order of the projections
     // delivered by the update kernel of the post population (no other writer
-    // of the conductance, no monitor reading it right after compute_psp);
-    // otherwise by the merged delivery launch after the scatters
+    // of the conductance, no monitor reading it right after compute_psp, at
+    // most AHIP_MAX_DELIVER per population); otherwise by the merged
+    // delivery launch after the scatters
     bool in_update = false;
-    DevBuf src_dev;               // DeliverSrc[n_src] (in_update)
 };
 // Spiking projections launched together: their targets are distinct, so the
 // atomics of one projection never interleave with another's into the same
@@ -274,9 +270,8 @@ struct ahip_net {
     hipEvent_t probe_ev[2] = {nullptr, nullptr};
     std::vector<ahip::ScatterGroup> scatter_groups;
     std::vector<ahip::OrderedTarget> ordered;   // targets delivered in the reference's order
-    ahip::DevBuf deliver_targets;               // DeliverTarget[ordered.size()]
-    ahip::DevBuf deliver_src;                   // DeliverSrc of every ordered target
-    int32_t deliver_n_tg = 0, deliver_n_src = 0, deliver_waves = 0;  // the merged delivery launch
+    ahip::DevBuf deliver_targets;               // DeliverTarget of the merged delivery launch
+    int32_t deliver_n_tg = 0, deliver_waves = 0;
     int32_t delivery = AHIP_DELIVERY_ORDERED;
     size_t elem() const { return precision == AHIP_F64 ? sizeof(double) : sizeof(float); }
 };

@@ -217,29 +217,35 @@ __global__ __launch_bounds__(256) void scatter_multi(const ScatterDesc* __restri
         const int32_t pre = d.list[list_slot(d, t) * d.list_stride + s];
         if (pre < 0 || pre >= d.n_cols) continue;
         const int64_t b = d.col_ptr[pre], e = d.col_ptr[pre + 1];
-        if (d.cnt) {
-            // ordered delivery: append key gid_off + j to the post neuron's
-            // bucket; each synapse fires at most once per step, so a bucket
-            // never holds more entries than the neuron's summed in-degree
+        const T* wc = static_cast<const T*>(d.w_col);
+        const T wk = (T)d.w_const;
+        if (d.head) {
+            // ordered delivery: append (key gid_off + j, weight) to the post
+            // neuron's bucket; each synapse fires at most once per step, so a
+            // bucket never holds more entries than the neuron's summed in-degree
             for (int64_t base = b + (int64_t)c * 256; base < e; base += (int64_t)d.chunks * 256) {
-                int32_t post[4], slot[4];
-                int64_t lo[4], hi[4];
+                int32_t post[4];
+                int64_t at[4], slot[4];
+                T w[4];
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const int64_t j = base + lane + u * kWave;
                     post[u] = j < e ? d.row_idx[j] : -1;
+                    w[u] = wc == nullptr ? wk : (j < e ? wc[j] : T(0));
                 }
 #pragma unroll
                 for (int u = 0; u < 4; u++)
                     if (post[u] >= 0) {
-                        slot[u] = atomicAdd(d.cnt + post[u], 1);
-                        lo[u] = d.boff[post[u]];
-                        hi[u] = d.boff[post[u] + 1];
+                        // one atomic returns both the bucket base and the slot
+                        const uint64_t old = atomicAdd((unsigned long long*)(d.head + post[u]), 1ull);
+                        slot[u] = (int64_t)(old & AHIP_HEAD_COUNT);
+                        at[u] = (int64_t)(old >> AHIP_HEAD_SHIFT) + slot[u];
                     }
 #pragma unroll
                 for (int u = 0; u < 4; u++)
-                    if (post[u] >= 0 && lo[u] + slot[u] < hi[u]) {
-                        d.bucket[lo[u] + slot[u]] = d.gid_off + (uint32_t)(base + lane + u * kWave);
+                    if (post[u] >= 0 && at[u] < d.cap) {
+                        d.bucket[at[u]] = d.gid_off + (uint32_t)(base + lane + u * kWave);
+                        static_cast<T*>(d.bucket_w)[at[u]] = w[u];
                         // the first event of a post neuron in this step lists it in its
                         // shard (targets delivered by their own launch)
                         if (slot[u] == 0 && d.touched) {
@@ -251,8 +257,6 @@ __global__ __launch_bounds__(256) void scatter_multi(const ScatterDesc* __restri
             }
             continue;
         }
-        const T* wc = static_cast<const T*>(d.w_col);
-        const T wk = (T)d.w_const;
         T* g = static_cast<T*>(d.g);
         for (int64_t base = b + (int64_t)c * 256; base < e; base += (int64_t)d.chunks * 256) {
             int32_t post[4];
@@ -274,23 +278,20 @@ __global__ __launch_bounds__(256) void scatter_multi(const ScatterDesc* __restri
 // touched post neurons of each target (the shards of a target follow one
 // another); the wave delivers the shard's neurons and clears its counter.
 template <typename T>
-__device__ inline void deliver_one(const DeliverTarget& d, const DeliverSrc* __restrict__ sk, int32_t i) {
-    const int64_t b = d.boff[i];
+__device__ inline void deliver_one(const DeliverTarget& d, int32_t i) {
+    const uint64_t h = d.head[i], hn = d.head[i + 1];
+    d.head[i] = h & ~AHIP_HEAD_COUNT;
+    const int64_t b = (int64_t)(h >> AHIP_HEAD_SHIFT);
     // a bucket never holds more events than the neuron's summed in-degree
-    const int32_t L = (int32_t)min((int64_t)d.cnt[i], d.boff[i + 1] - b);
-    d.cnt[i] = 0;
+    const int32_t L = (int32_t)min((int64_t)(h & AHIP_HEAD_COUNT), (int64_t)(hn >> AHIP_HEAD_SHIFT) - b);
     if (L <= 0) return;
     T* g = static_cast<T*>(d.g);
-    g[i] = ordered_sum<T>(g[i], d.bucket, b, L, sk, d.n_src);
+    g[i] = ordered_sum<T>(g[i], d.bucket, static_cast<const T*>(d.bucket_w), b, L);
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void deliver_ordered(const DeliverTarget* __restrict__ tg, int32_t n_tg,
-                                                       const DeliverSrc* __restrict__ src, int32_t n_src,
                                                        int32_t n_waves) {
-    __shared__ DeliverSrc ls[kMaxDeliverLds];
-    if (threadIdx.x < n_src) ls[threadIdx.x] = src[threadIdx.x];
-    __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const int32_t w = blockIdx.x * 4 + threadIdx.x / kWave;
     if (w >= n_waves) return;
@@ -301,7 +302,7 @@ __global__ __launch_bounds__(256) void deliver_ordered(const DeliverTarget* __re
     int32_t* sc = d.scnt + sh * kShardStride;
     const int32_t n = *sc;
     const int32_t* list = d.touched + (int64_t)sh * d.shard_cap;
-    for (int32_t q = lane; q < n; q += kWave) deliver_one<T>(d, ls + d.src0, list[q]);
+    for (int32_t q = lane; q < n; q += kWave) deliver_one<T>(d, list[q]);
     // every lane of the wave has read the count above
     if (lane == 0 && n) *sc = 0;
 }
@@ -412,10 +413,10 @@ int launch_deliver(const ahip_net* net, hipStream_t s) {
     const unsigned nb = (unsigned)((net->deliver_waves + 3) / 4);
     if (net->precision == AHIP_F64)
         hipLaunchKernelGGL(deliver_ordered<double>, dim3(nb), dim3(256), 0, s, net->deliver_targets.as<DeliverTarget>(),
-                           n_tg, net->deliver_src.as<DeliverSrc>(), net->deliver_n_src, net->deliver_waves);
+                           n_tg, net->deliver_waves);
     else
         hipLaunchKernelGGL(deliver_ordered<float>, dim3(nb), dim3(256), 0, s, net->deliver_targets.as<DeliverTarget>(),
-                           n_tg, net->deliver_src.as<DeliverSrc>(), net->deliver_n_src, net->deliver_waves);
+                           n_tg, net->deliver_waves);
     return (int)hipGetLastError();
 }
 

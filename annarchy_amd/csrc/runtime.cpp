@@ -405,35 +405,34 @@ int build_ordered_targets(ahip_net* net, const std::vector<const void*>& targets
             if (net->pops[p.post]->slots[p.target_slot].p != g) continue;
             o.pop = p.post;
             o.projs.push_back(p.id);
-            DeliverSrc d;
-            d.w_col = p.single_w ? nullptr : p.w_col.p;
-            d.w_const = p.w_const;
-            d.gid_off = (uint32_t)total;
+            o.key_off.push_back(total);
             total += p.nnz;
-            d.gid_end = (uint32_t)std::min<int64_t>(total, UINT32_MAX);
-            o.src.push_back(d);
         }
         if (total >= (int64_t)UINT32_MAX)
             return fail(AHIP_EINVAL, "ordered spike delivery keys 32-bit synapse indices: %lld synapses into one "
                         "conductance (use AHIP_DELIVERY_ATOMIC)", (long long)total);
-        if ((int32_t)o.src.size() > kMaxDeliverSrc)
-            return fail(AHIP_EINVAL, "ordered spike delivery: more than %d projections into one conductance",
-                        kMaxDeliverSrc);
         const Pop& post = *net->pops[o.pop];
         o.size = post.size;
         o.cap = total;
-        o.n_src = (int32_t)o.src.size();
         std::vector<int64_t> boff(o.size + 1, 0);
         for (int32_t id : o.projs) {
             const Proj& p = *net->projs[id];
             for (int32_t i = 0; i < o.size; i++) boff[i + 1] += p.h_row_ptr[i + 1] - p.h_row_ptr[i];
         }
+        int64_t max_in = 0;
+        for (int32_t i = 0; i < o.size; i++) max_in = std::max(max_in, boff[i + 1]);
+        if (max_in >= (int64_t)AHIP_HEAD_COUNT)
+            return fail(AHIP_EINVAL, "ordered spike delivery counts events in %d bits: in-degree %lld into one "
+                        "conductance (use AHIP_DELIVERY_ATOMIC)", AHIP_HEAD_SHIFT, (long long)max_in);
         for (int32_t i = 0; i < o.size; i++) boff[i + 1] += boff[i];
+        // head[i] = bucket base << AHIP_HEAD_SHIFT, event count 0
+        std::vector<uint64_t> head(o.size + 1);
+        for (int32_t i = 0; i <= o.size; i++) head[i] = (uint64_t)boff[i] << AHIP_HEAD_SHIFT;
         for (OrderedTarget& q : old)
             if (q.g == g) {
-                if (q.boff.bytes >= boff.size() * sizeof(int64_t)) o.boff = std::move(q.boff);
-                if (q.cnt.bytes >= (size_t)o.size * sizeof(int32_t)) o.cnt = std::move(q.cnt);
+                if (q.head.bytes >= head.size() * sizeof(uint64_t)) o.head = std::move(q.head);
                 if (q.bucket.bytes >= (size_t)o.cap * sizeof(uint32_t)) o.bucket = std::move(q.bucket);
+                if (q.bucket_w.bytes >= (size_t)o.cap * net->elem()) o.bucket_w = std::move(q.bucket_w);
                 if (q.touched.bytes >= (size_t)o.size * sizeof(int32_t)) o.touched = std::move(q.touched);
             }
         // shards of at most 64 neurons each (one wave), a power of two up to kMaxShards
@@ -441,15 +440,14 @@ int build_ordered_targets(ahip_net* net, const std::vector<const void*>& targets
         while (o.shards < kMaxShards && (int64_t)o.shards * 64 < o.size) o.shards *= 2;
         o.shard_cap = (o.size + o.shards - 1) / o.shards;
         int rc;
-        if (!o.boff.p && (rc = dev_alloc(o.boff, boff.size() * sizeof(int64_t)))) return rc;
-        if (!o.cnt.p && (rc = dev_alloc(o.cnt, (size_t)o.size * sizeof(int32_t)))) return rc;
+        if (!o.head.p && (rc = dev_alloc(o.head, head.size() * sizeof(uint64_t)))) return rc;
         if (!o.bucket.p && (rc = dev_alloc(o.bucket, (size_t)o.cap * sizeof(uint32_t)))) return rc;
+        if (!o.bucket_w.p && (rc = dev_alloc(o.bucket_w, (size_t)o.cap * net->elem()))) return rc;
         const size_t touched_n = (size_t)o.shards * o.shard_cap;
         if (o.touched.bytes < touched_n * sizeof(int32_t) && (rc = dev_alloc(o.touched, touched_n * sizeof(int32_t))))
             return rc;
         if ((rc = dev_alloc(o.scnt, (size_t)o.shards * kShardStride * sizeof(int32_t)))) return rc;
-        HIPCHK(hipMemcpy(o.boff.p, boff.data(), boff.size() * sizeof(int64_t), hipMemcpyHostToDevice));
-        HIPCHK(hipMemset(o.cnt.p, 0, (size_t)o.size * sizeof(int32_t)));
+        HIPCHK(hipMemcpy(o.head.p, head.data(), head.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
         // the post population's update adds the buckets when no other kernel
         // writes the conductance and no monitor records it after compute_psp
         o.in_update = true;
@@ -463,10 +461,10 @@ int build_ordered_targets(ahip_net* net, const std::vector<const void*>& targets
             const Monitor& m = *mp;
             if (m.after_psp && m.pop == o.pop && net->pops[o.pop]->slots[m.slot].p == g) o.in_update = false;
         }
-        if (o.in_update) {
-            if ((rc = dev_alloc(o.src_dev, o.src.size() * sizeof(DeliverSrc)))) return rc;
-            HIPCHK(hipMemcpy(o.src_dev.p, o.src.data(), o.src.size() * sizeof(DeliverSrc), hipMemcpyHostToDevice));
-        }
+        int32_t fused_here = 0;
+        for (const OrderedTarget& q : net->ordered)
+            if (q.in_update && q.pop == o.pop) fused_here++;
+        if (fused_here >= AHIP_MAX_DELIVER) o.in_update = false;
         HIPCHK(hipDeviceSynchronize());
         net->ordered.push_back(std::move(o));
     }
@@ -479,11 +477,9 @@ int build_ordered_targets(ahip_net* net, const std::vector<const void*>& targets
             ahip_deliver_desc d;
             std::memset(&d, 0, sizeof(d));
             d.g = o.g;
-            d.boff = o.boff.as<int64_t>();
-            d.cnt = o.cnt.as<int32_t>();
+            d.head = o.head.as<uint64_t>();
             d.bucket = o.bucket.as<uint32_t>();
-            d.src = o.src_dev.as<DeliverSrc>();
-            d.n_src = o.n_src;
+            d.bucket_w = o.bucket_w.p;
             dd.push_back(d);
         }
         p.n_deliver = (int32_t)dd.size();
@@ -497,40 +493,30 @@ int build_ordered_targets(ahip_net* net, const std::vector<const void*>& targets
     // the merged delivery launch: one DeliverTarget per conductance, their
     // sources in one table (staged in LDS by the kernel)
     std::vector<DeliverTarget> tg;
-    std::vector<DeliverSrc> src;
     int32_t waves = 0;
     for (OrderedTarget& o : net->ordered) {
         if (o.in_update) continue;
         DeliverTarget d;
         d.g = o.g;
-        d.boff = o.boff.as<int64_t>();
-        d.cnt = o.cnt.as<int32_t>();
+        d.head = o.head.as<uint64_t>();
         d.bucket = o.bucket.as<uint32_t>();
+        d.bucket_w = o.bucket_w.p;
         d.touched = o.touched.as<int32_t>();
         d.scnt = o.scnt.as<int32_t>();
         d.shards = o.shards;
         d.shard_cap = o.shard_cap;
         d.wave0 = waves;
-        d.src0 = (int32_t)src.size();
-        d.n_src = o.n_src;
-        src.insert(src.end(), o.src.begin(), o.src.end());
+        d.reserved = 0;
         tg.push_back(d);
         waves += o.shards;
     }
-    if ((int32_t)src.size() > kMaxDeliverLds)
-        return fail(AHIP_EINVAL, "ordered spike delivery: more than %d projections into ordered conductances",
-                    kMaxDeliverLds);
     net->deliver_targets.release();
-    net->deliver_src.release();
     net->deliver_n_tg = (int32_t)tg.size();
-    net->deliver_n_src = (int32_t)src.size();
     net->deliver_waves = waves;
     if (!tg.empty()) {
         int rc;
         if ((rc = dev_alloc(net->deliver_targets, tg.size() * sizeof(DeliverTarget)))) return rc;
-        if ((rc = dev_alloc(net->deliver_src, src.size() * sizeof(DeliverSrc)))) return rc;
         HIPCHK(hipMemcpy(net->deliver_targets.p, tg.data(), tg.size() * sizeof(DeliverTarget), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(net->deliver_src.p, src.data(), src.size() * sizeof(DeliverSrc), hipMemcpyHostToDevice));
     }
     return 0;
 }
@@ -615,9 +601,10 @@ int prepare(ahip_net* net) {
             d.w_const = p.w_const;
             d.n_cols = p.n_cols;
             d.chunks = p.scatter_chunks;
-            d.cnt = nullptr;
-            d.boff = nullptr;
+            d.head = nullptr;
+            d.cap = 0;
             d.bucket = nullptr;
+            d.bucket_w = nullptr;
             d.touched = nullptr;
             d.scnt = nullptr;
             d.shards = 1;
@@ -626,13 +613,12 @@ int prepare(ahip_net* net) {
             for (OrderedTarget& o : net->ordered) {
                 if (o.g != g) continue;
                 uint32_t off = 0;
-                for (int32_t id : o.projs) {
-                    if (id == p.id) break;
-                    off += (uint32_t)net->projs[id]->nnz;
-                }
-                d.cnt = o.cnt.as<int32_t>();
-                d.boff = o.boff.as<int64_t>();
+                for (size_t q = 0; q < o.projs.size(); q++)
+                    if (o.projs[q] == p.id) off = (uint32_t)o.key_off[q];
+                d.head = o.head.as<uint64_t>();
+                d.cap = o.cap;
                 d.bucket = o.bucket.as<uint32_t>();
+                d.bucket_w = o.bucket_w.p;
                 // targets delivered by the post update need no touched list
                 d.touched = o.in_update ? nullptr : o.touched.as<int32_t>();
                 d.scnt = o.scnt.as<int32_t>();

@@ -22,33 +22,28 @@
 extern "C" {
 #endif
 
-#define AHIP_MODEL_ABI 4
+#define AHIP_MODEL_ABI 6
 #define AHIP_MAX_SLOTS 48
 #define AHIP_MAX_RAND 8
 #define AHIP_UPDATE_BLOCK 256 /* threads per block of every update kernel */
 
-/* One projection feeding an ordered conductance: its event keys are
- * [gid_off, gid_end); key k names column-order synapse k - gid_off. */
-typedef struct ahip_deliver_src {
-    const void* w_col;                  /* weights in column order, or NULL: w_const */
-    double w_const;
-    uint32_t gid_off, gid_end;
-} ahip_deliver_src;
-
-/* Ordered spike delivery folded into the update of the post population: the
- * events of neuron i this step are keys bucket[boff[i] .. boff[i] + cnt[i]),
- * added to g[i] in ascending key order (projection by projection, ascending
- * pre-synaptic rank: the reference's summation order) before the equations
- * read g; cnt[i] is cleared. */
+/* Ordered spike delivery folded into the update of the post population.
+ * head[i] = (base << AHIP_HEAD_SHIFT) | count: neuron i's events of this
+ * step are bucket entries base .. base + count (head[i + 1] >> SHIFT bounds
+ * the range), each a key (the projection's key offset + the synapse's
+ * column-order index, so keys sort projection by projection and in
+ * ascending pre-synaptic rank: the reference's summation order) and its
+ * weight (bucket_w, in the network precision).  The weights are added to
+ * g[i] in ascending key before the equations read g; the count is cleared. */
 typedef struct ahip_deliver_desc {
     void* g;
-    const int64_t* boff;
-    int32_t* cnt;
+    uint64_t* head;                     /* size + 1 words */
     const uint32_t* bucket;
-    const ahip_deliver_src* src;
-    int32_t n_src;
-    int32_t reserved;
+    const void* bucket_w;
 } ahip_deliver_desc;
+#define AHIP_HEAD_SHIFT 24
+#define AHIP_HEAD_COUNT ((1ull << AHIP_HEAD_SHIFT) - 1ull)
+#define AHIP_MAX_DELIVER 4 /* ordered conductances one update delivers */
 
 /* Arguments of one population update launch.  Device pointers unless noted. */
 typedef struct ahip_pop_args {

@@ -208,3 +208,37 @@ def test_prepared_graph_survives_eager_steps(gpu):
     np.testing.assert_array_equal(idx, ref["idx"])
     assert np.array_equal(P.v.reshape(-1), ref["v"])
     assert np.array_equal(P.g_exc.reshape(-1), ref["g_exc"])
+
+
+@pytest.mark.parametrize("n_in", [3, 4, 5, 64])
+def test_buckets_delivered_by_the_update(gpu, n_in):
+    # no monitor reads g_exc after compute_psp, so the post population's
+    # update adds the buckets: n_in inputs fire together at step 0 into 5
+    # neurons (buckets of n_in + min(n_in, 40) events in step 1, in-register
+    # sort up to 4, repeated minimum beyond); v, g_exc bit-exact after 3 steps
+    Np, steps = 5, 3
+    rng = np.random.default_rng(7 + n_in)
+    net = ah.Network(dt=0.1, seed=2)
+    P = net.create(Np, ah.COBA)
+    Q = net.create(n_in, models.poisson_neuron(5.0))
+    w = 10.0 ** rng.uniform(-4, 2, n_in * Np)
+    k2 = min(n_in, 40)
+    csr1 = cn.from_arrays(Np, n_in, np.arange(Np + 1, dtype=np.int64) * n_in,
+                          np.tile(np.arange(n_in, dtype=np.int32), Np), w)
+    csr2 = cn.from_arrays(Np, n_in, np.arange(Np + 1, dtype=np.int64) * k2,
+                          np.tile(np.arange(k2, dtype=np.int32), Np), None, w_const=1e-3)
+    p1 = net.connect(Q, P, "exc").from_csr(csr1)
+    p2 = net.connect(Q, P, "exc").from_csr(csr2)
+    net.compile(graph_batch=2)
+    assert net.ordered_targets() == 1
+    P.Vt = 1e9
+    P.v = -60.0
+    unif = np.ones((steps, n_in))
+    unif[0, :] = 0.0
+    Q.inject_random(0, unif)
+    net.run(steps)
+    po = [orc.csrc_projection(p1.csr, 0, pre_is_input=True), orc.csrc_projection(p2.csr, 0, pre_is_input=True)]
+    ref = orc.spiking_run(0, Np, 0.1, np.full(Np, -60.0), steps, po, params=dict(Vt=1e9),
+                          poisson=dict(M=n_in, rates=5.0, uniforms=unif))
+    assert np.array_equal(P.g_exc.reshape(-1), ref["g_exc"])
+    assert np.array_equal(P.v.reshape(-1), ref["v"])
