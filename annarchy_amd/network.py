@@ -424,6 +424,8 @@ class Projection:
         self._generator = "numpy"
         self._delays = 0.0
         self._format = _lib.opts("csr", 64)
+        self._auto_format = True
+        self._selected = None
         self._syn_init = {}
         self._syn_attrs = None
         self.csr: conn.CSR | None = None
@@ -577,10 +579,39 @@ class Projection:
         # force_multiple_weights: one weight per synapse even when all are equal
         self._force_multi = bool(fmt.pop("force_multiple_weights", False))
         fmt.pop("storage_order", None)   # post_to_pre is the only order here
-        if storage_format is not None or fmt:
-            self._format = _lib.opts(storage_format or "csr", fmt.get("lanes", 0),
-                                     fmt.get("chunk", 0), fmt.get("sigma", 1), fmt.get("panel_cols", 0))
+        # no format given, or "auto": chosen when the connectivity is known
+        # (core/Projection.py:691-789; _select_format)
+        self._auto_format = storage_format in (None, "auto") and not fmt
+        if not self._auto_format:
+            self._format = _lib.opts(storage_format if storage_format not in (None, "auto") else "csr",
+                                     fmt.get("lanes", 0), fmt.get("chunk", 0), fmt.get("sigma", 1),
+                                     fmt.get("panel_cols", 0))
         return self
+
+    def _select_format(self):
+        """storage_format "auto" (the default): the reference's
+        _automatic_format_selection (core/Projection.py:691-789) with its
+        decision tree for rate-coded random connectivity replaced by the
+        learned selector.  All-to-all -> dense, one-to-one -> csr as there;
+        spiking projections use the CSRC view whatever the format; plastic
+        projections and per-synapse delays keep csr; every other rate-coded
+        pattern gets the configuration the trained MLP predicts from its
+        connectivity statistics (selector.select, mode "mlp")."""
+        if not getattr(self, "_auto_format", True) or self.csr is None:
+            return None
+        kind = self._connector[0] if self._connector else None
+        if _base(self.pre).spiking or self.synapse is not None:
+            return None
+        if not np.isscalar(self._delays):
+            return None
+        if kind == "all_to_all":
+            return dict(fmt="dense")
+        if kind == "one_to_one" or self.csr.nnz == 0:
+            return None
+        if self.csr.nnz >= 0.6 * float(self.csr.n_rows) * self.csr.n_cols:
+            return dict(fmt="dense")
+        from . import selector
+        return selector.select(self.csr, mode="mlp")
 
     def fixed_probability(self, probability, weights, delays=0.0, allow_self_connections=False,
                           storage_format=None, generator="numpy", **fmt):
@@ -742,10 +773,21 @@ class Projection:
             _lib.check(L.ahip_proj_set_synapse(self.net._h, self.id, str(lib_path).encode()))
         if self._generator == "device":
             self._generate_on_device()
+            self._selected = self._select_format()
+            if self._selected is not None:
+                sel = self._selected
+                self._format = _lib.opts(sel["fmt"], sel.get("lanes", 0), sel.get("chunk", 0), sel.get("sigma", 1),
+                                         sel.get("panel_cols", 0))
+                _lib.check(L.ahip_proj_set_format(self.net._h, self.id, C.byref(self._format)))
         else:
             self.csr = self._build_csr()
             if getattr(self, "_force_multi", False) and self.csr.w is None:
                 self.csr.w = np.full(self.csr.nnz, float(self.csr.w_const))
+            self._selected = self._select_format()
+            if self._selected is not None:
+                sel = self._selected
+                self._format = _lib.opts(sel["fmt"], sel.get("lanes", 0), sel.get("chunk", 0), sel.get("sigma", 1),
+                                         sel.get("panel_cols", 0))
             c = self.csr
             w = c.w
             _lib.check(L.ahip_proj_init_csr(self.net._h, self.id, c.n_rows, c.n_cols,
@@ -828,8 +870,14 @@ class Projection:
 
     def set_format(self, fmt="csr", lanes=0, chunk=0, sigma=1, panel_cols=0):
         self._format = _lib.opts(fmt, lanes, chunk, sigma, panel_cols)
+        self._auto_format = False
         if self.net.compiled:
             _lib.check(_lib.lib().ahip_proj_set_format(self.net._h, self.id, C.byref(self._format)))
+
+    @property
+    def selected_format(self) -> dict | None:
+        """The configuration storage_format "auto" chose (None: csr kept)."""
+        return self._selected
 
     def get_format(self) -> dict:
         o = _lib.FormatOpts()

@@ -37,7 +37,9 @@ CLASSES = [
     dict(fmt="sell", lanes=16, sigma=1024),
     dict(fmt="ellr"),
     dict(fmt="bsr", chunk=4),
-    dict(fmt="bsr", chunk=8),
+    # sliced tiles sized for 1e7-1e8-synapse matrices: 2048-row tiles,
+    # 20479-column panels (160 KiB of LDS in FP64), 8 waves per workgroup
+    dict(fmt="sell_panel", lanes=2, sigma=8, chunk=2048, panel_cols=20479),
 ]
 HEURISTIC_CSR = 5    # csr, 64 lanes per row
 HEURISTIC_SELL = 6   # sell, thread per row, no sorting
@@ -130,7 +132,7 @@ def feasible(csr: conn.CSR, cls: int, max_blowup: float = 8.0) -> bool:
     if cfg["fmt"] == "sell_panel":
         # one partial sum per (panel, row): worth it only with several
         # synapses per row and panel; bounds the host-side tile tables too
-        panels = -(-csr.n_cols // 4096)
+        panels = -(-csr.n_cols // cfg["panel_cols"])
         cells = panels * csr.n_rows
         return cells <= 3e7 and nnz >= 4 * cells
     return True
@@ -140,6 +142,8 @@ def feasible(csr: conn.CSR, cls: int, max_blowup: float = 8.0) -> bool:
 # Corpus of connectivity matrices
 # ---------------------------------------------------------------------------
 KINDS = ["fixed", "binomial", "lognormal", "powerlaw", "banded", "block"]
+# the large set: the in-degree laws of the 1e8-synapse rate network
+LARGE_KINDS = ["fixed", "lognormal"]
 
 
 def _sorted_unique_rows(n, rows, cols):
@@ -152,8 +156,31 @@ def _sorted_unique_rows(n, rows, cols):
     return row_ptr, cols
 
 
+def _generate_large(spec: dict) -> conn.CSR:
+    """Seeded square connectivity of 1e7-1e8 synapses: per row `deg` uniform
+    columns (with replacement; duplicates are rare at these densities),
+    sorted within the row; weights N(0, 1/sqrt(k))."""
+    rng = np.random.default_rng(spec["seed"])
+    n, k = int(spec["n"]), float(spec["k"])
+    if spec["kind"] == "fixed":
+        deg = np.full(n, int(round(k)), dtype=np.int64)
+    else:
+        deg = np.clip(np.round(rng.lognormal(math.log(k) - 0.5, 1.0, n)), 1, n).astype(np.int64)
+    row_ptr = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(deg, out=row_ptr[1:])
+    keys = np.repeat(np.arange(n, dtype=np.int64) * n, deg)
+    keys += rng.integers(0, n, keys.size)
+    keys.sort()
+    cols = (keys % n).astype(np.int32)
+    del keys
+    w = rng.normal(0.0, 1.0 / math.sqrt(max(k, 1.0)), cols.size)
+    return conn.CSR(n, n, row_ptr, cols, w)
+
+
 def generate(spec: dict) -> conn.CSR:
     """Seeded square connectivity of one corpus entry."""
+    if spec.get("large"):
+        return _generate_large(spec)
     rng = np.random.default_rng(spec["seed"])
     n, k, kind = int(spec["n"]), float(spec["k"]), spec["kind"]
     if kind == "block":
@@ -209,6 +236,20 @@ def corpus_specs(count: int = 2000, seed: int = 2024, nnz_max: float = 2e6):
         if kind == "banded":
             spec["spread"] = float(rng.uniform(0.6, 4.0))
         specs.append(spec)
+    return specs
+
+
+def large_specs(count: int = 40, seed: int = 4048):
+    """Matrices at the scale of the rate benchmark: N log-uniform in
+    [1e5, 1e6], mean in-degree log-uniform in [100, 1000], nnz = N k kept in
+    [1e7, 1e8], in-degree fixed or lognormal (sigma 1)."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(count):
+        n = int(round(10 ** rng.uniform(5.0, 6.0)))
+        k = 10 ** rng.uniform(2.0, 3.0)
+        k = float(np.clip(k, 1e7 / n, 1e8 / n))
+        specs.append(dict(n=n, k=k, kind=LARGE_KINDS[i % 2], seed=int(rng.integers(1 << 31)), large=True))
     return specs
 
 
@@ -329,6 +370,9 @@ class MLPSelector:
         return cls(d["params"], d["mean"], d["std"])
 
 
+_DEFAULT: MLPSelector | None = None   # the committed network, loaded once
+
+
 def select(csr: conn.CSR, mode: str = "mlp", selector: MLPSelector | None = None) -> dict:
     """Configuration for `csr`: mode "mlp" (trained network), "exhaustive"
     (autotuner) or "heuristic" (the reference's decision tree)."""
@@ -337,8 +381,12 @@ def select(csr: conn.CSR, mode: str = "mlp", selector: MLPSelector | None = None
         return dict(fmt="dense") if c == "dense" else CLASSES[c]
     if mode == "exhaustive":
         return CLASSES[int(np.argmin(autotune(csr)))]
-    sel = selector or MLPSelector.load()
-    c = int(sel.predict(features(csr))[0])
+    global _DEFAULT
+    if selector is None:
+        if _DEFAULT is None:
+            _DEFAULT = MLPSelector.load()
+        selector = _DEFAULT
+    c = int(selector.predict(features(csr))[0])
     if not feasible(csr, c):
         c = HEURISTIC_CSR
     return CLASSES[c]
@@ -355,16 +403,17 @@ def _gen_task(spec):
 
 
 def build_corpus(count: int, out_path, workers: int = 8, seed: int = 2024, log=print, tuner=None,
-                 start: int = 0, stop: int | None = None, time_limit: float | None = None):
-    """Autotune entries [start, stop) of the `count`-matrix corpus; writes
-    specs, features and times to `out_path` (npz) every 50 matrices and at
-    the end.  With `time_limit` (seconds) it stops early after the matrix
-    that crosses it; the file then holds the prefix done so far."""
+                 start: int = 0, stop: int | None = None, time_limit: float | None = None, specs=None):
+    """Autotune entries [start, stop) of the `count`-matrix corpus (or of
+    `specs`); writes specs, features and times to `out_path` (npz) every 50
+    matrices and at the end.  With `time_limit` (seconds) it stops early
+    after the matrix that crosses it; the file then holds the prefix done
+    so far."""
     import json
     import time
     from concurrent.futures import ThreadPoolExecutor
 
-    specs = corpus_specs(count, seed)[start:stop]
+    specs = (corpus_specs(count, seed) if specs is None else specs)[start:stop]
     feats, times, kept = [], [], []
     t0 = time.perf_counter()
 
@@ -385,7 +434,7 @@ def build_corpus(count: int, out_path, workers: int = 8, seed: int = 2024, log=p
         times.append(t)
         kept.append(spec)
         elapsed = time.perf_counter() - t0
-        if (i + 1) % 50 == 0 or i + 1 == len(specs):
+        if (i + 1) % 50 == 0 or i + 1 == len(specs) or spec.get("large"):
             save()
             log(f"corpus {start + i + 1}/{start + len(specs)}: {elapsed:.0f} s")
         if time_limit is not None and elapsed > time_limit:
@@ -395,14 +444,21 @@ def build_corpus(count: int, out_path, workers: int = 8, seed: int = 2024, log=p
     return np.array(feats), np.array(times), kept
 
 
-def holdout_split(T, heuristic, seed: int = 0):
+def holdout_split(T, heuristic, seed: int = 0, groups=None):
     """Indices (train, test) of the 80/20 split over the corpus entries that
-    have a finite timing and a sparse heuristic choice."""
+    have a finite timing and a sparse heuristic choice, split within each
+    group (small corpus, large set)."""
     T, heuristic = np.asarray(T), np.asarray(heuristic)
-    ok = np.flatnonzero(np.isfinite(T).any(axis=1) & (heuristic >= 0))
-    perm = ok[np.random.default_rng(seed).permutation(len(ok))]
-    ntr = int(0.8 * len(ok))
-    return perm[:ntr], perm[ntr:]
+    groups = np.zeros(len(T), dtype=int) if groups is None else np.asarray(groups)
+    ok = np.isfinite(T).any(axis=1) & (heuristic >= 0)
+    tr, te = [], []
+    for g in np.unique(groups[ok]):
+        idx = np.flatnonzero(ok & (groups == g))
+        perm = idx[np.random.default_rng(seed).permutation(len(idx))]
+        ntr = int(0.8 * len(idx))
+        tr.append(perm[:ntr])
+        te.append(perm[ntr:])
+    return np.concatenate(tr), np.concatenate(te)
 
 
 def with_fallback(T, choice):
@@ -413,32 +469,107 @@ def with_fallback(T, choice):
     return np.where(ok, choice, HEURISTIC_CSR)
 
 
-def evaluate(X, T, heuristic, save_path=None, seeds=(0, 1, 2), epochs=3000, log=print):
-    """80/20 split: regret of the MLP (per seed), of the reference heuristic and
-    of the best single configuration on the held-out matrices."""
-    X, T, heuristic = np.asarray(X), np.asarray(T), np.asarray(heuristic)
-    tr, te = holdout_split(T, heuristic)
-    Tt = T[te]
+def _regret_stats(Tt, choice):
+    rows = np.arange(len(Tt))
     best = Tt.min(axis=1)
-    rows = np.arange(len(te))
+    reg = Tt[rows, choice] / best - 1.0
+    return dict(n=int(len(Tt)), mean_regret=float(np.mean(reg)), median=float(np.median(reg)),
+                p90=float(np.quantile(reg, 0.9)), total_time_ratio=float(Tt[rows, choice].sum() / best.sum()))
+
+
+def evaluate(X, T, heuristic, save_path=None, seeds=(0, 1, 2), epochs=3000, log=print, groups=None,
+             group_names=("small", "large")):
+    """80/20 split (within each group): regret of the MLP (per seed), of the
+    reference heuristic and of the best single configuration on the
+    held-out matrices, over all of them and per group."""
+    X, T, heuristic = np.asarray(X), np.asarray(T), np.asarray(heuristic)
+    groups = np.zeros(len(T), dtype=int) if groups is None else np.asarray(groups)
+    tr, te = holdout_split(T, heuristic, groups=groups)
+    Tt = T[te]
     res = {"n_train": int(len(tr)), "n_test": int(len(te))}
-    res["heuristic"] = dict(mean_regret=float(np.mean(Tt[rows, heuristic[te]] / best - 1)),
-                            total_time_ratio=float(Tt[rows, heuristic[te]].sum() / best.sum()))
+    h = with_fallback(Tt, heuristic[te])
+    res["heuristic"] = _regret_stats(Tt, h)
     rel = np.where(np.isfinite(T[tr]), T[tr] / T[tr].min(axis=1, keepdims=True), np.nan)
     single = int(np.nanargmin(np.nanmean(rel, axis=0)))
-    res["best_single_class"] = dict(cls=single, mean_regret=float(np.mean(Tt[:, single] / best - 1)))
+    res["best_single_class"] = dict(cls=single, **_regret_stats(Tt, with_fallback(Tt, np.full(len(te), single))))
+    for gi in np.unique(groups[te]):
+        m = groups[te] == gi
+        res[f"heuristic_{group_names[gi]}"] = _regret_stats(Tt[m], h[m])
     for seed in seeds:
         sel = MLPSelector()
         curve = sel.fit(X[tr], T[tr], epochs=epochs, lr=3e-3, seed=seed)
         pr = with_fallback(Tt, sel.predict(X[te]))
-        reg = regret(Tt, pr)
-        res[f"mlp_seed{seed}"] = dict(mean_regret=float(np.mean(reg)), median=float(np.median(reg)),
-                                      p90=float(np.quantile(reg, 0.9)), loss0=float(curve[0]),
-                                      loss=float(curve[-1]),
-                                      total_time_ratio=float(Tt[rows, pr].sum() / best.sum()),
-                                      train_regret=float(np.mean(regret(T[tr], with_fallback(T[tr], sel.predict(X[tr]))))))
+        r = _regret_stats(Tt, pr)
+        r.update(loss0=float(curve[0]), loss=float(curve[-1]),
+                 train_regret=float(np.mean(regret(T[tr], with_fallback(T[tr], sel.predict(X[tr]))))))
+        for gi in np.unique(groups[te]):
+            m = groups[te] == gi
+            r[group_names[gi]] = _regret_stats(Tt[m], pr[m])
+        res[f"mlp_seed{seed}"] = r
         if seed == seeds[0] and save_path is not None:
             sel.save(save_path)
-        log(f"seed {seed}: {res[f'mlp_seed{seed}']}")
+        log(f"seed {seed}: {r}")
     res["label_histogram"] = np.bincount(np.argmin(T, axis=1), minlength=len(CLASSES)).tolist()
     return res
+
+
+# ---------------------------------------------------------------------------
+# Command line: tune slices of the corpus on the device, merge, evaluate
+# ---------------------------------------------------------------------------
+def _load_parts(paths):
+    import json
+
+    F, T, specs = [], [], []
+    for p in paths:
+        d = np.load(p)
+        F.append(d["features"])
+        T.append(d["times"])
+        specs += [json.loads(x) for x in d["specs"]]
+    return np.concatenate(F), np.concatenate(T), specs
+
+
+def main(argv=None):
+    """python -m annarchy_amd.selector tune --set small|large --start A --stop B --out F.npz [--time-limit S]
+    python -m annarchy_amd.selector evaluate --parts F1.npz F2.npz ... [--out-dir DIR]
+
+    tune: exhaustive timings of a slice of the 2000-matrix corpus ("small",
+    N 1e3..1e6, nnz <= 2e6) or of the 40-matrix set at the rate benchmark's
+    scale ("large", nnz 1e7..1e8).  evaluate: merges the parts, trains the
+    MLP on 80% of each set on the device and writes the corpus, the trained
+    network and the held-out regrets (selector_corpus.npz, selector_mlp.npz,
+    selector_results.json) into DIR (default: the package data folder)."""
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(prog="python -m annarchy_amd.selector")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("tune")
+    t.add_argument("--set", choices=["small", "large"], default="small")
+    t.add_argument("--start", type=int, default=0)
+    t.add_argument("--stop", type=int, default=None)
+    t.add_argument("--out", required=True)
+    t.add_argument("--time-limit", type=float, default=None)
+    t.add_argument("--workers", type=int, default=8)
+    e = sub.add_parser("evaluate")
+    e.add_argument("--parts", nargs="+", required=True)
+    e.add_argument("--out-dir", default=str(DATA_DIR))
+    e.add_argument("--epochs", type=int, default=3000)
+    a = ap.parse_args(argv)
+    if a.cmd == "tune":
+        specs = corpus_specs(2000) if a.set == "small" else large_specs()
+        build_corpus(len(specs), a.out, workers=a.workers, start=a.start, stop=a.stop, time_limit=a.time_limit,
+                     specs=specs, log=lambda m: print(m, flush=True))
+        return
+    F, T, specs = _load_parts(a.parts)
+    heur = np.array([s["heuristic"] for s in specs])
+    groups = np.array([1 if s.get("large") else 0 for s in specs])
+    out = Path(a.out_dir)
+    out.mkdir(parents=True, exist_ok=True)
+    np.savez(out / "selector_corpus.npz", features=F, times=T, specs=np.array([json.dumps(s) for s in specs]))
+    res = evaluate(F, T, heur, save_path=out / "selector_mlp.npz", epochs=a.epochs, groups=groups,
+                   log=lambda m: print(m, flush=True))
+    (out / "selector_results.json").write_text(json.dumps(res, indent=1))
+
+
+if __name__ == "__main__":
+    main()

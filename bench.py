@@ -51,7 +51,7 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 SUB_CONFIGS = ("c0", "c2", "c3", "c4")
-CHILD_TIMEOUT_S = {"c1": 240, "c0": 120, "c2": 240, "c3": 300, "c4": 180}
+CHILD_TIMEOUT_S = {"c1": 240, "c0": 120, "c2": 240, "c3": 300, "c4": 300}
 
 
 def log(msg):
@@ -551,10 +551,12 @@ def bench_c3(args):
 # ---------------------------------------------------------------------------
 def bench_c4(args):
     """Held-out regret of the trained selector against exhaustive tuning,
-    timed live: a sample of held-out corpus matrices is regenerated from its
-    seeds, every feasible configuration is timed on the device (the
-    exhaustive autotuner), and the MLP's and the reference heuristic's picks
-    are compared with the best."""
+    timed live: held-out matrices of both corpus sets (N 1e3..1e6 with
+    nnz <= 2e6, and the set at the rate benchmark's scale, nnz 1e7..1e8) are
+    regenerated from their seeds, every feasible configuration is timed on
+    the device (the exhaustive autotuner), and the MLP's and the reference
+    heuristic's picks are compared with the best; then the same for the
+    1e8-synapse rate network itself (in-degree fixed 1000)."""
     import json as js
 
     from annarchy_amd import selector as S
@@ -563,65 +565,88 @@ def bench_c4(args):
     d = np.load(S.DATA_DIR / "selector_corpus.npz")
     specs = [js.loads(s) for s in d["specs"]]
     heur = np.array([s["heuristic"] for s in specs])
-    _, te = S.holdout_split(d["times"], heur)
-    pick = te[:: max(1, len(te) // args.c4_sample)][: args.c4_sample]
+    groups = np.array([1 if s.get("large") else 0 for s in specs])
+    _, te = S.holdout_split(d["times"], heur, groups=groups)
+    small = [i for i in te if groups[i] == 0]
+    large = [i for i in te if groups[i] == 1]
+    pick = small[:: max(1, len(small) // args.c4_sample)][: args.c4_sample] + large[: args.c4_large]
     sel = S.MLPSelector.load()
-    T, choice, hc, nnz = [], [], [], []
+    T, choice, hc, nnz, grp = [], [], [], [], []
     t0 = time.perf_counter()
     for i in pick:
         csr = S.generate(specs[i])
         f = S.features(csr, specs[i]["seed"] & 0xFFFF)
-        c = int(sel.predict(f)[0])
-        t = S.autotune(csr)
-        T.append(t)
-        choice.append(c)
+        choice.append(int(sel.predict(f)[0]))
+        T.append(S.autotune(csr))
         hc.append(int(heur[i]))
         nnz.append(csr.nnz)
+        grp.append(int(groups[i]))
+        del csr
     el = time.perf_counter() - t0
-    T = np.array(T)
+    T, grp = np.array(T), np.array(grp)
     choice = S.with_fallback(T, np.array(choice))
+    hcs = S.with_fallback(T, np.array(hc))
     reg = S.regret(T, choice)
-    reg_h = S.regret(T, S.with_fallback(T, np.array(hc)))
-    # roofline: the selected configuration's psp on the largest matrix of the sample
-    k = int(np.argmax(nnz))
-    csr = S.generate(specs[pick[k]])
+    reg_h = S.regret(T, hcs)
+    # the rate network of the benchmark (1e8 synapses, in-degree fixed 1000)
+    c1 = build_c1(100_000, 1000, 1234)
+    c1_pick = S.select(c1, mode="mlp")
+    c1_cls = S.CLASSES.index(c1_pick)
+    t_c1 = S.autotune(c1)
+    c1_h = S.heuristic_class(c1)
+    c1_rec = {"workload": "rate network N=100000, in-degree fixed 1000, 1e8 synapses",
+              "mlp_choice": c1_pick, "exhaustive_best": S.CLASSES[int(np.argmin(t_c1))],
+              "mlp_regret": float(t_c1[c1_cls] / t_c1.min() - 1.0),
+              "heuristic_choice": S.CLASSES[c1_h], "heuristic_regret": float(t_c1[c1_h] / t_c1.min() - 1.0),
+              "times_ms": [None if not np.isfinite(x) else float(x) for x in t_c1]}
+    log(f"c4: C1 matrix: MLP {c1_pick} regret {c1_rec['mlp_regret']:.3f}, heuristic {c1_rec['heuristic_regret']:.3f}")
+    # roofline: the selected configuration's psp on the C1 matrix
     from annarchy_amd.models import RateNoisy
     from annarchy_amd.network import Network
     net = Network(dt=1.0, seed=0)
-    pop = net.create(csr.n_rows, RateNoisy)
+    pop = net.create(c1.n_rows, RateNoisy)
     proj = net.connect(pop, pop, "exc")
-    proj.from_csr(csr, storage_format="csr", lanes=64)
+    proj.from_csr(c1, storage_format="csr", lanes=64)
     net.compile(allow_build=False)
-    proj.set_format(**S.CLASSES[int(choice[k])])
+    proj.set_format(**c1_pick)
     ms = proj.time_psp(50)
     pb = proj.psp_bytes()
     ach = pb / (ms * 1e-3) / 1e9
     roof = roofline("hbm", ach, HBM_PEAK_GBS, "GB/s", None,
-                    f"psp of the selected configuration {S.CLASSES[int(choice[k])]}",
-                    kernel_ms=ms, algorithmic_bytes=pb, matrix_nnz=int(csr.nnz))
+                    f"psp of the selected configuration {c1_pick} on the 1e8-synapse rate network",
+                    kernel_ms=ms, algorithmic_bytes=pb, matrix_nnz=int(c1.nnz))
+    net.destroy()
     cpu = None
     if not args.no_cpu:
-        n = csr.n_rows
+        # the selector's own work on the host: the 16 features of the 1e8-synapse matrix
+        t1 = time.perf_counter()
+        S.features(c1)
+        tf = time.perf_counter() - t1
+        n = c1.n_rows
         r = np.random.default_rng(1).random(n)
         noise = np.zeros((1, n))
         t1, steps = time.perf_counter(), 0
         while time.perf_counter() - t1 < 2.0:
-            orc.rate_run(csr.row_ptr, csr.col_idx, csr.w, r, 10.0, 1.0, noise, 1, nthreads=1)
+            orc.rate_run(c1.row_ptr, c1.col_idx, c1.w, r, 10.0, 1.0, noise, 1, nthreads=1)
             steps += 1
         cel = time.perf_counter() - t1
-        cpu = {"value": csr.nnz * steps / cel, "unit": "synaptic events/s", "cores": 1, "kind": "port",
-               "sample": f"{steps} oracle CSR steps of the largest sampled matrix ({csr.nnz} synapses), single thread",
+        cpu = {"value": c1.nnz * steps / cel, "unit": "synaptic events/s", "cores": 1, "kind": "port",
+               "sample": f"{steps} oracle CSR steps of the 1e8-synapse matrix, single thread (the reference's "
+                         f"format for it, csr); feature extraction for the selector {tf:.2f} s",
                "host": host_cpu()}
-    net.destroy()
+    def stats(m):
+        return {"n": int(m.sum()), "mlp_mean_regret": float(reg[m].mean()) if m.any() else None,
+                "heuristic_mean_regret": float(reg_h[m].mean()) if m.any() else None}
     log(f"c4: {len(pick)} held-out matrices tuned in {el:.1f} s: MLP regret {reg.mean():.4f}, "
         f"heuristic {reg_h.mean():.3f}")
     return record("held-out regret vs exhaustive tuning", float(reg.mean()), "fraction", len(pick), 0,
                   el * 1e3 / len(pick), "f32",
-                  {"workload": f"learned selector, {len(pick)} held-out matrices of the 2000-matrix corpus "
-                               "(N 1e3..1e6), MLP 16-64-64-12, 12 configurations timed live"},
+                  {"workload": f"learned selector, {len(pick)} held-out matrices of the corpus (2000 matrices with "
+                               "N 1e3..1e6, 40 with nnz 1e7..1e8), MLP 16-64-64-12, 12 configurations timed live"},
                   roof, cpu, higher=False, heuristic_regret=float(reg_h.mean()),
                   regret_median=float(np.median(reg)), regret_p90=float(np.quantile(reg, 0.9)),
-                  total_time_ratio=float(T[np.arange(len(T)), choice].sum() / T.min(axis=1).sum()))
+                  total_time_ratio=float(T[np.arange(len(T)), choice].sum() / T.min(axis=1).sum()),
+                  small=stats(grp == 0), large=stats(grp == 1), c1_matrix=c1_rec)
 
 
 # ---------------------------------------------------------------------------
@@ -677,7 +702,8 @@ def main():
     ap.add_argument("--c3-steps", type=int, default=2000)
     ap.add_argument("--no-plain", action="store_true",
                     help="c3: skip the plain per-instance FP64 networks (and the error and speedup they give)")
-    ap.add_argument("--c4-sample", type=int, default=24)
+    ap.add_argument("--c4-sample", type=int, default=20)
+    ap.add_argument("--c4-large", type=int, default=2)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -102,22 +102,40 @@ def test_corpus_features_reproduce():
 
 
 def test_trained_selector_gain_over_heuristic():
-    # held-out 20% of the 2000-matrix corpus; the committed network's choices
+    # held-out 20% of the 2000-matrix corpus and of the 40 matrices at the
+    # rate benchmark's scale (nnz 1e7..1e8); the committed network's choices
     # are recomputed with the FP64 oracle forward pass
     X, T, specs, h = _corpus()
+    groups = np.array([1 if s.get("large") else 0 for s in specs])
+    assert (groups == 0).sum() == 2000 and (groups == 1).sum() == 40
     m = np.load(S.DATA_DIR / "selector_mlp.npz")
-    tr, te = S.holdout_split(T, h)
-    assert len(te) == 400
+    tr, te = S.holdout_split(T, h, groups=groups)
+    assert len(te) == 408 and (groups[te] == 1).sum() == 8
     Xn = ((X[te] - m["mean"]) / m["std"]).astype(np.float32).astype(np.float64)
     choice = S.with_fallback(T[te], np.argmax(orc.mlp_forward(Xn, m["params"].astype(np.float64)), axis=1))
     rows = np.arange(len(te))
     best = T[te].min(axis=1)
     mlp = T[te][rows, choice] / best - 1.0
-    heur = T[te][rows, h[te]] / best - 1.0
-    # within 10% of exhaustive tuning, and far better than the decision tree
+    heur = T[te][rows, S.with_fallback(T[te], h[te])] / best - 1.0
+    # within 10% of exhaustive tuning, and far better than the decision tree,
+    # on both sets
     assert np.mean(mlp) < 0.10
     assert np.mean(mlp) < 0.1 * np.mean(heur)
     assert T[te][rows, choice].sum() < 0.5 * T[te][rows, h[te]].sum()
+    big = groups[te] == 1
+    assert np.mean(mlp[big]) < 0.10
+    assert np.mean(mlp[big]) < 0.25 * np.mean(heur[big])
     # the committed evaluation record agrees
     res = json.loads((S.DATA_DIR / "selector_results.json").read_text())
     assert abs(res["mlp_seed0"]["mean_regret"] - np.mean(mlp)) < 0.01
+    assert abs(res["mlp_seed0"]["large"]["mean_regret"] - np.mean(mlp[big])) < 0.01
+
+
+def test_large_set_specs():
+    # the matrices at the rate benchmark's scale: nnz 1e7..1e8, fixed and
+    # lognormal in-degree, N 1e5..1e6
+    sp = S.large_specs()
+    assert len(sp) == 40 and {s["kind"] for s in sp} == {"fixed", "lognormal"}
+    nnz = np.array([s["n"] * s["k"] for s in sp])
+    assert nnz.min() >= 1e7 * 0.999 and nnz.max() <= 1e8 * 1.001
+    assert all(1e5 <= s["n"] <= 1e6 for s in sp)
