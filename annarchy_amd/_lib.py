@@ -142,6 +142,7 @@ _SIGS = {
     "ahip_batch_set_mode": ([_P, _I32], _I32),
     "ahip_batch_get_mode": ([_P, _PI32], _I32),
     "ahip_mlp_train": ([_PF, _PF, _I32, _PF, _I32, C.c_float, C.c_float, _PF, _PF], _I32),
+    "ahip_mlp_train_noalloc": ([_P, _P, _I32, _P, _I32, C.c_float, C.c_float, _P, _P, _P], _I32),
     "ahip_mlp_predict": ([_PF, _I32, _PF, _PI32, _PF], _I32),
 }
 

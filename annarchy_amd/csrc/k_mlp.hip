@@ -13,6 +13,8 @@
 //   3. Adam update of the parameters.
 // Layout of the parameter vector: [W1 (h1 x d_in), b1, W2 (h2 x h1), b2,
 // W3 (d_out x h2), b3], row-major.
+#include <mutex>
+
 #include "ahip_device.hpp"
 #include "internal.hpp"
 
@@ -142,6 +144,43 @@ __global__ __launch_bounds__(64) void mlp_predict(int32_t n, const float* __rest
         for (int o = 0; o < DOUT; o++) logits[(int64_t)s * DOUT + o] = z[o];
 }
 
+// Sum of the per-sample losses of one epoch into loss_curve[e]: one
+// workgroup, FP64 partial sums in a fixed order (deterministic).
+__global__ __launch_bounds__(256) void mlp_loss_sum(int32_t n, const float* __restrict__ L,
+                                                    float* __restrict__ out) {
+    __shared__ double part[256];
+    double a = 0.0;
+    for (int32_t s = threadIdx.x; s < n; s += 256) a += (double)L[s];
+    part[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = (float)part[0];
+}
+
+__global__ __launch_bounds__(256) void mlp_zero(float* __restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+__global__ __launch_bounds__(256) void mlp_copy(float* __restrict__ dst, const float* __restrict__ src, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// ---- training without allocations -----------------------------------------
+// Scratch of ahip_mlp_train_noalloc: the gradient and the Adam moments
+// (3 x NPAR floats) and the per-sample activations, deltas and losses of up
+// to kNoallocSamples samples.  Each GPU holds its own copy of the array for
+// the life of the process; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device.
+constexpr int32_t kNoallocSamples = 2048;
+constexpr int64_t kPerSample = 2 * H1 + 2 * H2 + DOUT + 1;
+constexpr int64_t kScratchFloats = 3 * (int64_t)NPAR + kNoallocSamples * kPerSample;
+static __device__ __align__(256) float scratch[kScratchFloats];
+
 }  // namespace
 
 struct MlpWork {
@@ -224,9 +263,82 @@ int mlp_predict_run(int32_t n, const float* X, const float* params, int32_t* cls
     return 0;
 }
 
+// The device address of `scratch` on the current device, looked up once per
+// device (hipGetSymbolAddress with that device current).
+static float* noalloc_scratch(int* rc) {
+    static std::mutex mu;
+    static float* ptr[64] = {nullptr};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        *rc = AHIP_EHIP;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            *rc = AHIP_EHIP;
+            return nullptr;
+        }
+        ptr[dev] = static_cast<float*>(p);
+    }
+    *rc = 0;
+    return ptr[dev];
+}
+
+int mlp_run_noalloc(int32_t n, const float* X, const float* T, float* P, int32_t epochs, float lr, float wd,
+                    float* loss_curve, float* grad_out, hipStream_t st) {
+    int rc = 0;
+    float* base = noalloc_scratch(&rc);
+    if (!base) return rc;
+    float* G = base;
+    float* M = G + NPAR;
+    float* V = M + NPAR;
+    float* A1 = V + NPAR;
+    float* A2 = A1 + (int64_t)n * H1;
+    float* G1 = A2 + (int64_t)n * H2;
+    float* G2 = G1 + (int64_t)n * H1;
+    float* DZ = G2 + (int64_t)n * H2;
+    float* L = DZ + (int64_t)n * DOUT;
+    // the previous call's contents stay in the array: clear what starts at 0
+    // (the moments; every other region is written before it is read)
+    hipLaunchKernelGGL(mlp_zero, dim3((2 * NPAR + 255) / 256), dim3(256), 0, st, M, (int64_t)2 * NPAR);
+    const int nbs = (n + 63) / 64, nbp = (NPAR + 255) / 256;
+    const float b1 = 0.9f, b2 = 0.999f;
+    float c1 = 1.f, c2 = 1.f;
+    const int32_t iters = epochs > 0 ? epochs : 1;
+    for (int32_t e = 0; e < iters; e++) {
+        hipLaunchKernelGGL(mlp_deltas, dim3(nbs), dim3(64), 0, st, n, X, T, P, A1, A2, G1, G2, DZ, L);
+        hipLaunchKernelGGL(mlp_grads, dim3(nbp), dim3(256), 0, st, n, X, A1, A2, G1, G2, DZ, G);
+        if (loss_curve) hipLaunchKernelGGL(mlp_loss_sum, dim3(1), dim3(256), 0, st, n, L, loss_curve + e);
+        if (epochs <= 0) break;  // gradient only
+        c1 *= b1;
+        c2 *= b2;
+        hipLaunchKernelGGL(mlp_adam, dim3(nbp), dim3(256), 0, st, P, G, M, V, lr, b1, b2, 1e-8f, 1.f - c1,
+                           1.f - c2, wd);
+    }
+    if (grad_out) hipLaunchKernelGGL(mlp_copy, dim3(nbp), dim3(256), 0, st, grad_out, G, (int64_t)NPAR);
+    return hipGetLastError() == hipSuccess ? 0 : AHIP_EHIP;
+}
+
 }  // namespace ahip
 
 extern "C" {
+
+// Training as ahip_mlp_train with caller-owned device buffers and no device
+// allocation: the scratch is the library's static array (up to 2048
+// samples); kernels only, enqueued on `stream`.  Calls that overlap on one
+// GPU share that array and race.
+int ahip_mlp_train_noalloc(const float* X, const float* T, int32_t n, float* params, int32_t epochs, float lr,
+                           float weight_decay, float* loss_curve, float* grad_out, void* stream) {
+    if (!X || !T || !params || n <= 0 || epochs < 0) return AHIP_EINVAL;
+    if (n > ahip::kNoallocSamples)
+        return ahip::report_error(AHIP_EINVAL, "ahip_mlp_train_noalloc: more than 2048 samples (the size of "
+                                               "its static scratch); use ahip_mlp_train");
+    return ahip::mlp_run_noalloc(n, X, T, params, epochs, lr, weight_decay, loss_curve, grad_out,
+                                 (hipStream_t)stream);
+}
+
 
 // Full-batch Adam training (epochs > 0) or one gradient evaluation
 // (epochs == 0: grad_out receives dLoss/dparams, loss_curve[0] the loss).

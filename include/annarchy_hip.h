@@ -397,6 +397,19 @@ int ahip_mlp_train(const float* X, const float* T, int32_t n, float* params, int
 /* Class (argmax) and logits (optional, n x 12) of every sample. */
 int ahip_mlp_predict(const float* X, int32_t n, const float* params, int32_t* classes,
                      float* logits);
+/* ahip_mlp_train without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  X (n x 16), T
+ * (n x 12), params (6028, updated in place), loss_curve (optional, one
+ * float per epoch) and grad_out (optional, 6028) are DEVICE pointers owned
+ * by the caller; the kernels are enqueued on `stream` (a hipStream_t, NULL:
+ * the null stream) and the call returns without waiting.  The scratch
+ * (gradient, Adam moments, per-sample activations) is one static device
+ * array of the library, about 2.3 MB per GPU, held until the process ends:
+ * n <= 2048, larger n is rejected with AHIP_EINVAL.  Every call on one GPU
+ * uses that GPU's array, so calls that overlap on one GPU race. */
+int ahip_mlp_train_noalloc(const float* X, const float* T, int32_t n, float* params, int32_t epochs,
+                           float lr, float weight_decay, float* loss_curve, float* grad_out, void* stream);
 
 #ifdef __cplusplus
 }
