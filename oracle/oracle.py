@@ -7,13 +7,16 @@ CPU-baseline leg of bench.py, never by the annarchy_amd package.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB = HERE / "build" / "liboracle.so"
+# AHIP_ORACLE_LIB: another build of oracle.cpp (the sanitizer build of the
+# CPU test that reruns the oracle's tests under ASan and UBSan)
+LIB = Path(os.environ["AHIP_ORACLE_LIB"]) if os.environ.get("AHIP_ORACLE_LIB") else HERE / "build" / "liboracle.so"
 
 _I32, _I64, _D = C.c_int32, C.c_int64, C.c_double
 _PI32, _PI64, _PD = C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_double)
