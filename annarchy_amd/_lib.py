@@ -112,6 +112,8 @@ _SIGS = {
     "ahip_time_kernel": ([_P, _I32, _I32, _I32, _PD], _I32),
     "ahip_scatter_groups": ([_P, _PI32], _I32),
     "ahip_net_set_spike_delivery": ([_P, _I32], _I32),
+    "ahip_net_set_persistent": ([_P, _I32], _I32),
+    "ahip_net_uses_persistent": ([_P, _PI32], _I32),
     "ahip_ordered_targets": ([_P, _PI32], _I32),
     "ahip_proj_psp_bytes": ([_P, _I32, _PD], _I32),
     "ahip_proj_time_psp": ([_P, _I32, _I32, _PD], _I32),

@@ -28,7 +28,7 @@ import re
 
 from .equations import DISTRIBUTIONS, NeuronDescription, analyse_neuron, equation_code
 
-ABI = 6  # AHIP_MODEL_ABI in include/annarchy_hip_model.h
+ABI = 8  # AHIP_MODEL_ABI in include/annarchy_hip_model.h
 
 _PRELUDE = r"""// Generated neuron-model library for the MI355X simulation backend.
 // Model: %(name)s
@@ -84,6 +84,7 @@ __global__ __launch_bounds__(AHIP_UPDATE_BLOCK) void local_update(ahip_pop_args 
     if (a.n_deliver > 0 && i < a.size) ahip::deliver_buckets<T>(a, i);
     asm volatile("" ::: "memory");  // the equations below read the delivered values
 %(global_loads)s
+%(div_consts)s
 %(local_ptrs)s
     bool _spike = false;
     if (i < a.size) {
@@ -111,6 +112,8 @@ int launch_f32(const ahip_pop_args* a, void* s) { return launch<float>(a, s); }
 
 %(preloop_fn)s
 
+%(persist_fn)s
+
 const char* const kSlotNames[] = {%(slot_names)s};
 const int32_t kSlotLocal[] = {%(slot_local)s};
 const double kSlotInit[] = {%(slot_init)s};
@@ -121,11 +124,344 @@ const double kRandArgs[] = {%(rand_args)s};
 const ahip_model_desc kDesc = {
     %(abi)d, "%(name)s", %(spiking)d, %(n_slots)d, kSlotNames, kSlotLocal, kSlotInit, kSlotKind,
     %(n_rand)d, kRandDist, kRandArgs, %(has_refractory)d, %(has_global)d,
-    launch_f64, launch_f32, %(preloop_ptr)s, "%(hash)s"};
+    launch_f64, launch_f32, %(preloop_ptr)s, "%(hash)s", %(persist_ptrs)s};
 
 }  // namespace
 
 extern "C" const ahip_model_desc* ahip_model_describe(void) { return &kDesc; }
+"""
+
+
+# Persistent steps of a small network (ahip_persist_args): one workgroup of
+# 1024 threads runs `pa.steps` ordinary steps.  Thread k owns the neurons
+# [k*per, k*per + per), so one block scan of the per-thread spike counts puts
+# the spike list in ascending order.  The conductances the projections write
+# live in LDS for the whole launch (the statements reach them through the
+# same slot pointers), the spike list and the monitor's fill counts too, so
+# no thread reads another thread's global writes within the launch.
+_PERSIST_KERNEL = r"""
+constexpr int kPersistItems = AHIP_PERSIST_ITEMS;
+constexpr int kPersistWaves = AHIP_PERSIST_THREADS / 64;
+static_assert(kPersistWaves == 16, "the wave-count scan covers one DPP row");
+constexpr int kPersistGroup = 16;                                   // lanes per item, small scatters
+constexpr int kPersistGroups = AHIP_PERSIST_THREADS / kPersistGroup;
+constexpr int kPersistSlotCount = %(n_pslots)d;
+constexpr bool kPersistRefractory = %(p_refr)s;
+
+// Workgroup barrier ordering LDS only: within the launch no thread reads the
+// global memory another thread writes (the spike list, the monitor and
+// last_spike are only written), so the barriers need not wait for global
+// stores as __syncthreads() does.
+__device__ inline void persist_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// exclusive scan of one value per thread over the workgroup with one
+// barrier (wave scans, then each thread adds the totals of the waves before
+// it); total = sum.  `wtot` must not be rewritten before the next barrier.
+template <typename V>
+__device__ inline V persist_scan(V v, V* wtot, V& total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    V x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const V y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wtot[wid] = x;
+    persist_sync();
+    V before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kPersistWaves; w++) {
+        const V u = wtot[w];
+        before += w < wid ? u : V(0);
+        all += u;
+    }
+    total = all;
+    return before + x - v;
+}
+
+// inclusive sum over the lanes of each row of 16 (lane l: lanes 16 (l / 16)
+// .. l), row_shr DPP moves: lanes shifted in from outside the row add 0
+__device__ inline uint32_t persist_row_scan(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);
+    return x;
+}
+
+// the projection of item `pi` of round r (the pi-th projection in that round)
+__device__ inline int32_t persist_round_proj(const ahip_persist_proj* proj, int32_t n_proj, int32_t r, int32_t pi) {
+    int32_t c = 0;
+    for (int32_t p = 0; p < n_proj; p++)
+        if (proj[p].round == r) {
+            if (c == pi) return p;
+            c++;
+        }
+    return 0;
+}
+
+// loads through the connectivity pointers of the projection table: the
+// table sits in LDS, so the compiler cannot tell that they address global
+// memory and would emit flat loads (which an LDS barrier waits for)
+template <typename V>
+__device__ inline V persist_ld(const V* p, int64_t e) {
+    return ((const __attribute__((address_space(1))) V*)p)[e];
+}
+
+// synapse range [bb, ee) of pre-synaptic rank `pre` in projection P
+__device__ inline void persist_range(const ahip_persist_proj& P, const int32_t* cols, int32_t pre, int64_t& bb,
+                                     int64_t& ee) {
+    bb = ee = 0;
+    if (pre >= P.n_cols) return;
+    if (P.lds_cols >= 0) {
+        bb = cols[P.lds_cols + pre];
+        ee = cols[P.lds_cols + pre + 1];
+    } else {
+        bb = persist_ld(P.col_ptr, pre);
+        ee = persist_ld(P.col_ptr, pre + 1);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(AHIP_PERSIST_THREADS) void persistent_steps(ahip_pop_args a_in, ahip_persist_args pa) {
+    extern __shared__ unsigned char lds_raw[];
+    __shared__ int32_t wsum[kPersistWaves];
+    __shared__ uint32_t wlo[kPersistWaves], whi[kPersistWaves];
+    __shared__ int64_t fill_s[2];
+    __shared__ int32_t nspk_s;
+    __shared__ ahip_persist_proj proj_s[AHIP_PERSIST_MAX_PROJ];   // read per lane: LDS, not kernel arguments
+    const int32_t N = a_in.size;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+    // LDS: the state the statements touch (slot by slot), the refractory
+    // counters, the spike list, the column ranges and the scatter's item
+    // tables
+    T* lds = reinterpret_cast<T*>(lds_raw);
+    int32_t* rr = reinterpret_cast<int32_t*>(lds + (size_t)kPersistSlotCount * N);
+    int32_t* spk = rr + (kPersistRefractory ? N : 0);
+    int32_t* cols = spk + N;   // column ranges of the projections (n_lds_cols entries)
+    const size_t beg_at = ((size_t)(reinterpret_cast<unsigned char*>(cols + pa.n_lds_cols) - lds_raw) + 7) &
+                          ~(size_t)7;
+    int64_t* beg = reinterpret_cast<int64_t*>(lds_raw + beg_at);
+    int32_t* off = reinterpret_cast<int32_t*>(beg + kPersistItems);
+    int32_t* projq = off + kPersistItems + 1;
+    const ahip_pop_args& a = a_in;
+    for (int32_t i = tid; i < N; i += AHIP_PERSIST_THREADS) {
+%(p_load)s
+        if (kPersistRefractory) rr[i] = a.refractory_remaining[i];
+    }
+    if (tid < pa.n_proj) proj_s[tid] = pa.proj[tid];
+    if (tid == 0) {
+        nspk_s = *pa.spike_count;
+        if (pa.rec_counts) {
+            fill_s[0] = pa.rec_fill[0];
+            fill_s[1] = pa.rec_fill[1];
+        }
+    }
+    persist_sync();
+    int32_t nspk = nspk_s;
+    for (int32_t i = tid; i < nspk; i += AHIP_PERSIST_THREADS) spk[i] = pa.spiked[i];
+    for (int32_t p = 0; p < pa.n_proj; p++) {
+        const ahip_persist_proj& P = pa.proj[p];
+        if (P.lds_cols < 0) continue;
+        for (int32_t j = tid; j <= P.n_cols; j += AHIP_PERSIST_THREADS) cols[P.lds_cols + j] = (int32_t)P.col_ptr[j];
+    }
+    const int64_t t0 = *a.t_base + a.step;
+    const T dt = (T)a.dt;
+    (void)dt;
+%(global_loads)s
+%(div_consts)s
+%(local_ptrs)s
+    int32_t* __restrict__ refrac_rem = rr;
+    (void)refrac_rem;
+    // neuron i = k * AHIP_PERSIST_THREADS + tid, round k < per (consecutive
+    // lanes, consecutive neurons)
+    const int32_t per = (N + AHIP_PERSIST_THREADS - 1) / AHIP_PERSIST_THREADS;
+    // the refractory periods (constant during the launch) of the first four
+    // rounds in registers: no global load when a neuron spikes
+    int32_t ref0 = 0, ref1 = 0, ref2 = 0, ref3 = 0;
+    if (kPersistRefractory && a.refractory) {
+        if (tid < N) ref0 = a.refractory[tid];
+        if (tid + AHIP_PERSIST_THREADS < N) ref1 = a.refractory[tid + AHIP_PERSIST_THREADS];
+        if (tid + 2 * AHIP_PERSIST_THREADS < N) ref2 = a.refractory[tid + 2 * AHIP_PERSIST_THREADS];
+        if (tid + 3 * AHIP_PERSIST_THREADS < N) ref3 = a.refractory[tid + 3 * AHIP_PERSIST_THREADS];
+    }
+    (void)ref0; (void)ref1; (void)ref2; (void)ref3;
+    persist_sync();
+
+    // compute_psp of the spike list spk[0, n): projection rounds in creation
+    // order, the whole workgroup on each round; ends at a barrier
+    auto scatter_list = [&](int32_t n) {
+        for (int32_t r = 0; r < pa.n_rounds; r++) {
+            int32_t nr = pa.n_proj;
+            if (pa.n_rounds > 1) {
+                nr = 0;
+                for (int32_t p = 0; p < pa.n_proj; p++) nr += proj_s[p].round == r;
+            }
+            const int32_t n_items = nr * n;
+            if (n_items <= kPersistGroups) {
+                // one group of lanes per (projection, spike) item: 64, 32 or
+                // 16 lanes, the most that give every item a group
+                const int gsh = n_items <= kPersistGroups / 4 ? 6 : n_items <= kPersistGroups / 2 ? 5 : 4;
+                const int32_t gsz = 1 << gsh;
+                const int32_t g = tid >> gsh, gl = tid & (gsz - 1);
+                if (g < n_items) {
+                    const ahip_persist_proj& P =
+                        proj_s[pa.n_rounds > 1 ? persist_round_proj(proj_s, pa.n_proj, r, g / n) : g / n];
+                    int64_t bb, ee;
+                    persist_range(P, cols, spk[g %% n], bb, ee);
+                    T* const gt = lds + (size_t)P.target * N;
+                    const T w = (T)P.w;
+                    for (int64_t e0 = bb + gl; e0 < ee; e0 += 4 * gsz) {
+                        int32_t row[4];
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const int64_t e = e0 + u * gsz;
+                            row[u] = e < ee ? persist_ld(P.row_idx, e) : -1;
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; u++)
+                            if (row[u] >= 0) atomicAdd(gt + row[u], w);
+                    }
+                }
+                persist_sync();
+                continue;
+            }
+            for (int32_t q0 = 0; q0 < n_items; q0 += kPersistItems) {
+                const int32_t m = min(kPersistItems, n_items - q0);
+                int32_t d = 0;
+                if (tid < m) {
+                    const int32_t q = q0 + tid;
+                    const int32_t pi = pa.n_rounds > 1 ? persist_round_proj(proj_s, pa.n_proj, r, q / n) : q / n;
+                    int64_t bb, ee;
+                    persist_range(proj_s[pi], cols, spk[q %% n], bb, ee);
+                    d = (int32_t)(ee - bb);
+                    beg[tid] = bb;
+                    projq[tid] = pi;
+                }
+                int32_t E;
+                const int32_t o = persist_scan<int32_t>(d, wsum, E);
+                if (tid < m) off[tid] = o;
+                persist_sync();
+                for (int32_t e = tid; e < E; e += AHIP_PERSIST_THREADS) {
+                    int32_t lo = 0, hi = m - 1;
+                    while (lo < hi) {
+                        const int32_t mid = (lo + hi + 1) >> 1;
+                        if (off[mid] <= e) lo = mid; else hi = mid - 1;
+                    }
+                    const ahip_persist_proj& P = proj_s[projq[lo]];
+                    const int32_t row = persist_ld(P.row_idx, beg[lo] + (e - off[lo]));
+                    atomicAdd(lds + (size_t)P.target * N + row, (T)P.w);
+                }
+                persist_sync();
+            }
+        }
+    };
+
+    for (int32_t s = 0; s < pa.steps; s++) {
+        const int64_t t = t0 + s;
+        // 1. compute_psp: the spikes of step t - 1
+        scatter_list(nspk);
+        // 2. neuron update (the statements of the per-kernel step)
+        uint32_t bits = 0;
+        for (int32_t k = 0; k < per; k++) {
+            const int32_t i = k * AHIP_PERSIST_THREADS + tid;
+            if (i >= N) break;
+            bool _spike = false;
+#define AHIP_REFRACTORY_I (k == 0 ? ref0 : k == 1 ? ref1 : k == 2 ? ref2 : k == 3 ? ref3 : a.refractory[i])
+%(reg_in)s
+%(refrac_in)s
+%(local_rand)s
+%(local_eqs)s
+%(refrac_dec)s
+%(gather)s
+%(reg_out)s
+            if (_spike) bits |= 1u << k;
+#undef AHIP_REFRACTORY_I
+        }
+        // 3. ascending spike list (round by round, in lane order: the
+        //    position of a spike is its lane's rank in the wave's ballot plus
+        //    the counts of the waves before, a 16-lane DPP scan of the wave
+        //    counts; four rounds per pass, one 16-bit count each), recording
+        int64_t slot = -1, f = 0;
+        if (pa.rec_counts) {
+            slot = t - pa.rec_t_start;
+            if (slot >= 0 && slot < pa.rec_capacity) f = fill_s[slot & 1];
+            else slot = -1;
+        }
+        int32_t base = 0;
+        for (int32_t g0 = 0; g0 < per; g0 += 4) {
+            unsigned long long msk[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) msk[k] = __ballot((bits >> (g0 + k)) & 1u);
+            const uint32_t clo = (uint32_t)__popcll(msk[0]) | ((uint32_t)__popcll(msk[1]) << 16);
+            const uint32_t chi = (uint32_t)__popcll(msk[2]) | ((uint32_t)__popcll(msk[3]) << 16);
+            if (lane == 0) {
+                wlo[wid] = clo;
+                whi[wid] = chi;
+            }
+            persist_sync();
+            const uint32_t xlo = persist_row_scan(lane < kPersistWaves ? wlo[lane] : 0u);
+            const uint32_t xhi = persist_row_scan(lane < kPersistWaves ? whi[lane] : 0u);
+            const uint32_t blo = (uint32_t)__builtin_amdgcn_readlane((int)xlo, wid) - clo;
+            const uint32_t bhi = (uint32_t)__builtin_amdgcn_readlane((int)xhi, wid) - chi;
+            const uint32_t alo = (uint32_t)__builtin_amdgcn_readlane((int)xlo, kPersistWaves - 1);
+            const uint32_t ahi = (uint32_t)__builtin_amdgcn_readlane((int)xhi, kPersistWaves - 1);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t before = ((k < 2 ? blo : bhi) >> (16 * (k & 1))) & 0xffffu;
+                const uint32_t all = ((k < 2 ? alo : ahi) >> (16 * (k & 1))) & 0xffffu;
+                if ((bits >> (g0 + k)) & 1u) {
+                    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(msk[k] >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)msk[k], 0u));
+                    const int32_t pos = base + (int32_t)(before + below);
+                    const int32_t i = (g0 + k) * AHIP_PERSIST_THREADS + tid;
+                    spk[pos] = i;
+                    pa.spiked[pos] = i;
+                    if (slot >= 0 && f + pos < pa.rec_idx_capacity) pa.rec_idx[f + pos] = i;
+                }
+                base += (int32_t)all;
+            }
+            if (g0 + 4 < per) persist_sync();   // wlo / whi are reused
+        }
+        if (tid == 0 && slot >= 0) {
+            pa.rec_counts[slot] = base;
+            fill_s[(slot + 1) & 1] = f + base;
+        }
+        nspk = base;
+        persist_sync();   // the list is complete before the next step reads it
+    }
+    for (int32_t i = tid; i < N; i += AHIP_PERSIST_THREADS) {
+%(p_store)s
+        if (kPersistRefractory) a.refractory_remaining[i] = rr[i];
+    }
+    if (tid == 0) {
+        *pa.spike_count = nspk;
+        if (pa.rec_counts) {
+            pa.rec_fill[0] = fill_s[0];
+            pa.rec_fill[1] = fill_s[1];
+        }
+    }
+}
+
+template <typename T>
+int persist(const ahip_pop_args* a, const ahip_persist_args* pa, size_t lds, void* stream) {
+    if (a->size > 32 * AHIP_PERSIST_THREADS) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(persistent_steps<T>, dim3(1), dim3(AHIP_PERSIST_THREADS), lds, (hipStream_t)stream, *a, *pa);
+    return (int)hipGetLastError();
+}
+int persist_f64(const ahip_pop_args* a, const ahip_persist_args* pa, size_t lds, void* s) {
+    return persist<double>(a, pa, lds, s);
+}
+int persist_f32(const ahip_pop_args* a, const ahip_persist_args* pa, size_t lds, void* s) {
+    return persist<float>(a, pa, lds, s);
+}
+const int32_t kPersistSlotsHost[] = {%(pslot_list)s};
 """
 
 
@@ -257,6 +593,15 @@ def generate(desc: NeuronDescription, stop_condition: str | None = None) -> Mode
     has_global = bool(global_eqs.strip())
     mask = "in_ref" if has_ref else None
     local_eqs = subst(equation_code(desc, "local", mask))
+    # divisions of the neuron equations by a population-wide value: the
+    # reciprocal steps once per kernel (ahip::DivConst)
+    div_names = [n for n in global_names
+                 if re.search(r"(?<!/)/\s*" + re.escape(_c_name(n)) + r"(?![\w\[(])", local_eqs)]
+    for n in div_names:
+        local_eqs = re.sub(r"(?<!/)(/\s*)" + re.escape(_c_name(n)) + r"(?![\w\[(])", r"\g<1>" + _c_name(n) + "__dc",
+                           local_eqs)
+    div_consts = "\n".join(f"    const ahip::DivConst<T> {_c_name(n)}__dc = ahip::div_const<T>({_c_name(n)});"
+                            for n in div_names)
 
     refrac_in = "const int in_ref = (a.refractory_remaining[i] > 0) ? 0 : 1;" if has_ref else ""
     refrac_dec = "a.refractory_remaining[i] -= (1 - in_ref);" if has_ref else ""
@@ -298,6 +643,57 @@ def generate(desc: NeuronDescription, stop_condition: str | None = None) -> Mode
         preloop_fn = ""
         preloop_ptr = "nullptr"
 
+    # persistent steps: spiking models without population-wide equations.
+    # LDS holds the local slots the statements touch and the conductances
+    # (g_<target>), in slot order; the statements index them with i as the
+    # per-kernel step does.
+    if desc.spiking and not has_global:
+        stmts = "\n".join([refrac_in, rand_code("local", "i"), local_eqs, refrac_dec, gather])
+        pslots = [n for n in local_names
+                  if n.startswith("g_") or re.search(r"(?<![\w.])" + re.escape(_c_name(n)) + r"\[i\]", stmts)]
+        persist_ptrs_code = "\n".join(
+            f"    T* __restrict__ {_c_name(n)} = lds + {k} * (size_t)N;" for k, n in enumerate(pslots))
+        rr = lambda c: c.replace("a.refractory_remaining", "refrac_rem").replace("a.refractory[i]", "AHIP_REFRACTORY_I")
+        parts = {"refrac_in": rr(refrac_in), "local_rand": rand_code("local", "i"), "local_eqs": rr(local_eqs),
+                 "refrac_dec": rr(refrac_dec), "gather": rr(gather)}
+        # the neuron's LDS values in registers for the statements: loaded
+        # before them, stored after them when written (one read and at most
+        # one write of each per neuron and step)
+        joined = "\n".join(parts.values())
+        reg_in, reg_out = [], []
+        for c, ty in [(_c_name(n), "T") for n in pslots] + ([("refrac_rem", "int32_t")] if has_ref else []):
+            pat = r"(?<![\w.])" + re.escape(c) + r"\[i\]"
+            if not re.search(pat, joined):
+                continue
+            reg_in.append(f"            {ty} {c}__r = {c}[i];")
+            if re.search(pat + r"\s*(?:[-+*/]?=(?!=)|\+\+|--)", joined):
+                reg_out.append(f"            {c}[i] = {c}__r;")
+            for key in parts:
+                parts[key] = re.sub(pat, c + "__r", parts[key])
+        persist_fn = _PERSIST_KERNEL % {
+            "n_pslots": len(pslots),
+            "p_refr": "true" if has_ref else "false",
+            "p_load": "\n".join(f"        lds[{k} * (size_t)N + i] = ((const T*)a.slot[{slot_of[n]}])[i];"
+                                for k, n in enumerate(pslots)),
+            "p_store": "\n".join(f"        ((T*)a.slot[{slot_of[n]}])[i] = lds[{k} * (size_t)N + i];"
+                                 for k, n in enumerate(pslots)),
+            "pslot_list": ", ".join(str(slot_of[n]) for n in pslots) or "0",
+            "global_loads": global_loads,
+            "div_consts": div_consts,
+            "local_ptrs": persist_ptrs_code,
+            "refrac_in": _indent(parts["refrac_in"], 12),
+            "local_rand": _indent(parts["local_rand"], 12),
+            "local_eqs": _indent(parts["local_eqs"], 12),
+            "refrac_dec": _indent(parts["refrac_dec"], 12),
+            "gather": _indent(parts["gather"], 12),
+            "reg_in": "\n".join(reg_in),
+            "reg_out": "\n".join(reg_out),
+        }
+        persist_ptrs = f"persist_f64, persist_f32, {len(pslots)}, kPersistSlotsHost"
+    else:
+        persist_fn = ""
+        persist_ptrs = "nullptr, nullptr, 0, nullptr"
+
     body = _PRELUDE % {"name": desc.name}
     if has_global:
         body += _GLOBAL_KERNEL % {
@@ -307,6 +703,7 @@ def generate(desc: NeuronDescription, stop_condition: str | None = None) -> Mode
         }
     src_wo_hash = _LOCAL_KERNEL % {
         "global_loads": global_loads,
+        "div_consts": div_consts,
         "local_ptrs": local_ptrs,
         "refrac_in": _indent(refrac_in, 8),
         "local_rand": _indent(rand_code("local", "i"), 8),
@@ -330,6 +727,8 @@ def generate(desc: NeuronDescription, stop_condition: str | None = None) -> Mode
         "has_refractory": int(has_ref),
         "has_global": int(has_global),
         "preloop_ptr": preloop_ptr,
+        "persist_fn": persist_fn,
+        "persist_ptrs": persist_ptrs,
         "hash": "%(hash)s",
     }
     cond, _ = parse_stop_condition(stop_condition)

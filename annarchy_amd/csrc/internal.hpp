@@ -273,6 +273,7 @@ struct ahip_net {
     ahip::DevBuf deliver_targets;               // DeliverTarget of the merged delivery launch
     int32_t deliver_n_tg = 0, deliver_waves = 0;
     int32_t delivery = AHIP_DELIVERY_ORDERED;
+    bool persist_enabled = true;   // small single-population networks: one launch per batch
     size_t elem() const { return precision == AHIP_F64 ? sizeof(double) : sizeof(float); }
 };
 

@@ -852,7 +852,109 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
     return 0;
 }
 
+// Persistent steps (ahip_persist_args, annarchy_hip_model.h): a network of
+// one active spiking population whose projections all start and end in it,
+// each with one weight, no delay and no synapse model, recorded at most by
+// one spike monitor, small enough for the state its statements touch, its
+// conductances and its spike list to sit in one workgroup's LDS, runs a
+// whole batch of steps in one launch of one workgroup.
+constexpr size_t kPersistLdsMax = 160 * 1024 - 1024;   // dynamic LDS, the kernel's static part aside
+
+bool persist_plan(const ahip_net* net, ahip_persist_args& pa, size_t& lds, int32_t& pop_id) {
+    if (!net->persist_enabled || net->pops.size() != 1) return false;
+    if (net->probe_kind >= 0 && net->probe_kind != AHIP_PROBE_PERSIST) return false;
+    const Pop& p = *net->pops[0];
+    const ahip_persist_fn fn = net->precision == AHIP_F64 ? p.model->persist_f64 : p.model->persist_f32;
+    if (!fn || !p.model->spiking || !p.active || p.depth > 1 || p.fr_window > 0 || p.n_deliver > 0) return false;
+    if (p.size > 32 * AHIP_PERSIST_THREADS) return false;
+    std::memset(&pa, 0, sizeof(pa));
+    const int32_t n_ps = p.model->persist_n_slots;
+    std::vector<int32_t> round_of(n_ps, -1);   // per LDS slot: the last round that writes it
+    for (auto& pp : net->projs) {
+        const Proj& q = *pp;
+        if (!q.spiking || q.pre != 0 || q.post != 0) return false;
+        if (q.syn || q.delay_per_syn || q.delay_uniform > 1 || !q.single_w) return false;
+        if (!transmits(net, q) || q.nnz == 0) continue;
+        if (pa.n_proj >= AHIP_PERSIST_MAX_PROJ) return false;
+        int32_t tgt = -1;
+        for (int32_t k = 0; k < n_ps; k++)
+            if (p.model->persist_slots[k] == q.target_slot) tgt = k;
+        if (tgt < 0) return false;
+        ahip_persist_proj& d = pa.proj[pa.n_proj++];
+        d.col_ptr = q.col_ptr.as<int64_t>();
+        d.row_idx = q.row_idx.as<int32_t>();
+        d.w = q.w_const;
+        d.n_cols = q.n_cols;
+        d.target = tgt;
+        // projections into one conductance add their events in creation order
+        d.round = round_of[tgt] + 1;
+        round_of[tgt] = d.round;
+        pa.n_rounds = std::max(pa.n_rounds, d.round + 1);
+    }
+    const Monitor* rec = nullptr;
+    for (auto& mp : net->mons) {
+        const Monitor& m = *mp;
+        if (m.paused) continue;
+        if (!m.spikes || m.pop != 0 || rec) return false;
+        rec = &m;
+    }
+    // LDS: the slots, the refractory counters, the spike list, the column
+    // ranges of the projections when they fit (else read from col_ptr), then
+    // (8-byte aligned) the scatter's item tables
+    const size_t es = net->elem(), n = (size_t)p.size;
+    const size_t fixed = (size_t)n_ps * n * es + (p.model->has_refractory ? n * sizeof(int32_t) : 0) +
+                         n * sizeof(int32_t);
+    const size_t tables = (size_t)AHIP_PERSIST_ITEMS * sizeof(int64_t) +
+                          (size_t)(2 * AHIP_PERSIST_ITEMS + 1) * sizeof(int32_t);
+    int32_t n_cols = 0;
+    bool narrow = true;   // column offsets fit in int32
+    for (int32_t k = 0; k < pa.n_proj; k++) n_cols += pa.proj[k].n_cols + 1;
+    for (auto& pp : net->projs) narrow &= pp->nnz < (int64_t)INT32_MAX;
+    const auto total = [&](int32_t cols) { return ((fixed + (size_t)cols * sizeof(int32_t) + 7) & ~(size_t)7) + tables; };
+    if (narrow && total(n_cols) <= kPersistLdsMax) {
+        pa.n_lds_cols = n_cols;
+        for (int32_t k = 0, o = 0; k < pa.n_proj; k++) {
+            pa.proj[k].lds_cols = o;
+            o += pa.proj[k].n_cols + 1;
+        }
+    } else {
+        pa.n_lds_cols = 0;
+        for (int32_t k = 0; k < pa.n_proj; k++) pa.proj[k].lds_cols = -1;
+    }
+    lds = total(pa.n_lds_cols);
+    if (lds > kPersistLdsMax) return false;
+    pa.spiked = p.spiked.as<int32_t>();
+    pa.spike_count = p.spike_count.as<int32_t>();
+    if (rec) {
+        pa.rec_counts = rec->counts.as<int32_t>();
+        pa.rec_idx = rec->data.as<int32_t>();
+        pa.rec_fill = rec->fill.as<int64_t>();
+        pa.rec_idx_capacity = rec->idx_capacity;
+        pa.rec_t_start = rec->t_start;
+        pa.rec_capacity = rec->capacity;
+    }
+    pop_id = 0;
+    return true;
+}
+
 int enqueue_batch(ahip_net* net, int32_t nsteps, const std::vector<int32_t>& zb) {
+    ahip_persist_args pa;
+    size_t lds = 0;
+    int32_t pid = -1;
+    if (nsteps > 0 && persist_plan(net, pa, lds, pid)) {
+        Pop& p = *net->pops[pid];
+        pa.steps = nsteps;
+        ahip_pop_args a = make_args(net, p, 0);
+        const ahip_persist_fn fn = net->precision == AHIP_F64 ? p.model->persist_f64 : p.model->persist_f32;
+        const bool probe = net->probe_kind == AHIP_PROBE_PERSIST;
+        if (probe) HIPCHK(hipEventRecord(net->probe_ev[0], net->stream));
+        const int rc = fn(&a, &pa, lds, (void*)net->stream);
+        if (probe && rc == 0) HIPCHK(hipEventRecord(net->probe_ev[1], net->stream));
+        if (rc != 0) return fail(AHIP_EHIP, "persistent steps of population %d failed: %s", p.id,
+                                 hipGetErrorString((hipError_t)rc));
+        LCHK(launch_advance_time(net->t_dev.as<int64_t>(), nsteps, net->stream));
+        return 0;
+    }
     for (int32_t j = 0; j < nsteps; j++) {
         int rc = enqueue_step(net, j, zb);
         if (rc) return rc;
@@ -2108,6 +2210,16 @@ int ahip_time_kernel(ahip_net* net, int32_t what, int32_t index, int32_t iters, 
             if (index != 0 || net->deliver_waves == 0)
                 return fail(AHIP_EINVAL, "no separate ordered delivery launch (index 0): the updates deliver");
             break;
+        case AHIP_PROBE_PERSIST: {
+            ahip_persist_args pa;
+            size_t lds = 0;
+            int32_t pid = -1;
+            net->probe_kind = AHIP_PROBE_PERSIST;
+            const bool yes = index == 0 && persist_plan(net, pa, lds, pid);
+            net->probe_kind = -1;
+            if (!yes) return fail(AHIP_EINVAL, "the network does not run persistent steps");
+            break;
+        }
         default:
             return fail(AHIP_EINVAL, "unknown kernel kind %d", what);
     }
@@ -2117,15 +2229,17 @@ int ahip_time_kernel(ahip_net* net, int32_t what, int32_t index, int32_t iters, 
     HIPCHK(hipEventCreate(&net->probe_ev[1]));
     net->probe_proj = index;
     net->probe_kind = what;
-    // `iters` ordinary steps, launched eagerly, with the events around the
-    // probed kernel: its time in the context of a real step
+    // `iters` ordinary steps (persistent steps: launches of a graph batch),
+    // launched eagerly, with the events around the probed kernel: its time
+    // in the context of a real step
+    const int32_t per_iter = what == AHIP_PROBE_PERSIST ? auto_batch(net) : 1;
     double total = 0.0;
     for (int32_t i = 0; i < iters && rc == 0; i++) {
-        rc = enqueue_batch(net, 1, zb);
+        rc = enqueue_batch(net, per_iter, zb);
         if (rc) break;
         hipError_t e = hipStreamSynchronize(net->stream);
         if (e != hipSuccess) { rc = fail(AHIP_EHIP, "step failed: %s", hipGetErrorString(e)); break; }
-        net->t += 1;
+        net->t += per_iter;
         float ms = 0.f;
         e = hipEventElapsedTime(&ms, net->probe_ev[0], net->probe_ev[1]);
         if (e != hipSuccess) { rc = fail(AHIP_EHIP, "probe events: %s", hipGetErrorString(e)); break; }
@@ -2138,6 +2252,28 @@ int ahip_time_kernel(ahip_net* net, int32_t what, int32_t index, int32_t iters, 
     net->probe_ev[0] = net->probe_ev[1] = nullptr;
     if (rc) return rc;
     *avg_ms = total / iters;
+    return 0;
+}
+
+int ahip_net_set_persistent(ahip_net* net, int32_t enabled) {
+    if (!net) return fail(AHIP_EINVAL, "null network");
+    if (net->persist_enabled != (enabled != 0)) {
+        HIPCHK(hipStreamSynchronize(net->stream));
+        net->persist_enabled = enabled != 0;
+        invalidate(net);
+    }
+    return 0;
+}
+
+int ahip_net_uses_persistent(ahip_net* net, int32_t* yes) {
+    if (!net || !yes) return fail(AHIP_EINVAL, "null argument");
+    if (!net->initialized) return fail(AHIP_ESTATE, "network not initialized");
+    int rc = prepare(net);
+    if (rc) return rc;
+    ahip_persist_args pa;
+    size_t lds = 0;
+    int32_t pid = -1;
+    *yes = persist_plan(net, pa, lds, pid) ? 1 : 0;
     return 0;
 }
 

@@ -1351,6 +1351,17 @@ class Network:
         _lib.check(_lib.lib().ahip_set_graph_enabled(self._h, int(enabled)))
         _lib.check(_lib.lib().ahip_set_graph_batch(self._h, int(batch)))
 
+    def set_persistent(self, enabled: bool = True):
+        """Persistent steps of a small single-population spiking network (one
+        launch of one workgroup per batch of steps; on by default)."""
+        _lib.check(_lib.lib().ahip_net_set_persistent(self._h, int(enabled)))
+
+    def uses_persistent(self) -> bool:
+        """Whether the next run() takes the persistent steps."""
+        y = C.c_int32()
+        _lib.check(_lib.lib().ahip_net_uses_persistent(self._h, C.byref(y)))
+        return bool(y.value)
+
     def prepare_graph(self):
         """Capture, instantiate and upload the step graph now (simulates
         nothing), so that the next run() only replays it."""

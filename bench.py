@@ -333,23 +333,47 @@ def bench_c0(args):
     def conn(net, P, ne):
         return [net.connect(P[:ne], P, "exc").fixed_probability(0.02, 0.6),
                 net.connect(P[ne:], P, "inh").fixed_probability(0.02, 6.7)]
-    net, P, _, projs, mon = _coba(4000, 3200, conn, steps + 200)
-    net.set_graph(True, 200)
+    G, probe_launches = 200, 5
+    net, P, _, projs, mon = _coba(4000, 3200, conn, steps + G * probe_launches + 400)
+    net.set_graph(True, G)
     net.prepare_graph()
+    persistent = net.uses_persistent()
     v0 = P.v.reshape(-1).copy()
     t0 = time.perf_counter()
     net.run(steps)
     el = time.perf_counter() - t0
-    counts, idx = mon.raw_spikes()
-    events = _events(projs, P, idx)
-    log(f"c0: {steps} steps in {el:.3f} s, {int(counts.sum())} spikes")
+    log(f"c0: {steps} steps in {el:.3f} s ({'persistent steps' if persistent else 'per-kernel steps'})")
     outdeg = _outdeg(projs, P)
-    kern, syn, spk = _spiking_kernels(net, P, projs, outdeg, mon, 8, steps)
+    es = 8
+    done = steps
+    kern = {}
+    if persistent:
+        # one launch runs G steps in one workgroup: state in and out of LDS
+        # once, N (3 s + 4) each way; per step the spike list and monitor
+        # entries (4 + 4 B per spike), last_spike (8 B), the column range
+        # (16 B) per spike and row index (4 B) per event
+        ms = net.time_kernel(7, 0, probe_launches)
+        counts, idx = mon.raw_spikes()
+        starts = np.concatenate([[0], np.cumsum(counts)])
+        us = range(done - 1, done + G * probe_launches - 1)
+        syn = float(np.mean([outdeg[idx[starts[u]:starts[u + 1]]].sum() for u in us]))
+        spk = float(np.mean([counts[u] for u in us]))
+        b = 2 * 4000 * (3 * es + 4) + G * (spk * (4 + 4 + 8 + 16) + syn * 4)
+        kern["persistent"] = {"ms": ms, "bytes": float(b), "steps_per_launch": G, "us_per_step": ms * 1e3 / G}
+        done += G * probe_launches
+        log(f"c0: persistent launch {ms:.3f} ms / {G} steps = {ms * 1e3 / G:.2f} us per step")
+    per_k, syn, spk = _spiking_kernels(net, P, projs, outdeg, mon, es, done)
+    if not persistent:
+        kern = per_k
+    counts, idx = mon.raw_spikes()
+    events = _events(projs, P, idx[:int(counts[:steps].sum())])
     dom = max(kern, key=lambda k: kern[k]["ms"])
     ach = kern[dom]["bytes"] / (kern[dom]["ms"] * 1e-3) / 1e9
+    note = ("latency-bound: one workgroup runs 200 dependent steps per launch (scatter, update, spike list "
+            "with barriers between them); the state stays in LDS" if persistent else
+            "latency-bound: three launches of a few microseconds per step")
     roof = roofline("hbm", ach, HBM_PEAK_GBS, "GB/s", None, dom, kernel_ms=kern[dom]["ms"],
-                    algorithmic_bytes=kern[dom]["bytes"],
-                    note="latency-bound: three launches of a few microseconds per step")
+                    algorithmic_bytes=kern[dom]["bytes"], note=note)
     cpu = None
     if not args.no_cpu:
         po = [orc.csrc_projection(projs[0].csr, 0), orc.csrc_projection(projs[1].csr, 1)]
@@ -367,9 +391,11 @@ def bench_c0(args):
                "host": h}
         log(f"c0: oracle {steps / cel:.0f} steps/s, bit-exact {exact}")
     return record("steps/s", steps / el, "steps/s", steps, 0, el * 1e3 / steps, "f64",
-                  {"workload": "COBA LIF N=4000 (3200E/800I) p=0.02, dt=0.1ms, 10000 steps", "graph_steps": 200},
+                  {"workload": "COBA LIF N=4000 (3200E/800I) p=0.02, dt=0.1ms, 10000 steps", "graph_steps": G,
+                   "persistent_steps": persistent},
                   roof, cpu, synaptic_events_per_s=events / el, spikes=int(counts[:steps].sum()),
-                  mean_rate_hz=float(counts[:steps].sum() / 4000 / (steps * 1e-4)), kernels=kern)
+                  mean_rate_hz=float(counts[:steps].sum() / 4000 / (steps * 1e-4)), kernels=kern,
+                  per_kernel_step_kernels=per_k if persistent else None)
 
 
 def bench_c2(args):

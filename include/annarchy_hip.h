@@ -266,6 +266,8 @@ int ahip_proj_set_learning(ahip_net* net, int32_t proj, int32_t update, int32_t 
 #define AHIP_PROBE_SCATTER 5
 int ahip_time_kernel(ahip_net* net, int32_t what, int32_t index, int32_t iters, double* avg_ms);
 #define AHIP_PROBE_DELIVER 6   /* the ordered delivery launch of every conductance (index 0) */
+#define AHIP_PROBE_PERSIST 7   /* the persistent steps (index 0): each of the `iters` runs one launch of
+                                  a graph batch of steps; *avg_ms = mean launch duration */
 /* Number of fused spike-scatter launches per step (spiking projections
  * without per-synapse delays or synapse models, grouped by target). */
 int ahip_scatter_groups(ahip_net* net, int32_t* n_groups);
@@ -284,6 +286,18 @@ int ahip_scatter_groups(ahip_net* net, int32_t* n_groups);
 #define AHIP_DELIVERY_ORDERED 0
 #define AHIP_DELIVERY_ATOMIC 1
 int ahip_net_set_spike_delivery(ahip_net* net, int32_t mode);
+/* Persistent steps of a small network (on by default): a network of one
+ * active spiking population whose projections all start and end in it, each
+ * with one weight, no delay and no synapse model, recorded at most by one
+ * spike monitor, with its conductances and spike list within one
+ * workgroup's LDS, runs a whole batch of steps in one launch of one
+ * workgroup (the same statements in the same order as the per-kernel step;
+ * the reference's single-thread loop, SingleThread/CSR.py:867-888, is the
+ * shape it restates).  No counterpart in the reference.  enabled = 0 keeps
+ * the per-kernel step. */
+int ahip_net_set_persistent(ahip_net* net, int32_t enabled);
+/* *yes = 1 when the next run would use the persistent steps. */
+int ahip_net_uses_persistent(ahip_net* net, int32_t* yes);
 /* Number of conductances delivered in order (all in one launch, probed as
  * AHIP_PROBE_DELIVER index 0). */
 int ahip_ordered_targets(ahip_net* net, int32_t* n);

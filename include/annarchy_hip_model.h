@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define AHIP_MODEL_ABI 6
+#define AHIP_MODEL_ABI 8
 #define AHIP_MAX_SLOTS 48
 #define AHIP_MAX_RAND 8
 #define AHIP_UPDATE_BLOCK 256 /* threads per block of every update kernel */
@@ -68,6 +68,47 @@ typedef struct ahip_pop_args {
 } ahip_pop_args;
 
 typedef int (*ahip_launch_fn)(const ahip_pop_args* args, void* stream);
+
+/* ---- persistent steps of a small network --------------------------------
+ * A network of one spiking population whose projections all start and end
+ * in it (constant weight each, no delays, no synapse model) can run a whole
+ * batch of steps in ONE launch of one workgroup: per step the scatter of the
+ * previous step's spikes into the conductances, the update of every neuron
+ * and the ascending spike list with its recording.  The state the
+ * statements touch (the model's persist_slots) and the refractory counters
+ * live in LDS for the whole launch.  The steps are the ordinary ones (the
+ * same statements, the same order), so spike trains and state equal those of
+ * the per-kernel step. */
+#define AHIP_PERSIST_MAX_PROJ 8
+#define AHIP_PERSIST_THREADS 1024
+#define AHIP_PERSIST_ITEMS 512      /* (projection, spike) items per scatter pass */
+typedef struct ahip_persist_proj {
+    const int64_t* col_ptr;             /* CSRC backward view (pre rank -> synapse range) */
+    const int32_t* row_idx;             /* post rank of every synapse, column order */
+    double w;                           /* the projection's one weight */
+    int32_t n_cols;                     /* pre ranks covered by col_ptr */
+    int32_t target;                     /* index of the conductance in the model's persist_slots */
+    int32_t round;                      /* scatter round: same-target projections in creation order */
+    int32_t lds_cols;                   /* offset of its column ranges in the LDS table (-1: col_ptr) */
+} ahip_persist_proj;
+typedef struct ahip_persist_args {
+    int32_t steps;                      /* steps of this launch */
+    int32_t n_proj, n_rounds;
+    int32_t n_lds_cols;                 /* entries of the LDS column-range table (0: none) */
+    ahip_persist_proj proj[AHIP_PERSIST_MAX_PROJ];
+    int32_t* spiked;                    /* the population's spike list and count (kept up to date) */
+    int32_t* spike_count;
+    /* the population's spike monitor (rec_counts == NULL: none), as filled
+     * by the compaction of the per-kernel step */
+    int32_t* rec_counts;
+    int32_t* rec_idx;
+    int64_t* rec_fill;
+    int64_t rec_idx_capacity;
+    int64_t rec_t_start;
+    int64_t rec_capacity;
+} ahip_persist_args;
+typedef int (*ahip_persist_fn)(const ahip_pop_args* args, const ahip_persist_args* pa, size_t lds_bytes,
+                               void* stream);
 /* Host-side evaluation of the hoisted step sizes (exponential Euler
  * "pre_loop" values, parser/Equation.py:559-604) from the global slots; writes
  * one value per pre-loop slot.  Host libm is used on purpose so the values
@@ -94,6 +135,10 @@ typedef struct ahip_model_desc {
     ahip_launch_fn launch_f32;
     ahip_preloop_fn preloop;            /* NULL when there is no pre-loop value */
     const char* source_hash;
+    ahip_persist_fn persist_f64;        /* NULL for rate-coded models or with a global update */
+    ahip_persist_fn persist_f32;
+    int32_t persist_n_slots;            /* local slots the persistent steps hold in LDS, in LDS order: */
+    const int32_t* persist_slots;       /* those the statements touch and the conductances */
 } ahip_model_desc;
 
 /* Every model library exports this symbol. */
