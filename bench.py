@@ -267,13 +267,14 @@ def _spiking_kernels(net, P, projs, outdeg, mon, es, n_steps_done):
     steps.  Kernel bytes:
       update      N * (6 s + 12): v, g_exc, g_inh read and written, refractory
                   read, refractory_remaining read and written; when it
-                  delivers the ordered conductances also N * 4 per
-                  conductance (counters), per event 4 + s (key, weight) and
-                  per touched neuron 20 (bucket range, counter cleared);
+                  delivers the ordered conductances also N * 8 per
+                  conductance (packed bucket heads), per event 4 + s (key,
+                  weight) and per touched neuron 20 (bucket offset, head
+                  cleared);
       scatter     per spike 20 B (list entry, column range), per synapse event
                   4 + 3 s (post rank, weight, conductance read and written) with
-                  atomic delivery or one weight, and 4 + 8 + 16 + 4 (post rank,
-                  bucket counter read and written, bucket range, key) ordered;
+                  atomic delivery or one weight, and 4 + 16 + 4 + 8 (post rank,
+                  packed bucket head read and written, key, weight) ordered;
       deliver     per touched post neuron 4 + 4 + 16 + 2 s (list entry, counter,
                   bucket range, g read and written) + per event 4 + s (key,
                   weight).
@@ -308,13 +309,14 @@ def _spiking_kernels(net, P, projs, outdeg, mon, es, n_steps_done):
     syn = np.mean([outdeg[idx[starts[u]:starts[u + 1]]].sum() for u in us]) if len(us) else 0.0
     spk = np.mean([counts[u] for u in us]) if len(us) else 0.0
     N = P.size
-    per_syn = (4 + 8 + 16 + 4) if no else (4 + 3 * es)
+    per_syn = (4 + 16 + 4 + 8) if no else (4 + 3 * es)
     # touched post neurons: the expected number of distinct targets of `syn`
     # events spread over N neurons, N (1 - exp(-syn / N))
     touched = N * -np.expm1(-syn / N)
-    # update: + per ordered conductance N * 4 (counters) when it delivers,
-    # and per event 4 + s (key, weight), per touched neuron 16 + 4 (range, clear)
-    deliver_in_update = (no * N * 4 + syn * (4 + es) + touched * 20) if (no and not sep) else 0.0
+    # update: + per ordered conductance N * 8 (packed heads) when it
+    # delivers, and per event 4 + s (key, weight), per touched neuron 20
+    # (bucket offset, head cleared)
+    deliver_in_update = (no * N * 8 + syn * (4 + es) + touched * 20) if (no and not sep) else 0.0
     b = {"update": N * (6 * es + 12) + deliver_in_update, "compact": N / 8 + spk * 4 + 8,
          "scatter": syn * per_syn + spk * 20,
          "deliver": (syn * (4 + es) + touched * (24 + 2 * es)) if sep else 0.0}
@@ -438,7 +440,9 @@ def bench_c2(args):
     kern, syn, spk = _spiking_kernels(net, P, projs[:2], outdeg, mon, es, warmup + steps)
     dom = max(kern, key=lambda k: kern[k]["ms"])
     ach = kern[dom]["bytes"] / (kern[dom]["ms"] * 1e-3) / 1e9
-    traffic = traffic_from_profiles(f"c2_{dom}_{'f64' if prec == 'double' else 'f32'}_traffic.txt")
+    kname = {"scatter": "scatter_multi", "update": "local_update", "compact": "compact_spikes",
+             "deliver": "deliver_ordered"}[dom]
+    traffic = traffic_from_profiles(f"{kname}_{'f64' if prec == 'double' else 'f32'}_c2_coba400k_traffic.txt")
     roof = roofline("hbm", ach, HBM_PEAK_GBS, "GB/s", traffic, dom, kernel_ms=kern[dom]["ms"],
                     algorithmic_bytes=kern[dom]["bytes"])
     cpu = None
@@ -539,19 +543,27 @@ def bench_c3(args):
         f"per launch of {info['chunk_steps']} steps")
     sps_b = steps / el_b
     sps_p = worst = pop_err = None
+    plain_sample = None
     if not args.no_plain:
-        nets = sw.plain_networks()
+        # every 8th instance as a plain FP64 network of its own (CSR
+        # everywhere): its time scaled to the 256 instances, and the batched
+        # rates against its rates after the same steps
+        plain_sample = list(range(0, sw.n_inst, 8))
+        t_p = time.perf_counter()
+        nets = sw.plain_networks(plain_sample)
+        log(f"c3: {len(nets)} plain instances built in {time.perf_counter() - t_p:.0f} s")
         plain_steps = min(steps, 200)
-        el_p = time_plain(nets, plain_steps)
+        el_p = time_plain(nets, plain_steps) * sw.n_inst / len(nets)
         for net, exc in nets:
             net.run(20 + steps - plain_steps)
         got = b.rates()
         worst, pop_err = 0.0, 0.0
-        for i, (net, exc) in enumerate(nets):
+        for i, (net, exc) in zip(plain_sample, nets):
             ref = exc.r.reshape(-1)
             worst = max(worst, float(np.max(np.abs(got[i] - ref)) / max(np.max(np.abs(ref)), 1e-3)))
             pop_err = max(pop_err, abs(float(got[i].mean() - ref.mean())) / max(abs(ref.mean()), 1e-9))
         sps_p = plain_steps / el_p
+        log(f"c3: plain instances {sps_p:.2f} steps/s (all 256, scaled), worst rate error {worst:.2e}")
     achieved = info["sparse_bytes"] / (ms_spmv * 1e-3) / 1e9
     # one GEMM launch covers the input psp of a whole chunk (graph batch) of steps
     gemm_flop = info["dense_flops"] * info["chunk_steps"]
@@ -564,6 +576,7 @@ def bench_c3(args):
                   {"workload": "256 rate instances N~U{500..8000} p=0.05 + shared dense 2000x2000 input, 2000 steps",
                    "neurons": sw.total, "synapses": sw.nnz, "graph_steps": G},
                   roof, cpu, plain_steps_per_s=sps_p, speedup_vs_plain=sps_b / sps_p if sps_p else None,
+                  plain_instances_compared=plain_sample,
                   max_rel_rate_error_vs_fp64=worst, max_rel_population_rate_error=pop_err,
                   gemm={"kernel": "batch_gemm", "ms_per_launch": ms_gemm, "steps_per_launch": info["chunk_steps"],
                         "ms_per_step": ms_gemm / info["chunk_steps"], "useful_flop_per_launch": gemm_flop,
@@ -730,7 +743,12 @@ def main():
                     help="c3: skip the plain per-instance FP64 networks (and the error and speedup they give)")
     ap.add_argument("--c4-sample", type=int, default=20)
     ap.add_argument("--c4-large", type=int, default=2)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only time the streaming copy of 2 GiB (the kernel that calibrates HBM counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        print(json.dumps({"copy_probe_GBps": copy_probe()}), flush=True)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every GPU box of this setup has one GPU, and the simulation "
