@@ -19,6 +19,8 @@ the GPU):
       spike trains against the oracle);
   c2  COBA LIF, 400,000 neurons, fixed_number_pre 800 E + 200 I drawn on the
       device, 4e8 synapses with per-synapse weights, Poisson 5 Hz into 10%;
+  stdp  STDP on 5e7 plastic synapses (100,000 Poisson pre neurons at 20 Hz
+      -> 50,000 Poisson post neurons at 10 Hz, fixed_number_pre 1000);
   c3  256 rate-network instances of uneven size in one batched launch, plain
       per-instance FP64 loop beside it;
   c4  the learned storage-format selector: held-out regret against
@@ -50,8 +52,8 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
-SUB_CONFIGS = ("c0", "c2", "c3", "c4")
-CHILD_TIMEOUT_S = {"c1": 240, "c0": 120, "c2": 240, "c3": 300, "c4": 300}
+SUB_CONFIGS = ("c0", "c2", "stdp", "c3", "c4")
+CHILD_TIMEOUT_S = {"c1": 240, "c0": 120, "c2": 240, "stdp": 240, "c3": 300, "c4": 300}
 
 
 def log(msg):
@@ -510,6 +512,109 @@ def c3_cpu_baseline(sw, steps, workers):
     return steps / wall, wall, float(sum(per))
 
 
+def bench_stdp(args):
+    """STDP (models/Synapses.py:232-310) on 5e7 plastic synapses: 100,000
+    Poisson pre-synaptic neurons at 20 Hz -> 50,000 post-synaptic Poisson
+    neurons at 10 Hz (their spikes do not depend on g), fixed_number_pre
+    1000 drawn on the device, weights U(0.2, 0.8), dt = 1 ms.  metric =
+    synaptic events/s: pre spikes x out-degree + post spikes x in-degree
+    (core/Utils.py:223-239).  roofline: the pre-event and post-event kernels
+    timed with HIP events inside 20 further steps each; algorithmic bytes per synapse
+    event: pre 20 + 8*s (row index, w, x, y, _last_event read and written,
+    the conductance atomic), post 24 + 6*s (state position, w, x, y,
+    _last_event), s = bytes per value.  cpu_baseline: the oracle's plastic
+    projection (orc_plastic_run) over the warm-up steps' recorded spike
+    trains, single thread; its weights are compared with the GPU's."""
+    import annarchy_amd as ah
+    from annarchy_amd import connectivity as cn
+
+    steps = args.steps if args.steps != 200 else 1000
+    warmup = args.warmup if args.warmup != 20 else 200
+    n_pre, n_post, k = 100_000, 50_000, 1000
+    prec = args.precision
+    es = 8 if prec == "double" else 4
+    pre_n = ah.Neuron(parameters="rates = 40.0", equations="p = Uniform(0.0, 1.0) * 1000.0 / dt",
+                      spike="p < rates", name="PoissonPre")
+    post_n = ah.Neuron(parameters="rates = 30.0", equations="p = Uniform(0.0, 1.0) * 1000.0 / dt\nmp = g_exc",
+                       spike="p < rates", name="PoissonPostG")
+    stdp = ah.STDP(tau_plus=20.0, tau_minus=20.0, A_plus=0.01, A_minus=0.0105, w_min=0.0, w_max=1.0)
+    t_b = time.perf_counter()
+    net = ah.Network(dt=1.0, seed=17, precision=prec)
+    pre = net.create(n_pre, pre_n)
+    post = net.create(n_post, post_n)
+    proj = net.connect(pre, post, "exc", synapse=stdp).fixed_number_pre(k, cn.Uniform(0.2, 0.8),
+                                                                         generator="device")
+    m_pre = net.monitor(pre, ["spike"], capacity=warmup + steps + 40)
+    m_post = net.monitor(post, ["spike"], capacity=warmup + steps + 40)
+    net.compile(allow_build=False)
+    pre.rates = 20.0
+    post.rates = 10.0
+    net.set_graph(True, 50)
+    log(f"stdp network ({proj.nb_synapses} plastic synapses) set up in {time.perf_counter() - t_b:.1f} s")
+    w0 = proj.w.copy()
+    net.run(warmup)
+    w_warm = proj.w.copy()
+    t0 = time.perf_counter()
+    net.run(steps)
+    el = time.perf_counter() - t0
+    pc, pi = m_pre.raw_spikes()
+    qc, qi = m_post.raw_spikes()
+    pc, qc = pc[:warmup + steps], qc[:warmup + steps]
+    pi, qi = pi[:int(pc.sum())], qi[:int(qc.sum())]
+    csr = proj.csr
+    outdeg = np.bincount(csr.col_idx, minlength=n_pre)
+    n_pw = int(pc[:warmup].sum())
+    ev_pre = int(outdeg[pi[n_pw:]].sum())
+    ev_post = int(qc[warmup:].sum()) * k
+    events = ev_pre + ev_post
+    log(f"stdp: {steps} steps in {el:.3f} s ({el / steps * 1e6:.1f} us/step), {events / steps:.3e} synapse "
+        f"events/step")
+    # kernel times inside 2 x 20 further steps (HIP events around the kernel
+    # in each step); algorithmic bytes of exactly those steps' events
+    n_done = warmup + steps
+    t_pre = proj.time_events("pre", 20)
+    t_post = proj.time_events("post", 20)
+    pc2, pi2 = m_pre.raw_spikes()
+    qc2, qi2 = m_post.raw_spikes()
+    starts = np.concatenate([[0], np.cumsum(pc2)])
+    # pre events of step u handle the pre spikes of step u - 1
+    pre_steps = range(n_done - 1, n_done + 19)
+    syn_pre = sum(int(outdeg[pi2[starts[u]:starts[u + 1]]].sum()) for u in pre_steps) / 20.0
+    spk_pre = sum(int(pc2[u]) for u in pre_steps) / 20.0
+    spk_post = float(qc2[n_done + 20:n_done + 40].mean())
+    syn_post = spk_post * k
+    b_pre = syn_pre * (20 + 8 * es) + spk_pre * 20
+    b_post = syn_post * (24 + 6 * es) + spk_post * 20
+    copy_gbs = copy_probe()
+    kern = {"pre_event": {"ms": t_pre, "bytes": b_pre, "syn_events": syn_pre},
+            "post_event": {"ms": t_post, "bytes": b_post, "syn_events": syn_post}}
+    dom = "pre_event" if t_pre >= t_post else "post_event"
+    ach = kern[dom]["bytes"] / (kern[dom]["ms"] * 1e-3) / 1e9
+    traffic = traffic_from_profiles(f"{dom}_f64_stdp5e7_traffic.txt") if prec == "double" else None
+    roof = roofline("hbm", ach, HBM_PEAK_GBS, "GB/s", traffic, dom, kernel_ms=kern[dom]["ms"],
+                    algorithmic_bytes=kern[dom]["bytes"])
+    cpu, parity = None, None
+    if not args.no_cpu and prec == "double":
+        from oracle import oracle as orc
+        n_qw = int(qc[:warmup].sum())
+        t1 = time.perf_counter()
+        ref = orc.plastic_run(0, csr.row_ptr, csr.col_idx, n_pre, w0, [20.0, 20.0, 0.01, 0.0105, 0.0, 1.0], 1.0,
+                              pc[:warmup], pi[:n_pw], qc[:warmup], qi[:n_qw])
+        cel = time.perf_counter() - t1
+        ev_w = int(outdeg[pi[:n_pw]].sum()) + n_qw * k
+        parity = float(np.max(np.abs(ref["w"] - w_warm)))
+        cpu = {"value": ev_w / cel, "unit": "synaptic events/s", "cores": 1, "kind": "port",
+               "sample": f"the {warmup} warm-up steps ({ev_w:.3e} synapse events) of the same network, "
+                         f"single thread; max |w_gpu - w_oracle| = {parity:.2e}", "host": host_cpu()}
+        log(f"stdp: oracle {ev_w / cel:.3e} events/s, max |w_gpu - w_oracle| {parity:.2e}")
+    return record("synaptic events/s", events / el, "synaptic events/s", steps, warmup, el * 1e3 / steps,
+                 "f64" if prec == "double" else "f32",
+                 {"workload": "STDP 100000 Poisson(20 Hz) -> 50000 Poisson(10 Hz), fixed_number_pre 1000 "
+                              "(5e7 plastic synapses), dt=1ms", "precision": prec},
+                 roof, cpu, kernels=kern, w_max_abs_diff_vs_oracle=parity, hbm_copy_gbs=copy_gbs,
+                 pre_spikes_per_step=float(pc[warmup:].mean()), post_spikes_per_step=float(qc[warmup:].mean()))
+
+
 def bench_c3(args):
     """Batched sweep: 256 instances, N_i ~ U{500..8000}, recurrent p = 0.05,
     shared dense 2000 x 2000 input, gains linspace(0.1, 2.0)."""
@@ -734,7 +839,7 @@ def main():
                     help="sell_panel: rows per tile (0: 2048 in FP64, 1024 in FP32)")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="all", choices=["all", "c0", "c1", "c2", "c3", "c4"],
+    ap.add_argument("--config", default="all", choices=["all", "c0", "c1", "c2", "stdp", "c3", "c4"],
                     help="all (default): c1 headline + c0, c2, c3, c4 records, each in a child process")
     ap.add_argument("--single", action="store_true", help="run one config in this process")
     ap.add_argument("--c2-steps", type=int, default=2000)
