@@ -15,6 +15,8 @@ step / step time.
 Under "configs" the line holds one record of the same shape per further
 workload, each run in a child process of its own (this process never touches
 the GPU):
+  c1_f32, c1_lognormal  the headline network in FP32 (with its psp error
+      against the float64 product) and with lognormal in-degrees;
   c0  COBA LIF, 4000 neurons, p = 0.02, 10,000 steps (parity run, bit-exact
       spike trains against the oracle);
   c2  COBA LIF, 400,000 neurons, fixed_number_pre 800 E + 200 I drawn on the
@@ -52,8 +54,9 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
-SUB_CONFIGS = ("c0", "c2", "stdp", "c3", "c4")
-CHILD_TIMEOUT_S = {"c1": 240, "c0": 120, "c2": 240, "stdp": 240, "c3": 300, "c4": 300}
+SUB_CONFIGS = ("c1_f32", "c1_lognormal", "c0", "c2", "stdp", "c3", "c4")
+CHILD_TIMEOUT_S = {"c1": 240, "c1_f32": 180, "c1_lognormal": 180, "c0": 120, "c2": 240, "stdp": 240,
+                   "c3": 300, "c4": 300}
 
 
 def log(msg):
@@ -208,6 +211,18 @@ def bench_c1(args):
     ms_step = el * 1000.0 / args.steps
     events = csr.nnz * args.steps / el
 
+    # FP32: the psp of one more step against the float64 product of the same
+    # rates (the FP32 weights widened), as a relative max-norm error
+    psp_err = None
+    if args.precision == "float":
+        import scipy.sparse as sp
+        r_in = np.asarray(pop.r, dtype=np.float64).reshape(-1)
+        net.run(1)
+        got = np.asarray(pop.sum("exc"), dtype=np.float64).reshape(-1)
+        w32 = np.asarray(csr.w, dtype=np.float32).astype(np.float64)
+        ref = sp.csr_matrix((w32, csr.col_idx, csr.row_ptr), shape=(args.neurons, args.neurons)) @ r_in
+        psp_err = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+        log(f"c1: FP32 psp error against the float64 product {psp_err:.2e}")
     copy_gbs = copy_probe()
     psp_ms = proj.time_psp(50)
     psp_bytes = proj.psp_bytes()
@@ -225,7 +240,23 @@ def bench_c1(args):
                                f"dt=1ms, {args.format} {dtype}",
                    "neurons": args.neurons, "synapses": csr.nnz, "format": proj.get_format(),
                    "graph_steps": G, "parallelism": "single GPU"},
-                  roof, cpu, hbm_gbs_step=psp_bytes / (ms_step * 1e-3) / 1e9, hbm_copy_gbs=copy_gbs)
+                  roof, cpu, hbm_gbs_step=psp_bytes / (ms_step * 1e-3) / 1e9, hbm_copy_gbs=copy_gbs,
+                  psp_max_rel_error_vs_fp64=psp_err)
+
+
+def bench_c1_f32(args):
+    """The headline network in FP32 (Kahan-compensated sums, 1024-row tiles),
+    with the psp's error against the float64 product stated."""
+    args.precision = "float"
+    args.cpu_seconds = min(args.cpu_seconds, 3.0)
+    return bench_c1(args)
+
+
+def bench_c1_lognormal(args):
+    """The headline network with lognormal in-degrees (mean 1000): uneven rows."""
+    args.lognormal = True
+    args.cpu_seconds = min(args.cpu_seconds, 3.0)
+    return bench_c1(args)
 
 
 # ---------------------------------------------------------------------------
@@ -839,7 +870,7 @@ def main():
                     help="sell_panel: rows per tile (0: 2048 in FP64, 1024 in FP32)")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="all", choices=["all", "c0", "c1", "c2", "stdp", "c3", "c4"],
+    ap.add_argument("--config", default="all", choices=["all", "c0", "c1", "c1_f32", "c1_lognormal", "c2", "stdp", "c3", "c4"],
                     help="all (default): c1 headline + c0, c2, c3, c4 records, each in a child process")
     ap.add_argument("--single", action="store_true", help="run one config in this process")
     ap.add_argument("--c2-steps", type=int, default=2000)
