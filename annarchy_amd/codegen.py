@@ -492,6 +492,11 @@ def _c_name(name: str) -> str:
     return re.sub(r"\W", "_", name)
 
 
+def _u(name: str) -> str:
+    """The C identifier of a model's attribute in the generated kernels."""
+    return "u_" + _c_name(name)
+
+
 _STOP_KERNEL = r"""
 namespace {
 // stop_condition of the population (Population/OpenMPGenerator.py:488-555):
@@ -557,23 +562,42 @@ def generate(desc: NeuronDescription, stop_condition: str | None = None) -> Mode
             slots.append((v.pre_loop["name"], False, 0.0, 3))
     slot_of = {s[0]: k for k, s in enumerate(slots)}
 
+    # user identifiers are emitted with a prefix (u_v, u_tau, ...), so no
+    # model name can collide with the kernels' own identifiers (a, i, t, k,
+    # N, ...); the parser's C text marks each of them with an index
+    # placeholder, the pre-loop step sizes aside
+    user_names = sorted({s[0] for s in slots} | {r.name for r in desc.randoms}, key=len, reverse=True)
+    bare_names = sorted({pl["name"] for pl in preloops}, key=len, reverse=True)
+    name_re = re.compile(r"(?<![\w.])(" + "|".join(re.escape(n) for n in user_names) + r")(?=%\()") \
+        if user_names else None
+    bare_re = re.compile(r"(?<![\w.])(" + "|".join(re.escape(n) for n in bare_names) + r")(?![\w])") \
+        if bare_names else None
+
+    def mangle(code: str) -> str:
+        if name_re is not None:
+            code = name_re.sub(lambda m: _u(m.group(1)), code)
+        if bare_re is not None:
+            code = bare_re.sub(lambda m: _u(m.group(1)), code)
+        return code
+
     refractory = desc.refractory if desc.spiking else None
     has_ref = refractory is not None and refractory != 0 and refractory != 0.0
 
     # -- population-wide attributes -----------------------------------------
     global_names = [s[0] for s in slots if not s[1]]
     global_loads = "\n".join(
-        f"    const T {_c_name(n)} = ((const T*)a.slot[{slot_of[n]}])[0];" for n in global_names)
+        f"    const T {_u(n)} = ((const T*)a.slot[{slot_of[n]}])[0];" for n in global_names)
     global_refs = "\n".join(
-        f"    T& {_c_name(n)} = ((T*)a.slot[{slot_of[n]}])[0];" for n in global_names)
+        f"    T& {_u(n)} = ((T*)a.slot[{slot_of[n]}])[0];" for n in global_names)
     local_names = [s[0] for s in slots if s[1]]
     local_ptrs = "\n".join(
-        f"    T* __restrict__ {_c_name(n)} = (T*)a.slot[{slot_of[n]}];" for n in local_names)
+        f"    T* __restrict__ {_u(n)} = (T*)a.slot[{slot_of[n]}];" for n in local_names)
 
     def subst(code: str) -> str:
+        code = mangle(code)
         for r in desc.randoms:
-            code = code.replace(r.name + "%(local_index)s", r.name)
-            code = code.replace(r.name + "%(global_index)s", r.name)
+            code = code.replace(_u(r.name) + "%(local_index)s", _u(r.name))
+            code = code.replace(_u(r.name) + "%(global_index)s", _u(r.name))
         code = code.replace("in_ref%(local_index)s", "in_ref")
         code = code.replace("%(local_index)s", "[i]").replace("%(global_index)s", "")
         code = code.replace("%(semiglobal_index)s", "")
@@ -585,7 +609,7 @@ def generate(desc: NeuronDescription, stop_condition: str | None = None) -> Mode
             if r.locality != locality:
                 continue
             a0, a1 = r.args
-            out.append(f"const T {r.name} = ahip::random_value<T>(a, {idx}, {k}, t, "
+            out.append(f"const T {_u(r.name)} = ahip::random_value<T>(a, {idx}, {k}, t, "
                        f"{DISTRIBUTIONS[r.dist]}, {a0!r}, {a1!r});")
         return "\n".join(out)
 
@@ -596,11 +620,11 @@ def generate(desc: NeuronDescription, stop_condition: str | None = None) -> Mode
     # divisions of the neuron equations by a population-wide value: the
     # reciprocal steps once per kernel (ahip::DivConst)
     div_names = [n for n in global_names
-                 if re.search(r"(?<!/)/\s*" + re.escape(_c_name(n)) + r"(?![\w\[(])", local_eqs)]
+                 if re.search(r"(?<!/)/\s*" + re.escape(_u(n)) + r"(?![\w\[(])", local_eqs)]
     for n in div_names:
-        local_eqs = re.sub(r"(?<!/)(/\s*)" + re.escape(_c_name(n)) + r"(?![\w\[(])", r"\g<1>" + _c_name(n) + "__dc",
+        local_eqs = re.sub(r"(?<!/)(/\s*)" + re.escape(_u(n)) + r"(?![\w\[(])", r"\g<1>" + _u(n) + "__dc",
                            local_eqs)
-    div_consts = "\n".join(f"    const ahip::DivConst<T> {_c_name(n)}__dc = ahip::div_const<T>({_c_name(n)});"
+    div_consts = "\n".join(f"    const ahip::DivConst<T> {_u(n)}__dc = ahip::div_const<T>({_u(n)});"
                             for n in div_names)
 
     refrac_in = "const int in_ref = (a.refractory_remaining[i] > 0) ? 0 : 1;" if has_ref else ""
@@ -611,7 +635,7 @@ def generate(desc: NeuronDescription, stop_condition: str | None = None) -> Mode
         if isinstance(refractory, str):
             rvar = refractory
             idx = "[i]" if any(s[0] == rvar and s[1] for s in slots) else ""
-            refrac_set = f"a.refractory_remaining[i] = int({rvar}{idx}/dt);"
+            refrac_set = f"a.refractory_remaining[i] = int({_u(rvar)}{idx}/dt);"
         elif has_ref:
             refrac_set = "a.refractory_remaining[i] = a.refractory[i];"
         else:
@@ -632,9 +656,9 @@ def generate(desc: NeuronDescription, stop_condition: str | None = None) -> Mode
     if preloops:
         lines = ["void preloop(const double* g, double dt, double* out) {"]
         for n in global_names:
-            lines.append(f"    const double {_c_name(n)} = g[{slot_of[n]}]; (void){_c_name(n)};")
+            lines.append(f"    const double {_u(n)} = g[{slot_of[n]}]; (void){_u(n)};")
         for pl in preloops:
-            val = pl["value"].replace("%(global_index)s", "").replace("%(local_index)s", "")
+            val = mangle(pl["value"]).replace("%(global_index)s", "").replace("%(local_index)s", "")
             lines.append(f"    out[{slot_of[pl['name']]}] = {val};")
         lines.append("}")
         preloop_fn = "\n".join(lines)
@@ -644,32 +668,34 @@ def generate(desc: NeuronDescription, stop_condition: str | None = None) -> Mode
         preloop_ptr = "nullptr"
 
     # persistent steps: spiking models without population-wide equations.
-    # LDS holds the local slots the statements touch and the conductances
-    # (g_<target>), in slot order; the statements index them with i as the
-    # per-kernel step does.
+    # LDS holds the local slots the statements write and the conductances
+    # (g_<target>), in slot order; local slots they only read stay in global
+    # memory (constant during the launch, L2-resident).  The statements see
+    # the neuron's values in registers: loaded before them, stored after
+    # them when written.
     if desc.spiking and not has_global:
-        stmts = "\n".join([refrac_in, rand_code("local", "i"), local_eqs, refrac_dec, gather])
-        pslots = [n for n in local_names
-                  if n.startswith("g_") or re.search(r"(?<![\w.])" + re.escape(_c_name(n)) + r"\[i\]", stmts)]
-        persist_ptrs_code = "\n".join(
-            f"    T* __restrict__ {_c_name(n)} = lds + {k} * (size_t)N;" for k, n in enumerate(pslots))
         rr = lambda c: c.replace("a.refractory_remaining", "refrac_rem").replace("a.refractory[i]", "AHIP_REFRACTORY_I")
         parts = {"refrac_in": rr(refrac_in), "local_rand": rand_code("local", "i"), "local_eqs": rr(local_eqs),
                  "refrac_dec": rr(refrac_dec), "gather": rr(gather)}
-        # the neuron's LDS values in registers for the statements: loaded
-        # before them, stored after them when written (one read and at most
-        # one write of each per neuron and step)
         joined = "\n".join(parts.values())
+        ref_pat = lambda c: r"(?<![\w.])" + re.escape(c) + r"\[i\]"
+        writes = lambda c: re.search(ref_pat(c) + r"\s*(?:[-+*/]?=(?!=)|\+\+|--)", joined) is not None
+        reads = lambda c: re.search(ref_pat(c), joined) is not None
+        pslots = [n for n in local_names if n.startswith("g_") or writes(_u(n))]
+        ro_slots = [n for n in local_names if n not in pslots and reads(_u(n))]
+        persist_ptrs_code = "\n".join(
+            [f"    T* __restrict__ {_u(n)} = lds + {k} * (size_t)N;" for k, n in enumerate(pslots)] +
+            [f"    const T* __restrict__ {_u(n)} = (const T*)a.slot[{slot_of[n]}];" for n in ro_slots])
         reg_in, reg_out = [], []
-        for c, ty in [(_c_name(n), "T") for n in pslots] + ([("refrac_rem", "int32_t")] if has_ref else []):
-            pat = r"(?<![\w.])" + re.escape(c) + r"\[i\]"
-            if not re.search(pat, joined):
+        for c, ty in ([(_u(n), "T") for n in pslots + ro_slots] +
+                      ([("refrac_rem", "int32_t")] if has_ref else [])):
+            if not reads(c):
                 continue
             reg_in.append(f"            {ty} {c}__r = {c}[i];")
-            if re.search(pat + r"\s*(?:[-+*/]?=(?!=)|\+\+|--)", joined):
+            if writes(c):
                 reg_out.append(f"            {c}[i] = {c}__r;")
             for key in parts:
-                parts[key] = re.sub(pat, c + "__r", parts[key])
+                parts[key] = re.sub(ref_pat(c), c + "__r", parts[key])
         persist_fn = _PERSIST_KERNEL % {
             "n_pslots": len(pslots),
             "p_refr": "true" if has_ref else "false",

@@ -39,12 +39,13 @@ def _line(equations, needle):
 
 
 def test_ite_code():
-    assert _line("r = if (t is 1): 1.0 else: 0.0", "r[i] =") == "r[i] = ((double(t)*dt) == 1 ? 1.0 : 0.0);"
-    assert _line("dr/dt = if t > 5: 1 else: 0", "_r =") == "double _r = ((double(t)*dt) > 5 ? 1 : 0);"
+    # the kernels name a model's attributes u_<name>
+    assert _line("r = if (t is 1): 1.0 else: 0.0", "u_r[i] =") == "u_r[i] = ((double(t)*dt) == 1 ? 1.0 : 0.0);"
+    assert _line("dr/dt = if t > 5: 1 else: 0", "double _r =") == "double _r = ((double(t)*dt) > 5 ? 1 : 0);"
 
 
 def test_logical_operators():
-    code = _line("dr/dt = ite((t > 5) and (t < 20), 1, 0)", "_r =")
+    code = _line("dr/dt = ite((t > 5) and (t < 20), 1, 0)", "double _r =")
     assert "&&" in code and "< 20" in code
-    code = _line("r = ite((t > 5) or not (t > 8), 1, 0)", "r[i] =")
+    code = _line("r = ite((t > 5) or not (t > 8), 1, 0)", "u_r[i] =")
     assert "||" in code and "<= 8" in code
