@@ -28,7 +28,7 @@ import re
 
 from .equations import DISTRIBUTIONS, NeuronDescription, analyse_neuron, equation_code
 
-ABI = 8  # AHIP_MODEL_ABI in include/annarchy_hip_model.h
+ABI = 9  # AHIP_MODEL_ABI in include/annarchy_hip_model.h
 
 _PRELUDE = r"""// Generated neuron-model library for the MI355X simulation backend.
 // Model: %(name)s

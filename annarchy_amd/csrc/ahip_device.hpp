@@ -171,29 +171,77 @@ __device__ inline void publish_spikes(const ahip_pop_args& a, bool spk) {
     }
 }
 
+// ---- global-memory views -------------------------------------------------
+// A pointer read from a table in memory is generic to the compiler, which
+// then emits flat instructions (counted against the LDS wait counter as
+// well); these views of pointers known to address device memory give
+// global ones.
+template <typename V>
+using gptr = __attribute__((address_space(1))) V*;
+template <typename V>
+__device__ inline gptr<V> as_global(V* p) {
+    return (gptr<V>)p;
+}
+template <typename V>
+__device__ inline gptr<const V> as_global(const V* p) {
+    return (gptr<const V>)p;
+}
+
 // ---- ordered spike delivery ----------------------------------------------
-// A post neuron's events of one step are keys of a bucket; adding their
-// weights in ascending key (projection by projection in creation order, in
-// ascending pre-synaptic rank) reproduces every rounding of the reference's
-// CSRC loop g[post] += w (generator/Projection/OpenMP/CSR.py:870-885).
+// A post neuron's events of one step are records (key, weight) of a bucket;
+// adding their weights in ascending key (projection by projection in
+// creation order, in ascending pre-synaptic rank) reproduces every rounding
+// of the reference's CSRC loop g[post] += w
+// (generator/Projection/OpenMP/CSR.py:870-885).  One record is one store of
+// the scatter and one load of the delivery.
+template <typename T>
+struct BucketRec;
+template <>
+struct alignas(16) BucketRec<double> {
+    uint32_t key, pad;
+    double w;
+};
+template <>
+struct alignas(8) BucketRec<float> {
+    uint32_t key;
+    float w;
+};
+
+static_assert(sizeof(BucketRec<double>) == 16 && sizeof(BucketRec<float>) == 8,
+              "the runtime sizes the buckets by these records");
+
+// record `at` of a bucket in device memory: one 16-byte (FP64) or 8-byte
+// (FP32) store
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+__device__ inline void store_rec(BucketRec<double>* bucket, int64_t at, uint32_t key, double w) {
+    const uint64_t bits = (uint64_t)__double_as_longlong(w);
+    const u32x4_t v = {key, 0u, (uint32_t)bits, (uint32_t)(bits >> 32)};
+    ((gptr<u32x4_t>)(reinterpret_cast<u32x4_t*>(bucket)))[at] = v;
+}
+__device__ inline void store_rec(BucketRec<float>* bucket, int64_t at, uint32_t key, float w) {
+    const u32x2_t v = {key, (uint32_t)__float_as_uint(w)};
+    ((gptr<u32x2_t>)(reinterpret_cast<u32x2_t*>(bucket)))[at] = v;
+}
+
 constexpr int kDeliverSort = 4;
 
-// acc + the weights w[b .. b + L) in ascending key[b .. b + L).  Up to 4
-// (key, weight) pairs are sorted in registers by a bitonic network on the
-// keys (a neuron rarely receives more events in one step, and a small
-// network keeps the update kernel's registers, hence its occupancy, low);
-// a longer bucket is walked by repeated minimum, O(L^2) reads of the keys.
+// acc + the weights of records rec[b .. b + L) in ascending key.  Up to 4
+// records are sorted in registers by a bitonic network on the keys (a
+// neuron rarely receives more events in one step, and a small network keeps
+// the update kernel's registers, hence its occupancy, low); a longer bucket
+// is walked by repeated minimum, O(L^2) reads of the keys.
 template <typename T>
-__device__ inline T ordered_sum(T acc, const uint32_t* __restrict__ key, const T* __restrict__ w, int64_t b,
-                                int32_t L) {
-    if (L == 1) return acc + w[b];
+__device__ inline T ordered_sum(T acc, const BucketRec<T>* __restrict__ rec, int64_t b, int32_t L) {
+    if (L == 1) return acc + rec[b].w;
     if (L <= kDeliverSort) {
         uint32_t k[kDeliverSort];
         T v[kDeliverSort];
 #pragma unroll
         for (int m = 0; m < kDeliverSort; m++) {
-            k[m] = m < L ? key[b + m] : 0xffffffffu;
-            v[m] = m < L ? w[b + m] : T(0);
+            const BucketRec<T> r = m < L ? rec[b + m] : BucketRec<T>{};
+            k[m] = m < L ? r.key : 0xffffffffu;
+            v[m] = m < L ? r.w : T(0);
         }
 #pragma unroll
         for (int s = 2; s <= kDeliverSort; s <<= 1)
@@ -223,13 +271,13 @@ __device__ inline T ordered_sum(T acc, const uint32_t* __restrict__ key, const T
         uint32_t best = 0xffffffffu;
         int64_t at = b;
         for (int32_t q = 0; q < L; q++) {
-            const uint32_t x = key[b + q];
+            const uint32_t x = rec[b + q].key;
             if ((m == 0 || x > last) && x < best) {
                 best = x;
                 at = b + q;
             }
         }
-        acc += w[at];
+        acc += rec[at].w;
         last = best;
     }
     return acc;
@@ -258,7 +306,7 @@ __device__ inline void deliver_buckets(const ahip_pop_args& a, int32_t i) {
         const int32_t L = (int32_t)min(c, (int64_t)(hn[k] >> AHIP_HEAD_SHIFT) - b);
         if (L <= 0) continue;
         T* g = static_cast<T*>(d.g);
-        g[i] = ordered_sum<T>(g[i], d.bucket, static_cast<const T*>(d.bucket_w), b, L);
+        g[i] = ordered_sum<T>(g[i], static_cast<const BucketRec<T>*>(d.bucket), b, L);
     }
 }
 

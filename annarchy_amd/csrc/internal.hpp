@@ -182,9 +182,8 @@ struct ScatterDesc {
     // appended as key gid_off + j to the bucket of its post neuron instead of
     // being added to g; deliver_ordered adds the bucket in ascending keys
     uint64_t* head;           // post neuron i: (bucket base << AHIP_HEAD_SHIFT) | events this step
-    int64_t cap;              // bucket entries
-    uint32_t* bucket;
-    void* bucket_w;           // the events' weights beside their keys
+    int64_t cap;              // bucket records
+    void* bucket;             // records: key and weight (BucketRec of the precision)
     // post neurons with events this step, sharded by post & (shards - 1):
     // shard s lists its neurons at touched[s * shard_cap ..] and counts them
     // in scnt[s * kShardStride] (the delivery clears it)
@@ -201,8 +200,7 @@ constexpr int kMaxScatter = 8;
 struct DeliverTarget {
     void* g;
     uint64_t* head;
-    const uint32_t* bucket;
-    const void* bucket_w;
+    const void* bucket;       // BucketRec records
     const int32_t* touched;
     int32_t* scnt;
     int32_t shards, shard_cap;
@@ -224,8 +222,7 @@ struct OrderedTarget {
     std::vector<int32_t> projs;   // projection ids, creation order
     std::vector<int64_t> key_off; // key offset of each projection (its keys: key_off + column-order index)
     DevBuf head;                  // uint64[size + 1]: (bucket base << AHIP_HEAD_SHIFT) | count, count 0 between steps
-    DevBuf bucket;                // uint32[cap] keys
-    DevBuf bucket_w;              // [cap] weights, network precision
+    DevBuf bucket;                // [cap] records (key, weight): 16 B in FP64, 8 B in FP32
     int32_t shards = 1, shard_cap = 0;
     DevBuf touched;               // int32[shards * shard_cap]
     DevBuf scnt;                  // int32[shards * kShardStride]

@@ -216,8 +216,10 @@ __global__ __launch_bounds__(256) void scatter_multi(const ScatterDesc* __restri
         const int32_t c = (int32_t)(local - s * d.chunks);
         const int32_t pre = d.list[list_slot(d, t) * d.list_stride + s];
         if (pre < 0 || pre >= d.n_cols) continue;
-        const int64_t b = d.col_ptr[pre], e = d.col_ptr[pre + 1];
+        const int64_t b = as_global(d.col_ptr)[pre], e = as_global(d.col_ptr)[pre + 1];
+        const auto ri = as_global(d.row_idx);
         const T* wc = static_cast<const T*>(d.w_col);
+        const auto wg = as_global(wc);
         const T wk = (T)d.w_const;
         if (d.head) {
             // ordered delivery: append (key gid_off + j, weight) to the post
@@ -230,22 +232,22 @@ __global__ __launch_bounds__(256) void scatter_multi(const ScatterDesc* __restri
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const int64_t j = base + lane + u * kWave;
-                    post[u] = j < e ? d.row_idx[j] : -1;
-                    w[u] = wc == nullptr ? wk : (j < e ? wc[j] : T(0));
+                    post[u] = j < e ? ri[j] : -1;
+                    w[u] = wc == nullptr ? wk : (j < e ? wg[j] : T(0));
                 }
 #pragma unroll
                 for (int u = 0; u < 4; u++)
                     if (post[u] >= 0) {
                         // one atomic returns both the bucket base and the slot
-                        const uint64_t old = atomicAdd((unsigned long long*)(d.head + post[u]), 1ull);
+                        const uint64_t old = atomicAdd((unsigned long long*)as_global(d.head + post[u]), 1ull);
                         slot[u] = (int64_t)(old & AHIP_HEAD_COUNT);
                         at[u] = (int64_t)(old >> AHIP_HEAD_SHIFT) + slot[u];
                     }
 #pragma unroll
                 for (int u = 0; u < 4; u++)
                     if (post[u] >= 0 && at[u] < d.cap) {
-                        d.bucket[at[u]] = d.gid_off + (uint32_t)(base + lane + u * kWave);
-                        static_cast<T*>(d.bucket_w)[at[u]] = w[u];
+                        store_rec(static_cast<BucketRec<T>*>(d.bucket), at[u],
+                                  d.gid_off + (uint32_t)(base + lane + u * kWave), w[u]);
                         // the first event of a post neuron in this step lists it in its
                         // shard (targets delivered by their own launch)
                         if (slot[u] == 0 && d.touched) {
@@ -264,12 +266,12 @@ __global__ __launch_bounds__(256) void scatter_multi(const ScatterDesc* __restri
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int64_t j = base + lane + u * kWave;
-                post[u] = j < e ? d.row_idx[j] : -1;
-                v[u] = wc == nullptr ? wk : (j < e ? wc[j] : T(0));
+                post[u] = j < e ? ri[j] : -1;
+                v[u] = wc == nullptr ? wk : (j < e ? wg[j] : T(0));
             }
 #pragma unroll
             for (int u = 0; u < 4; u++)
-                if (post[u] >= 0) unsafeAtomicAdd(g + post[u], v[u]);
+                if (post[u] >= 0) unsafeAtomicAdd((T*)as_global(g + post[u]), v[u]);
         }
     }
 }
@@ -286,7 +288,7 @@ __device__ inline void deliver_one(const DeliverTarget& d, int32_t i) {
     const int32_t L = (int32_t)min((int64_t)(h & AHIP_HEAD_COUNT), (int64_t)(hn >> AHIP_HEAD_SHIFT) - b);
     if (L <= 0) return;
     T* g = static_cast<T*>(d.g);
-    g[i] = ordered_sum<T>(g[i], d.bucket, static_cast<const T*>(d.bucket_w), b, L);
+    g[i] = ordered_sum<T>(g[i], static_cast<const BucketRec<T>*>(d.bucket), b, L);
 }
 
 template <typename T>

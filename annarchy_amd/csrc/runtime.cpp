@@ -425,14 +425,16 @@ int build_ordered_targets(ahip_net* net, const std::vector<const void*>& targets
             return fail(AHIP_EINVAL, "ordered spike delivery counts events in %d bits: in-degree %lld into one "
                         "conductance (use AHIP_DELIVERY_ATOMIC)", AHIP_HEAD_SHIFT, (long long)max_in);
         for (int32_t i = 0; i < o.size; i++) boff[i + 1] += boff[i];
+        // bucket records: key and weight, 16 B in FP64 and 8 B in FP32
+        // (ahip::BucketRec of the precision)
+        const size_t rec = net->precision == AHIP_F64 ? 16 : 8;
         // head[i] = bucket base << AHIP_HEAD_SHIFT, event count 0
         std::vector<uint64_t> head(o.size + 1);
         for (int32_t i = 0; i <= o.size; i++) head[i] = (uint64_t)boff[i] << AHIP_HEAD_SHIFT;
         for (OrderedTarget& q : old)
             if (q.g == g) {
                 if (q.head.bytes >= head.size() * sizeof(uint64_t)) o.head = std::move(q.head);
-                if (q.bucket.bytes >= (size_t)o.cap * sizeof(uint32_t)) o.bucket = std::move(q.bucket);
-                if (q.bucket_w.bytes >= (size_t)o.cap * net->elem()) o.bucket_w = std::move(q.bucket_w);
+                if (q.bucket.bytes >= (size_t)o.cap * rec) o.bucket = std::move(q.bucket);
                 if (q.touched.bytes >= (size_t)o.size * sizeof(int32_t)) o.touched = std::move(q.touched);
             }
         // shards of at most 64 neurons each (one wave), a power of two up to kMaxShards
@@ -441,8 +443,7 @@ int build_ordered_targets(ahip_net* net, const std::vector<const void*>& targets
         o.shard_cap = (o.size + o.shards - 1) / o.shards;
         int rc;
         if (!o.head.p && (rc = dev_alloc(o.head, head.size() * sizeof(uint64_t)))) return rc;
-        if (!o.bucket.p && (rc = dev_alloc(o.bucket, (size_t)o.cap * sizeof(uint32_t)))) return rc;
-        if (!o.bucket_w.p && (rc = dev_alloc(o.bucket_w, (size_t)o.cap * net->elem()))) return rc;
+        if (!o.bucket.p && (rc = dev_alloc(o.bucket, (size_t)o.cap * rec))) return rc;
         const size_t touched_n = (size_t)o.shards * o.shard_cap;
         if (o.touched.bytes < touched_n * sizeof(int32_t) && (rc = dev_alloc(o.touched, touched_n * sizeof(int32_t))))
             return rc;
@@ -478,8 +479,7 @@ int build_ordered_targets(ahip_net* net, const std::vector<const void*>& targets
             std::memset(&d, 0, sizeof(d));
             d.g = o.g;
             d.head = o.head.as<uint64_t>();
-            d.bucket = o.bucket.as<uint32_t>();
-            d.bucket_w = o.bucket_w.p;
+            d.bucket = o.bucket.p;
             dd.push_back(d);
         }
         p.n_deliver = (int32_t)dd.size();
@@ -499,8 +499,7 @@ int build_ordered_targets(ahip_net* net, const std::vector<const void*>& targets
         DeliverTarget d;
         d.g = o.g;
         d.head = o.head.as<uint64_t>();
-        d.bucket = o.bucket.as<uint32_t>();
-        d.bucket_w = o.bucket_w.p;
+        d.bucket = o.bucket.p;
         d.touched = o.touched.as<int32_t>();
         d.scnt = o.scnt.as<int32_t>();
         d.shards = o.shards;
@@ -604,7 +603,6 @@ int prepare(ahip_net* net) {
             d.head = nullptr;
             d.cap = 0;
             d.bucket = nullptr;
-            d.bucket_w = nullptr;
             d.touched = nullptr;
             d.scnt = nullptr;
             d.shards = 1;
@@ -617,8 +615,7 @@ int prepare(ahip_net* net) {
                     if (o.projs[q] == p.id) off = (uint32_t)o.key_off[q];
                 d.head = o.head.as<uint64_t>();
                 d.cap = o.cap;
-                d.bucket = o.bucket.as<uint32_t>();
-                d.bucket_w = o.bucket_w.p;
+                d.bucket = o.bucket.p;
                 // targets delivered by the post update need no touched list
                 d.touched = o.in_update ? nullptr : o.touched.as<int32_t>();
                 d.scnt = o.scnt.as<int32_t>();

@@ -22,24 +22,26 @@
 extern "C" {
 #endif
 
-#define AHIP_MODEL_ABI 8
+#define AHIP_MODEL_ABI 9
 #define AHIP_MAX_SLOTS 48
 #define AHIP_MAX_RAND 8
 #define AHIP_UPDATE_BLOCK 256 /* threads per block of every update kernel */
 
 /* Ordered spike delivery folded into the update of the post population.
  * head[i] = (base << AHIP_HEAD_SHIFT) | count: neuron i's events of this
- * step are bucket entries base .. base + count (head[i + 1] >> SHIFT bounds
- * the range), each a key (the projection's key offset + the synapse's
- * column-order index, so keys sort projection by projection and in
- * ascending pre-synaptic rank: the reference's summation order) and its
- * weight (bucket_w, in the network precision).  The weights are added to
- * g[i] in ascending key before the equations read g; the count is cleared. */
+ * step are bucket records base .. base + count (head[i + 1] >> SHIFT bounds
+ * the range).  A record is a key (the projection's key offset + the
+ * synapse's column-order index, so keys sort projection by projection and in
+ * ascending pre-synaptic rank: the reference's summation order) and the
+ * event's weight: uint32 key, uint32 padding, double weight in FP64 (16
+ * bytes); uint32 key, float weight in FP32 (8 bytes).  The weights are added
+ * to g[i] in ascending key before the equations read g; the count is
+ * cleared. */
 typedef struct ahip_deliver_desc {
     void* g;
     uint64_t* head;                     /* size + 1 words */
-    const uint32_t* bucket;
-    const void* bucket_w;
+    const void* bucket;                 /* records */
+    const void* reserved;
 } ahip_deliver_desc;
 #define AHIP_HEAD_SHIFT 24
 #define AHIP_HEAD_COUNT ((1ull << AHIP_HEAD_SHIFT) - 1ull)
