@@ -21,6 +21,7 @@ the GPU):
       spike trains against the oracle);
   c2  COBA LIF, 400,000 neurons, fixed_number_pre 800 E + 200 I drawn on the
       device, 4e8 synapses with per-synapse weights, Poisson 5 Hz into 10%;
+  c2_f32  the same in FP32, its population rate and ISI CV against FP64;
   stdp  STDP on 5e7 plastic synapses (100,000 Poisson pre neurons at 20 Hz
       -> 50,000 Poisson post neurons at 10 Hz, fixed_number_pre 1000);
   c3  256 rate-network instances of uneven size in one batched launch, plain
@@ -54,9 +55,9 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
-SUB_CONFIGS = ("c1_f32", "c1_lognormal", "c0", "c2", "stdp", "c3", "c4")
-CHILD_TIMEOUT_S = {"c1": 240, "c1_f32": 180, "c1_lognormal": 180, "c0": 120, "c2": 240, "stdp": 240,
-                   "c3": 300, "c4": 300}
+SUB_CONFIGS = ("c1_f32", "c1_lognormal", "c0", "c2", "c2_f32", "stdp", "c3", "c4")
+CHILD_TIMEOUT_S = {"c1": 240, "c1_f32": 180, "c1_lognormal": 180, "c0": 120, "c2": 240, "c2_f32": 240,
+                   "stdp": 240, "c3": 300, "c4": 300}
 
 
 def log(msg):
@@ -478,6 +479,24 @@ def bench_c2(args):
     traffic = traffic_from_profiles(f"{kname}_{'f64' if prec == 'double' else 'f32'}_c2_coba400k_traffic.txt")
     roof = roofline("hbm", ach, HBM_PEAK_GBS, "GB/s", traffic, dom, kernel_ms=kern[dom]["ms"],
                     algorithmic_bytes=kern[dom]["bytes"])
+    fp32_vs_fp64 = None
+    if prec == "float":
+        # the same seeded network in FP64 (connectivity, initial state and
+        # Poisson draws are the same streams): population rate and mean ISI
+        # CV over the same steps
+        rate32, cv32 = _rate_and_cv(c_t, idx_t, N, 0.1)
+        del net, P, Q, mon
+        net64, P64, _, _, mon64 = _coba(N, ne, conn, steps + warmup + 200, poisson=(N // 10, 5.0, 0.6),
+                                        precision="double")
+        net64.set_graph(True, G)
+        net64.run(warmup + steps)
+        c64, i64 = mon64.raw_spikes()
+        rate64, cv64 = _rate_and_cv(c64[warmup:warmup + steps],
+                                    i64[int(c64[:warmup].sum()):int(c64[:warmup + steps].sum())], N, 0.1)
+        fp32_vs_fp64 = {"rate_hz": [rate32, rate64], "rate_rel_diff": abs(rate32 - rate64) / rate64,
+                        "isi_cv": [cv32, cv64], "isi_cv_rel_diff": abs(cv32 - cv64) / cv64}
+        log(f"c2: FP32 rate {rate32:.3f} Hz / FP64 {rate64:.3f} Hz, ISI CV {cv32:.4f} / {cv64:.4f}")
+        del net64, P64, mon64
     cpu = None
     if not args.no_cpu and prec == "double":
         from oracle import oracle as orc
@@ -503,7 +522,35 @@ def bench_c2(args):
                    "delivery": "ordered (the reference's summation order)", "graph_steps": G,
                    "connectivity": "drawn on the device (Philox stream)"},
                   roof, cpu, kernels=kern, spikes_per_step=float(c_t.mean()),
-                  mean_rate_hz=float(c_t.sum() / N / (steps * 1e-4)))
+                  mean_rate_hz=float(c_t.sum() / N / (steps * 1e-4)), fp32_vs_fp64=fp32_vs_fp64)
+
+
+def bench_c2_f32(args):
+    """C2 in FP32, with its population rate and mean ISI CV against an FP64
+    run of the same seeded network."""
+    args.precision = "float"
+    return bench_c2(args)
+
+
+def _rate_and_cv(counts, idx, n, dt_ms):
+    """Population rate (Hz) and mean ISI coefficient of variation over the
+    neurons with at least two intervals."""
+    counts = np.asarray(counts, dtype=np.int64)
+    idx = np.asarray(idx, dtype=np.int64)
+    rate = float(counts.sum()) / n / (len(counts) * dt_ms * 1e-3)
+    steps = np.repeat(np.arange(len(counts)), counts)
+    order = np.lexsort((steps, idx))
+    nn, tt = idx[order], steps[order]
+    same = np.diff(nn) == 0
+    isi = np.diff(tt)[same].astype(np.float64)
+    who = nn[1:][same]
+    cnt = np.bincount(who, minlength=n)
+    s1 = np.bincount(who, weights=isi, minlength=n)
+    s2 = np.bincount(who, weights=isi * isi, minlength=n)
+    ok = cnt >= 2
+    mean = s1[ok] / cnt[ok]
+    var = np.maximum(s2[ok] / cnt[ok] - mean * mean, 0.0)
+    return rate, float(np.mean(np.sqrt(var) / mean)) if ok.any() else float("nan")
 
 
 # ---------------------------------------------------------------------------
@@ -870,10 +917,10 @@ def main():
                     help="sell_panel: rows per tile (0: 2048 in FP64, 1024 in FP32)")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="all", choices=["all", "c0", "c1", "c1_f32", "c1_lognormal", "c2", "stdp", "c3", "c4"],
+    ap.add_argument("--config", default="all", choices=["all", "c0", "c1", "c1_f32", "c1_lognormal", "c2", "c2_f32", "stdp", "c3", "c4"],
                     help="all (default): c1 headline + c0, c2, c3, c4 records, each in a child process")
     ap.add_argument("--single", action="store_true", help="run one config in this process")
-    ap.add_argument("--c2-steps", type=int, default=2000)
+    ap.add_argument("--c2-steps", type=int, default=10000)
     ap.add_argument("--c3-steps", type=int, default=2000)
     ap.add_argument("--no-plain", action="store_true",
                     help="c3: skip the plain per-instance FP64 networks (and the error and speedup they give)")
