@@ -72,9 +72,20 @@ __global__ __launch_bounds__(kCompactThreads) void compact_spikes(int32_t size,
     __shared__ int32_t lds_base[kCompactThreads / kWave];
     const int32_t chunk0 = blockIdx.x * kCompactChunk;
     const int32_t ub0 = chunk0 / AHIP_UPDATE_BLOCK;  // first update block of the chunk
-    // base = sum(bcnt[0..ub0))
+    // every global load first (they do not depend on one another): this
+    // thread's mask word, its share of the block counts before the chunk,
+    // the monitor's fill count
+    const int32_t n0 = chunk0 + threadIdx.x * kCompactPerThread;
+    const uint64_t word = n0 < size ? mask[n0 / 64] : 0ull;
     int32_t part = 0;
     for (int32_t k = threadIdx.x; k < ub0; k += kCompactThreads) part += bcnt[k];
+    int64_t slot = -1, f = 0;
+    if (rec.counts) {
+        slot = *rec.t_base + rec.step - rec.t_start;
+        if (slot >= 0 && slot < rec.capacity) f = rec.fill[slot & 1];
+        else slot = -1;
+    }
+    // base = sum(bcnt[0..ub0))
     part = group_sum<kWave>(part);
     if ((threadIdx.x & (kWave - 1)) == 0) lds_base[threadIdx.x / kWave] = part;
     __syncthreads();
@@ -82,10 +93,8 @@ __global__ __launch_bounds__(kCompactThreads) void compact_spikes(int32_t size,
 #pragma unroll
     for (int w = 0; w < kCompactThreads / kWave; w++) base += lds_base[w];
     // this thread's 4 neurons
-    const int32_t n0 = chunk0 + threadIdx.x * kCompactPerThread;
     uint32_t bits = 0;
     if (n0 < size) {
-        const uint64_t word = mask[n0 / 64];
         bits = (uint32_t)(word >> (n0 % 64)) & 0xFu;
         const int32_t valid = size - n0;
         if (valid < kCompactPerThread) bits &= (1u << valid) - 1u;
@@ -93,12 +102,6 @@ __global__ __launch_bounds__(kCompactThreads) void compact_spikes(int32_t size,
     int32_t total;
     const int32_t excl = block_exclusive_scan(__popc(bits), lds_waves, total);
     int32_t pos = base + excl;
-    int64_t slot = -1, f = 0;
-    if (rec.counts) {
-        slot = *rec.t_base + rec.step - rec.t_start;
-        if (slot >= 0 && slot < rec.capacity) f = rec.fill[slot & 1];
-        else slot = -1;
-    }
 #pragma unroll
     for (int k = 0; k < kCompactPerThread; k++)
         if (bits & (1u << k)) {
