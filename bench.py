@@ -834,7 +834,12 @@ def bench_c4(args):
     ms = proj.time_psp(50)
     pb = proj.psp_bytes()
     ach = pb / (ms * 1e-3) / 1e9
-    roof = roofline("hbm", ach, HBM_PEAK_GBS, "GB/s", None,
+    # the default sliced-tile layout of the headline network has its HBM
+    # counters committed (the same kernel on the same matrix)
+    sp_default = c1_pick.get("fmt") == "sell_panel" and c1_pick.get("lanes") == 2 and c1_pick.get("sigma") == 8 \
+        and c1_pick.get("chunk") == 2048 and c1_pick.get("panel_cols") == 20479
+    roof = roofline("hbm", ach, HBM_PEAK_GBS, "GB/s",
+                    traffic_from_profiles("sell_panel_psp_f64_c1_fixed1000_traffic.txt") if sp_default else None,
                     f"psp of the selected configuration {c1_pick} on the 1e8-synapse rate network",
                     kernel_ms=ms, algorithmic_bytes=pb, matrix_nnz=int(c1.nnz))
     net.destroy()
