@@ -704,8 +704,14 @@ def generate(desc: NeuronDescription, stop_condition: str | None = None) -> Mode
             "p_store": "\n".join(f"        ((T*)a.slot[{slot_of[n]}])[i] = lds[{k} * (size_t)N + i];"
                                  for k, n in enumerate(pslots)),
             "pslot_list": ", ".join(str(slot_of[n]) for n in pslots) or "0",
-            "global_loads": global_loads,
-            "div_consts": div_consts,
+            # the population-wide values in vector registers for the whole
+            # launch (the kernel's scalar registers are taken)
+            "global_loads": "\n".join(
+                [f"    T {_u(n)} = ((const T*)a.slot[{slot_of[n]}])[0];" for n in global_names] +
+                [f"    ahip::in_vgpr({_u(n)});" for n in global_names]),
+            "div_consts": "\n".join(
+                [f"    ahip::DivConst<T> {_u(n)}__dc = ahip::div_const<T>({_u(n)});" for n in div_names] +
+                [f"    ahip::in_vgpr({_u(n)}__dc.v);\n    ahip::in_vgpr({_u(n)}__dc.r);" for n in div_names]),
             "local_ptrs": persist_ptrs_code,
             "refrac_in": _indent(parts["refrac_in"], 12),
             "local_rand": _indent(parts["local_rand"], 12),

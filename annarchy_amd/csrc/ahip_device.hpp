@@ -322,6 +322,14 @@ __device__ inline void deliver_buckets(const ahip_pop_args& a, int32_t i) {
 // infinities, NaN, extreme exponents) take the division itself.  Operands
 // of other types divide as written.
 // ---------------------------------------------------------------------------
+// Keep a value in vector registers (an empty asm that reads and writes it
+// there): a loop that would otherwise reload spilled scalar registers every
+// iteration reads the copy instead.  No arithmetic changes.
+template <typename V>
+__device__ inline void in_vgpr(V& x) {
+    asm volatile("" : "+v"(x));
+}
+
 template <typename T>
 struct DivConst {
     T v;
