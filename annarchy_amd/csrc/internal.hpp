@@ -236,8 +236,9 @@ struct OrderedTarget {
 // atomics of one projection never interleave with another's into the same
 // conductance and every sum keeps the reference's projection order.
 struct ScatterGroup {
-    DevBuf table;             // ScatterDesc[n]
+    std::vector<ScatterDesc> descs;   // passed by value in the launch's arguments
     int32_t n = 0;
+    int32_t waves_per = 1;            // waves of each descriptor
     int64_t blocks = 1;
 };
 }  // namespace ahip

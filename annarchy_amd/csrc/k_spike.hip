@@ -2,6 +2,8 @@
 #include "ahip_device.hpp"
 #include "internal.hpp"
 
+#include <cstring>
+
 namespace ahip {
 namespace {
 
@@ -187,37 +189,46 @@ int launch_scatter_t(const Proj& p, const Pop& pre, void* target, hipStream_t s)
 }
 
 // All projections of a scatter group in one launch.  Work items of
-// descriptor k are (spike s, chunk c) for s < count_k, c < chunks_k, laid out
-// after those of descriptors 0..k-1; a wave maps its item to (k, s, c) and
-// proceeds as scatter_spikes.  Descriptors of a group have distinct targets.
+// descriptor k are (spike s, chunk c) for s < count_k, c < chunks_k; the
+// waves of descriptor k take items lw, lw + waves_per, ... and proceed as
+// scatter_spikes.  Descriptors of a group have distinct targets.
 __device__ inline int64_t list_slot(const ScatterDesc& d, int64_t t) {
     if (d.depth <= 1) return 0;
     const int64_t m = (t - d.delay) % d.depth;
     return m < 0 ? m + d.depth : m;
 }
 
+// The descriptors of a group travel by value in the kernel arguments (no
+// dependent load of a table before the spike counts); descriptor k owns waves
+// [k * waves_per, (k + 1) * waves_per).
+struct ScatterArgs {
+    ScatterDesc d[kMaxScatter];
+    int32_t nd;
+    int32_t waves_per;
+};
+
 template <typename T>
-__global__ __launch_bounds__(256) void scatter_multi(const ScatterDesc* __restrict__ desc, int32_t nd,
-                                                     const int64_t* __restrict__ t_base, int32_t step) {
+__global__ __launch_bounds__(256) void scatter_multi(const ScatterArgs A, const int64_t* __restrict__ t_base,
+                                                     int32_t step) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int wid = threadIdx.x / kWave;
-    const int64_t t = *t_base + step;
-    int64_t ends[kMaxScatter];
-    int64_t total = 0;
-#pragma unroll
-    for (int k = 0; k < kMaxScatter; k++) {
-        if (k < nd) total += (int64_t)desc[k].count[list_slot(desc[k], t)] * desc[k].chunks;
-        ends[k] = total;
-    }
-    const int64_t waves = (int64_t)gridDim.x * 4;
-    for (int64_t it = (int64_t)blockIdx.x * 4 + wid; it < total; it += waves) {
-        int k = 0;
-        while (k < nd - 1 && it >= ends[k]) k++;
-        const ScatterDesc& d = desc[k];
-        const int64_t local = it - (k > 0 ? ends[k - 1] : 0);
-        const int64_t s = local / d.chunks;
-        const int32_t c = (int32_t)(local - s * d.chunks);
-        const int32_t pre = d.list[list_slot(d, t) * d.list_stride + s];
+    // the wave's index, uniform: the descriptor is read from the kernel
+    // arguments with scalar loads
+    const int32_t gw = __builtin_amdgcn_readfirstlane((int32_t)blockIdx.x * 4 + (int32_t)(threadIdx.x / kWave));
+    const int32_t k = gw / A.waves_per;
+    if (k >= A.nd) return;
+    const ScatterDesc& d = A.d[k];
+    const int32_t lw = gw - k * A.waves_per;
+    const int64_t sl = d.depth <= 1 ? 0 : list_slot(d, *t_base + step);
+    const auto list = as_global(d.list + sl * d.list_stride);
+    // the list entry of the wave's first item loads together with the count
+    // (a list never holds more than n_cols entries)
+    const int64_t s0 = lw / d.chunks;
+    const int32_t pre0 = s0 < d.n_cols ? list[s0] : -1;
+    const int64_t items = (int64_t)as_global(d.count)[sl] * d.chunks;
+    for (int64_t it = lw; it < items; it += A.waves_per) {
+        const int64_t s = it / d.chunks;
+        const int32_t c = (int32_t)(it - s * d.chunks);
+        const int32_t pre = it == lw ? pre0 : list[s];
         if (pre < 0 || pre >= d.n_cols) continue;
         const int64_t b = as_global(d.col_ptr)[pre], e = as_global(d.col_ptr)[pre + 1];
         const auto ri = as_global(d.row_idx);
@@ -403,12 +414,18 @@ int launch_compact(const Pop& p, const Monitor* m, const int64_t* t_base, int32_
 
 int launch_scatter_group(const ahip_net* net, const ScatterGroup& g, int32_t step, hipStream_t s) {
     if (g.n == 0) return 0;
+    if (g.n > kMaxScatter) return (int)hipErrorInvalidValue;
+    ScatterArgs A;
+    std::memset(&A, 0, sizeof(A));
+    for (int32_t k = 0; k < g.n; k++) A.d[k] = g.descs[k];
+    A.nd = g.n;
+    A.waves_per = g.waves_per;
     if (net->precision == AHIP_F64)
-        hipLaunchKernelGGL(scatter_multi<double>, dim3(g.blocks), dim3(256), 0, s,
-                           g.table.as<ScatterDesc>(), g.n, net->t_dev.as<int64_t>(), step);
+        hipLaunchKernelGGL(scatter_multi<double>, dim3(g.blocks), dim3(256), 0, s, A, net->t_dev.as<int64_t>(),
+                           step);
     else
-        hipLaunchKernelGGL(scatter_multi<float>, dim3(g.blocks), dim3(256), 0, s,
-                           g.table.as<ScatterDesc>(), g.n, net->t_dev.as<int64_t>(), step);
+        hipLaunchKernelGGL(scatter_multi<float>, dim3(g.blocks), dim3(256), 0, s, A, net->t_dev.as<int64_t>(),
+                           step);
     return (int)hipGetLastError();
 }
 

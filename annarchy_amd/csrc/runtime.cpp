@@ -627,14 +627,15 @@ int prepare(ahip_net* net) {
             targets[k].push_back(g);
             blocks[k] += ((int64_t)pre.size * p.scatter_chunks + 3) / 4;
         }
+        // each descriptor of a group gets an equal share of the launch's
+        // waves (up to 2048 in all)
         for (size_t q = 0; q < groups.size(); q++) {
             ScatterGroup sg;
+            sg.descs = groups[q];
             sg.n = (int32_t)groups[q].size();
-            sg.blocks = std::max<int64_t>(1, std::min<int64_t>(512, blocks[q]));
-            int rc = dev_alloc(sg.table, groups[q].size() * sizeof(ScatterDesc));
-            if (rc) return rc;
-            HIPCHK(hipMemcpy(sg.table.p, groups[q].data(), groups[q].size() * sizeof(ScatterDesc),
-                             hipMemcpyHostToDevice));
+            const int64_t waves = 4 * std::max<int64_t>(1, std::min<int64_t>(512, blocks[q]));
+            sg.waves_per = (int32_t)std::max<int64_t>(1, waves / sg.n);
+            sg.blocks = ((int64_t)sg.n * sg.waves_per + 3) / 4;
             net->scatter_groups.push_back(std::move(sg));
         }
         net->scatter_dirty = false;
