@@ -80,12 +80,16 @@ __global__ __launch_bounds__(AHIP_UPDATE_BLOCK) void local_update(ahip_pop_args 
     const int64_t t = ahip::sim_time(a);
     const T dt = (T)a.dt;
     (void)t; (void)dt;
-    // this step's spike events of the ordered conductances (end of compute_psp)
-    if (a.n_deliver > 0 && i < a.size) ahip::deliver_buckets<T>(a, i);
-    asm volatile("" ::: "memory");  // the equations below read the delivered values
 %(global_loads)s
 %(div_consts)s
 %(local_ptrs)s
+    // the neuron's values the delivery below does not write, in registers
+    // (their loads overlap the delivery's)
+%(reg_pre)s
+    // this step's spike events of the ordered conductances (end of compute_psp)
+    if (a.n_deliver > 0 && i < a.size) ahip::deliver_buckets<T>(a, i);
+    asm volatile("" ::: "memory");  // the equations below read the delivered values
+%(reg_post)s
     bool _spike = false;
     if (i < a.size) {
 %(refrac_in)s
@@ -93,6 +97,7 @@ __global__ __launch_bounds__(AHIP_UPDATE_BLOCK) void local_update(ahip_pop_args 
 %(local_eqs)s
 %(refrac_dec)s
 %(gather)s
+%(reg_out)s
     }
     (void)_spike;
 %(publish)s
@@ -726,6 +731,37 @@ def generate(desc: NeuronDescription, stop_condition: str | None = None) -> Mode
         persist_fn = ""
         persist_ptrs = "nullptr, nullptr, 0, nullptr"
 
+    # the per-kernel update: the statements work on register copies of the
+    # neuron's values (X[i] -> X__r); the local slots other than the
+    # conductances (g_*, which the ordered delivery may write) and the
+    # refractory counter are loaded before the delivery, the conductances
+    # after it; written values are stored after the statements
+    lparts = {"refrac_in": refrac_in, "local_rand": rand_code("local", "i"), "local_eqs": local_eqs,
+              "refrac_dec": refrac_dec, "gather": gather}
+    for key in lparts:
+        lparts[key] = lparts[key].replace("a.refractory_remaining[i]", "refrac_rem[i]")
+    ljoined = "\n".join(lparts.values())
+    lref = lambda c: r"(?<![\w.])" + re.escape(c) + r"\[i\]"
+    lwrites = lambda c: re.search(lref(c) + r"\s*(?:[-+*/]?=(?!=)|\+\+|--)", ljoined) is not None
+    # read: any reference other than the target of a plain assignment
+    lreads = lambda c: re.search(lref(c) + r"(?!\s*=(?!=))", ljoined) is not None
+    reg_pre, reg_post, reg_out = [], [], []
+    cands = [(_u(n), "T", f"{_u(n)}[i]", n.startswith("g_")) for n in local_names]
+    if has_ref:
+        cands.append(("refrac_rem", "int32_t", "a.refractory_remaining[i]", False))
+    for c, ty, mem, after in cands:
+        if not lreads(c):
+            if lwrites(c):
+                # written without being read: stays a store
+                for key in lparts:
+                    lparts[key] = re.sub(lref(c), mem.replace("\\", "\\\\"), lparts[key])
+            continue
+        (reg_post if after else reg_pre).append(f"    {ty} {c}__r = i < a.size ? {mem} : {ty}(0);")
+        if lwrites(c):
+            reg_out.append(f"        {mem} = {c}__r;")
+        for key in lparts:
+            lparts[key] = re.sub(lref(c), c + "__r", lparts[key])
+
     body = _PRELUDE % {"name": desc.name}
     if has_global:
         body += _GLOBAL_KERNEL % {
@@ -737,11 +773,14 @@ def generate(desc: NeuronDescription, stop_condition: str | None = None) -> Mode
         "global_loads": global_loads,
         "div_consts": div_consts,
         "local_ptrs": local_ptrs,
-        "refrac_in": _indent(refrac_in, 8),
-        "local_rand": _indent(rand_code("local", "i"), 8),
-        "local_eqs": _indent(local_eqs, 8),
-        "refrac_dec": _indent(refrac_dec, 8),
-        "gather": _indent(gather, 8),
+        "refrac_in": _indent(lparts["refrac_in"], 8),
+        "local_rand": _indent(lparts["local_rand"], 8),
+        "local_eqs": _indent(lparts["local_eqs"], 8),
+        "refrac_dec": _indent(lparts["refrac_dec"], 8),
+        "gather": _indent(lparts["gather"], 8),
+        "reg_pre": "\n".join(reg_pre),
+        "reg_post": "\n".join(reg_post),
+        "reg_out": "\n".join(reg_out),
         "publish": publish,
         "launch_global": launch_global,
         "preloop_fn": preloop_fn,
