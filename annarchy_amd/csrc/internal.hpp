@@ -272,12 +272,6 @@ struct ahip_net {
     int32_t deliver_n_tg = 0, deliver_waves = 0;
     int32_t delivery = AHIP_DELIVERY_ORDERED;
     bool persist_enabled = true;   // small single-population networks: one launch per batch
-    // the populations of a step after the first run on streams of their own
-    // (parallel branches of the captured graph), forked from and joined
-    // back into `stream` within the step
-    std::vector<hipStream_t> branch;
-    std::vector<hipEvent_t> branch_join;
-    hipEvent_t branch_fork = nullptr;
     size_t elem() const { return precision == AHIP_F64 ? sizeof(double) : sizeof(float); }
 };
 

@@ -718,24 +718,6 @@ const Monitor* first_spike_monitor(const ahip_net* net, int32_t pop) {
     return nullptr;
 }
 
-// Streams for the population branches of a step: `n` besides the network
-// stream (default priority, like every stream here), with their join events
-// and the fork event, created once and kept with the network.
-constexpr int kMaxBranches = 4;
-static int ensure_branches(ahip_net* net, int n) {
-    n = std::min(n, kMaxBranches);
-    if (!net->branch_fork) HIPCHK(hipEventCreateWithFlags(&net->branch_fork, hipEventDisableTiming));
-    while ((int)net->branch.size() < n) {
-        hipStream_t b = nullptr;
-        HIPCHK(hipStreamCreateWithFlags(&b, hipStreamNonBlocking));
-        net->branch.push_back(b);
-        hipEvent_t e = nullptr;
-        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        net->branch_join.push_back(e);
-    }
-    return 0;
-}
-
 // Enqueue one simulation step (index `step` inside the current batch).
 int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_before) {
     hipStream_t s = net->stream;
@@ -797,22 +779,7 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
         LCHK(launch_record_values(m, p.slots[m.slot].p, p.size, (int32_t)es,
                                   net->t_dev.as<int64_t>(), step, s));
     }
-    // 3. neuron updates (+ spike flags), 4. compaction.  Each active
-    // population's launches (update, compaction, firing-rate window, delay
-    // push) read and write only that population's buffers and what the
-    // psp stage above wrote, so with several active populations the second
-    // and later ones run on branch streams forked here and joined below:
-    // parallel branches of the step graph.
-    int32_t n_active = 0;
-    for (auto& pp : net->pops) n_active += pp->active ? 1 : 0;
-    const int n_branch = std::min(n_active - 1, kMaxBranches);
-    if (n_branch > 0) {
-        int rcb = ensure_branches(net, n_branch);
-        if (rcb) return rcb;
-        HIPCHK(hipEventRecord(net->branch_fork, s));
-        for (int b = 0; b < n_branch; b++) HIPCHK(hipStreamWaitEvent(net->branch[b], net->branch_fork, 0));
-    }
-    int32_t k_active = 0;
+    // 3. neuron updates (+ spike flags), 4. compaction
     for (auto& pp : net->pops) {
         Pop& p = *pp;
         if (!p.active) {
@@ -825,41 +792,32 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
             if (p.model->spiking && p.fr_window > 0) LCHK(launch_update_rate(net, p, step, s, false));
             continue;
         }
-        // the population's stream: the network stream for the first, a
-        // branch for the others
-        const int32_t k = k_active++;
-        hipStream_t ps = (k == 0 || n_branch <= 0) ? s : net->branch[(k - 1) % n_branch];
         ahip_pop_args a = make_args(net, p, step);
         ahip_launch_fn fn = net->precision == AHIP_F64 ? p.model->launch_f64 : p.model->launch_f32;
         const bool probe_u = net->probe_kind == AHIP_PROBE_UPDATE && net->probe_proj == p.id;
-        if (probe_u) HIPCHK(hipEventRecord(net->probe_ev[0], ps));
-        int rc = fn(&a, (void*)ps);
+        if (probe_u) HIPCHK(hipEventRecord(net->probe_ev[0], s));
+        int rc = fn(&a, (void*)s);
         if (rc != 0) return fail(AHIP_EHIP, "update of population %d failed: %s", p.id,
                                  hipGetErrorString((hipError_t)rc));
-        if (probe_u) HIPCHK(hipEventRecord(net->probe_ev[1], ps));
+        if (probe_u) HIPCHK(hipEventRecord(net->probe_ev[1], s));
         if (p.model->spiking) {
             const bool probe_c = net->probe_kind == AHIP_PROBE_COMPACT && net->probe_proj == p.id;
-            if (probe_c) HIPCHK(hipEventRecord(net->probe_ev[0], ps));
+            if (probe_c) HIPCHK(hipEventRecord(net->probe_ev[0], s));
             // the population's first spike monitor is filled by the compaction
-            LCHK(launch_compact(p, first_spike_monitor(net, p.id), net->t_dev.as<int64_t>(), step, ps));
-            if (probe_c) HIPCHK(hipEventRecord(net->probe_ev[1], ps));
-            if (p.fr_window > 0) LCHK(launch_update_rate(net, p, step, ps));
+            LCHK(launch_compact(p, first_spike_monitor(net, p.id), net->t_dev.as<int64_t>(), step, s));
+            if (probe_c) HIPCHK(hipEventRecord(net->probe_ev[1], s));
+            if (p.fr_window > 0) LCHK(launch_update_rate(net, p, step, s));
         }
         // delay queues: push this step's values (after the update, as the
         // reference's update_delay)
         if (p.depth > 1) {
             if (p.model->spiking) {
-                LCHK(launch_push_spikes(net, p, step, ps));
+                LCHK(launch_push_spikes(net, p, step, s));
             } else {
                 const int32_t rs = p.slot_index("r");
-                if (rs >= 0) LCHK(launch_push_values(net, p.slots[rs].p, p.hist.p, p.size, p.depth, step, ps));
+                if (rs >= 0) LCHK(launch_push_values(net, p.slots[rs].p, p.hist.p, p.size, p.depth, step, s));
             }
         }
-    }
-    // join the branches before anything reads another population's results
-    for (int b = 0; b < n_branch; b++) {
-        HIPCHK(hipEventRecord(net->branch_join[b], net->branch[b]));
-        HIPCHK(hipStreamWaitEvent(s, net->branch_join[b], 0));
     }
     // plastic projections, after every population update: update_synapse,
     // then the post events (OpenMPBaseTemplate.py:217-325)
@@ -1148,9 +1106,6 @@ int ahip_net_destroy(ahip_net* net) {
     }
     net->pops.clear();
     net->t_dev.release();
-    for (hipStream_t b : net->branch) hipStreamDestroy(b);
-    for (hipEvent_t e : net->branch_join) hipEventDestroy(e);
-    if (net->branch_fork) hipEventDestroy(net->branch_fork);
     hipStreamDestroy(net->stream);
     delete net;
     return 0;
