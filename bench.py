@@ -472,6 +472,11 @@ def bench_c2(args):
     events = int(outdeg[idx_t].sum())
     log(f"c2: {steps} steps in {el:.3f} s ({el / steps * 1e6:.1f} us/step), {events / el:.3e} events/s")
     kern, syn, spk = _spiking_kernels(net, P, projs[:2], outdeg, mon, es, warmup + steps)
+    # every step the monitor recorded (warm-up, timed and probe steps): the
+    # mean spikes and synaptic events per step, one scatter launch each
+    c_all, i_all = mon.raw_spikes()
+    all_steps = {"steps": int(len(c_all)), "spikes_per_step": float(c_all.mean()) if len(c_all) else 0.0,
+                 "events_per_step": float(outdeg[i_all].sum()) / max(1, len(c_all))}
     dom = max(kern, key=lambda k: kern[k]["ms"])
     ach = kern[dom]["bytes"] / (kern[dom]["ms"] * 1e-3) / 1e9
     kname = {"scatter": "scatter_multi", "update": "local_update", "compact": "compact_spikes",
@@ -521,7 +526,7 @@ def bench_c2(args):
                                "weights, Poisson 5 Hz into 10%, dt=0.1ms", "precision": prec,
                    "delivery": "ordered (the reference's summation order)", "graph_steps": G,
                    "connectivity": "drawn on the device (Philox stream)"},
-                  roof, cpu, kernels=kern, spikes_per_step=float(c_t.mean()),
+                  roof, cpu, kernels=kern, all_recorded_steps=all_steps, spikes_per_step=float(c_t.mean()),
                   mean_rate_hz=float(c_t.sum() / N / (steps * 1e-4)), fp32_vs_fp64=fp32_vs_fp64)
 
 
