@@ -716,7 +716,8 @@ def generate(desc: NeuronDescription, stop_condition: str | None = None) -> Mode
                 [f"    ahip::in_vgpr({_u(n)});" for n in global_names]),
             "div_consts": "\n".join(
                 [f"    ahip::DivConst<T> {_u(n)}__dc = ahip::div_const<T>({_u(n)});" for n in div_names] +
-                [f"    ahip::in_vgpr({_u(n)}__dc.v);\n    ahip::in_vgpr({_u(n)}__dc.r);" for n in div_names]),
+                [f"    ahip::in_vgpr({_u(n)}__dc.v);\n    ahip::in_vgpr({_u(n)}__dc.r);\n"
+                 f"    ahip::in_vgpr({_u(n)}__dc.span);" for n in div_names]),
             "local_ptrs": persist_ptrs_code,
             "refrac_in": _indent(parts["refrac_in"], 12),
             "local_rand": _indent(parts["local_rand"], 12),

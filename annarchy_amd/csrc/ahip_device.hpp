@@ -310,18 +310,6 @@ __device__ inline void deliver_buckets(const ahip_pop_args& a, int32_t i) {
     }
 }
 
-// ---------------------------------------------------------------------------
-// Division by a population-wide value (constant within a kernel): x / d as
-// the compiler's IEEE double division computes it (reciprocal, two
-// Newton-Raphson steps, quotient, residual, corrected quotient, the last
-// step with the operands scaled and the result fixed up only for extreme
-// exponents), with the reciprocal steps done once per kernel instead of per
-// neuron.  For d in [2^-100, 2^100] and |x| in [2^-600, 2^600] the scaling
-// and the fix-up do nothing, so the three remaining operations give the
-// division's own result (correctly rounded); other operands (zeros,
-// infinities, NaN, extreme exponents) take the division itself.  Operands
-// of other types divide as written.
-// ---------------------------------------------------------------------------
 // Keep a value in vector registers (an empty asm that reads and writes it
 // there): a loop that would otherwise reload spilled scalar registers every
 // iteration reads the copy instead.  No arithmetic changes.
@@ -330,11 +318,23 @@ __device__ inline void in_vgpr(V& x) {
     asm volatile("" : "+v"(x));
 }
 
+// ---------------------------------------------------------------------------
+// Division by a population-wide value (constant within a kernel): x / d as
+// the compiler's IEEE double division computes it (reciprocal, two
+// Newton-Raphson steps, quotient, residual, corrected quotient, the last
+// step with the operands scaled and the result fixed up only for extreme
+// exponents), with the reciprocal steps done once per kernel instead of per
+// neuron.  For d in [2^-100, 2^100] and |x| in [2^-600, 2^600) the scaling
+// and the fix-up do nothing, so the three remaining operations give the
+// division's own result (correctly rounded); other operands (zeros,
+// infinities, NaN, extreme exponents) take the division itself.  Operands
+// of other types divide as written.
+// ---------------------------------------------------------------------------
 template <typename T>
 struct DivConst {
     T v;
-    double r;   // the reciprocal of (double) v after the Newton-Raphson steps
-    bool fast;
+    double r;        // the reciprocal of (double) v after the Newton-Raphson steps
+    uint32_t span;   // 1200: the quotient steps serve |x| in [2^-600, 2^600); 0: never
 };
 
 template <typename T>
@@ -342,14 +342,14 @@ __device__ inline DivConst<T> div_const(T v) {
     DivConst<T> c;
     c.v = v;
     c.r = 0.0;
-    c.fast = false;
+    c.span = 0u;
     const double d = (double)v;
     const double ad = __builtin_fabs(d);
     if (ad >= 0x1p-100 && ad <= 0x1p100) {
         const double r0 = __builtin_amdgcn_rcp(d);
         const double r1 = __builtin_fma(r0, __builtin_fma(-d, r0, 1.0), r0);
         c.r = __builtin_fma(r1, __builtin_fma(-d, r1, 1.0), r1);
-        c.fast = true;
+        c.span = 1200u;
     }
     return c;
 }
@@ -359,8 +359,10 @@ __device__ inline auto operator/(X x, const DivConst<T>& c) -> decltype(x / c.v)
     using R = decltype(x / c.v);
     if constexpr (std::is_same<R, double>::value) {
         const double xd = (double)x;
-        const double ax = __builtin_fabs(xd);
-        if (c.fast && ax >= 0x1p-600 && ax <= 0x1p600) {
+        // |x| in [2^-600, 2^600): biased exponent in [423, 1622], one
+        // integer test on the high word (span 0 fails it for any x)
+        const uint32_t ex = ((uint32_t)((uint64_t)__double_as_longlong(xd) >> 52)) & 0x7ffu;
+        if (ex - 423u < c.span) {
             const double q = xd * c.r;
             return __builtin_fma(__builtin_fma(-(double)c.v, q, xd), c.r, q);
         }
