@@ -374,6 +374,7 @@ __global__ __launch_bounds__(AHIP_PERSIST_THREADS) void persistent_steps(ahip_po
         scatter_list(nspk);
         // 2. neuron update (the statements of the per-kernel step)
         uint32_t bits = 0;
+#pragma unroll 2
         for (int32_t k = 0; k < per; k++) {
             const int32_t i = k * AHIP_PERSIST_THREADS + tid;
             if (i >= N) break;
