@@ -282,7 +282,15 @@ int launch_rate_psp(const ahip_net* net, const Proj& p, const void* pre_r, void*
                     hipStream_t s);
 int launch_spike_psp(const ahip_net* net, const Proj& p, const Pop& pre, void* target,
                      hipStream_t s);
-int launch_compact(const Pop& p, const Monitor* m, const int64_t* t_base, int32_t step, hipStream_t s);
+// the spike compactions of the populations `pops` (one launch per 8)
+int launch_compact(const ahip_net* net, const std::vector<int32_t>& pops, const int64_t* t_base, int32_t step,
+                   hipStream_t s);
+// The first spike monitor of a population (filled by its compaction kernel).
+inline const Monitor* first_spike_monitor(const ahip_net* net, int32_t pop) {
+    for (auto& mp : net->mons)
+        if (mp->spikes && mp->pop == pop && !mp->paused) return mp.get();
+    return nullptr;
+}
 int launch_update_rate(const ahip_net* net, const Pop& p, int32_t step, hipStream_t s, bool active = true);
 int launch_scatter_group(const ahip_net* net, const ScatterGroup& g, int32_t step, hipStream_t s);
 int launch_deliver(const ahip_net* net, hipStream_t s);

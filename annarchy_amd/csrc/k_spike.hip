@@ -59,31 +59,38 @@ struct SpikeRec {
     int64_t idx_capacity;
     int64_t t_start;
     int64_t capacity;
-    const int64_t* t_base;
-    int32_t step;
 };
 
-__global__ __launch_bounds__(kCompactThreads) void compact_spikes(int32_t size,
-                                                                  int32_t n_update_blocks,
-                                                                  const uint64_t* __restrict__ mask,
-                                                                  const int32_t* __restrict__ bcnt,
-                                                                  int32_t* __restrict__ spiked,
-                                                                  int32_t* __restrict__ count,
-                                                                  SpikeRec rec) {
+// One population's compaction: its buffers and its first spike monitor.
+struct CompactArgs {
+    int32_t size;
+    int32_t nb;                   // workgroups (chunks of kCompactChunk neurons)
+    const uint64_t* mask;
+    const int32_t* bcnt;
+    int32_t* spiked;
+    int32_t* count;
+    SpikeRec rec;
+};
+
+// Chunk b of a population's compaction at step t_base + step.
+__device__ inline void compact_chunk(const CompactArgs& c, int32_t b, const int64_t* __restrict__ t_base,
+                                     int32_t step) {
     __shared__ int32_t lds_waves[kCompactThreads / kWave + 1];
     __shared__ int32_t lds_base[kCompactThreads / kWave];
-    const int32_t chunk0 = blockIdx.x * kCompactChunk;
+    const int32_t size = c.size;
+    const SpikeRec& rec = c.rec;
+    const int32_t chunk0 = b * kCompactChunk;
     const int32_t ub0 = chunk0 / AHIP_UPDATE_BLOCK;  // first update block of the chunk
     // every global load first (they do not depend on one another): this
     // thread's mask word, its share of the block counts before the chunk,
     // the monitor's fill count
     const int32_t n0 = chunk0 + threadIdx.x * kCompactPerThread;
-    const uint64_t word = n0 < size ? mask[n0 / 64] : 0ull;
+    const uint64_t word = n0 < size ? c.mask[n0 / 64] : 0ull;
     int32_t part = 0;
-    for (int32_t k = threadIdx.x; k < ub0; k += kCompactThreads) part += bcnt[k];
+    for (int32_t k = threadIdx.x; k < ub0; k += kCompactThreads) part += c.bcnt[k];
     int64_t slot = -1, f = 0;
     if (rec.counts) {
-        slot = *rec.t_base + rec.step - rec.t_start;
+        slot = *t_base + step - rec.t_start;
         if (slot >= 0 && slot < rec.capacity) f = rec.fill[slot & 1];
         else slot = -1;
     }
@@ -107,19 +114,35 @@ __global__ __launch_bounds__(kCompactThreads) void compact_spikes(int32_t size,
 #pragma unroll
     for (int k = 0; k < kCompactPerThread; k++)
         if (bits & (1u << k)) {
-            spiked[pos] = n0 + k;
+            c.spiked[pos] = n0 + k;
             if (slot >= 0 && f + pos < rec.idx_capacity) rec.idx[f + pos] = n0 + k;
             pos++;
         }
     // grand total: the last chunk knows base + its own total
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
-        *count = base + total;
+    if (b == c.nb - 1 && threadIdx.x == 0) {
+        *c.count = base + total;
         if (slot >= 0) {
             rec.counts[slot] = base + total;
             rec.fill[(slot + 1) & 1] = f + base + total;
         }
     }
-    (void)n_update_blocks;
+}
+
+// The compactions of up to kMaxCompact populations in one launch: workgroup
+// ranges one after another, population k's chunks at [b0[k], b0[k + 1]).
+constexpr int kMaxCompact = 8;
+struct CompactBatch {
+    CompactArgs c[kMaxCompact];
+    int32_t b0[kMaxCompact + 1];
+    int32_t n;
+};
+
+__global__ __launch_bounds__(kCompactThreads) void compact_spikes(const CompactBatch B,
+                                                                  const int64_t* __restrict__ t_base, int32_t step) {
+    const int32_t bid = __builtin_amdgcn_readfirstlane((int32_t)blockIdx.x);
+    int32_t k = 0;
+    while (k < B.n - 1 && bid >= B.b0[k + 1]) k++;
+    compact_chunk(B.c[k], bid - B.b0[k], t_base, step);
 }
 
 // Event-driven propagation over the compacted spike list, CSRC backward view
@@ -392,24 +415,38 @@ int launch_update_rate(const ahip_net* net, const Pop& p, int32_t step, hipStrea
     return (int)hipGetLastError();
 }
 
-int launch_compact(const Pop& p, const Monitor* m, const int64_t* t_base, int32_t step, hipStream_t s) {
-    const int32_t nb = (p.size + kCompactChunk - 1) / kCompactChunk;
-    if (nb == 0) return 0;
-    SpikeRec rec{};
-    if (m) {
-        rec.counts = m->counts.as<int32_t>();
-        rec.idx = m->data.as<int32_t>();
-        rec.fill = m->fill.as<int64_t>();
-        rec.idx_capacity = m->idx_capacity;
-        rec.t_start = m->t_start;
-        rec.capacity = m->capacity;
-        rec.t_base = t_base;
-        rec.step = step;
+int launch_compact(const ahip_net* net, const std::vector<int32_t>& pops, const int64_t* t_base, int32_t step,
+                   hipStream_t s) {
+    for (size_t first = 0; first < pops.size(); first += kMaxCompact) {
+        CompactBatch B;
+        std::memset(&B, 0, sizeof(B));
+        for (size_t q = first; q < pops.size() && B.n < kMaxCompact; q++) {
+            const Pop& p = *net->pops[pops[q]];
+            const Monitor* m = first_spike_monitor(net, p.id);
+            CompactArgs& c = B.c[B.n];
+            c.size = p.size;
+            c.nb = (p.size + kCompactChunk - 1) / kCompactChunk;
+            c.mask = p.spike_mask.as<uint64_t>();
+            c.bcnt = p.block_counts.as<int32_t>();
+            c.spiked = p.spiked.as<int32_t>();
+            c.count = p.spike_count.as<int32_t>();
+            if (m) {
+                c.rec.counts = m->counts.as<int32_t>();
+                c.rec.idx = m->data.as<int32_t>();
+                c.rec.fill = m->fill.as<int64_t>();
+                c.rec.idx_capacity = m->idx_capacity;
+                c.rec.t_start = m->t_start;
+                c.rec.capacity = m->capacity;
+            }
+            B.b0[B.n + 1] = B.b0[B.n] + c.nb;
+            B.n++;
+        }
+        if (B.b0[B.n] == 0) continue;
+        hipLaunchKernelGGL(compact_spikes, dim3((unsigned)B.b0[B.n]), dim3(kCompactThreads), 0, s, B, t_base, step);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return (int)e;
     }
-    hipLaunchKernelGGL(compact_spikes, dim3(nb), dim3(kCompactThreads), 0, s, p.size, p.nblocks(),
-                       p.spike_mask.as<uint64_t>(), p.block_counts.as<int32_t>(),
-                       p.spiked.as<int32_t>(), p.spike_count.as<int32_t>(), rec);
-    return (int)hipGetLastError();
+    return 0;
 }
 
 int launch_scatter_group(const ahip_net* net, const ScatterGroup& g, int32_t step, hipStream_t s) {

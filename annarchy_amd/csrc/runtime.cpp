@@ -711,12 +711,6 @@ ahip_pop_args make_args(ahip_net* net, Pop& p, int32_t step) {
     return a;
 }
 
-// The first spike monitor of a population (filled by its compaction kernel).
-const Monitor* first_spike_monitor(const ahip_net* net, int32_t pop) {
-    for (auto& mp : net->mons)
-        if (mp->spikes && mp->pop == pop && !mp->paused) return mp.get();
-    return nullptr;
-}
 
 // Enqueue one simulation step (index `step` inside the current batch).
 int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_before) {
@@ -779,7 +773,11 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
         LCHK(launch_record_values(m, p.slots[m.slot].p, p.size, (int32_t)es,
                                   net->t_dev.as<int64_t>(), step, s));
     }
-    // 3. neuron updates (+ spike flags), 4. compaction
+    // 3. neuron updates (+ spike flags) of every population, 4. the spike
+    // compactions of the updated spiking populations in one launch (no
+    // update reads another population's compaction), then each population's
+    // firing-rate window and delay push
+    std::vector<int32_t> compact;
     for (auto& pp : net->pops) {
         Pop& p = *pp;
         if (!p.active) {
@@ -800,16 +798,22 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
         if (rc != 0) return fail(AHIP_EHIP, "update of population %d failed: %s", p.id,
                                  hipGetErrorString((hipError_t)rc));
         if (probe_u) HIPCHK(hipEventRecord(net->probe_ev[1], s));
-        if (p.model->spiking) {
-            const bool probe_c = net->probe_kind == AHIP_PROBE_COMPACT && net->probe_proj == p.id;
-            if (probe_c) HIPCHK(hipEventRecord(net->probe_ev[0], s));
-            // the population's first spike monitor is filled by the compaction
-            LCHK(launch_compact(p, first_spike_monitor(net, p.id), net->t_dev.as<int64_t>(), step, s));
-            if (probe_c) HIPCHK(hipEventRecord(net->probe_ev[1], s));
-            if (p.fr_window > 0) LCHK(launch_update_rate(net, p, step, s));
-        }
-        // delay queues: push this step's values (after the update, as the
-        // reference's update_delay)
+        if (p.model->spiking) compact.push_back(p.id);
+    }
+    if (!compact.empty()) {
+        const bool probe_c = net->probe_kind == AHIP_PROBE_COMPACT &&
+                             std::find(compact.begin(), compact.end(), net->probe_proj) != compact.end();
+        if (probe_c) HIPCHK(hipEventRecord(net->probe_ev[0], s));
+        // each population's first spike monitor is filled by its compaction
+        LCHK(launch_compact(net, compact, net->t_dev.as<int64_t>(), step, s));
+        if (probe_c) HIPCHK(hipEventRecord(net->probe_ev[1], s));
+    }
+    for (auto& pp : net->pops) {
+        Pop& p = *pp;
+        if (!p.active) continue;
+        if (p.model->spiking && p.fr_window > 0) LCHK(launch_update_rate(net, p, step, s));
+        // delay queues: push this step's values (after the update and the
+        // compaction, as the reference's update_delay)
         if (p.depth > 1) {
             if (p.model->spiking) {
                 LCHK(launch_push_spikes(net, p, step, s));

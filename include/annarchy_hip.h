@@ -256,7 +256,8 @@ int ahip_proj_set_learning(ahip_net* net, int32_t proj, int32_t update, int32_t 
  * with HIP events on the network stream around one kernel inside each step;
  * *avg_ms = mean time of that kernel within real steps.  what / index:
  * plastic projection `index`: its pre events, update_synapse or post
- * events; population `index`: its update or its spike compaction; fused
+ * events; population `index`: its update or the spike-compaction launch
+ * (one launch compacts every spiking population updated in the step); fused
  * spike-scatter launch `index` (0 .. ahip_scatter_groups - 1). */
 #define AHIP_PROBE_PRE_EVENTS 0
 #define AHIP_PROBE_SYN_UPDATE 1
