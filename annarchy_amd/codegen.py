@@ -28,7 +28,7 @@ import re
 
 from .equations import DISTRIBUTIONS, NeuronDescription, analyse_neuron, equation_code
 
-ABI = 9  # AHIP_MODEL_ABI in include/annarchy_hip_model.h
+ABI = 10  # AHIP_MODEL_ABI in include/annarchy_hip_model.h
 
 _PRELUDE = r"""// Generated neuron-model library for the MI355X simulation backend.
 // Model: %(name)s
@@ -748,9 +748,15 @@ def generate(desc: NeuronDescription, stop_condition: str | None = None) -> Mode
     # read: any reference other than the target of a plain assignment
     lreads = lambda c: re.search(lref(c) + r"(?!\s*=(?!=))", ljoined) is not None
     reg_pre, reg_post, reg_out = [], [], []
+    kind_of = {s_[0]: s_[3] for s_ in slots}
     cands = [(_u(n), "T", f"{_u(n)}[i]", n.startswith("g_")) for n in local_names]
     if has_ref:
         cands.append(("refrac_rem", "int32_t", "a.refractory_remaining[i]", False))
+    for n in local_names:
+        # a psp sum the psp may leave in panel partials: completed (and
+        # stored) by the update, read or not
+        if kind_of.get(n) == 2 and not lreads(_u(n)):
+            reg_pre.append(f"    if (a.n_psum > 0 && i < a.size) (void)ahip::psp_sum<T>(a, {slot_of[n]}, {_u(n)}, i);")
     for c, ty, mem, after in cands:
         if not lreads(c):
             if lwrites(c):
@@ -758,7 +764,11 @@ def generate(desc: NeuronDescription, stop_condition: str | None = None) -> Mode
                 for key in lparts:
                     lparts[key] = re.sub(lref(c), mem.replace("\\", "\\\\"), lparts[key])
             continue
-        (reg_post if after else reg_pre).append(f"    {ty} {c}__r = i < a.size ? {mem} : {ty}(0);")
+        load = mem
+        name = {_u(n): n for n in local_names}.get(c)
+        if name is not None and kind_of.get(name) == 2:
+            load = f"ahip::psp_sum<T>(a, {slot_of[name]}, {c}, i)"
+        (reg_post if after else reg_pre).append(f"    {ty} {c}__r = i < a.size ? {load} : {ty}(0);")
         if lwrites(c):
             reg_out.append(f"        {mem} = {c}__r;")
         for key in lparts:

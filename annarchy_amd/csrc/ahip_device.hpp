@@ -134,6 +134,35 @@ __device__ inline float group_sum_kahan_rt(float s, float c, int w) {
 
 __device__ inline int64_t sim_time(const ahip_pop_args& a) { return *a.t_base + a.step; }
 
+// The panel partial sums of row r of a sliced-tile psp, added in ascending
+// panel order (FP32: compensated), the one sequence both the psp's own
+// reduction and an update that completes the sum use.
+__device__ inline double panel_sum(const double* __restrict__ part, int32_t n_panels, int64_t stride, int64_t r) {
+    double s = 0.0;
+    for (int32_t q = 0; q < n_panels; q++) s += part[(int64_t)q * stride + r];
+    return s;
+}
+__device__ inline float panel_sum(const float* __restrict__ part, int32_t n_panels, int64_t stride, int64_t r) {
+    Kahan k{0.0f, 0.0f};
+    for (int32_t q = 0; q < n_panels; q++) k.add(part[(int64_t)q * stride + r]);
+    return k.value();
+}
+
+// Psp sum slot `slot` of neuron i for the update: completed from the panel
+// partials when the psp left it to the update (and stored, so the slot holds
+// it after the step), else the slot's value.
+template <typename T>
+__device__ inline T psp_sum(const ahip_pop_args& a, int32_t slot, T* x, int32_t i) {
+    for (int k = 0; k < a.n_psum && k < AHIP_MAX_PSUM; k++)
+        if (a.psum_slot[k] == slot) {
+            const T v = panel_sum(static_cast<const T*>(a.psum_part[k]), a.psum_panels[k], a.psum_stride[k], i);
+            const T y = a.psum_acc[k] ? x[i] + v : v;
+            x[i] = y;
+            return y;
+        }
+    return x[i];
+}
+
 // Random variable k of neuron i at the current step: injected stream when one
 // is set (row = t - rand_t0, wrapped into the rows available), else Philox.
 template <typename T>

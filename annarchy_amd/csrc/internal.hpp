@@ -98,6 +98,7 @@ struct Proj {
     int32_t target_slot = -1;     // slot of post: _sum_<target> (rate) or g_<target> (spiking)
     int32_t pre_slot = -1;        // rate: slot "r" of pre
     bool accumulate = false;      // rate: add into the target instead of overwriting it
+    bool defer_reduce = false;    // rate, panel layout: the post update adds the panel partial sums
     bool single_w = false;
     double w_const = 0.0;
     bool transmission = true;     // _transmission: compute_psp (and the learning rules) on / off
@@ -278,8 +279,10 @@ struct ahip_net {
 namespace ahip {
 
 // ---- kernel launchers (return hipError_t as int) ---------------------------
+// reduce = false: a sliced-tile layout leaves its panel partial sums to the
+// post population's update (ahip_pop_args.psum_*)
 int launch_rate_psp(const ahip_net* net, const Proj& p, const void* pre_r, void* target,
-                    hipStream_t s);
+                    hipStream_t s, bool reduce = true);
 int launch_spike_psp(const ahip_net* net, const Proj& p, const Pop& pre, void* target,
                      hipStream_t s);
 // the spike compactions of the populations `pops` (one launch per 8)

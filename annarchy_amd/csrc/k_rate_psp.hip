@@ -447,13 +447,11 @@ __global__ __launch_bounds__(kBlock) void tile_reduce(int32_t n_rows, int32_t n_
                                                       int32_t acc) {
     const int32_t r = blockIdx.x * kBlock + threadIdx.x;
     if (r >= n_rows) return;
-    Acc<T> a;
-    for (int32_t q = 0; q < n_panels; q++) a.add(part[(int64_t)q * n_rows + r]);
-    store_target(y, r, a.template reduce<1>(), acc != 0);
+    store_target(y, r, panel_sum(part, n_panels, n_rows, r), acc != 0);
 }
 
 template <typename T>
-int launch_rate_t(const ahip_net* net, const Proj& p, const T* x, T* y, hipStream_t s) {
+int launch_rate_t(const ahip_net* net, const Proj& p, const T* x, T* y, hipStream_t s, bool reduce) {
     const RateLayout& L = p.rate;
     const int32_t acc = p.accumulate ? 1 : 0;
     (void)net;
@@ -598,6 +596,8 @@ int launch_rate_t(const ahip_net* net, const Proj& p, const T* x, T* y, hipStrea
             }
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) return (int)e;
+            // reduce = false: the post population's update adds the panels
+            if (!reduce) return 0;
             hipLaunchKernelGGL((tile_reduce<T>), dim3((p.n_rows + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
                                p.n_rows, L.n_units, part, y, acc);
             return (int)hipGetLastError();
@@ -628,6 +628,8 @@ int launch_rate_t(const ahip_net* net, const Proj& p, const T* x, T* y, hipStrea
             }
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) return (int)e;
+            // reduce = false: the post population's update adds the panels
+            if (!reduce) return 0;
             hipLaunchKernelGGL((tile_reduce<T>), dim3((p.n_rows + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
                                p.n_rows, L.n_units, part, y, acc);
             return (int)hipGetLastError();
@@ -640,10 +642,10 @@ int launch_rate_t(const ahip_net* net, const Proj& p, const T* x, T* y, hipStrea
 }  // namespace
 
 int launch_rate_psp(const ahip_net* net, const Proj& p, const void* pre_r, void* target,
-                    hipStream_t s) {
+                    hipStream_t s, bool reduce) {
     if (net->precision == AHIP_F64)
-        return launch_rate_t<double>(net, p, (const double*)pre_r, (double*)target, s);
-    return launch_rate_t<float>(net, p, (const float*)pre_r, (float*)target, s);
+        return launch_rate_t<double>(net, p, (const double*)pre_r, (double*)target, s, reduce);
+    return launch_rate_t<float>(net, p, (const float*)pre_r, (float*)target, s, reduce);
 }
 
 }  // namespace ahip

@@ -661,6 +661,34 @@ int prepare(ahip_net* net) {
     return 0;
 }
 
+// Rate projections whose panel partial sums the post population's update
+// adds instead of a reduction launch: a panel layout with rows = the post
+// population, the only projection writing its target, an active rate-coded
+// post population without a population-wide update kernel and without a
+// monitor of the target right after compute_psp, at most AHIP_MAX_PSUM per
+// population.
+static void assign_deferred_reduce(ahip_net* net) {
+    std::vector<int32_t> n_def(net->pops.size(), 0);
+    for (auto& pp : net->projs) {
+        Proj& p = *pp;
+        p.defer_reduce = false;
+        if (p.spiking || p.delay_per_syn || !transmits(net, p)) continue;
+        const Pop& post = *net->pops[p.post];
+        if (post.model->spiking || post.model->has_global_update) continue;
+        const RateLayout& L = p.rate;
+        if (L.format != AHIP_FMT_SELL_PANEL && L.format != AHIP_FMT_CSR_PANEL) continue;
+        if ((int64_t)L.n_units * L.width == 0 || p.n_rows != post.size || !L.perm.p) continue;
+        bool alone = true;
+        for (auto& qq : net->projs)
+            if (qq.get() != &p && qq->post == p.post && qq->target_slot == p.target_slot) alone = false;
+        for (auto& mp : net->mons)
+            if (mp->after_psp && mp->pop == p.post && mp->slot == p.target_slot) alone = false;
+        if (!alone || n_def[p.post] >= AHIP_MAX_PSUM) continue;
+        n_def[p.post]++;
+        p.defer_reduce = true;
+    }
+}
+
 // Which projections write first into their target.
 void assign_accumulate(ahip_net* net, std::vector<int32_t>& zero_before) {
     zero_before.assign(net->projs.size(), 0);
@@ -682,6 +710,7 @@ void assign_accumulate(ahip_net* net, std::vector<int32_t>& zero_before) {
             zero_before[p.id] = 1;
         }
     }
+    assign_deferred_reduce(net);
 }
 
 ahip_pop_args make_args(ahip_net* net, Pop& p, int32_t step) {
@@ -708,6 +737,16 @@ ahip_pop_args make_args(ahip_net* net, Pop& p, int32_t step) {
     a.block_counts = p.block_counts.as<int32_t>();
     a.deliver = p.deliver.as<ahip_deliver_desc>();
     a.n_deliver = p.n_deliver;
+    for (auto& pp : net->projs) {
+        const Proj& q = *pp;
+        if (!q.defer_reduce || q.post != p.id || a.n_psum >= AHIP_MAX_PSUM) continue;
+        const int32_t k = a.n_psum++;
+        a.psum_slot[k] = q.target_slot;
+        a.psum_panels[k] = q.rate.n_units;
+        a.psum_acc[k] = q.accumulate ? 1 : 0;
+        a.psum_stride[k] = q.n_rows;
+        a.psum_part[k] = q.rate.perm.p;
+    }
     return a;
 }
 
@@ -760,9 +799,9 @@ int enqueue_step(ahip_net* net, int32_t step, const std::vector<int32_t>& zero_b
             LCHK(launch_rate_psp_delayed(net, p, pre, target, step, s));
         } else if (p.delay_uniform >= 2) {
             LCHK(launch_gather_delayed(net, pre.hist.p, p.xd.p, pre.size, pre.depth, p.delay_uniform, step, s));
-            LCHK(launch_rate_psp(net, p, p.xd.p, target, s));
+            LCHK(launch_rate_psp(net, p, p.xd.p, target, s, !p.defer_reduce));
         } else {
-            LCHK(launch_rate_psp(net, p, pre.slots[p.pre_slot].p, target, s));
+            LCHK(launch_rate_psp(net, p, pre.slots[p.pre_slot].p, target, s, !p.defer_reduce));
         }
     }
     // 2. record targets

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define AHIP_MODEL_ABI 9
+#define AHIP_MODEL_ABI 10
 #define AHIP_MAX_SLOTS 48
 #define AHIP_MAX_RAND 8
 #define AHIP_UPDATE_BLOCK 256 /* threads per block of every update kernel */
@@ -46,6 +46,7 @@ typedef struct ahip_deliver_desc {
 #define AHIP_HEAD_SHIFT 24
 #define AHIP_HEAD_COUNT ((1ull << AHIP_HEAD_SHIFT) - 1ull)
 #define AHIP_MAX_DELIVER 4 /* ordered conductances one update delivers */
+#define AHIP_MAX_PSUM 2    /* psp sums one update completes from panel partials */
 
 /* Arguments of one population update launch.  Device pointers unless noted. */
 typedef struct ahip_pop_args {
@@ -66,7 +67,17 @@ typedef struct ahip_pop_args {
     int32_t* block_counts;              /* ceil(size/256) spike counts per update block */
     const ahip_deliver_desc* deliver;   /* ordered conductances delivered by this update, or NULL */
     int32_t n_deliver;
-    int32_t reserved;
+    /* psp sums the update completes: slot psum_slot[k] (a _sum_<target>) is
+     * the sum over the panels q < psum_panels[k] of psum_part[k][q *
+     * psum_stride[k] + i], added in ascending q (compensated in FP32), plus
+     * the slot's own value when psum_acc[k]; the update stores it to the
+     * slot before its statements read it */
+    int32_t n_psum;
+    int32_t psum_slot[AHIP_MAX_PSUM];
+    int32_t psum_panels[AHIP_MAX_PSUM];
+    int32_t psum_acc[AHIP_MAX_PSUM];
+    int32_t psum_stride[AHIP_MAX_PSUM];
+    const void* psum_part[AHIP_MAX_PSUM];
 } ahip_pop_args;
 
 typedef int (*ahip_launch_fn)(const ahip_pop_args* args, void* stream);

@@ -201,6 +201,35 @@ def test_sell_panel_graph_run_matches_oracle(gpu):
     assert _rel_err(pop.r.reshape(-1), r_ref) < 1e-9
 
 
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_panel_sums_completed_by_the_update(gpu, precision):
+    # a panel layout's partial sums are added by the rate update itself
+    # (no reduction launch) unless a monitor reads the sum right after
+    # compute_psp: both networks give bit-identical r and sums (the same
+    # panel order, compensated in FP32), equal to the oracle within 1e-9
+    n, steps = 3000, 40
+    csr = _random_csr(n, 120, 8, ragged=True)
+    rng = np.random.default_rng(12)
+    r0 = rng.random(n)
+    noise = rng.normal(0.0, 0.1, (steps, n))
+    out = []
+    for watch_sum in (False, True):
+        net = ah.Network(dt=1.0, seed=1, precision=precision)
+        pop = net.create(n, ah.RateNoisy)
+        net.connect(pop, pop, "exc").from_csr(csr, storage_format="sell_panel", lanes=2, sigma=8,
+                                              panel_cols=700, chunk=512)
+        mon = net.monitor(pop, ["_sum_exc"] if watch_sum else ["r"], capacity=steps)
+        net.compile(graph_batch=16)
+        pop.r = r0
+        pop.inject_random(0, noise)
+        net.run(steps)
+        out.append((pop.r.reshape(-1).copy(), pop.get("_sum_exc").reshape(-1).copy()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    if precision == "double":
+        r_ref, _, _ = orc.rate_run(csr.row_ptr, csr.col_idx, csr.w, r0, 10.0, 1.0, noise, steps)
+        assert _rel_err(out[0][0], r_ref) < 1e-9
+
+
 def test_rate_graph_and_eager_agree_bitwise(gpu):
     n, steps = 1000, 40
     csr = _random_csr(n, 50, 4)
