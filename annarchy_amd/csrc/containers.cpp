@@ -77,6 +77,18 @@ bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 // Panel width for at most `pmax` columns per panel: the same number of
 // panels, all of (about) equal width, so the last panel is not a sliver with
 // too little work per tile.
+// Compute units of the current device (256 on MI355X; that figure when no
+// device answers, as on a build host).
+static int32_t device_cus() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) {
+        (void)hipGetLastError();
+        return 256;
+    }
+    return cus;
+}
+
 static int32_t equal_panels(int32_t n_cols, int32_t pmax) {
     if (n_cols <= 0 || pmax <= 0) return pmax;
     const int32_t np = (n_cols + pmax - 1) / pmax;
@@ -305,16 +317,19 @@ int build_layout(const Proj& p, int32_t precision, const ahip_format_opts& o, Ra
             // of a row, -1 for the others.
             const size_t es = precision == AHIP_F64 ? 8 : 4;
             // automatic: panels of up to 20479 columns (160 KiB of LDS in
-            // FP64, one workgroup per CU), and the tallest tiles (2048 rows
-            // down to 256) that still give every CU a workgroup
+            // FP64, one workgroup per CU), and tiles of at most 2048 rows,
+            // as tall as lets every tile run in one round on the device's
+            // workgroup slots (CUs x the panels one CU's LDS holds): every
+            // slot busy, none waiting for a second round
             int32_t pc = o.panel_cols > 0 ? o.panel_cols : 20479;
             if (pc > 65535 || (size_t)(pc + 1) * es > 160 * 1024) { err = "panel exceeds 160 KiB of LDS or 65535 columns"; return AHIP_EINVAL; }
             pc = equal_panels(p.n_cols, pc);
             int32_t rb_rows = o.chunk;
             if (rb_rows <= 0) {
                 const int64_t panels = std::max<int32_t>(1, (p.n_cols + pc - 1) / pc);
-                rb_rows = 2048;
-                while (rb_rows > 256 && panels * ((R + rb_rows - 1) / rb_rows) < 256) rb_rows /= 2;
+                const int64_t slots = (int64_t)device_cus() * std::max<int64_t>(1, (160 * 1024) / ((int64_t)(pc + 1) * es));
+                const int64_t nrb = std::max<int64_t>(1, slots / panels);
+                rb_rows = (int32_t)std::min<int64_t>(2048, std::max<int64_t>(256, ((R + nrb - 1) / nrb + 63) / 64 * 64));
             }
             if (rb_rows < 64 || rb_rows > 8192 || rb_rows % 64) { err = "tile rows must be a multiple of 64 in 64..8192"; return AHIP_EINVAL; }
             const int32_t steps = o.lanes_per_row > 0 ? o.lanes_per_row : 4;

@@ -194,7 +194,9 @@ def bench_c1(args):
     net = ah.Network(dt=1.0, seed=42, precision=args.precision)
     pop = net.create(args.neurons, ah.RateNoisy)
     proj = net.connect(pop, pop, "exc")
-    chunk = args.chunk or (2048 if args.precision == "double" else 1024)
+    # chunk 0: the builder sizes the tiles to the device's workgroup slots
+    # (FP64 1e5 rows: 1984-row tiles, 255 workgroups; FP32: 1024 rows)
+    chunk = args.chunk
     proj.from_csr(csr, storage_format=args.format, lanes=args.lanes,
                   sigma=args.waves if args.format == "sell_panel" else args.sigma,
                   panel_cols=args.panel_cols, chunk=chunk)
@@ -846,7 +848,7 @@ def bench_c4(args):
     sp_default = c1_pick.get("fmt") == "sell_panel" and c1_pick.get("lanes") == 2 and c1_pick.get("sigma") == 8 \
         and c1_pick.get("chunk") == 2048 and c1_pick.get("panel_cols") == 20479
     roof = roofline("hbm", ach, HBM_PEAK_GBS, "GB/s",
-                    traffic_from_profiles("sell_panel_psp_f64_c1_fixed1000_traffic.txt") if sp_default else None,
+                    traffic_from_profiles("sell_panel_psp_f64_rb2048_c1_fixed1000_traffic.txt") if sp_default else None,
                     f"psp of the selected configuration {c1_pick} on the 1e8-synapse rate network",
                     kernel_ms=ms, algorithmic_bytes=pb, matrix_nnz=int(c1.nnz))
     net.destroy()
