@@ -340,24 +340,56 @@ int build_layout(const Proj& p, int32_t precision, const ahip_format_opts& o, Ra
                 return AHIP_EINVAL;
             }
             const int32_t np = std::max<int32_t>(1, (p.n_cols + pc - 1) / pc);
-            const int32_t nrb = (R + rb_rows - 1) / rb_rows;
+            // ceil(R / rb_rows) row blocks whose boundaries split the synapses
+            // evenly (the psp ends with its heaviest tile; uneven in-degrees
+            // would make equal-height blocks uneven), a block at most 65535
+            // rows (the tile's row field) and 4 x rb_rows
+            std::vector<int32_t> row0(1, 0);
+            {
+                const int32_t nrb0 = std::max<int32_t>(1, (R + rb_rows - 1) / rb_rows);
+                const int64_t nnz = rp[R] - rp[0];
+                const int32_t hcap = std::min<int32_t>(65535, 4 * rb_rows);
+                int32_t r = 0;
+                for (int32_t k = 1; k < nrb0 && r < R; k++) {
+                    const int64_t target = rp[0] + (nnz * k + nrb0 - 1) / nrb0;
+                    int32_t e = r;
+                    while (e < R && rp[e] < target && e - r < hcap) e++;
+                    // whole 64-row slices: the height rounded to the nearest
+                    // multiple of 64 (equal in-degrees keep equal blocks)
+                    const int32_t h = std::min<int32_t>({std::max<int32_t>(64, (e - r + 32) / 64 * 64), hcap, R - r});
+                    row0.push_back(r + h);
+                    r += h;
+                }
+                while (R - r > hcap) {
+                    row0.push_back(r + hcap);
+                    r += hcap;
+                }
+                if (r < R || row0.size() == 1) row0.push_back(R);
+            }
+            const int32_t nrb = (int32_t)row0.size() - 1;
+            int32_t hmax = 1;
+            std::vector<int32_t> rbof(R);
+            for (int32_t b = 0; b < nrb; b++) {
+                hmax = std::max(hmax, row0[b + 1] - row0[b]);
+                for (int32_t r = row0[b]; r < row0[b + 1]; r++) rbof[r] = b;
+            }
             const int64_t ntiles = (int64_t)np * nrb;
-            std::vector<int32_t> cnt((size_t)ntiles * rb_rows, 0);
+            std::vector<int32_t> cnt((size_t)ntiles * hmax, 0);
             for (int32_t r = 0; r < R; r++) {
-                const int32_t rb = r / rb_rows, lr = r % rb_rows;
+                const int32_t rb = rbof[r], lr = r - row0[rb];
                 for (int64_t j = rp[r]; j < rp[r + 1]; j++)
-                    cnt[((size_t)(ci[j] / pc) * nrb + rb) * rb_rows + lr]++;
+                    cnt[((size_t)(ci[j] / pc) * nrb + rb) * hmax + lr]++;
             }
             // slot[(t, lr)] = slice-in-tile << 12 | first lane << 6 | (lanes - 1)
-            std::vector<int32_t> slot((size_t)ntiles * rb_rows, 0);
+            std::vector<int32_t> slot((size_t)ntiles * hmax, 0);
             std::vector<int32_t> tsl(ntiles + 1, 0);
             std::vector<int64_t> off(1, 0);
             std::vector<int32_t> rowof;
-            std::vector<int32_t> order(rb_rows);
+            std::vector<int32_t> order(hmax);
             for (int64_t t = 0; t < ntiles; t++) {
                 const int32_t rb = (int32_t)(t % nrb);
-                const int32_t nr = std::min(rb_rows, R - rb * rb_rows);
-                const int32_t* c = cnt.data() + (size_t)t * rb_rows;
+                const int32_t nr = row0[rb + 1] - row0[rb];
+                const int32_t* c = cnt.data() + (size_t)t * hmax;
                 int64_t tile_nnz = 0;
                 for (int32_t lr = 0; lr < nr; lr++) tile_nnz += c[lr];
                 const int64_t cap = std::max<int64_t>(64, (2 * tile_nnz / nr + 3) / 4 * 4);
@@ -376,7 +408,7 @@ int build_layout(const Proj& p, int32_t precision, const ahip_format_opts& o, Ra
                         const int64_t g = used == 0 ? g0 : std::max<int64_t>(1, W > 0 ? (len + 4 * W - 1) / (4 * W) : 1);
                         if (used + g > 64) break;
                         ro[used] = (int32_t)((g << 16) | order[i]);
-                        slot[(size_t)t * rb_rows + order[i]] = (ns << 12) | (used << 6) | (int32_t)(g - 1);
+                        slot[(size_t)t * hmax + order[i]] = (ns << 12) | (used << 6) | (int32_t)(g - 1);
                         used += (int32_t)g;
                         i++;
                     }
@@ -389,12 +421,12 @@ int build_layout(const Proj& p, int32_t precision, const ahip_format_opts& o, Ra
             std::vector<double> val(total, 0.0);
             std::fill(cnt.begin(), cnt.end(), 0);
             for (int32_t r = 0; r < R; r++) {
-                const int32_t rb = r / rb_rows, lr = r % rb_rows;
+                const int32_t rb = rbof[r], lr = r - row0[rb];
                 for (int64_t j = rp[r]; j < rp[r + 1]; j++) {
                     const int32_t q = ci[j] / pc;
                     const int64_t t = (int64_t)q * nrb + rb;
-                    const int64_t k = cnt[(size_t)t * rb_rows + lr]++;
-                    const int32_t sv = slot[(size_t)t * rb_rows + lr];
+                    const int64_t k = cnt[(size_t)t * hmax + lr]++;
+                    const int32_t sv = slot[(size_t)t * hmax + lr];
                     const int64_t gs = tsl[t] + (sv >> 12);
                     const int64_t b = (sv >> 6) & 63, g = (sv & 63) + 1;
                     const int64_t e = k / 4;
@@ -413,6 +445,7 @@ int build_layout(const Proj& p, int32_t precision, const ahip_format_opts& o, Ra
             if ((rc = upload(L.ptr, off.data(), off.size() * sizeof(int64_t), err))) return rc;
             if ((rc = upload(L.aux, rowof.data(), rowof.size() * sizeof(int32_t), err))) return rc;
             if ((rc = upload(L.tiles, tsl.data(), tsl.size() * sizeof(int32_t), err))) return rc;
+            if ((rc = upload(L.rl, row0.data(), row0.size() * sizeof(int32_t), err))) return rc;  // row-block starts
             if ((rc = upload(L.idx, col.data(), col.size() * sizeof(uint16_t), err))) return rc;
             if ((rc = upload_vals(L.val, val, precision, err))) return rc;
             if ((rc = upload(L.perm, nullptr, (size_t)np * R * es, err))) return rc;  // partial sums

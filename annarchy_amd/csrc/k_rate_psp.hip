@@ -353,6 +353,7 @@ __global__ __launch_bounds__(WAVES * kWave) void sell_panel_psp(int32_t n_rows, 
                                                          const int64_t* __restrict__ off,
                                                          const int32_t* __restrict__ rowof,
                                                          const int32_t* __restrict__ tile_slices,
+                                                         const int32_t* __restrict__ rb_row0,
                                                          const uint16_t* __restrict__ col,
                                                          const T* __restrict__ w,
                                                          const T* __restrict__ x, T* __restrict__ part) {
@@ -382,7 +383,8 @@ __global__ __launch_bounds__(WAVES * kWave) void sell_panel_psp(int32_t n_rows, 
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
-    T* pq = part + (int64_t)q * n_rows + (int64_t)rb * rb_rows;
+    (void)rb_rows;
+    T* pq = part + (int64_t)q * n_rows + rb_row0[rb];
     const int32_t s0 = tile_slices[t], s_end = tile_slices[t + 1];
     // slices are sorted by length (descending); in every round of 2*WAVES
     // slices wave w takes the w-th and the (2*WAVES-1-w)-th, so each wave's
@@ -613,7 +615,7 @@ int launch_rate_t(const ahip_net* net, const Proj& p, const T* x, T* y, hipStrea
         hipLaunchKernelGGL((sell_panel_psp<T, N, W>), dim3(ntiles), dim3(W * kWave), lds, s,        \
                            p.n_rows, p.n_cols, L.panel_cols, L.chunk, L.width,                      \
                            L.ptr.as<int64_t>(), L.aux.as<int32_t>(), L.tiles.as<int32_t>(),         \
-                           L.idx.as<uint16_t>(), L.val.as<T>(), x, part);                           \
+                           L.rl.as<int32_t>(), L.idx.as<uint16_t>(), L.val.as<T>(), x, part);       \
         break;
                 AHIP_SP_CASE(1, 4)
                 AHIP_SP_CASE(2, 4)
